@@ -1,0 +1,42 @@
+// ORACLE (test infrastructure only): C entry points for the Python tests (ctypes). Arrays are
+// caller-owned host buffers; each function restates one reference routine (see the C++ side).
+#include <cstring>
+
+#include "cfa_oracle.hpp"
+
+using namespace oracle;
+
+extern "C" {
+
+// Fixed-mesh run. rows: steps x 13 doubles (step, t, C_D, C_L, dP, nodes, elements, p_front,
+// p_back, C_D_surface, C_L_surface, it_u + 1000 it_v, it_p); mms: 9 doubles (6 norms, cyl, probe,
+// h_max). Returns the number of rows written.
+int oracle_run_fixed(int steps, double dt, int uniform_level, int min_level, int mms, double mom_tol,
+                     double ppe_tol, const char* prefix, double* rows, double* mms_out, double* wall_s) {
+    RunConfig c;
+    c.steps = steps;
+    c.dt = dt;
+    c.uniform_level = uniform_level;
+    c.min_level = min_level;
+    c.mms = mms != 0;
+    c.mom_tol = mom_tol;
+    c.ppe_tol = ppe_tol;
+    c.quiet = true;
+    if (prefix) c.prefix = prefix;
+    RunResult r = run_fixed(c);
+    for (size_t k = 0; k < r.rows.size(); ++k) {
+        const StepRow& s = r.rows[k];
+        double* o = rows + 13 * k;
+        o[0] = s.step;  o[1] = s.t;  o[2] = s.cd;  o[3] = s.cl;  o[4] = s.dp;  o[5] = s.nodes;
+        o[6] = s.elems;  o[7] = s.p_front;  o[8] = s.p_back;  o[9] = s.cd_s;  o[10] = s.cl_s;
+        o[11] = s.it_u + 1000.0 * s.it_v;  o[12] = s.it_p;
+    }
+    if (mms_out) {
+        for (int k = 0; k < 6; ++k) mms_out[k] = r.mms[k];
+        mms_out[6] = r.cyl_max;  mms_out[7] = r.probe_max;  mms_out[8] = r.h_max;
+    }
+    if (wall_s) *wall_s = r.wall_s;
+    return (int)r.rows.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Device-side helpers shared by the gfx950 kernels: error mapping, wave64 reductions,
+// deterministic block partial sums, CSR row products with several lanes per row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../../include/cfa_hip.h"
+
+namespace cfa_dev {
+
+constexpr int WAVE = 64;     // CDNA wavefront width
+constexpr int BLOCK = 256;   // default workgroup: 4 waves
+
+// Boundary tags (src/cfa/common.cuh:68-74)
+constexpr int BC_INTERIOR = 0, BC_INLET = 1, BC_OUTLET = 2, BC_WALL = 3, BC_CYLINDER = 4;
+
+inline hipStream_t as_stream(cfa_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int launch_status() {
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? CFA_OK : (int)e;
+}
+
+inline int blocks_for(long long n, int block = BLOCK) {
+    return (int)((n + block - 1) / block);
+}
+
+// Sum over a wavefront; lane 0 holds the result.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_down(v, o, WAVE);
+    return v;
+}
+
+// Sum over the workgroup in a fixed order (lanes by tree, then waves in index order); the result
+// is valid in thread 0. lds: at least blockDim.x / 64 doubles.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + WAVE - 1) / WAVE;
+        for (int w = 0; w < nw; ++w) s += lds[w];
+    }
+    __syncthreads();
+    return s;
+}
+
+// Sum of `count` partials (stride `stride`) by one workgroup in a fixed order; valid in all threads.
+__device__ __forceinline__ double sum_partials(const double* part, int count, int stride, double* lds) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < count; k += blockDim.x) s += part[(size_t)k * stride];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) lds[0] = s;
+    __syncthreads();
+    s = lds[0];
+    __syncthreads();
+    return s;
+}
+
+// Sum over TPR consecutive lanes (TPR a power of two <= 64); lane 0 of the group holds it.
+template <int TPR>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int o = TPR / 2; o > 0; o >>= 1) v += __shfl_down(v, o, TPR);
+    return v;
+}
+
+// Row product sum_k val[k] x[ci[k]] over [s, e) with TPR lanes (lane = rank inside the group)
+template <int TPR>
+__device__ __forceinline__ double row_dot(const int* __restrict__ ci, const double* __restrict__ val,
+                                          const double* __restrict__ x, int s, int e, int lane) {
+    double acc = 0.0;
+    for (int k = s + lane; k < e; k += TPR) acc += val[k] * x[ci[k]];
+    return group_sum<TPR>(acc);
+}
+
+}  // namespace cfa_dev

@@ -1,0 +1,1303 @@
+// Sparse linear algebra for gfx950: CSR SpMV (several lanes per row, wave64 reductions), dot
+// products with fixed-order two-stage reductions, multicolour ILU(0), the SA-AMG V-cycle and the
+// Krylov solvers of src/cfa/linalg.cuh.
+//
+// Krylov design: the reference copies every dot product to the host (three blocking transfers
+// per CG iteration, linalg.cuh:172-177). Here all scalars stay on the device: a one-workgroup
+// "final" kernel reduces the block partials in a fixed order and applies the scalar logic of the
+// reference (coefficients, breakdown guards, stopping rules); every vector kernel of the
+// iteration reads a device state word and returns at once when the solve has stopped. The
+// host enqueues iterations ahead and follows a progress word that the final kernels write to
+// mapped pinned memory, so the queue never drains between iterations.
+#include <algorithm>
+#include <cmath>
+#include <thread>
+
+#include "cfa_device.hpp"
+
+using namespace cfa_dev;
+
+namespace {
+
+constexpr int RED_BLOCKS = 1024;   // partial sums per reduction (grid-stride kernels)
+constexpr int OP_BLOCKS = 4096;    // grid cap of the fused operator kernels (partials <= workspace)
+
+// Solve state (device ints after the scalar area of the workspace)
+enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_STOPPED = 2 };
+
+__device__ __forceinline__ bool halted(const int* state) { return state && state[0] != ST_RUNNING; }
+
+__device__ __forceinline__ void publish(int* status, int progress, int state) {
+    if (!status) return;
+    __hip_atomic_store(&status[1], state, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&status[0], progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// ---- CSR products --------------------------------------------------------------------------------
+enum CsrOp { OP_SPMV, OP_ADD, OP_RESID, OP_JACOBI };
+
+// y = A x | y += A x | y = b - A x | y = x + omega dinv (b - A x)   (src/cfa/linalg.cuh:884-901)
+template <int TPR, int OP>
+__global__ void __launch_bounds__(BLOCK)
+k_csr_op(int rows, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ val,
+         const double* __restrict__ x, const double* __restrict__ b, const double* __restrict__ dinv,
+         double omega, double* __restrict__ y, const int* __restrict__ state) {
+    if (halted(state)) return;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = g / TPR, lane = g % TPR;
+    double s = 0.0;
+    if (row < rows) {
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) s += val[k] * x[ci[k]];
+    }
+    s = group_sum<TPR>(s);
+    if (row >= rows || lane != 0) return;
+    if (OP == OP_SPMV) y[row] = s;
+    else if (OP == OP_ADD) y[row] += s;
+    else if (OP == OP_RESID) y[row] = b[row] - s;
+    else y[row] = x[row] + omega * dinv[row] * (b[row] - s);
+}
+
+// First two steps of the V-cycle smoother fused: x0 = omega dinv b (src/cfa/linalg.cuh:864-868),
+// y = x0 + omega dinv (b - A x0), with x0 formed on the fly from b and dinv at the neighbours.
+template <int TPR>
+__global__ void __launch_bounds__(BLOCK)
+k_jacobi_from_zero(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
+                   const double* __restrict__ val, const double* __restrict__ b, const double* __restrict__ dinv,
+                   double omega, double* __restrict__ y, const int* __restrict__ state) {
+    if (halted(state)) return;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = g / TPR, lane = g % TPR;
+    double s = 0.0;
+    if (row < rows)
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
+            const int j = ci[k];
+            s += val[k] * (omega * dinv[j] * b[j]);
+        }
+    s = group_sum<TPR>(s);
+    if (row >= rows || lane != 0) return;
+    const double x0 = omega * dinv[row] * b[row];
+    y[row] = x0 + omega * dinv[row] * (b[row] - s);
+}
+
+__global__ void k_jacobi_init(int n, double omega, const double* __restrict__ dinv, const double* __restrict__ b,
+                              double* __restrict__ x, const int* __restrict__ state) {
+    if (halted(state)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = omega * dinv[i] * b[i];
+}
+
+// y = Ainv x with the dense coarse inverse, one wavefront per row (src/cfa/linalg.cuh:929-937)
+__global__ void __launch_bounds__(BLOCK)
+k_dense_matvec(int n, const double* __restrict__ A, const double* __restrict__ x, double* __restrict__ y,
+               const int* __restrict__ state) {
+    if (halted(state)) return;
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+    double s = 0.0;
+    if (i < n)
+        for (int j = lane; j < n; j += WAVE) s += A[(size_t)i * n + j] * x[j];
+    s = wave_sum(s);
+    if (i < n && lane == 0) y[i] = s;
+}
+
+// The levels from `first` down to the coarse solve and back, inside one workgroup: every phase
+// is a pass over the rows of one level, separated by workgroup barriers. Used for the small
+// levels, where a kernel launch per phase would cost more than the work.
+__device__ __forceinline__ double row_product(const int* __restrict__ rp, const int* __restrict__ ci,
+                                              const double* __restrict__ v, const double* __restrict__ x, int row) {
+    double s = 0.0;
+    for (int k = rp[row]; k < rp[row + 1]; ++k) s += v[k] * x[ci[k]];
+    return s;
+}
+
+__global__ void __launch_bounds__(1024)
+k_vcycle_small(const cfa_amg_level_t* __restrict__ lv, int first, int n_levels, int smooth, int coarse_n,
+               const double* __restrict__ cinv, double* __restrict__ cb, double* __restrict__ cx,
+               const int* __restrict__ state) {
+    if (halted(state)) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    // downward sweep
+    for (int l = first; l < n_levels; ++l) {
+        const cfa_amg_level_t L = lv[l];
+        const double* b = L.b;
+        double* xa = L.x;
+        double* xb = L.ax;
+        for (int i = tid; i < L.rows; i += nt) xa[i] = L.omega * L.dinv[i] * b[i];
+        __syncthreads();
+        for (int s = 1; s < smooth; ++s) {
+            for (int i = tid; i < L.rows; i += nt)
+                xb[i] = xa[i] + L.omega * L.dinv[i] * (b[i] - row_product(L.a_rp, L.a_ci, L.a_v, xa, i));
+            __syncthreads();
+            double* t = xa; xa = xb; xb = t;
+        }
+        for (int i = tid; i < L.rows; i += nt) L.r[i] = b[i] - row_product(L.a_rp, L.a_ci, L.a_v, xa, i);
+        __syncthreads();
+        double* next_b = (l + 1 < n_levels) ? lv[l + 1].b : cb;
+        for (int i = tid; i < L.cols; i += nt) next_b[i] = row_product(L.r_rp, L.r_ci, L.r_v, L.r, i);
+        __syncthreads();
+        // keep the pre-smoothed iterate in L.x for the upward sweep
+        if (xa != L.x) {
+            for (int i = tid; i < L.rows; i += nt) L.x[i] = xa[i];
+            __syncthreads();
+        }
+    }
+    // coarse solve
+    for (int i = tid; i < coarse_n; i += nt) {
+        double s = 0.0;
+        for (int j = 0; j < coarse_n; ++j) s += cinv[(size_t)i * coarse_n + j] * cb[j];
+        cx[i] = s;
+    }
+    __syncthreads();
+    // upward sweep
+    for (int l = n_levels - 1; l >= first; --l) {
+        const cfa_amg_level_t L = lv[l];
+        const double* b = L.b;
+        const double* xc = (l + 1 < n_levels) ? lv[l + 1].x : cx;
+        double* xa = L.x;
+        double* xb = L.ax;
+        for (int i = tid; i < L.rows; i += nt) xa[i] += row_product(L.p_rp, L.p_ci, L.p_v, xc, i);
+        __syncthreads();
+        for (int s = 0; s < smooth; ++s) {
+            for (int i = tid; i < L.rows; i += nt)
+                xb[i] = xa[i] + L.omega * L.dinv[i] * (b[i] - row_product(L.a_rp, L.a_ci, L.a_v, xa, i));
+            __syncthreads();
+            double* t = xa; xa = xb; xb = t;
+        }
+        // the level's result always ends in L.x (read by the next finer level)
+        if (xa != L.x) {
+            for (int i = tid; i < L.rows; i += nt) L.x[i] = xa[i];
+            __syncthreads();
+        }
+    }
+}
+
+// ---- reductions ------------------------------------------------------------------------------------
+// partial[b] = sum over the block's grid-stride share of x . y
+__global__ void __launch_bounds__(BLOCK)
+k_dot_partial(int n, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ part,
+              const int* __restrict__ state) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (halted(state)) return;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += x[i] * y[i];
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ void k_sum_to(int nb, const double* __restrict__ part, double* __restrict__ out, int take_sqrt) {
+    __shared__ double lds[BLOCK / WAVE];
+    const double s = sum_partials(part, nb, 1, lds);
+    if (threadIdx.x == 0) *out = take_sqrt ? sqrt(s) : s;
+}
+
+inline int red_blocks(long long n) {
+    int nb = blocks_for(n > 0 ? n : 1);
+    return nb > RED_BLOCKS ? RED_BLOCKS : nb;
+}
+
+// ---- elementwise ------------------------------------------------------------------------------------
+__global__ void k_axpy(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+__global__ void k_xpay(int n, const double* __restrict__ x, double b, double* __restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = x[i] + b * y[i];
+}
+__global__ void k_vec_mul(int n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out,
+                          const int* __restrict__ state) {
+    if (halted(state)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i] * b[i];
+}
+__global__ void k_fill_zero(int n, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = 0.0;
+}
+__global__ void k_copy(int n, const double* __restrict__ a, double* __restrict__ out, const int* __restrict__ state) {
+    if (halted(state)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = a[i];
+}
+// inverse diagonal, 1 where |d| <= eps (src/cfa/linalg.cuh:28-48 with eps 1e-14, :871-878 with
+// eps 1e-300; a missing diagonal counts as 1 resp. 0, both giving 1)
+__global__ void k_inv_diag(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ val, double eps, double* __restrict__ inv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = eps > 1.0e-100 ? 1.0 : 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] == i) { d = val[k]; break; }
+    inv[i] = (fabs(d) > eps) ? (1.0 / d) : 1.0;
+}
+
+template <int TPR, int NRHS>
+__global__ void __launch_bounds__(BLOCK)
+k_spmv_multi(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ val,
+             const double* __restrict__ x, double* __restrict__ y) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = g / TPR, lane = g % TPR;
+    double s[NRHS];
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) s[c] = 0.0;
+    if (row < n)
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
+            const int j = ci[k];
+            const double a = val[k];
+#pragma unroll
+            for (int c = 0; c < NRHS; ++c) s[c] += a * x[(size_t)c * n + j];
+        }
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) s[c] = group_sum<TPR>(s[c]);
+    if (row < n && lane == 0)
+#pragma unroll
+        for (int c = 0; c < NRHS; ++c) y[(size_t)c * n + row] = s[c];
+}
+
+// ======================================================================================================
+// PCG (src/cfa/linalg.cuh:183-256)
+// ======================================================================================================
+// scalar slots
+enum { S_BNORM = 0, S_RZ_OLD, S_PQ, S_ALPHA, S_RNORM, S_REL, S_BETA, S_TOL };
+// state ints: [0] state, [1] iterations
+
+// r = b - q; partial r.r
+__global__ void __launch_bounds__(BLOCK)
+k_pcg_residual(int n, const double* __restrict__ b, const double* __restrict__ q, double* __restrict__ r,
+               double* __restrict__ part) {
+    __shared__ double lds[BLOCK / WAVE];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double ri = b[i] + (-1.0) * q[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// alpha = rz_old / (p.q); breakdown when |p.q| < 1e-30
+__global__ void k_pcg_final_alpha(int nb, const double* __restrict__ part, double* __restrict__ sc,
+                                  int* __restrict__ st, int* __restrict__ status, int it) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (st[0] != ST_RUNNING) return;
+    const double pq = sum_partials(part, nb, 1, lds);
+    if (threadIdx.x != 0) return;
+    sc[S_PQ] = pq;
+    if (fabs(pq) < 1.0e-30) {
+        st[0] = ST_STOPPED;
+        st[1] = it;
+        publish(status, it, ST_STOPPED);
+        return;
+    }
+    sc[S_ALPHA] = sc[S_RZ_OLD] / pq;
+}
+
+// x += alpha p; r -= alpha q; partial r.r
+__global__ void __launch_bounds__(BLOCK)
+k_pcg_update_xr(int n, const double* __restrict__ sc, const double* __restrict__ p, const double* __restrict__ q,
+                double* __restrict__ x, double* __restrict__ r, double* __restrict__ part, const int* __restrict__ st) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (halted(st)) return;
+    const double alpha = sc[S_ALPHA];
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] + (-alpha) * q[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// ||r|| < tol ||b|| -> converged after it + 1 iterations
+__global__ void k_pcg_final_rnorm(int nb, const double* __restrict__ part, double* __restrict__ sc,
+                                  int* __restrict__ st, int* __restrict__ status, int it) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (st[0] != ST_RUNNING) return;
+    const double rr = sum_partials(part, nb, 1, lds);
+    if (threadIdx.x != 0) return;
+    const double r_norm = sqrt(rr);
+    sc[S_RNORM] = r_norm;
+    sc[S_REL] = r_norm / sc[S_BNORM];
+    if (r_norm < sc[S_TOL] * sc[S_BNORM]) {
+        st[0] = ST_CONVERGED;
+        st[1] = it + 1;
+        publish(status, it + 1, ST_CONVERGED);
+    }
+}
+
+// beta = rz_new / (rz_old + 1e-30); rz_old = rz_new (first == 1: only rz_old, the prologue)
+__global__ void k_pcg_final_beta(int nb, const double* __restrict__ part, double* __restrict__ sc,
+                                 const int* __restrict__ st, int* __restrict__ status, int it, int first) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (st[0] != ST_RUNNING) return;
+    const double rz = sum_partials(part, nb, 1, lds);
+    if (threadIdx.x != 0) return;
+    if (!first) sc[S_BETA] = rz / (sc[S_RZ_OLD] + 1.0e-30);
+    sc[S_RZ_OLD] = rz;
+    publish(status, it + 1, ST_RUNNING);
+}
+
+// p = z + beta p
+__global__ void k_pcg_p(int n, const double* __restrict__ z, const double* __restrict__ sc, double* __restrict__ p,
+                        const int* __restrict__ st) {
+    if (halted(st)) return;
+    const double beta = sc[S_BETA];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+// Fused operator, stage 1: q = Kp p (CSR) and g = M^-1 B p, both on the same row pattern
+template <int TPR>
+__global__ void __launch_bounds__(BLOCK)
+k_apply_K_B(int n, int nnz, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ val,
+            const double* __restrict__ oss, const double* __restrict__ inv_m, const double* __restrict__ in,
+            double* __restrict__ q, double* __restrict__ gx, double* __restrict__ gy, const int* __restrict__ st) {
+    if (halted(st)) return;
+    const int groups = gridDim.x * (blockDim.x / TPR);
+    const int lane = threadIdx.x % TPR;
+    for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < n; row += groups) {
+        double s = 0.0, sx = 0.0, sy = 0.0;
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
+            const double xj = in[ci[k]];
+            s += val[k] * xj;
+            sx += oss[k] * xj;
+            sy += oss[nnz + k] * xj;
+        }
+        s = group_sum<TPR>(s);
+        sx = group_sum<TPR>(sx);
+        sy = group_sum<TPR>(sy);
+        if (lane == 0) {
+            q[row] = s;
+            gx[row] = sx * inv_m[row];
+            gy[row] = sy * inv_m[row];
+        }
+    }
+}
+
+// Fused operator, stage 2: q -= B^T g on non-Dirichlet rows; partial p.q per block. The grid is
+// capped (grid-stride over rows) so the partials fit the workspace.
+template <int TPR>
+__global__ void __launch_bounds__(BLOCK)
+k_apply_BT_dot(int n, int nnz, const int* __restrict__ rp, const int* __restrict__ ci,
+               const double* __restrict__ oss, const int* __restrict__ tag, const double* __restrict__ gx,
+               const double* __restrict__ gy, const double* __restrict__ p, double* __restrict__ q,
+               double* __restrict__ part, const int* __restrict__ st) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (halted(st)) return;
+    const double* BTx = oss + 2 * (size_t)nnz;
+    const double* BTy = oss + 3 * (size_t)nnz;
+    const int groups = gridDim.x * (blockDim.x / TPR);
+    const int lane = threadIdx.x % TPR;
+    double contrib = 0.0;
+    for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < n; row += groups) {
+        const bool active = tag[row] != BC_OUTLET;
+        double s = 0.0;
+        if (active)
+            for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
+                const int c = ci[k];
+                s += BTx[k] * gx[c] + BTy[k] * gy[c];
+            }
+        s = group_sum<TPR>(s);
+        if (lane == 0) {
+            double qi = q[row];
+            if (active) {
+                qi -= s;
+                q[row] = qi;
+            }
+            contrib += p[row] * qi;
+        }
+    }
+    contrib = block_sum(contrib, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = contrib;
+}
+
+// ======================================================================================================
+// multicolour ILU(0) (src/cfa/linalg.cuh:361-568)
+// ======================================================================================================
+// Rows of colour c (c >= 1): l_ik = a_ik / u_kk for the earlier-colour entries k in colour order,
+// a_ij -= l_ik u_kj on the later-colour entries of the pattern; updated in place in lu (own row).
+__global__ void k_ilu_factor_colour(int m, const int* __restrict__ rows, const int* __restrict__ rp,
+                                    const int* __restrict__ ci, const int* __restrict__ colour,
+                                    const int* __restrict__ diag_pos, double* __restrict__ lu) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int i = rows[t];
+    const int s = rp[i], len = rp[i + 1] - s;
+    const int ci_i = colour[i];
+    for (int c = 0; c < ci_i; ++c)
+        for (int q = 0; q < len; ++q) {
+            const int k = ci[s + q];
+            if (colour[k] != c) continue;
+            const double lik = lu[s + q] / lu[diag_pos[k]];
+            lu[s + q] = lik;
+            for (int mm = rp[k]; mm < rp[k + 1]; ++mm) {
+                const int j = ci[mm];
+                for (int b = 0; b < len; ++b)
+                    if (ci[s + b] == j) {
+                        if (colour[j] > c) lu[s + b] -= lik * lu[mm];
+                        break;
+                    }
+            }
+        }
+}
+
+__global__ void k_ilu_gather(int nl, int nu, int n, const int* __restrict__ l_src, const int* __restrict__ u_src,
+                             const int* __restrict__ d_src, const double* __restrict__ lu, double* __restrict__ lv,
+                             double* __restrict__ uv, double* __restrict__ dinv, float* __restrict__ lv32,
+                             float* __restrict__ uv32, float* __restrict__ dinv32) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nl) { const double v = lu[l_src[e]]; lv[e] = v; if (lv32) lv32[e] = (float)v; }
+    if (e < nu) { const double v = lu[u_src[e]]; uv[e] = v; if (uv32) uv32[e] = (float)v; }
+    if (e < n) { const double v = 1.0 / lu[d_src[e]]; dinv[e] = v; if (dinv32) dinv32[e] = (float)v; }
+}
+
+// Forward sweep over the rows of one colour, all components per thread (the L row is read once)
+template <int NRHS, typename T>
+__global__ void __launch_bounds__(BLOCK)
+k_ilu_lower(int m, int n, const int* __restrict__ rows, const int* __restrict__ lp, const int* __restrict__ lc,
+            const T* __restrict__ lv, const T* __restrict__ dinv, int last, const double* __restrict__ r,
+            double* __restrict__ z, const int* __restrict__ st) {
+    if (halted(st)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int i = rows[t];
+    double acc[NRHS];
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) acc[c] = r[(size_t)c * n + i];
+    for (int e = lp[t]; e < lp[t + 1]; ++e) {
+        const double l = (double)lv[e];
+        const int j = lc[e];
+#pragma unroll
+        for (int c = 0; c < NRHS; ++c) acc[c] -= l * z[(size_t)c * n + j];
+    }
+    const double d = (double)dinv[t];
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) z[(size_t)c * n + i] = last ? acc[c] * d : acc[c];
+}
+
+template <int NRHS, typename T>
+__global__ void __launch_bounds__(BLOCK)
+k_ilu_upper(int m, int n, const int* __restrict__ rows, const int* __restrict__ up, const int* __restrict__ uc,
+            const T* __restrict__ uv, const T* __restrict__ dinv, double* __restrict__ z,
+            const int* __restrict__ st) {
+    if (halted(st)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m) return;
+    const int i = rows[t];
+    double acc[NRHS];
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) acc[c] = z[(size_t)c * n + i];
+    for (int e = up[t]; e < up[t + 1]; ++e) {
+        const double u = (double)uv[e];
+        const int j = uc[e];
+#pragma unroll
+        for (int c = 0; c < NRHS; ++c) acc[c] -= u * z[(size_t)c * n + j];
+    }
+    const double d = (double)dinv[t];
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) z[(size_t)c * n + i] = acc[c] * d;
+}
+
+// ======================================================================================================
+// Two-component BiCGSTAB (src/cfa/linalg.cuh:642-779)
+// ======================================================================================================
+// Per-component scalars: slot(c) = base + c
+enum {
+    B_RN0 = 0, B_BD = 2, B_RHO = 4, B_RHO_OLD = 6, B_ALPHA = 8, B_OMEGA = 10, B_REL = 12,
+    B_CB = 14, B_CO = 16, B_CA = 18, B_A1 = 20, B_A2 = 22, B_CW = 24, B_TOL = 26
+};
+// ints: [0] state (all inactive -> stopped), [2 + c] active, [4 + c] iters, [6 + c] converged,
+// [8 + c] s_done
+
+// Two-component partial sums: part[b * 6 + 2 q + c]
+__device__ __forceinline__ void write_partials2(double* part, int nq, double acc[][2], double* lds) {
+    for (int q = 0; q < nq; ++q)
+        for (int c = 0; c < 2; ++c) {
+            const double s = block_sum(acc[q][c], lds);
+            if (threadIdx.x == 0) part[(size_t)blockIdx.x * 6 + 2 * q + c] = s;
+        }
+}
+
+// r = b - A x (both components), r0 = p = r, v = 0; partial r.r
+template <int TPR>
+__global__ void __launch_bounds__(BLOCK)
+k_bicg_init(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ val,
+            const double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r,
+            double* __restrict__ r0, double* __restrict__ p, double* __restrict__ v, double* __restrict__ part) {
+    __shared__ double lds[BLOCK / WAVE];
+    const int groups = gridDim.x * (blockDim.x / TPR);
+    const int g0 = (blockIdx.x * blockDim.x + threadIdx.x) / TPR, lane = threadIdx.x % TPR;
+    double acc[1][2] = {{0.0, 0.0}};
+    for (int row = g0; row < n; row += groups) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
+            const int j = ci[k];
+            const double a = val[k];
+            s0 += a * x[j];
+            s1 += a * x[n + j];
+        }
+        s0 = group_sum<TPR>(s0);
+        s1 = group_sum<TPR>(s1);
+        if (lane == 0) {
+            const double q0 = b[row] - 1.0 * s0, q1 = b[n + row] - 1.0 * s1;
+            r[row] = q0;  r[n + row] = q1;
+            r0[row] = q0; r0[n + row] = q1;
+            p[row] = q0;  p[n + row] = q1;
+            v[row] = 0.0; v[n + row] = 0.0;
+            acc[0][0] += q0 * q0;
+            acc[0][1] += q1 * q1;
+        }
+    }
+    write_partials2(part, 1, acc, lds);
+}
+
+__device__ __forceinline__ void bicg_stop(int* st, int c, int it, int ok) {
+    st[2 + c] = 0;
+    st[4 + c] = it;
+    st[6 + c] = ok;
+}
+
+__global__ void k_bicg_final_init(int nb, const double* __restrict__ part, double* __restrict__ sc,
+                                  int* __restrict__ st, int* __restrict__ status) {
+    __shared__ double lds[BLOCK / WAVE];
+    double rr[2];
+    for (int c = 0; c < 2; ++c) rr[c] = sum_partials(part + c, nb, 6, lds);
+    if (threadIdx.x != 0) return;
+    for (int c = 0; c < 2; ++c) {
+        sc[B_RN0 + c] = sqrt(rr[c]);
+        sc[B_RHO + c] = rr[c];
+        sc[B_BD + c] = 1.0e-30 * sc[B_RN0 + c] * sc[B_RN0 + c] + 1.0e-300;
+        sc[B_RHO_OLD + c] = 1.0;
+        sc[B_ALPHA + c] = 1.0;
+        sc[B_OMEGA + c] = 1.0;
+        sc[B_REL + c] = 0.0;
+        st[2 + c] = sc[B_RN0 + c] >= 1.0e-13 ? 1 : 0;
+        st[4 + c] = 0;
+        st[6 + c] = 1;
+        st[8 + c] = 0;
+    }
+    st[0] = (st[2] || st[3]) ? ST_RUNNING : ST_STOPPED;
+    publish(status, 0, st[0]);
+}
+
+// Start of iteration it: breakdown check on rho, coefficients of the p update
+__global__ void k_bicg_pre(double* __restrict__ sc, int* __restrict__ st, int* __restrict__ status, int it) {
+    if (st[0] != ST_RUNNING) return;
+    for (int c = 0; c < 2; ++c) {
+        sc[B_CB + c] = 0.0;
+        sc[B_CO + c] = 0.0;
+        if (st[2 + c] && fabs(sc[B_RHO + c]) < sc[B_BD + c]) bicg_stop(st, c, it, 0);
+        if (st[2 + c] && it > 0) {
+            sc[B_CB + c] = (sc[B_RHO + c] / sc[B_RHO_OLD + c]) * (sc[B_ALPHA + c] / (sc[B_OMEGA + c] + 1.0e-30));
+            sc[B_CO + c] = sc[B_OMEGA + c];
+        }
+    }
+    if (!(st[2] || st[3])) { st[0] = ST_STOPPED; publish(status, it, ST_STOPPED); }
+}
+
+// p = r + cb (p - co v)
+__global__ void k_bicg_p(int n, const double* __restrict__ r, const double* __restrict__ v,
+                         const double* __restrict__ sc, double* __restrict__ p, const int* __restrict__ st) {
+    if (halted(st)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * n) return;
+    const int c = i >= n;
+    p[i] = r[i] + sc[B_CB + c] * (p[i] - sc[B_CO + c] * v[i]);
+}
+
+// y = A x for both components; with dots: partial y.w0 (nq = 1) or y.y, y.w0 (nq = 2)
+template <int TPR>
+__global__ void __launch_bounds__(BLOCK)
+k_spmv2_dots(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ val,
+             const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ w0, int mode,
+             double* __restrict__ part, const int* __restrict__ st) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (halted(st)) return;
+    const int groups = gridDim.x * (blockDim.x / TPR);
+    const int g0 = (blockIdx.x * blockDim.x + threadIdx.x) / TPR, lane = threadIdx.x % TPR;
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int row = g0; row < n; row += groups) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
+            const int j = ci[k];
+            const double a = val[k];
+            s0 += a * x[j];
+            s1 += a * x[n + j];
+        }
+        s0 = group_sum<TPR>(s0);
+        s1 = group_sum<TPR>(s1);
+        if (lane == 0) {
+            y[row] = s0;
+            y[n + row] = s1;
+            if (mode == 1) {   // r0 . v
+                acc[0][0] += w0[row] * s0;
+                acc[0][1] += w0[n + row] * s1;
+            } else {           // t . t, t . s
+                acc[0][0] += s0 * s0;
+                acc[0][1] += s1 * s1;
+                acc[1][0] += s0 * w0[row];
+                acc[1][1] += s1 * w0[n + row];
+            }
+        }
+    }
+    write_partials2(part, mode == 1 ? 1 : 2, acc, lds);
+}
+
+// alpha = rho / (r0 . v), breakdown guard
+__global__ void k_bicg_final_alpha(int nb, const double* __restrict__ part, double* __restrict__ sc,
+                                   int* __restrict__ st, int* __restrict__ status, int it) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (st[0] != ST_RUNNING) return;
+    double o[2];
+    for (int c = 0; c < 2; ++c) o[c] = sum_partials(part + c, nb, 6, lds);
+    if (threadIdx.x != 0) return;
+    for (int c = 0; c < 2; ++c) {
+        sc[B_CA + c] = 0.0;
+        if (st[2 + c] && fabs(o[c]) < sc[B_BD + c]) bicg_stop(st, c, it, 0);
+        if (st[2 + c]) {
+            sc[B_ALPHA + c] = sc[B_RHO + c] / o[c];
+            sc[B_CA + c] = sc[B_ALPHA + c];
+        }
+    }
+    if (!(st[2] || st[3])) { st[0] = ST_STOPPED; publish(status, it, ST_STOPPED); }
+}
+
+// s = r - ca v; partial s.s
+__global__ void __launch_bounds__(BLOCK)
+k_bicg_s(int n, const double* __restrict__ r, const double* __restrict__ v, const double* __restrict__ sc,
+         double* __restrict__ s, double* __restrict__ part, const int* __restrict__ st) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (halted(st)) return;
+    double acc[1][2] = {{0.0, 0.0}};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += gridDim.x * blockDim.x) {
+        const int c = i >= n;
+        const double si = r[i] - sc[B_CA + c] * v[i];
+        s[i] = si;
+        acc[0][c] += si * si;
+    }
+    write_partials2(part, 1, acc, lds);
+}
+
+// omega = (t . s) / (t . t) unless the half step already converged
+__global__ void k_bicg_final_omega(int nb_s, const double* __restrict__ part_s, int nb_t,
+                                   const double* __restrict__ part_t, double* __restrict__ sc,
+                                   int* __restrict__ st) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (st[0] != ST_RUNNING) return;
+    double ss[2], tt[2], ts[2];
+    for (int c = 0; c < 2; ++c) {
+        ss[c] = sum_partials(part_s + c, nb_s, 6, lds);
+        tt[c] = sum_partials(part_t + c, nb_t, 6, lds);
+        ts[c] = sum_partials(part_t + 2 + c, nb_t, 6, lds);
+    }
+    if (threadIdx.x != 0) return;
+    for (int c = 0; c < 2; ++c) {
+        sc[B_A1 + c] = 0.0;
+        sc[B_A2 + c] = 0.0;
+        sc[B_CW + c] = 0.0;
+        st[8 + c] = 0;
+        if (!st[2 + c]) continue;
+        sc[B_A1 + c] = sc[B_ALPHA + c];
+        const double s_norm = sqrt(ss[c]);
+        if (s_norm < sc[B_TOL] * sc[B_RN0 + c] || s_norm < 1.0e-11) { st[8 + c] = 1; continue; }
+        sc[B_OMEGA + c] = (tt[c] > sc[B_BD + c]) ? (ts[c] / tt[c]) : 0.0;
+        sc[B_A2 + c] = sc[B_OMEGA + c];
+        sc[B_CW + c] = sc[B_OMEGA + c];
+    }
+}
+
+// x += a1 phat + a2 shat; r = s - cw t; partial r.r, r0.r
+__global__ void __launch_bounds__(BLOCK)
+k_bicg_xr(int n, const double* __restrict__ ph, const double* __restrict__ sh, const double* __restrict__ s,
+          const double* __restrict__ t, const double* __restrict__ r0, const double* __restrict__ sc,
+          double* __restrict__ x, double* __restrict__ r, double* __restrict__ part, const int* __restrict__ st) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (halted(st)) return;
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += gridDim.x * blockDim.x) {
+        const int c = i >= n;
+        x[i] += sc[B_A1 + c] * ph[i] + sc[B_A2 + c] * sh[i];
+        const double ri = s[i] - sc[B_CW + c] * t[i];
+        r[i] = ri;
+        acc[0][c] += ri * ri;
+        acc[1][c] += r0[i] * ri;
+    }
+    write_partials2(part, 2, acc, lds);
+}
+
+// end of iteration it: half-step stops, residual test, rho update, iteration cap
+__global__ void k_bicg_final_post(int nb, const double* __restrict__ part, double* __restrict__ sc,
+                                  int* __restrict__ st, int* __restrict__ status, int it, int max_iter) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (st[0] != ST_RUNNING) return;
+    double rr[2], r0r[2];
+    for (int c = 0; c < 2; ++c) {
+        rr[c] = sum_partials(part + c, nb, 6, lds);
+        r0r[c] = sum_partials(part + 2 + c, nb, 6, lds);
+    }
+    if (threadIdx.x != 0) return;
+    for (int c = 0; c < 2; ++c)
+        if (st[8 + c]) bicg_stop(st, c, it + 1, 1);
+    if (st[2] || st[3]) {
+        for (int c = 0; c < 2; ++c) {
+            if (!st[2 + c]) continue;
+            const double r_norm = sqrt(rr[c]);
+            sc[B_REL + c] = r_norm / sc[B_RN0 + c];
+            if (r_norm < sc[B_TOL] * sc[B_RN0 + c] || r_norm < 1.0e-11) { bicg_stop(st, c, it + 1, 1); continue; }
+            sc[B_RHO_OLD + c] = sc[B_RHO + c];
+            sc[B_RHO + c] = r0r[c];
+            if (it + 1 == max_iter) bicg_stop(st, c, max_iter, 0);
+        }
+    }
+    const int state = (st[2] || st[3]) ? ST_RUNNING : ST_STOPPED;
+    st[0] = state;
+    publish(status, it + 1, state);
+}
+
+// ======================================================================================================
+// Host helpers
+// ======================================================================================================
+template <int OP>
+void csr_op_launch(int tpr, int rows, const int* rp, const int* ci, const double* val, const double* x,
+                   const double* b, const double* dinv, double omega, double* y, const int* st, hipStream_t s) {
+    const int blocks = blocks_for((long long)rows * tpr);
+    switch (tpr) {
+        case 1: hipLaunchKernelGGL((k_csr_op<1, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
+        case 2: hipLaunchKernelGGL((k_csr_op<2, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
+        case 4: hipLaunchKernelGGL((k_csr_op<4, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
+        case 8: hipLaunchKernelGGL((k_csr_op<8, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
+        case 16: hipLaunchKernelGGL((k_csr_op<16, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
+        case 32: hipLaunchKernelGGL((k_csr_op<32, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
+        default: hipLaunchKernelGGL((k_csr_op<64, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
+    }
+}
+
+void jacobi_from_zero_launch(int tpr, int rows, const int* rp, const int* ci, const double* val, const double* b,
+                             const double* dinv, double omega, double* y, const int* st, hipStream_t s) {
+    const int blocks = blocks_for((long long)rows * tpr);
+    switch (tpr) {
+        case 1: hipLaunchKernelGGL(k_jacobi_from_zero<1>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
+        case 2: hipLaunchKernelGGL(k_jacobi_from_zero<2>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
+        case 4: hipLaunchKernelGGL(k_jacobi_from_zero<4>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
+        case 8: hipLaunchKernelGGL(k_jacobi_from_zero<8>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
+        case 16: hipLaunchKernelGGL(k_jacobi_from_zero<16>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
+        case 32: hipLaunchKernelGGL(k_jacobi_from_zero<32>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
+        default: hipLaunchKernelGGL(k_jacobi_from_zero<64>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
+    }
+}
+
+// One V-cycle for level l with right-hand side b; the result of level 0 goes to `out` (if given).
+// The descriptors' x/ax buffers are used as a ping-pong pair; `xl` returns where the level's
+// iterate ends up.
+struct VcycleRun {
+    const cfa_amg_t* amg;
+    const int* st;
+    hipStream_t s;
+    std::vector<cfa_amg_level_t> lv;   // host copy with the current buffer roles
+
+    // returns the buffer holding level l's result
+    double* level(int l, const double* b, double* out) {
+        cfa_amg_level_t& L = lv[l];
+        const int n = L.rows;
+        const int ss = amg->smooth_steps;
+        double* xa = L.x;
+        double* xb = L.ax;
+        if (ss >= 2) {
+            jacobi_from_zero_launch(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, b, L.dinv, L.omega, xa, st, s);
+            for (int k = 2; k < ss; ++k) {
+                csr_op_launch<OP_JACOBI>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, xb, st, s);
+                std::swap(xa, xb);
+            }
+        } else {
+            hipLaunchKernelGGL(k_jacobi_init, dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, L.omega, L.dinv, b, xa, st);
+        }
+        csr_op_launch<OP_RESID>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, nullptr, 0.0, L.r, st, s);
+        const bool deeper = l + 1 < (int)lv.size();
+        double* cb = deeper ? lv[l + 1].b : amg->cb;
+        csr_op_launch<OP_SPMV>(L.tpr_r, L.cols, L.r_rp, L.r_ci, L.r_v, L.r, nullptr, nullptr, 0.0, cb, st, s);
+        const double* cx;
+        if (deeper && amg->small_rows > 0 && lv[l + 1].rows <= amg->small_rows && amg->d_levels) {
+            // all coarser levels in one workgroup; level l + 1's result ends in its x buffer
+            hipLaunchKernelGGL(k_vcycle_small, dim3(1), dim3(1024), 0, s, amg->d_levels, l + 1, (int)lv.size(),
+                               ss, amg->coarse_n, amg->coarse_inv, amg->cb, amg->cx, st);
+            cx = lv[l + 1].x;
+        } else if (deeper) {
+            cx = level(l + 1, lv[l + 1].b, nullptr);
+        } else {
+            hipLaunchKernelGGL(k_dense_matvec, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
+                               amg->coarse_n, amg->coarse_inv, amg->cb, amg->cx, st);
+            cx = amg->cx;
+        }
+        csr_op_launch<OP_ADD>(L.tpr_p, n, L.p_rp, L.p_ci, L.p_v, cx, nullptr, nullptr, 0.0, xa, st, s);
+        for (int k = 0; k < ss; ++k) {
+            double* dst = (l == 0 && out && k == ss - 1) ? out : xb;
+            csr_op_launch<OP_JACOBI>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, dst, st, s);
+            if (dst == out) { xa = out; break; }
+            std::swap(xa, xb);
+        }
+        return xa;
+    }
+};
+
+hipStream_t S(cfa_stream_t s) { return as_stream(s); }
+
+#define CFA_TRY(expr) do { const hipError_t e_ = (expr); if (e_ != hipSuccess) return (int)e_; } while (0)
+
+// Host wait on the progress word: returns when progress >= target, the solve stopped, or the
+// stream is idle.
+void wait_progress(const cfa_krylov_ws_t* ws, int target, hipStream_t s) {
+    volatile int* st = ws->status_host;
+    for (int spin = 0;; ++spin) {
+        if (st[0] >= target || st[1] != ST_RUNNING) return;
+        if ((spin & 63) == 63) {
+            if (hipStreamQuery(s) == hipSuccess) return;
+            std::this_thread::yield();
+        }
+    }
+}
+
+struct KrylovState {
+    double* sc;
+    int* st;
+};
+KrylovState kstate(cfa_krylov_ws_t* ws) { return {ws->scal, reinterpret_cast<int*>(ws->scal + 40)}; }
+
+constexpr int LOOKAHEAD = 3;   // iterations enqueued ahead of the last one known to have finished
+
+}  // namespace
+
+namespace {
+template <int NRHS, typename T>
+void ilu_apply_impl(const cfa_ilu_t* ilu, const T* lv, const T* uv, const T* dinv, const double* r, double* z,
+                    const int* st, hipStream_t s) {
+    const int nc = ilu->n_colours;
+    for (int c = 0; c < nc; ++c) {
+        const int cs = ilu->colour_start[c], m = ilu->colour_start[c + 1] - cs;
+        if (m == 0) continue;
+        hipLaunchKernelGGL((k_ilu_lower<NRHS, T>), dim3(blocks_for(m)), dim3(BLOCK), 0, s, m, ilu->n, ilu->rows + cs,
+                           ilu->lp + cs, ilu->lc, lv, dinv + cs, c == nc - 1 ? 1 : 0, r, z, st);
+    }
+    for (int c = nc - 2; c >= 0; --c) {
+        const int cs = ilu->colour_start[c], m = ilu->colour_start[c + 1] - cs;
+        if (m == 0) continue;
+        hipLaunchKernelGGL((k_ilu_upper<NRHS, T>), dim3(blocks_for(m)), dim3(BLOCK), 0, s, m, ilu->n, ilu->rows + cs,
+                           ilu->up + cs, ilu->uc, uv, dinv + cs, z, st);
+    }
+}
+
+void ilu_apply_any(const cfa_ilu_t* ilu, int use_f32, const double* r, double* z, int nrhs, const int* st,
+                   hipStream_t s) {
+    if (use_f32 && ilu->lv32) {
+        if (nrhs == 2) ilu_apply_impl<2, float>(ilu, ilu->lv32, ilu->uv32, ilu->dinv32, r, z, st, s);
+        else ilu_apply_impl<1, float>(ilu, ilu->lv32, ilu->uv32, ilu->dinv32, r, z, st, s);
+    } else {
+        if (nrhs == 2) ilu_apply_impl<2, double>(ilu, ilu->lv, ilu->uv, ilu->dinv, r, z, st, s);
+        else ilu_apply_impl<1, double>(ilu, ilu->lv, ilu->uv, ilu->dinv, r, z, st, s);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int cfa_spmv(int n, const int* rp, const int* ci, const double* val, const double* x, double* y, int nrhs,
+             cfa_stream_t stream) {
+    if (n < 0 || !rp || !ci || !val || !x || !y || nrhs < 1 || nrhs > 2) return CFA_E_ARG;
+    if (n == 0) return CFA_OK;
+    constexpr int TPR = 8;
+    const int blocks = blocks_for((long long)n * TPR);
+    if (nrhs == 1)
+        hipLaunchKernelGGL((k_spmv_multi<TPR, 1>), dim3(blocks), dim3(BLOCK), 0, S(stream), n, rp, ci, val, x, y);
+    else
+        hipLaunchKernelGGL((k_spmv_multi<TPR, 2>), dim3(blocks), dim3(BLOCK), 0, S(stream), n, rp, ci, val, x, y);
+    return launch_status();
+}
+
+int cfa_dot(int n, const double* x, const double* y, double* result, double* ws, cfa_stream_t stream) {
+    if (n < 0 || !x || !y || !result || !ws) return CFA_E_ARG;
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, S(stream), n, x, y, ws, nullptr);
+    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(BLOCK), 0, S(stream), nb, ws, ws + RED_BLOCKS, 0);
+    int rc = launch_status();
+    if (rc) return rc;
+    const hipError_t e = hipMemcpyAsync(result, ws + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, S(stream));
+    if (e != hipSuccess) return (int)e;
+    const hipError_t e2 = hipStreamSynchronize(S(stream));
+    return e2 == hipSuccess ? CFA_OK : (int)e2;
+}
+
+int cfa_axpy(int n, double a, const double* x, double* y, cfa_stream_t stream) {
+    if (n < 0 || !x || !y) return CFA_E_ARG;
+    if (n == 0) return CFA_OK;
+    hipLaunchKernelGGL(k_axpy, dim3(blocks_for(n)), dim3(BLOCK), 0, S(stream), n, a, x, y);
+    return launch_status();
+}
+
+int cfa_xpay(int n, const double* x, double b, double* y, cfa_stream_t stream) {
+    if (n < 0 || !x || !y) return CFA_E_ARG;
+    if (n == 0) return CFA_OK;
+    hipLaunchKernelGGL(k_xpay, dim3(blocks_for(n)), dim3(BLOCK), 0, S(stream), n, x, b, y);
+    return launch_status();
+}
+
+int cfa_vec_mul(int n, const double* a, const double* b, double* out, cfa_stream_t stream) {
+    if (n < 0 || !a || !b || !out) return CFA_E_ARG;
+    if (n == 0) return CFA_OK;
+    hipLaunchKernelGGL(k_vec_mul, dim3(blocks_for(n)), dim3(BLOCK), 0, S(stream), n, a, b, out, nullptr);
+    return launch_status();
+}
+
+int cfa_inv_diag(int n, const int* rp, const int* ci, const double* val, double eps, double* inv_diag,
+                 cfa_stream_t stream) {
+    if (n < 0 || !rp || !ci || !val || !inv_diag) return CFA_E_ARG;
+    if (n == 0) return CFA_OK;
+    hipLaunchKernelGGL(k_inv_diag, dim3(blocks_for(n)), dim3(BLOCK), 0, S(stream), n, rp, ci, val, eps, inv_diag);
+    return launch_status();
+}
+
+int cfa_amg_apply(const cfa_amg_t* amg, const double* r, double* z, cfa_stream_t stream) {
+    if (!amg || !r || !z) return CFA_E_ARG;
+    if (amg->n_levels < 0 || (amg->n_levels > 0 && !amg->levels)) return CFA_E_ARG;
+    hipStream_t s = S(stream);
+    if (amg->n_levels == 0) {
+        hipLaunchKernelGGL(k_dense_matvec, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
+                           amg->coarse_n, amg->coarse_inv, r, z, nullptr);
+        return launch_status();
+    }
+    VcycleRun run{amg, nullptr, s, std::vector<cfa_amg_level_t>(amg->levels, amg->levels + amg->n_levels)};
+    run.level(0, r, z);
+    return launch_status();
+}
+
+int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double* b, double* x, int max_iter,
+            double tol, const cfa_oss_op_t* oss, cfa_op_cb extra_cb, void* extra_ctx, const cfa_amg_t* amg,
+            cfa_op_cb prec_cb, void* prec_ctx, cfa_krylov_ws_t* ws, cfa_solve_result_t* result,
+            cfa_stream_t stream) {
+    if (n < 0 || !rp || !ci || !val || !b || !x || !ws || !result || ws->cap < n || !ws->status_host) return CFA_E_ARG;
+    if (oss && (!oss->oss_val || !oss->inv_m || !oss->bc_tag || !oss->gx || !oss->gy)) return CFA_E_ARG;
+    hipStream_t s = S(stream);
+    result->iters[0] = 0;
+    result->converged[0] = 1;
+    result->rel_residual[0] = 0.0;
+    if (n == 0) return CFA_OK;
+    double* r = ws->vec[0];
+    double* z = ws->vec[1];
+    double* p = ws->vec[2];
+    double* q = ws->vec[3];
+    double* inv_diag = ws->vec[4];
+    double* part = ws->part;
+    KrylovState ks = kstate(ws);
+    const int nb = red_blocks(n);
+    const int blocks = blocks_for(n);
+
+    // prologue (src/cfa/linalg.cuh:205-227): ||b||, early exits, r = b - A x, z = M^-1 r, p = z
+    double host_sc[2];
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, s, n, b, b, part, nullptr);
+    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(BLOCK), 0, s, nb, part, ks.sc + S_BNORM, 1);
+    CFA_TRY(hipMemcpyAsync(&host_sc[0], ks.sc + S_BNORM, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (hipStreamSynchronize(s) != hipSuccess) return launch_status();
+    const double b_norm = host_sc[0];
+    if (b_norm < 1.0e-14) {
+        hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(BLOCK), 0, s, n, x);
+        return launch_status();
+    }
+    if (!amg && !prec_cb) hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(BLOCK), 0, s, n, rp, ci, val, 1.0e-14, inv_diag);
+
+    int* st = ks.st;
+    // state words: running, 0 iterations; tolerance into the scalar area
+    int init_state[2] = {ST_RUNNING, 0};
+    double tol_v = tol;
+    CFA_TRY(hipMemcpyAsync(st, init_state, sizeof(init_state), hipMemcpyHostToDevice, s));
+    CFA_TRY(hipMemcpyAsync(ks.sc + S_TOL, &tol_v, sizeof(double), hipMemcpyHostToDevice, s));
+    ws->status_host[0] = 0;
+    ws->status_host[1] = ST_RUNNING;
+
+    constexpr int TPR = 4;
+    const int blocks_tpr = std::min(blocks_for((long long)n * TPR), OP_BLOCKS);
+    // The fused operator needs nnz for the [Bx | By | BTx | BTy] layout: read it from rp[n] once.
+    int nnz = 0;
+    CFA_TRY(hipMemcpyAsync(&nnz, rp + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (hipStreamSynchronize(s) != hipSuccess) return launch_status();
+
+    auto apply_op = [&](const double* in, double* out) {
+        if (oss) {
+            hipLaunchKernelGGL(k_apply_K_B<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, nnz, rp, ci, val,
+                               oss->oss_val, oss->inv_m, in, out, oss->gx, oss->gy, st);
+        } else {
+            csr_op_launch<OP_SPMV>(TPR, n, rp, ci, val, in, nullptr, nullptr, 0.0, out, st, s);
+            if (extra_cb) extra_cb(in, out, extra_ctx);
+        }
+    };
+    // second stage of the fused operator (q -= B^T g) with the p.q partials, or a plain dot
+    auto finish_op_dot = [&](const double* pv, double* qv) {
+        if (oss) {
+            hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss->oss_val,
+                               oss->bc_tag, oss->gx, oss->gy, pv, qv, part, st);
+            return blocks_tpr;
+        }
+        hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, s, n, pv, qv, part, st);
+        return nb;
+    };
+    auto apply_prec = [&](const double* rv, double* zv) {
+        if (amg) {
+            if (amg->n_levels == 0) {
+                hipLaunchKernelGGL(k_dense_matvec, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
+                                   amg->coarse_n, amg->coarse_inv, rv, zv, st);
+            } else {
+                VcycleRun run{amg, st, s, std::vector<cfa_amg_level_t>(amg->levels, amg->levels + amg->n_levels)};
+                run.level(0, rv, zv);
+            }
+        } else if (prec_cb) {
+            prec_cb(rv, zv, prec_ctx);
+        } else {
+            hipLaunchKernelGGL(k_vec_mul, dim3(blocks), dim3(BLOCK), 0, s, n, inv_diag, rv, zv, st);
+        }
+    };
+
+    // r = b - A x, ||r||
+    apply_op(x, q);
+    if (oss) {
+        // the BT stage without a dot: reuse the dot kernel's stage with p = q (result unused)
+        hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss->oss_val,
+                           oss->bc_tag, oss->gx, oss->gy, q, q, part, st);
+    }
+    hipLaunchKernelGGL(k_pcg_residual, dim3(nb), dim3(BLOCK), 0, s, n, b, q, r, part);
+    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(BLOCK), 0, s, nb, part, ks.sc + S_RNORM, 1);
+    CFA_TRY(hipMemcpyAsync(&host_sc[1], ks.sc + S_RNORM, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (hipStreamSynchronize(s) != hipSuccess) return launch_status();
+    if (host_sc[1] < tol * b_norm) return launch_status();
+
+    apply_prec(r, z);
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(BLOCK), 0, s, n, z, p, st);
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, s, n, r, z, part, st);
+    hipLaunchKernelGGL(k_pcg_final_beta, dim3(1), dim3(BLOCK), 0, s, nb, part, ks.sc, st, ws->status_dev, -1, 1);
+
+    // iterations (src/cfa/linalg.cuh:231-253)
+    int it = 0;
+    for (; it < max_iter; ++it) {
+        if (it >= LOOKAHEAD) {
+            wait_progress(ws, it - LOOKAHEAD + 1, s);
+            if (ws->status_host[1] != ST_RUNNING) break;
+        }
+        apply_op(p, q);
+        const int npart = finish_op_dot(p, q);
+        hipLaunchKernelGGL(k_pcg_final_alpha, dim3(1), dim3(BLOCK), 0, s, npart, part, ks.sc, st, ws->status_dev, it);
+        hipLaunchKernelGGL(k_pcg_update_xr, dim3(nb), dim3(BLOCK), 0, s, n, ks.sc, p, q, x, r, part, st);
+        hipLaunchKernelGGL(k_pcg_final_rnorm, dim3(1), dim3(BLOCK), 0, s, nb, part, ks.sc, st, ws->status_dev, it);
+        apply_prec(r, z);
+        hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, s, n, r, z, part, st);
+        hipLaunchKernelGGL(k_pcg_final_beta, dim3(1), dim3(BLOCK), 0, s, nb, part, ks.sc, st, ws->status_dev, it, 0);
+        hipLaunchKernelGGL(k_pcg_p, dim3(blocks), dim3(BLOCK), 0, s, n, z, ks.sc, p, st);
+    }
+    int fin[2];
+    double rel;
+    CFA_TRY(hipMemcpyAsync(fin, st, sizeof(fin), hipMemcpyDeviceToHost, s));
+    CFA_TRY(hipMemcpyAsync(&rel, ks.sc + S_REL, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (hipStreamSynchronize(s) != hipSuccess) return launch_status();
+    if (fin[0] == ST_RUNNING) {   // iteration cap
+        result->iters[0] = max_iter;
+        result->converged[0] = 0;
+    } else {
+        result->iters[0] = fin[1];
+        result->converged[0] = fin[0] == ST_CONVERGED;
+    }
+    result->rel_residual[0] = rel;
+    return launch_status();
+}
+
+int cfa_ilu0_factor(const cfa_ilu_t* ilu, const double* val, cfa_stream_t stream) {
+    if (!ilu || !val || !ilu->lu || !ilu->colour_start) return CFA_E_ARG;
+    hipStream_t s = S(stream);
+    if (ilu->n == 0) return CFA_OK;
+    CFA_TRY(hipMemcpyAsync(ilu->lu, val, (size_t)ilu->nnz * sizeof(double), hipMemcpyDeviceToDevice, s));
+    for (int c = 1; c < ilu->n_colours; ++c) {
+        const int m = ilu->colour_start[c + 1] - ilu->colour_start[c];
+        if (m == 0) continue;
+        hipLaunchKernelGGL(k_ilu_factor_colour, dim3(blocks_for(m, 128)), dim3(128), 0, s, m,
+                           ilu->rows + ilu->colour_start[c], ilu->row_ptr, ilu->col_ind, ilu->colour, ilu->diag_pos,
+                           ilu->lu);
+    }
+    const int len = std::max(ilu->n, std::max(ilu->nl, ilu->nu));
+    hipLaunchKernelGGL(k_ilu_gather, dim3(blocks_for(len)), dim3(BLOCK), 0, s, ilu->nl, ilu->nu, ilu->n, ilu->l_src,
+                       ilu->u_src, ilu->d_src, ilu->lu, ilu->lv, ilu->uv, ilu->dinv, ilu->lv32, ilu->uv32, ilu->dinv32);
+    return launch_status();
+}
+
+
+int cfa_ilu0_apply(const cfa_ilu_t* ilu, const double* r, double* z, int nrhs, cfa_stream_t stream) {
+    if (!ilu || !r || !z || nrhs < 1 || nrhs > 2 || !ilu->colour_start) return CFA_E_ARG;
+    if (ilu->n == 0) return CFA_OK;
+    ilu_apply_any(ilu, 0, r, z, nrhs, nullptr, S(stream));
+    return launch_status();
+}
+
+int cfa_bicgstab2(int n, const int* rp, const int* ci, const double* val, const double* b_u, const double* b_v,
+                  double* x_u, double* x_v, int max_iter, double tol, const cfa_ilu_t* ilu, int use_f32,
+                  cfa_krylov_ws_t* ws, cfa_solve_result_t* result, cfa_stream_t stream) {
+    if (n < 0 || !rp || !ci || !val || !b_u || !b_v || !x_u || !x_v || !ilu || !ws || !result || ws->cap < n ||
+        !ws->status_host)
+        return CFA_E_ARG;
+    hipStream_t s = S(stream);
+    for (int c = 0; c < 2; ++c) { result->iters[c] = 0; result->converged[c] = 1; result->rel_residual[c] = 0.0; }
+    if (n == 0) return CFA_OK;
+    const size_t bytes = (size_t)n * sizeof(double);
+    double* x = ws->vec[0];
+    double* bb = ws->vec[1];
+    double* r = ws->vec[2];
+    double* r0 = ws->vec[3];
+    double* p = ws->vec[4];
+    double* v = ws->vec[5];
+    double* sv = ws->vec[6];
+    double* t = ws->vec[7];
+    double* ph = ws->vec[8];
+    double* sh = ws->vec[9];
+    double* part_a = ws->part;                 // 6 * RED_BLOCKS each
+    double* part_b = ws->part + 6 * RED_BLOCKS;
+    double* part_c = ws->part + 12 * RED_BLOCKS;
+    KrylovState ks = kstate(ws);
+    int* st = ks.st;
+    CFA_TRY(hipMemcpyAsync(x, x_u, bytes, hipMemcpyDeviceToDevice, s));
+    CFA_TRY(hipMemcpyAsync(x + n, x_v, bytes, hipMemcpyDeviceToDevice, s));
+    CFA_TRY(hipMemcpyAsync(bb, b_u, bytes, hipMemcpyDeviceToDevice, s));
+    CFA_TRY(hipMemcpyAsync(bb + n, b_v, bytes, hipMemcpyDeviceToDevice, s));
+    double tol_v = tol;
+    CFA_TRY(hipMemcpyAsync(ks.sc + B_TOL, &tol_v, sizeof(double), hipMemcpyHostToDevice, s));
+    ws->status_host[0] = 0;
+    ws->status_host[1] = ST_RUNNING;
+
+    constexpr int TPR = 4;
+    const int groups_per_block = BLOCK / TPR;
+    int nb_rows = (n + groups_per_block - 1) / groups_per_block;
+    if (nb_rows > RED_BLOCKS) nb_rows = RED_BLOCKS;
+    const int nb2 = red_blocks(2LL * n);
+    const int blocks2 = blocks_for(2LL * n);
+
+    hipLaunchKernelGGL(k_bicg_init<TPR>, dim3(nb_rows), dim3(BLOCK), 0, s, n, rp, ci, val, bb, x, r, r0, p, v, part_a);
+    hipLaunchKernelGGL(k_bicg_final_init, dim3(1), dim3(BLOCK), 0, s, nb_rows, part_a, ks.sc, st, ws->status_dev);
+
+    for (int it = 0; it < max_iter; ++it) {
+        if (it >= LOOKAHEAD) {
+            wait_progress(ws, it - LOOKAHEAD + 1, s);
+            if (ws->status_host[1] != ST_RUNNING) break;
+        }
+        hipLaunchKernelGGL(k_bicg_pre, dim3(1), dim3(1), 0, s, ks.sc, st, ws->status_dev, it);
+        if (it > 0) hipLaunchKernelGGL(k_bicg_p, dim3(blocks2), dim3(BLOCK), 0, s, n, r, v, ks.sc, p, st);
+        ilu_apply_any(ilu, use_f32, p, ph, 2, st, s);
+        hipLaunchKernelGGL(k_spmv2_dots<TPR>, dim3(nb_rows), dim3(BLOCK), 0, s, n, rp, ci, val, ph, v, r0, 1, part_a, st);
+        hipLaunchKernelGGL(k_bicg_final_alpha, dim3(1), dim3(BLOCK), 0, s, nb_rows, part_a, ks.sc, st, ws->status_dev, it);
+        hipLaunchKernelGGL(k_bicg_s, dim3(nb2), dim3(BLOCK), 0, s, n, r, v, ks.sc, sv, part_b, st);
+        ilu_apply_any(ilu, use_f32, sv, sh, 2, st, s);
+        hipLaunchKernelGGL(k_spmv2_dots<TPR>, dim3(nb_rows), dim3(BLOCK), 0, s, n, rp, ci, val, sh, t, sv, 2, part_c, st);
+        hipLaunchKernelGGL(k_bicg_final_omega, dim3(1), dim3(BLOCK), 0, s, nb2, part_b, nb_rows, part_c, ks.sc, st);
+        hipLaunchKernelGGL(k_bicg_xr, dim3(nb2), dim3(BLOCK), 0, s, n, ph, sh, sv, t, r0, ks.sc, x, r, part_a, st);
+        hipLaunchKernelGGL(k_bicg_final_post, dim3(1), dim3(BLOCK), 0, s, nb2, part_a, ks.sc, st, ws->status_dev, it,
+                           max_iter);
+    }
+    int fin[10];
+    double rel[2];
+    CFA_TRY(hipMemcpyAsync(fin, st, sizeof(fin), hipMemcpyDeviceToHost, s));
+    CFA_TRY(hipMemcpyAsync(rel, ks.sc + B_REL, sizeof(rel), hipMemcpyDeviceToHost, s));
+    CFA_TRY(hipMemcpyAsync(x_u, x, bytes, hipMemcpyDeviceToDevice, s));
+    CFA_TRY(hipMemcpyAsync(x_v, x + n, bytes, hipMemcpyDeviceToDevice, s));
+    if (hipStreamSynchronize(s) != hipSuccess) return launch_status();
+    for (int c = 0; c < 2; ++c) {
+        if (fin[2 + c]) {   // still active when the loop ended (cap reached without the in-kernel stop)
+            result->iters[c] = max_iter;
+            result->converged[c] = 0;
+        } else {
+            result->iters[c] = fin[4 + c];
+            result->converged[c] = fin[6 + c];
+        }
+        result->rel_residual[c] = rel[c];
+    }
+    return launch_status();
+}
+
+int cfa_bicgstab_jacobi(int n, const int* rp, const int* ci, const double* val, const double* b, double* x,
+                        int max_iter, double tol, cfa_krylov_ws_t* ws, cfa_solve_result_t* result,
+                        cfa_stream_t stream) {
+    // Host-side scalar logic with one transfer per dot product, as the reference's one-component
+    // solver (src/cfa/linalg.cuh:261-347); it is off the default path (--momentum-precond jacobi).
+    if (n < 0 || !rp || !ci || !val || !b || !x || !ws || !result || ws->cap < n) return CFA_E_ARG;
+    hipStream_t s = S(stream);
+    result->iters[0] = 0;
+    result->converged[0] = 1;
+    result->rel_residual[0] = 0.0;
+    if (n == 0) return CFA_OK;
+    double* r = ws->vec[0];
+    double* r0 = ws->vec[1];
+    double* p = ws->vec[2];
+    double* v = ws->vec[3];
+    double* sv = ws->vec[4];
+    double* t = ws->vec[5];
+    double* ph = ws->vec[6];
+    double* sh = ws->vec[7];
+    double* dinv = ws->vec[8];
+    const int blocks = blocks_for(n);
+    auto dot = [&](const double* a, const double* bb) {
+        double out = 0.0;
+        cfa_dot(n, a, bb, &out, ws->part, stream);
+        return out;
+    };
+    auto spmv = [&](const double* in, double* out) { cfa_spmv(n, rp, ci, val, in, out, 1, stream); };
+    hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(BLOCK), 0, s, n, rp, ci, val, 1.0e-14, dinv);
+    spmv(x, v);
+    CFA_TRY(hipMemcpyAsync(r, b, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(BLOCK), 0, s, n, -1.0, v, r);
+    CFA_TRY(hipMemcpyAsync(r0, r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    CFA_TRY(hipMemcpyAsync(p, r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(BLOCK), 0, s, n, v);
+    const double rn0 = std::sqrt(dot(r, r));
+    if (rn0 < 1.0e-13) return launch_status();
+    double rho_old = 1.0, alpha = 1.0, omega = 1.0;
+    const double bd = 1.0e-30 * rn0 * rn0 + 1.0e-300;
+    int iter = 0;
+    bool done = false;
+    for (; iter < max_iter; ++iter) {
+        const double rho_new = dot(r0, r);
+        if (std::fabs(rho_new) < bd) break;
+        if (iter > 0) {
+            const double beta = (rho_new / rho_old) * (alpha / (omega + 1.0e-30));
+            // p = r + beta (p - omega v)
+            hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(BLOCK), 0, s, n, -omega, v, p);
+            hipLaunchKernelGGL(k_xpay, dim3(blocks), dim3(BLOCK), 0, s, n, r, beta, p);
+        }
+        hipLaunchKernelGGL(k_vec_mul, dim3(blocks), dim3(BLOCK), 0, s, n, dinv, p, ph, nullptr);
+        spmv(ph, v);
+        const double r0v = dot(r0, v);
+        if (std::fabs(r0v) < bd) break;
+        alpha = rho_new / r0v;
+        CFA_TRY(hipMemcpyAsync(sv, r, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(BLOCK), 0, s, n, -alpha, v, sv);
+        const double s_norm = std::sqrt(dot(sv, sv));
+        if (s_norm < tol * rn0 || s_norm < 1.0e-11) {
+            hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(BLOCK), 0, s, n, alpha, ph, x);
+            ++iter;
+            done = true;
+            break;
+        }
+        hipLaunchKernelGGL(k_vec_mul, dim3(blocks), dim3(BLOCK), 0, s, n, dinv, sv, sh, nullptr);
+        spmv(sh, t);
+        const double tt = dot(t, t);
+        omega = (tt > bd) ? (dot(t, sv) / tt) : 0.0;
+        hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(BLOCK), 0, s, n, alpha, ph, x);
+        hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(BLOCK), 0, s, n, omega, sh, x);
+        CFA_TRY(hipMemcpyAsync(r, sv, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(BLOCK), 0, s, n, -omega, t, r);
+        const double r_norm = std::sqrt(dot(r, r));
+        result->rel_residual[0] = r_norm / rn0;
+        if (r_norm < tol * rn0 || r_norm < 1.0e-11) {
+            ++iter;
+            done = true;
+            break;
+        }
+        rho_old = rho_new;
+    }
+    result->iters[0] = iter;
+    result->converged[0] = done ? 1 : 0;
+    const hipError_t e = hipStreamSynchronize(s);
+    return e == hipSuccess ? launch_status() : (int)e;
+}
+
+}  // extern "C"

@@ -1,0 +1,86 @@
+// Device state of one mesh and the GPU driver routines shared by time stepping and remeshing.
+// Interfaces of src/cfa/device.cuh:21-574 (DeviceMeshBuffers, compute_error_indicator,
+// apply_oss_projection[_matrix_free], assemble_oss_projection, project_scheme_consistent,
+// project_divergence_free, set_initial_condition, print_amr_report, RemeshProfile) and of the
+// cylinder probes of src/cfa/diagnostics.cuh:8-45.
+#pragma once
+
+#include <functional>
+
+#include "amr.hpp"
+#include "gpu.hpp"
+#include "linalg.hpp"
+
+namespace cfa {
+
+struct PressureProbe {
+    int a = 0, b = 0;
+    double w = 0.0;   // p = (1 - w) p_a + w p_b
+};
+PressureProbe find_cylinder_probe(const Mesh2D& mesh, double theta);
+
+struct DeviceMeshBuffers {
+    int n_nodes = 0, n_elems = 0, nnz = 0;
+    int n_cyl_edges = 0, n_layer = 0;
+
+    DevArray<int> d_elem_v, d_cyl_edges, d_bc_tag, d_row_ptr, d_col_ind;
+    DevArray<int> d_inc_ptr, d_inc_elem, d_inc_pos, d_layer;   // owner-computes tables
+    DevArray<double> d_x, d_y, d_u, d_v, d_u_old, d_v_old, d_u_prev, d_v_prev, d_p, d_dp, d_vort;
+    DevArray<double> d_A_val, d_Kp_val, d_rhs_u, d_rhs_v, d_rhs_p, d_mass_lumped;
+    DevArray<double> d_grad_dp_x, d_grad_dp_y, d_vort_num, d_elem_error, d_grad_rec, d_div_num;
+    DevArray<double> d_elem_s, d_oss_gx, d_oss_gy, d_diag, d_oss_val, d_inv_m, d_phi, d_ws;
+
+    PressureProbe probe_front, probe_back;
+
+    const cfa_mesh_t* view() const { return &view_; }
+    cfa_oss_op_t oss_op() const;
+
+    void sync_from_host(const Mesh2D& mesh, const std::vector<double>& h_u, const std::vector<double>& h_v,
+                        const std::vector<double>& h_p, const std::vector<double>& h_uo,
+                        const std::vector<double>& h_vo);
+    void copy_solution_to_host(std::vector<double>& h_u, std::vector<double>& h_v, std::vector<double>& h_p,
+                               std::vector<double>& h_uo, std::vector<double>& h_vo, std::vector<double>& h_vort,
+                               std::vector<double>& h_elem_err);
+private:
+    cfa_mesh_t view_{};
+};
+
+// node -> incident element table of the owner-computes kernels (ascending element order) and
+// the CSR position of each (node, element vertex) pair
+void build_incidence(const Mesh2D& mesh, std::vector<int>& inc_ptr, std::vector<int>& inc_elem,
+                     std::vector<int>& inc_pos);
+
+void compute_error_indicator(DeviceMeshBuffers& g, double inflow_peak = Sim::U_MAX);
+void apply_oss_projection_matrix_free(DeviceMeshBuffers& g, const double* in, double* out);
+void assemble_oss_projection(DeviceMeshBuffers& g);
+void apply_oss_projection(DeviceMeshBuffers& g, const double* in, double* out);
+int project_scheme_consistent(DeviceMeshBuffers& g, GpuLinearSolver& solver, double dt_proj, double stab,
+                              AmgPreconditioner* amg, int& nonconverged,
+                              const std::function<void()>& build_amg = nullptr, bool both_levels = true,
+                              double* phi_out = nullptr, double tol = 1.0e-8);
+int project_divergence_free(DeviceMeshBuffers& g, GpuLinearSolver& solver, double dt, AmgPreconditioner* amg,
+                            bool setup_amg, double* u, double* v, double* vort_out, double tol = 1.0e-8);
+void set_initial_condition(const Mesh2D& mesh, std::vector<double>& h_u, std::vector<double>& h_v,
+                           std::vector<double>& h_p, double inflow_peak = Sim::U_MAX);
+void print_amr_report(const char* tag, const AmrStats& st, const Mesh2D& mesh, const MeshQuality& q,
+                      int proj_iters);
+
+struct RemeshProfile {
+    int count = 0;
+    double indicator = 0.0;
+    double lookahead = 0.0, mark = 0.0, neighbours = 0.0, coarsen = 0.0, refine = 0.0;
+    double extract = 0.0, csr = 0.0, validate = 0.0, upload = 0.0, colouring = 0.0;
+    double amg_setup = 0.0, projection = 0.0;
+    double overlapped = 0.0, join_wait = 0.0, transfer = 0.0;
+    void add(const AmrStats& st) {
+        ++count;
+        lookahead += st.ms_lookahead;
+        mark += st.ms_mark;
+        neighbours += st.ms_neighbours;
+        coarsen += st.ms_coarsen;
+        refine += st.ms_refine;
+    }
+    void report(double total_ms) const;
+};
+
+}  // namespace cfa

@@ -1,0 +1,742 @@
+// Solver classes over the C ABI, and the host smoothed-aggregation set-up.
+// The set-up restates src/cfa/linalg.cuh:995-1060 and 1276-1431: strength |a_ij| >=
+// theta sqrt(|a_ii a_jj|) (theta 0.08), three-phase greedy aggregation in row order, tentative
+// prolongator 1/sqrt(|aggregate|), one damped Jacobi smoothing step with omega = 4 / (3 * 1.05 *
+// rho(D^-1 A)) from 20 power iterations, Galerkin R A P with R = P^T, stop at <= 300 rows,
+// 15 levels or when P.cols >= 0.85 rows, dense Cholesky inverse of the coarsest operator. Every
+// parallel loop writes disjoint rows and each sum keeps its serial order, so the hierarchy does
+// not depend on the thread count.
+#include "linalg.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace cfa {
+
+// ---------------------------------------------------------------------------------------------
+KrylovWorkspace::~KrylovWorkspace() {
+    if (status_host_) cfa_host_free(status_host_);
+}
+
+cfa_krylov_ws_t* KrylovWorkspace::get(int n) {
+    if (!status_host_) {
+        void* d_alias = nullptr;
+        GPU_CHECK(cfa_host_malloc(&status_host_, &d_alias, 16 * sizeof(int)));
+        std::memset(status_host_, 0, 16 * sizeof(int));
+        ws_.status_host = static_cast<int*>(status_host_);
+        ws_.status_dev = static_cast<int*>(d_alias);
+        part_.reserve(8 * 4096, 0);
+        scal_.reserve(64, 0);
+        ws_.part = part_.ptr;
+        ws_.scal = scal_.ptr;
+    }
+    if (n > ws_.cap) {
+        const int cap = n + n / 2 + 256;
+        for (int k = 0; k < 12; ++k) {
+            vec_[k].release();
+            vec_[k].reserve(2 * (size_t)cap, 0);
+            ws_.vec[k] = vec_[k].ptr;
+        }
+        ws_.cap = cap;
+    }
+    return &ws_;
+}
+
+// ---------------------------------------------------------------------------------------------
+// GpuLinearSolver (src/cfa/linalg.cuh:120-348)
+// ---------------------------------------------------------------------------------------------
+namespace {
+void call_op(const double* in, double* out, void* ctx) {
+    (*static_cast<const std::function<void(const double*, double*)>*>(ctx))(in, out);
+}
+}  // namespace
+
+void GpuLinearSolver::allocate(int n) {
+    ws_.get(n);
+    max_n = ws_.capacity();
+}
+
+double GpuLinearSolver::dot(int n, const double* d_x, const double* d_y) {
+    dot_ws_.reserve(2048, 0);
+    double r = 0.0;
+    GPU_CHECK(cfa_dot(n, d_x, d_y, &r, dot_ws_.ptr, solver_stream()));
+    return r;
+}
+
+int GpuLinearSolver::solve_pcg(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
+                               const double* d_b, double* d_x, int max_iter, double tol,
+                               const std::function<void(const double*, double*)>* extra_op,
+                               const std::function<void(const double*, double*)>* precond) {
+    cfa_solve_result_t res{};
+    GPU_CHECK(cfa_pcg(n, d_row_ptr, d_col_ind, d_val, d_b, d_x, max_iter, tol, nullptr,
+                      extra_op ? call_op : nullptr, const_cast<void*>(static_cast<const void*>(extra_op)), nullptr,
+                      precond ? call_op : nullptr, const_cast<void*>(static_cast<const void*>(precond)), ws_.get(n),
+                      &res, solver_stream()));
+    max_n = ws_.capacity();
+    converged = res.converged[0] != 0;
+    last_rel_residual = res.rel_residual[0];
+    last_iters = res.iters[0];
+    return res.iters[0];
+}
+
+int GpuLinearSolver::solve_pcg_fused(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
+                                     const double* d_b, double* d_x, int max_iter, double tol,
+                                     const cfa_oss_op_t* oss, AmgPreconditioner* amg) {
+    cfa_solve_result_t res{};
+    GPU_CHECK(cfa_pcg(n, d_row_ptr, d_col_ind, d_val, d_b, d_x, max_iter, tol, oss, nullptr, nullptr,
+                      amg ? amg->view() : nullptr, nullptr, nullptr, ws_.get(n), &res, solver_stream()));
+    max_n = ws_.capacity();
+    converged = res.converged[0] != 0;
+    last_rel_residual = res.rel_residual[0];
+    last_iters = res.iters[0];
+    return res.iters[0];
+}
+
+int GpuLinearSolver::solve_bicgstab(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
+                                    const double* d_b, double* d_x, int max_iter, double tol) {
+    cfa_solve_result_t res{};
+    GPU_CHECK(cfa_bicgstab_jacobi(n, d_row_ptr, d_col_ind, d_val, d_b, d_x, max_iter, tol, ws_.get(n), &res,
+                                  solver_stream()));
+    max_n = ws_.capacity();
+    converged = res.converged[0] != 0;
+    last_rel_residual = res.rel_residual[0];
+    last_iters = res.iters[0];
+    return res.iters[0];
+}
+
+// ---------------------------------------------------------------------------------------------
+// MulticolourIlu0 (src/cfa/linalg.cuh:435-568)
+// ---------------------------------------------------------------------------------------------
+void MulticolourIlu0::set_pattern(int n_, const std::vector<int>& rp, const std::vector<int>& ci, const int* d_rp,
+                                  const int* d_ci) {
+    const auto t0 = Clock::now();
+    n = n_;
+    nnz = rp[n];
+    d_row_ptr = d_rp;
+    d_col_ind = d_ci;
+    // greedy colouring in node order: the smallest colour not used by an earlier neighbour
+    std::vector<int> colour(n, -1), diag(n, -1), mark;
+    n_colours = 0;
+    for (int i = 0; i < n; ++i) {
+        if (rp[i + 1] - rp[i] > 32) {
+            std::fprintf(stderr, "[ILU] row %d has %d entries (limit %d)\n", i, rp[i + 1] - rp[i], 32);
+            std::exit(EXIT_FAILURE);
+        }
+        for (int a = rp[i]; a < rp[i + 1]; ++a) {
+            const int c = colour[ci[a]];
+            if (c >= 0) mark[c] = i;
+            if (ci[a] == i) diag[i] = a;
+        }
+        int c = 0;
+        while (c < n_colours && mark[c] == i) ++c;
+        if (c == n_colours) { ++n_colours; mark.push_back(-1); }
+        colour[i] = c;
+    }
+    colour_start.assign(n_colours + 1, 0);
+    for (int i = 0; i < n; ++i) colour_start[colour[i] + 1]++;
+    for (int c = 0; c < n_colours; ++c) colour_start[c + 1] += colour_start[c];
+    std::vector<int> rows(n), fill(colour_start.begin(), colour_start.end() - 1);
+    for (int i = 0; i < n; ++i) rows[fill[colour[i]]++] = i;
+    // split pattern in colour-row order: L = earlier colours, U = later colours
+    std::vector<int> lp(n + 1, 0), up(n + 1, 0);
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < n; ++t) {
+        const int i = rows[t];
+        int l = 0, u = 0;
+        for (int a = rp[i]; a < rp[i + 1]; ++a) {
+            if (ci[a] == i) continue;
+            if (colour[ci[a]] < colour[i]) ++l; else ++u;
+        }
+        lp[t + 1] = l;
+        up[t + 1] = u;
+    }
+    for (int t = 0; t < n; ++t) { lp[t + 1] += lp[t]; up[t + 1] += up[t]; }
+    nl = lp[n];
+    nu = up[n];
+    std::vector<int> lc(nl), uc(nu), l_src(nl), u_src(nu), d_src(n);
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < n; ++t) {
+        const int i = rows[t];
+        int l = lp[t], u = up[t];
+        for (int a = rp[i]; a < rp[i + 1]; ++a) {
+            const int j = ci[a];
+            if (j == i) d_src[t] = a;
+            else if (colour[j] < colour[i]) { lc[l] = j; l_src[l++] = a; }
+            else { uc[u] = j; u_src[u++] = a; }
+        }
+    }
+    d_rows.upload(rows);
+    d_colour.upload(colour);
+    d_diag_pos.upload(diag);
+    d_lp.upload(lp);
+    d_up.upload(up);
+    d_lc.upload(lc);
+    d_uc.upload(uc);
+    d_l_src.upload(l_src);
+    d_u_src.upload(u_src);
+    d_d_src.upload(d_src);
+    d_lu.reserve(nnz, 256);
+    d_lv.reserve(std::max(nl, 1), 256);
+    d_uv.reserve(std::max(nu, 1), 256);
+    d_dinv.reserve(n, 64);
+    if (keep_fp32) {
+        d_lv32.reserve(std::max(nl, 1), 256);
+        d_uv32.reserve(std::max(nu, 1), 256);
+        d_dinv32.reserve(n, 64);
+    }
+    refresh_view();
+    colour_ms += ms_since(t0);
+}
+
+void MulticolourIlu0::refresh_view() {
+    view_.n = n;
+    view_.nnz = nnz;
+    view_.n_colours = n_colours;
+    view_.nl = nl;
+    view_.nu = nu;
+    view_.colour_start = colour_start.data();
+    view_.rows = d_rows;
+    view_.colour = d_colour;
+    view_.diag_pos = d_diag_pos;
+    view_.row_ptr = d_row_ptr;
+    view_.col_ind = d_col_ind;
+    view_.lu = d_lu;
+    view_.lp = d_lp;
+    view_.up = d_up;
+    view_.lc = d_lc;
+    view_.uc = d_uc;
+    view_.l_src = d_l_src;
+    view_.u_src = d_u_src;
+    view_.d_src = d_d_src;
+    view_.lv = d_lv;
+    view_.uv = d_uv;
+    view_.dinv = d_dinv;
+    view_.lv32 = keep_fp32 ? d_lv32.ptr : nullptr;
+    view_.uv32 = keep_fp32 ? d_uv32.ptr : nullptr;
+    view_.dinv32 = keep_fp32 ? d_dinv32.ptr : nullptr;
+}
+
+void MulticolourIlu0::factor(const double* d_val) { GPU_CHECK(cfa_ilu0_factor(&view_, d_val, solver_stream())); }
+
+void MulticolourIlu0::apply(const double* r, double* z, int nrhs) const {
+    GPU_CHECK(cfa_ilu0_apply(&view_, r, z, nrhs, solver_stream()));
+}
+
+// ---------------------------------------------------------------------------------------------
+// GpuBicgstab2 (src/cfa/linalg.cuh:642-779)
+// ---------------------------------------------------------------------------------------------
+void GpuBicgstab2::solve(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val, const double* b_u,
+                         const double* b_v, double* x_u, double* x_v, int max_iter, double tol,
+                         const MulticolourIlu0& ilu) {
+    cfa_solve_result_t res{};
+    GPU_CHECK(cfa_bicgstab2(n, d_row_ptr, d_col_ind, d_val, b_u, b_v, x_u, x_v, max_iter, tol, ilu.view(),
+                            precond_fp32 ? 1 : 0, ws_.get(n), &res, solver_stream()));
+    for (int c = 0; c < 2; ++c) {
+        iters[c] = res.iters[c];
+        converged[c] = res.converged[c] != 0;
+        last_rel_residual[c] = res.rel_residual[c];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host CSR products (src/cfa/linalg.cuh:799-862)
+// ---------------------------------------------------------------------------------------------
+HostCsr csr_multiply(const HostCsr& A, const HostCsr& B) {
+    HostCsr C;
+    C.rows = A.rows;
+    C.cols = B.cols;
+    C.rp.assign(A.rows + 1, 0);
+#pragma omp parallel
+    {
+        std::vector<int> marker(B.cols, -1);
+#pragma omp for schedule(static)
+        for (int i = 0; i < A.rows; ++i) {
+            int count = 0;
+            for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
+                for (int b = B.rp[A.ci[a]]; b < B.rp[A.ci[a] + 1]; ++b) {
+                    const int j = B.ci[b];
+                    if (marker[j] != i) { marker[j] = i; ++count; }
+                }
+            C.rp[i + 1] = count;
+        }
+    }
+    for (int i = 0; i < A.rows; ++i) C.rp[i + 1] += C.rp[i];
+    C.ci.resize(C.rp[A.rows]);
+    C.v.resize(C.rp[A.rows]);
+#pragma omp parallel
+    {
+        std::vector<int> marker(B.cols, -1), cols;
+        std::vector<double> acc(B.cols, 0.0);
+#pragma omp for schedule(static)
+        for (int i = 0; i < A.rows; ++i) {
+            cols.clear();
+            for (int a = A.rp[i]; a < A.rp[i + 1]; ++a) {
+                const int k = A.ci[a];
+                const double av = A.v[a];
+                for (int b = B.rp[k]; b < B.rp[k + 1]; ++b) {
+                    const int j = B.ci[b];
+                    if (marker[j] != i) { marker[j] = i; acc[j] = 0.0; cols.push_back(j); }
+                    acc[j] += av * B.v[b];
+                }
+            }
+            std::sort(cols.begin(), cols.end());
+            int o = C.rp[i];
+            for (int j : cols) { C.ci[o] = j; C.v[o] = acc[j]; ++o; }
+        }
+    }
+    return C;
+}
+
+HostCsr csr_transpose(const HostCsr& A) {
+    HostCsr T;
+    T.rows = A.cols;
+    T.cols = A.rows;
+    T.rp.assign(A.cols + 1, 0);
+    for (int c : A.ci) T.rp[c + 1]++;
+    for (int i = 0; i < A.cols; ++i) T.rp[i + 1] += T.rp[i];
+    T.ci.resize(A.ci.size());
+    T.v.resize(A.v.size());
+    std::vector<int> fill(T.rp.begin(), T.rp.end() - 1);
+    for (int i = 0; i < A.rows; ++i)
+        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a) {
+            const int k = fill[A.ci[a]]++;
+            T.ci[k] = i;
+            T.v[k] = A.v[a];
+        }
+    return T;
+}
+
+int csr_lanes_per_row(int rows, long long nnz) {
+    const double per_row = rows > 0 ? (double)nnz / rows : 1.0;
+    const double target = per_row / (rows < 20000 ? 2.0 : 4.0);
+    int tpr = 1;
+    while (tpr < 64 && 2 * tpr <= target) tpr *= 2;
+    return tpr;
+}
+
+// ---------------------------------------------------------------------------------------------
+// AmgPreconditioner
+// ---------------------------------------------------------------------------------------------
+double AmgPreconditioner::spectral_radius_DinvA(const HostCsr& A, const std::vector<double>& dinv) {
+    const int n = A.rows, chunk = 4096, nch = (n + chunk - 1) / chunk;
+    std::vector<double> x(n), y(n), py(nch), px(nch);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) x[i] = 1.0 + 0.1 * std::sin(1.0 + i);
+    double lambda = 1.0;
+    for (int it = 0; it < 20; ++it) {
+#pragma omp parallel for schedule(static)
+        for (int c = 0; c < nch; ++c) {
+            double sy = 0.0, sx = 0.0;
+            for (int i = c * chunk; i < std::min(n, (c + 1) * chunk); ++i) {
+                double s = 0.0;
+                for (int a = A.rp[i]; a < A.rp[i + 1]; ++a) s += A.v[a] * x[A.ci[a]];
+                y[i] = dinv[i] * s;
+                sy += y[i] * y[i];
+                sx += x[i] * x[i];
+            }
+            py[c] = sy;
+            px[c] = sx;
+        }
+        double nrm = 0.0, xn = 0.0;
+        for (int c = 0; c < nch; ++c) { nrm += py[c]; xn += px[c]; }
+        nrm = std::sqrt(nrm);
+        lambda = nrm / std::sqrt(xn);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < n; ++i) x[i] = y[i] / nrm;
+    }
+    return std::max(lambda, 1.0e-12) * 1.05;
+}
+
+HostCsr AmgPreconditioner::smoothed_prolongator(const HostCsr& A, const std::vector<double>& dinv, double omega,
+                                                std::vector<int>* agg_out) const {
+    const int n = A.rows;
+    std::vector<double> diag(n);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) diag[i] = 1.0 / dinv[i];
+    std::vector<char> strong(A.ci.size());
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i)
+        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a) {
+            const int j = A.ci[a];
+            strong[a] = j != i && std::fabs(A.v[a]) >= theta * std::sqrt(std::fabs(diag[i] * diag[j]));
+        }
+    // phase 1: aggregates seeded at rows whose strong neighbourhood is still free (row order)
+    std::vector<int> agg(n, -1);
+    int na = 0;
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        bool free_nb = true;
+        for (int a = A.rp[i]; a < A.rp[i + 1] && free_nb; ++a)
+            if (strong[a] && agg[A.ci[a]] >= 0) free_nb = false;
+        if (!free_nb) continue;
+        agg[i] = na;
+        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
+            if (strong[a]) agg[A.ci[a]] = na;
+        ++na;
+    }
+    // phase 2: unassigned rows join the aggregate of their first strong neighbour from phase 1
+    std::vector<int> agg2(agg);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
+            if (strong[a] && agg[A.ci[a]] >= 0) { agg2[i] = agg[A.ci[a]]; break; }
+    }
+    agg.swap(agg2);
+    // phase 3: the rest seed new aggregates with their unassigned strong neighbours
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        agg[i] = na;
+        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
+            if (strong[a] && agg[A.ci[a]] < 0) agg[A.ci[a]] = na;
+        ++na;
+    }
+    if (agg_out) *agg_out = agg;
+    std::vector<int> size(na, 0);
+    for (int i = 0; i < n; ++i) size[agg[i]]++;
+    HostCsr P0;
+    P0.rows = n;
+    P0.cols = na;
+    P0.rp.resize(n + 1);
+    P0.ci.resize(n);
+    P0.v.resize(n);
+    for (int i = 0; i < n; ++i) {
+        P0.rp[i] = i;
+        P0.ci[i] = agg[i];
+        P0.v[i] = 1.0 / std::sqrt((double)size[agg[i]]);
+    }
+    P0.rp[n] = n;
+    // P = P0 - omega D^-1 A P0, rows merged in column order, exact zeros dropped
+    const HostCsr AP0 = csr_multiply(A, P0);
+    HostCsr P;
+    P.rows = n;
+    P.cols = na;
+    P.rp.assign(n + 1, 0);
+    auto build_row = [&](int i, int* cols, double* vals) {
+        int out = 0;
+        bool own = false;
+        const int oc = agg[i];
+        for (int a = AP0.rp[i]; a < AP0.rp[i + 1]; ++a) {
+            const int j = AP0.ci[a];
+            if (!own && j > oc) {
+                if (cols) { cols[out] = oc; vals[out] = P0.v[i]; }
+                ++out;
+                own = true;
+            }
+            double val = -omega * dinv[i] * AP0.v[a];
+            if (j == oc) { val += P0.v[i]; own = true; }
+            if (val != 0.0) {
+                if (cols) { cols[out] = j; vals[out] = val; }
+                ++out;
+            }
+        }
+        if (!own) {
+            if (cols) { cols[out] = oc; vals[out] = P0.v[i]; }
+            ++out;
+        }
+        return out;
+    };
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) P.rp[i + 1] = build_row(i, nullptr, nullptr);
+    for (int i = 0; i < n; ++i) P.rp[i + 1] += P.rp[i];
+    P.ci.resize(P.rp[n]);
+    P.v.resize(P.rp[n]);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) build_row(i, P.ci.data() + P.rp[i], P.v.data() + P.rp[i]);
+    return P;
+}
+
+std::vector<double> AmgPreconditioner::dense_spd_inverse(std::vector<double> M, int n) {
+    for (int j = 0; j < n; ++j) {
+        double d = M[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= M[(size_t)j * n + k] * M[(size_t)j * n + k];
+        if (!(d > 0.0)) {
+            std::fprintf(stderr, "[AMG] coarse operator not SPD (pivot %.3e), aborting\n", d);
+            std::exit(EXIT_FAILURE);
+        }
+        const double ljj = std::sqrt(d);
+        M[(size_t)j * n + j] = ljj;
+        for (int i = j + 1; i < n; ++i) {
+            double s = M[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) s -= M[(size_t)i * n + k] * M[(size_t)j * n + k];
+            M[(size_t)i * n + j] = s / ljj;
+        }
+    }
+    std::vector<double> inv((size_t)n * n, 0.0);
+#pragma omp parallel
+    {
+        std::vector<double> col(n);
+#pragma omp for schedule(dynamic, 8)
+        for (int c = 0; c < n; ++c) {
+            for (int i = 0; i < n; ++i) {
+                double s = (i == c) ? 1.0 : 0.0;
+                for (int k = 0; k < i; ++k) s -= M[(size_t)i * n + k] * col[k];
+                col[i] = s / M[(size_t)i * n + i];
+            }
+            for (int i = n - 1; i >= 0; --i) {
+                double s = col[i];
+                for (int k = i + 1; k < n; ++k) s -= M[(size_t)k * n + i] * col[k];
+                col[i] = s / M[(size_t)i * n + i];
+            }
+            for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = col[i];
+        }
+    }
+    return inv;
+}
+
+void AmgPreconditioner::upload_csr(const HostCsr& M, DevCsr& D) {
+    D.rows = M.rows;
+    D.nnz = (int)M.ci.size();
+    D.tpr = csr_lanes_per_row(D.rows, D.nnz);
+    D.rp.upload(M.rp);
+    if (D.nnz) {
+        D.ci.upload(M.ci);
+        D.v.upload(M.v);
+    } else {
+        D.ci.reserve(1);
+        D.v.reserve(1);
+    }
+}
+
+void AmgPreconditioner::upload_csr_from_device(int rows, int nnz, const int* rp, const int* ci, const double* v,
+                                               DevCsr& D) {
+    D.rows = rows;
+    D.nnz = nnz;
+    D.tpr = csr_lanes_per_row(rows, nnz);
+    D.rp.reserve(rows + 1);
+    D.ci.reserve(std::max(nnz, 1));
+    D.v.reserve(std::max(nnz, 1));
+    GPU_CHECK(cfa_memcpy(D.rp.ptr, rp, (rows + 1) * sizeof(int), 2, solver_stream()));
+    if (nnz) {
+        GPU_CHECK(cfa_memcpy(D.ci.ptr, ci, nnz * sizeof(int), 2, solver_stream()));
+        GPU_CHECK(cfa_memcpy(D.v.ptr, v, nnz * sizeof(double), 2, solver_stream()));
+    }
+}
+
+void AmgPreconditioner::ensure_level_vectors(Level& L, int n) {
+    L.dinv.reserve(n, 16);
+    L.x.reserve(n, 16);
+    L.b.reserve(n, 16);
+    L.r.reserve(n, 16);
+    L.Ax.reserve(n, 16);
+}
+
+void AmgPreconditioner::release() {
+    for (auto& L : levels) spare_.push_back(std::move(L));
+    levels.clear();
+    coarse_n = 0;
+}
+
+void AmgPreconditioner::setup(int n, int nnz, const int* d_row_ptr, const int* d_col_ind, const double* d_val) {
+    const auto t0 = Clock::now();
+    // keep the buffers of the previous hierarchy for reuse, finest first
+    std::vector<std::unique_ptr<Level>> old;
+    old.swap(levels);
+    for (auto it = spare_.begin(); it != spare_.end(); ++it) old.push_back(std::move(*it));
+    spare_.clear();
+    auto take_level = [&]() {
+        if (!old.empty()) {
+            auto L = std::move(old.front());
+            old.erase(old.begin());
+            return L;
+        }
+        return std::make_unique<Level>();
+    };
+    phase_start();
+    HostCsr A;
+    A.rows = A.cols = n;
+    A.rp.resize(n + 1);
+    A.ci.resize(nnz);
+    A.v.resize(nnz);
+    GPU_CHECK(cfa_memcpy(A.rp.data(), d_row_ptr, (n + 1) * sizeof(int), 1, solver_stream()));
+    GPU_CHECK(cfa_memcpy(A.ci.data(), d_col_ind, nnz * sizeof(int), 1, solver_stream()));
+    GPU_CHECK(cfa_memcpy(A.v.data(), d_val, nnz * sizeof(double), 1, solver_stream()));
+    phase_end(PH_DOWNLOAD);
+    last_aggregates.clear();
+
+    while (A.rows > coarse_target && (int)levels.size() < max_levels - 1) {
+        std::vector<double> dinv(A.rows);
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < A.rows; ++i) {
+            double d = 0.0;
+            for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
+                if (A.ci[a] == i) d = A.v[a];
+            dinv[i] = (std::fabs(d) > 1.0e-300) ? 1.0 / d : 1.0;
+        }
+        const double omega = 4.0 / (3.0 * spectral_radius_DinvA(A, dinv));
+        phase_end(PH_RHO);
+        phase_start();
+        std::vector<int> agg;
+        HostCsr P = smoothed_prolongator(A, dinv, omega, levels.empty() ? &agg : nullptr);
+        phase_end(PH_SMOOTH);
+        if (P.cols >= (int)(0.85 * A.rows) || P.cols == 0) break;
+        if (levels.empty()) last_aggregates = agg;
+        HostCsr R = csr_transpose(P);
+        phase_end(PH_TRANSPOSE);
+        HostCsr Ac = csr_multiply(R, csr_multiply(A, P));
+        phase_end(PH_GALERKIN);
+
+        auto L = take_level();
+        if (levels.empty()) upload_csr_from_device(A.rows, nnz, d_row_ptr, d_col_ind, d_val, L->A);
+        else upload_csr(A, L->A);
+        upload_csr(P, L->P);
+        upload_csr(R, L->R);
+        L->omega = omega;
+        ensure_level_vectors(*L, A.rows);
+        L->dinv.upload(dinv.data(), dinv.size());
+        if (levels.empty()) P0_host = std::move(P);
+        levels.push_back(std::move(L));
+        A = std::move(Ac);
+        phase_end(PH_UPLOAD);
+    }
+    for (auto& L : old) spare_.push_back(std::move(L));
+
+    phase_start();
+    coarse_n = A.rows;
+    std::vector<double> D((size_t)coarse_n * coarse_n, 0.0);
+    for (int i = 0; i < coarse_n; ++i)
+        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a) D[(size_t)i * coarse_n + A.ci[a]] = A.v[a];
+    const std::vector<double> inv = dense_spd_inverse(D, coarse_n);
+    d_coarse_inv.upload(inv);
+    d_cx.reserve(std::max(coarse_n, 1), 16);
+    d_cb.reserve(std::max(coarse_n, 1), 16);
+    refresh_view();
+    dev_sync();
+    phase_end(PH_COARSE);
+    setup_ms = ms_since(t0);
+}
+
+void AmgPreconditioner::refresh_view() {
+    desc_.resize(levels.size());
+    for (size_t l = 0; l < levels.size(); ++l) {
+        const Level& L = *levels[l];
+        cfa_amg_level_t& d = desc_[l];
+        d.rows = L.A.rows;
+        d.cols = L.P.rows > 0 ? (int)(L.R.rows) : 0;
+        d.a_nnz = L.A.nnz;
+        d.p_nnz = L.P.nnz;
+        d.r_nnz = L.R.nnz;
+        d.tpr_a = L.A.tpr;
+        d.tpr_p = L.P.tpr;
+        d.tpr_r = L.R.tpr;
+        d.a_rp = L.A.rp; d.a_ci = L.A.ci; d.a_v = L.A.v;
+        d.p_rp = L.P.rp; d.p_ci = L.P.ci; d.p_v = L.P.v;
+        d.r_rp = L.R.rp; d.r_ci = L.R.ci; d.r_v = L.R.v;
+        d.dinv = L.dinv;
+        d.omega = L.omega;
+        d.x = L.x; d.b = L.b; d.r = L.r; d.ax = L.Ax;
+    }
+    if (!desc_.empty()) d_desc_.upload(desc_.data(), desc_.size());
+    view_.n_levels = (int)levels.size();
+    view_.levels = desc_.data();
+    view_.d_levels = desc_.empty() ? nullptr : d_desc_.ptr;
+    view_.coarse_n = coarse_n;
+    view_.coarse_inv = d_coarse_inv;
+    view_.cb = d_cb;
+    view_.cx = d_cx;
+    view_.smooth_steps = smooth_steps;
+    view_.small_rows = small_rows;
+}
+
+void AmgPreconditioner::apply(const double* d_r, double* d_z) {
+    GPU_CHECK(cfa_amg_apply(&view_, d_r, d_z, solver_stream()));
+}
+
+// Carries the level-0 prolongator to a new mesh (src/cfa/linalg.cuh:1074-1157): kept nodes keep
+// their row, a new node gets the mean of the rows of its already-set graph neighbours, pass by
+// pass; coarser levels, omega and the coarse inverse stay.
+void AmgPreconditioner::remap(int n, int nnz, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
+                              const int* h_rp, const int* h_ci, const std::vector<int>& new_to_old) {
+    if (levels.empty() || P0_host.rows == 0) {
+        setup(n, nnz, d_row_ptr, d_col_ind, d_val);
+        return;
+    }
+    const auto t0 = Clock::now();
+    phase_start();
+    const HostCsr& Po = P0_host;
+    HostCsr P;
+    P.rows = n;
+    P.cols = Po.cols;
+    P.rp.assign(n + 1, 0);
+    std::vector<int> extra_of(n, -1);
+    std::vector<std::vector<std::pair<int, double>>> extra;
+    std::vector<int> pending;
+    for (int i = 0; i < n; ++i)
+        if (new_to_old[i] < 0) pending.push_back(i);
+    std::vector<char> has_row(n);
+    for (int i = 0; i < n; ++i) has_row[i] = new_to_old[i] >= 0;
+    std::vector<std::pair<int, double>> acc;
+    while (!pending.empty()) {
+        std::vector<int> still, set_now;
+        for (int i : pending) {
+            acc.clear();
+            int cnt = 0;
+            for (int a = h_rp[i]; a < h_rp[i + 1]; ++a) {
+                const int k = h_ci[a];
+                if (k == i || !has_row[k]) continue;
+                ++cnt;
+                if (new_to_old[k] >= 0) {
+                    const int j = new_to_old[k];
+                    for (int b = Po.rp[j]; b < Po.rp[j + 1]; ++b) acc.push_back({Po.ci[b], Po.v[b]});
+                } else {
+                    for (const auto& e : extra[extra_of[k]]) acc.push_back(e);
+                }
+            }
+            if (cnt == 0) { still.push_back(i); continue; }
+            std::sort(acc.begin(), acc.end(),
+                      [](const std::pair<int, double>& x, const std::pair<int, double>& y) { return x.first < y.first; });
+            std::vector<std::pair<int, double>> row;
+            for (const auto& e : acc) {
+                if (!row.empty() && row.back().first == e.first) row.back().second += e.second;
+                else row.push_back(e);
+            }
+            for (auto& e : row) e.second /= cnt;
+            extra_of[i] = (int)extra.size();
+            extra.push_back(std::move(row));
+            set_now.push_back(i);
+        }
+        if (set_now.empty()) {
+            setup(n, nnz, d_row_ptr, d_col_ind, d_val);
+            return;
+        }
+        for (int i : set_now) has_row[i] = 1;
+        pending.swap(still);
+    }
+    for (int i = 0; i < n; ++i) {
+        const int j = new_to_old[i];
+        P.rp[i + 1] = j >= 0 ? Po.rp[j + 1] - Po.rp[j] : (int)extra[extra_of[i]].size();
+    }
+    for (int i = 0; i < n; ++i) P.rp[i + 1] += P.rp[i];
+    P.ci.resize(P.rp[n]);
+    P.v.resize(P.rp[n]);
+    for (int i = 0; i < n; ++i) {
+        const int j = new_to_old[i];
+        int o = P.rp[i];
+        if (j >= 0) {
+            std::copy(Po.ci.begin() + Po.rp[j], Po.ci.begin() + Po.rp[j + 1], P.ci.begin() + o);
+            std::copy(Po.v.begin() + Po.rp[j], Po.v.begin() + Po.rp[j + 1], P.v.begin() + o);
+        } else {
+            for (const auto& e : extra[extra_of[i]]) { P.ci[o] = e.first; P.v[o] = e.second; ++o; }
+        }
+    }
+    HostCsr R = csr_transpose(P);
+    Level& L = *levels[0];
+    upload_csr_from_device(n, nnz, d_row_ptr, d_col_ind, d_val, L.A);
+    upload_csr(P, L.P);
+    upload_csr(R, L.R);
+    ensure_level_vectors(L, n);
+    GPU_CHECK(cfa_inv_diag(n, L.A.rp, L.A.ci, L.A.v, 1.0e-300, L.dinv, solver_stream()));
+    P0_host = std::move(P);
+    ++remaps;
+    refresh_view();
+    dev_sync();
+    phase_end(PH_REMAP);
+    setup_ms = ms_since(t0);
+}
+
+}  // namespace cfa

@@ -1,0 +1,183 @@
+// Solver classes with the reference's interfaces (src/cfa/linalg.cuh): GpuLinearSolver
+// (:120-348), MulticolourIlu0 (:435-568), GpuBicgstab2 (:642-779), AmgPreconditioner
+// (:939-1432). The numerical work runs in the gfx950 kernels behind include/cfa_hip.h; the
+// smoothed-aggregation set-up runs on the host (OpenMP), as in the reference.
+#pragma once
+
+#include <functional>
+#include <memory>
+#include <vector>
+
+#include "gpu.hpp"
+
+namespace cfa {
+
+// Krylov workspace shared by the solvers (vectors, partial sums, device scalars, status word)
+class KrylovWorkspace {
+public:
+    KrylovWorkspace() = default;
+    KrylovWorkspace(const KrylovWorkspace&) = delete;
+    KrylovWorkspace& operator=(const KrylovWorkspace&) = delete;
+    ~KrylovWorkspace();
+    cfa_krylov_ws_t* get(int n);   // capacity >= n
+    int capacity() const { return ws_.cap; }
+private:
+    cfa_krylov_ws_t ws_{};
+    DevArray<double> vec_[12];
+    DevArray<double> part_, scal_;
+    void* status_host_ = nullptr;
+};
+
+class AmgPreconditioner;
+
+class GpuLinearSolver {
+public:
+    int max_n = 0;
+    bool converged = true;
+    double last_rel_residual = 0.0;
+    int last_iters = 0;
+
+    void allocate(int n);
+    double dot(int n, const double* d_x, const double* d_y);
+
+    // Preconditioned CG on CSR val (+ extra_op) with precond (AMG apply etc.), else Jacobi.
+    int solve_pcg(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
+                  const double* d_b, double* d_x, int max_iter = 400, double tol = 1.0e-7,
+                  const std::function<void(const double*, double*)>* extra_op = nullptr,
+                  const std::function<void(const double*, double*)>* precond = nullptr);
+    // The same solve with the OSS operator and the AMG V-cycle fused into the iteration.
+    int solve_pcg_fused(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
+                        const double* d_b, double* d_x, int max_iter, double tol,
+                        const cfa_oss_op_t* oss, AmgPreconditioner* amg);
+    int solve_bicgstab(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
+                       const double* d_b, double* d_x, int max_iter = 200, double tol = 1.0e-7);
+private:
+    KrylovWorkspace ws_;
+    DevArray<double> dot_ws_;
+};
+
+class MulticolourIlu0 {
+public:
+    int n = 0, nnz = 0, n_colours = 0, nl = 0, nu = 0;
+    double colour_ms = 0.0;
+    std::vector<int> colour_start;
+    DevArray<int> d_rows, d_colour, d_diag_pos;
+    DevArray<double> d_lu;
+    const int* d_row_ptr = nullptr;
+    const int* d_col_ind = nullptr;
+    DevArray<int> d_lp, d_up, d_lc, d_uc, d_l_src, d_u_src, d_d_src;
+    DevArray<double> d_lv, d_uv, d_dinv;
+    DevArray<float> d_lv32, d_uv32, d_dinv32;   // FP32 factors (mixed precision)
+    bool keep_fp32 = false;
+
+    MulticolourIlu0() = default;
+    MulticolourIlu0(const MulticolourIlu0&) = delete;
+    MulticolourIlu0& operator=(const MulticolourIlu0&) = delete;
+
+    void set_pattern(int n, const std::vector<int>& rp, const std::vector<int>& ci, const int* d_rp,
+                     const int* d_ci);
+    void factor(const double* d_val);
+    void apply(const double* r, double* z, int nrhs = 1) const;
+    const cfa_ilu_t* view() const { return &view_; }
+private:
+    cfa_ilu_t view_{};
+    void refresh_view();
+};
+
+class GpuBicgstab2 {
+public:
+    int iters[2] = {0, 0};
+    bool converged[2] = {true, true};
+    double last_rel_residual[2] = {0.0, 0.0};
+    bool precond_fp32 = false;   // apply the FP32 copy of the ILU factors
+
+    GpuBicgstab2() = default;
+    GpuBicgstab2(const GpuBicgstab2&) = delete;
+    GpuBicgstab2& operator=(const GpuBicgstab2&) = delete;
+
+    void solve(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
+               const double* b_u, const double* b_v, double* x_u, double* x_v, int max_iter,
+               double tol, const MulticolourIlu0& ilu);
+private:
+    KrylovWorkspace ws_;
+};
+
+// Host CSR matrix used by the AMG set-up (src/cfa/linalg.cuh:791-795)
+struct HostCsr {
+    int rows = 0, cols = 0;
+    std::vector<int> rp, ci;
+    std::vector<double> v;
+};
+HostCsr csr_multiply(const HostCsr& A, const HostCsr& B);
+HostCsr csr_transpose(const HostCsr& A);
+
+class AmgPreconditioner {
+public:
+    struct DevCsr {
+        int rows = 0, nnz = 0, tpr = 1;
+        DevArray<int> rp, ci;
+        DevArray<double> v;
+    };
+    struct Level {
+        DevCsr A, P, R;
+        DevArray<double> dinv;
+        double omega = 0.0;
+        DevArray<double> x, b, r, Ax;
+    };
+
+    int smooth_steps = 2;
+    double theta = 0.08;
+    int coarse_target = 300;
+    int max_levels = 15;
+    int small_rows = 2048;   // levels at most this size run in one workgroup
+
+    std::vector<std::unique_ptr<Level>> levels;
+    int coarse_n = 0;
+    DevArray<double> d_coarse_inv, d_cx, d_cb;
+    double setup_ms = 0.0;
+    enum { PH_DOWNLOAD, PH_RHO, PH_AGG, PH_SMOOTH, PH_TRANSPOSE, PH_GALERKIN, PH_UPLOAD, PH_COARSE, PH_REMAP, PH_COUNT };
+    HostCsr P0_host;
+    int remaps = 0;
+    double phase_ms[PH_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<int> last_aggregates;   // level-0 aggregate of each row (set-up record)
+
+    AmgPreconditioner() = default;
+    AmgPreconditioner(const AmgPreconditioner&) = delete;
+    AmgPreconditioner& operator=(const AmgPreconditioner&) = delete;
+
+    int num_levels() const { return (int)levels.size() + 1; }
+    void setup(int n, int nnz, const int* d_row_ptr, const int* d_col_ind, const double* d_val);
+    void remap(int n, int nnz, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
+               const int* h_rp, const int* h_ci, const std::vector<int>& new_to_old);
+    void apply(const double* d_r, double* d_z);
+    const cfa_amg_t* view() const { return &view_; }
+
+    // host-side set-up steps (src/cfa/linalg.cuh:1276-1431)
+    static double spectral_radius_DinvA(const HostCsr& A, const std::vector<double>& dinv);
+    HostCsr smoothed_prolongator(const HostCsr& A, const std::vector<double>& dinv, double omega,
+                                 std::vector<int>* agg_out) const;
+    static std::vector<double> dense_spd_inverse(std::vector<double> M, int n);
+private:
+    std::vector<std::unique_ptr<Level>> spare_;
+    std::vector<cfa_amg_level_t> desc_;
+    DevArray<cfa_amg_level_t> d_desc_;
+    cfa_amg_t view_{};
+    Clock::time_point phase_t_;
+    void phase_start() { phase_t_ = Clock::now(); }
+    void phase_end(int ph) {
+        auto now = Clock::now();
+        phase_ms[ph] += ms_between(phase_t_, now);
+        phase_t_ = now;
+    }
+    void release();
+    void refresh_view();
+    static void upload_csr(const HostCsr& M, DevCsr& D);
+    static void upload_csr_from_device(int rows, int nnz, const int* rp, const int* ci, const double* v, DevCsr& D);
+    static void ensure_level_vectors(Level& L, int n);
+};
+
+// lanes per row of the V-cycle kernels from the mean row length (wave64 form of
+// src/cfa/linalg.cuh:919-926)
+int csr_lanes_per_row(int rows, long long nnz);
+
+}  // namespace cfa

@@ -1,0 +1,312 @@
+/*
+ * C ABI of the MI355X (gfx950) kernels for the per-step projection solve of cylinder_fem_amr.
+ *
+ * The reference has no plugin ABI: its solver path is a set of C++ classes and free functions
+ * inside one CUDA translation unit (src/cylinder_fem_amr.cu includes the eight src/cfa headers). Each entry
+ * point below replaces one kernel group of that path; the comment on it names the reference
+ * code it stands in for. The C++ host layer (Mesh2D, AdaptiveMesh, GpuLinearSolver,
+ * MulticolourIlu0, GpuBicgstab2, AmgPreconditioner, DeviceMeshBuffers, the CLI) calls only
+ * these functions, and so can a Python or other FFI caller.
+ *
+ * Conventions
+ *   - All pointers named d_* or documented as "device" are device memory; sizes are element
+ *     counts unless they say bytes. Indices are int32, values FP64 unless a name ends in _f32.
+ *   - Every call is asynchronous on the given stream (NULL = the default stream) unless noted.
+ *   - Return value: 0 on success, a HIP error code (> 0) on a runtime failure, or a negative
+ *     CFA_E_* code for an invalid argument. cfa_error_string() describes any code.
+ *   - No entry point allocates device memory except cfa_malloc / cfa_host_malloc: scratch
+ *     comes from caller-provided workspaces.
+ */
+#ifndef CFA_HIP_H
+#define CFA_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* cfa_stream_t; /* hipStream_t */
+
+enum {
+    CFA_OK = 0,
+    CFA_E_ARG = -1,      /* invalid argument (null pointer, negative size, ...) */
+    CFA_E_ROWLEN = -2,   /* a CSR row longer than the kernel limit (ILU: 32, as linalg.cuh:361) */
+    CFA_E_NOTREADY = -3  /* object not set up */
+};
+
+/* ---------------------------------------------------------------------------------------------
+ * Runtime plumbing (replaces the CUDA runtime calls and CUDA_CHECK of src/cfa/common.cuh:33-43)
+ * ------------------------------------------------------------------------------------------- */
+int cfa_init(int device);
+const char* cfa_error_string(int code);
+int cfa_malloc(void** d_ptr, size_t bytes);
+int cfa_free(void* d_ptr);
+/* pinned host memory, mapped into the device address space (d_alias may be NULL) */
+int cfa_host_malloc(void** h_ptr, void** d_alias, size_t bytes);
+int cfa_host_free(void* h_ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+int cfa_memcpy(void* dst, const void* src, size_t bytes, int kind, cfa_stream_t stream);
+/* zero fill by a kernel (no memset node, safe inside captured graphs) */
+int cfa_zero(void* d_ptr, size_t bytes, cfa_stream_t stream);
+int cfa_stream_sync(cfa_stream_t stream);
+int cfa_device_sync(void);
+/* GPU timers for the benchmark: events recorded on `stream` */
+int cfa_timer_create(void** timer);
+int cfa_timer_destroy(void* timer);
+int cfa_timer_record(void* timer, cfa_stream_t stream);
+int cfa_timer_elapsed_ms(void* t_begin, void* t_end, float* ms);
+
+/* ---------------------------------------------------------------------------------------------
+ * Mesh description on the device. Besides the reference's arrays (elem_v, x, y, bc_tag, CSR
+ * graph; src/cfa/device.cuh:21-68) it carries the node -> incident-element table used by the
+ * atomic-free owner-computes assembly: slot s in [inc_ptr[i], inc_ptr[i+1]) of node i holds
+ * inc_elem[s] = 3 e + a (node i is vertex a of element e) and inc_pos[3 s + b] = CSR position
+ * of entry (i, v_b(e)) in row i.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct {
+    int n_nodes, n_elems, nnz;
+    const int* elem_v;   /* [3 n_elems] */
+    const double* x;     /* [n_nodes] */
+    const double* y;
+    const int* bc_tag;   /* [n_nodes], BoundaryTag values of src/cfa/common.cuh:68-74 */
+    const int* row_ptr;  /* [n_nodes + 1] */
+    const int* col_ind;  /* [nnz] */
+    const int* inc_ptr;  /* [n_nodes + 1] */
+    const int* inc_elem; /* [3 n_elems] */
+    const int* inc_pos;  /* [9 n_elems] */
+} cfa_mesh_t;
+
+/* Momentum system (src/cfa/fem_kernels.cuh:40-181 k_assemble_momentum_fem, fused with
+ * :411-453 k_apply_bc_momentum). Writes A_val, rhs_u, rhs_v and mass (no prior zeroing), then
+ * Dirichlet rows (inlet/wall/cylinder) become identity rows whose rhs and u, v entries hold the
+ * boundary value: the inflow parabola with peak u_max, zero on walls and cylinder, or the
+ * manufactured solution at mms_time when mms_time >= 0. u_n, v_n must not alias u, v.
+ * fx, fy (nodal body force) may be NULL. */
+int cfa_assemble_momentum(const cfa_mesh_t* m, const double* u_n, const double* v_n,
+                          const double* u_prev, const double* v_prev, const double* p,
+                          double nu, double rho, double a0, double a1, double a2, double c1,
+                          double c2, double tau_time, double skew, const double* fx,
+                          const double* fy, double u_max, double mms_time, double* A_val,
+                          double* rhs_u, double* rhs_v, double* mass, double* u, double* v,
+                          cfa_stream_t stream);
+
+/* Manufactured-solution body force at nodes (src/cfa/fem_kernels.cuh:348-355). */
+int cfa_mms_forcing(int n, const double* x, const double* y, double t, double* fx, double* fy,
+                    cfa_stream_t stream);
+
+/* OSS weights s_K = rho tau_K / dt (src/cfa/fem_kernels.cuh:697-719). */
+int cfa_oss_weights(const cfa_mesh_t* m, const double* u, const double* v, double dt, double rho,
+                    double nu, double stab, double* elem_s, cfa_stream_t stream);
+
+/* G(f)_i = sum_K w_K |K|/3 grad f|_K, w_K = sqrt(elem_s[K]) or 1 if elem_s is NULL
+ * (src/cfa/fem_kernels.cuh:726-757). Writes gx, gy (no prior zeroing). */
+int cfa_scalar_gradient(const cfa_mesh_t* m, const double* f, const double* elem_s, double* gx,
+                        double* gy, cfa_stream_t stream);
+
+/* Lumped P1 mass m_i = sum_K |K|/3 (src/cfa/fem_kernels.cuh:890-909). Writes mass. */
+int cfa_lumped_mass(const cfa_mesh_t* m, double* mass, cfa_stream_t stream);
+
+/* Stabilised pressure Poisson system (src/cfa/fem_kernels.cuh:201-282
+ * k_assemble_pressure_poisson, fused with the outlet Dirichlet rows and the symmetric column
+ * elimination of :460-490 k_apply_bc_pressure, with the OSS operator of :807-852
+ * k_assemble_oss_operator / k_inverse_lumped_mass when oss_val is not NULL). Writes Kp, rhs,
+ * div_num, oss_val [Bx|By|BTx|BTy] (4 nnz) and inv_m; sets dp = 0 on Dirichlet rows. gpx, gpy
+ * hold G(p^n) (cfa_scalar_gradient with elem_s). stab = 0 gives the plain PPE (elem_s, gpx,
+ * gpy unused). */
+int cfa_assemble_pressure(const cfa_mesh_t* m, const double* u_star, const double* v_star,
+                          const double* p, const double* gpx, const double* gpy,
+                          const double* mass, const double* elem_s, double dt, double rho,
+                          double stab, double* Kp, double* rhs, double* div_num, double* dp,
+                          double* oss_val, double* inv_m, cfa_stream_t stream);
+
+/* OSS operator alone (src/cfa/device.cuh:304-311 assemble_oss_projection: fem_kernels.cuh:807-852
+ * k_assemble_oss_operator and k_inverse_lumped_mass): writes oss_val [Bx|By|BTx|BTy] and inv_m
+ * from the weights elem_s and the lumped mass. */
+int cfa_assemble_oss_operator(const cfa_mesh_t* m, const double* elem_s, const double* mass,
+                              double* oss_val, double* inv_m, cfa_stream_t stream);
+
+/* out -= B^T M_L^{-1} B in on non-Dirichlet rows (src/cfa/fem_kernels.cuh:855-885,
+ * src/cfa/device.cuh:316-323). gx, gy: n-vector scratch. */
+int cfa_oss_apply(const cfa_mesh_t* m, const double* oss_val, const double* inv_m,
+                  const double* in, double* out, double* gx, double* gy, cfa_stream_t stream);
+
+/* The same operator in matrix-free element form (src/cfa/device.cuh:287-297 with
+ * src/cfa/fem_kernels.cuh:763-797), for checking the assembled form. */
+int cfa_oss_apply_matrix_free(const cfa_mesh_t* m, const double* elem_s, const double* mass,
+                              const double* in, double* out, double* gx, double* gy,
+                              cfa_stream_t stream);
+
+/* out = c K f, K the P1 stiffness matrix (src/cfa/fem_kernels.cuh:627-649, applied to a zeroed
+ * output as the driver does at src/cylinder_fem_amr.cu:467-472). */
+int cfa_stiffness_apply(const cfa_mesh_t* m, const double* f, double c, double* out,
+                        cfa_stream_t stream);
+
+/* Velocity correction and pressure update of src/cfa/fem_kernels.cuh:536-582
+ * (k_accumulate_grad_dp_and_vorticity) and :592-622 (k_velocity_pressure_correction):
+ *   u -= (dt/rho) M^-1 G dp on interior and outlet rows, p += p_update (dp - rot div_num / m),
+ *   vort = M^-1 (N, omega_h(u*)). gx, gy, vort_num receive the nodal sums. */
+int cfa_correction(const cfa_mesh_t* m, double dt, double rho, double p_update, double rot,
+                   const double* div_num, const double* mass, const double* dp, double* u,
+                   double* v, double* p, double* vort, double* gx, double* gy, double* vort_num,
+                   cfa_stream_t stream);
+
+/* Forces and probes of one step (src/cfa/fem_kernels.cuh:1046-1123 k_volume_forces,
+ * :1129-1139 k_sample_pressure_probes, :657-692 k_surface_forces). diag[6] receives
+ * [F_x, F_y volume brackets, p_front, p_back, F_x, F_y surface]; probe = {a0, b0, a1, b1}
+ * (host array) with weights w0, w1; cyl_edges = [3 n_cyl] (a, b, owning element); layer =
+ * the n_layer elements with a cylinder vertex (the only ones with a non-zero volume term;
+ * NULL = scan all elements); ws: 8 * 1024 doubles. */
+int cfa_forces(const cfa_mesh_t* m, const int* cyl_edges, int n_cyl, const int* layer,
+               int n_layer, const double* u, const double* v, const double* u_old,
+               const double* v_old, const double* u_prev, const double* v_prev, const double* p,
+               double b0, double b1, double b2, double nu, double rho, const int* probe, double w0,
+               double w1, double* diag, double* ws, cfa_stream_t stream);
+
+/* AMR indicators (src/cfa/device.cuh:265-281 with src/cfa/fem_kernels.cuh:921-1035): lumped
+ * mass, recovered disturbance gradient grad_rec [4 n], elem_error [2 ne] = [eta_v | eta_i]. */
+int cfa_error_indicator(const cfa_mesh_t* m, const double* u, const double* v, double u_max,
+                        double* mass, double* grad_rec, double* elem_error, cfa_stream_t stream);
+
+/* Squared MMS error and norm integrals (src/cfa/fem_kernels.cuh:362-405): out[6], summed in a
+ * fixed order. ws: 6 * 1024 doubles. */
+int cfa_mms_errors(const cfa_mesh_t* m, const double* u, const double* v, const double* p,
+                   double t, double* out, double* ws, cfa_stream_t stream);
+
+/* ---------------------------------------------------------------------------------------------
+ * Sparse linear algebra (src/cfa/linalg.cuh)
+ * ------------------------------------------------------------------------------------------- */
+/* y = A x (linalg.cuh:8-26); nrhs vectors of stride n (linalg.cuh:572-581 for nrhs = 2). */
+int cfa_spmv(int n, const int* rp, const int* ci, const double* val, const double* x, double* y,
+             int nrhs, cfa_stream_t stream);
+/* Synchronous dot product, the reference's GpuLinearSolver::dot (linalg.cuh:172-177).
+ * ws: 1024 doubles. */
+int cfa_dot(int n, const double* x, const double* y, double* result, double* ws,
+            cfa_stream_t stream);
+/* BLAS-1 of linalg.cuh:50-90: y += a x; y = x + b y; out = a .* b. inv_diag = 1 / a_ii, or 1
+ * where |a_ii| <= eps (eps 1e-14: linalg.cuh:28-48; eps 1e-300: the AMG rule of :871-878). */
+int cfa_axpy(int n, double a, const double* x, double* y, cfa_stream_t stream);
+int cfa_xpay(int n, const double* x, double b, double* y, cfa_stream_t stream);
+int cfa_vec_mul(int n, const double* a, const double* b, double* out, cfa_stream_t stream);
+int cfa_inv_diag(int n, const int* rp, const int* ci, const double* val, double eps,
+                 double* inv_diag, cfa_stream_t stream);
+
+/* OSS operator for the fused PCG apply. */
+typedef struct {
+    const double* oss_val; /* [4 nnz] */
+    const double* inv_m;   /* [n] */
+    const int* bc_tag;     /* [n] */
+    double* gx;            /* [n] scratch */
+    double* gy;
+} cfa_oss_op_t;
+
+/* One SA-AMG level (src/cfa/linalg.cuh:939-957 Level): operator A (rows x rows), prolongator
+ * P (rows x cols), restriction R = P^T, inverse diagonal and Jacobi weight, work vectors.
+ * tpr_*: lanes per row of the fused CSR kernels (1..64). */
+typedef struct {
+    int rows, cols;
+    int a_nnz, p_nnz, r_nnz;
+    int tpr_a, tpr_p, tpr_r;
+    const int* a_rp; const int* a_ci; const double* a_v;
+    const int* p_rp; const int* p_ci; const double* p_v;
+    const int* r_rp; const int* r_ci; const double* r_v;
+    const double* dinv;
+    double omega;
+    double* x; double* b; double* r; double* ax;
+} cfa_amg_level_t;
+
+/* SA-AMG hierarchy for the V(ss, ss) cycle (src/cfa/linalg.cuh:1161-1206).
+ * levels: host array of n_levels descriptors; d_levels: the same array in device memory
+ * (read by the single-workgroup kernel that runs the small levels); coarse_inv: dense
+ * inverse (coarse_n x coarse_n, row-major) of the coarsest operator; cb, cx: [coarse_n].
+ * Levels whose row count is at most small_rows run inside one workgroup (0 = never). */
+typedef struct {
+    int n_levels;
+    const cfa_amg_level_t* levels;
+    const cfa_amg_level_t* d_levels;
+    int coarse_n;
+    const double* coarse_inv;
+    double* cb; double* cx;
+    int smooth_steps;
+    int small_rows;
+} cfa_amg_t;
+
+/* z = M^{-1} r, one V-cycle (src/cfa/linalg.cuh:1161-1169 AmgPreconditioner::apply). */
+int cfa_amg_apply(const cfa_amg_t* amg, const double* r, double* z, cfa_stream_t stream);
+
+/* Host callback that enqueues out = op(in) (or out += op(in) for an extra operator) on the
+ * stream; the generic form of the std::function arguments of GpuLinearSolver::solve_pcg. */
+typedef void (*cfa_op_cb)(const double* in, double* out, void* ctx);
+
+/* Krylov workspace: device vectors of capacity >= n (x 2 for the two-component solver),
+ * device scalars, and a pinned status word the kernels update so the host can follow the
+ * iteration without a synchronisation per dot product. Layout sizes: vec[12] vectors of
+ * 2 * cap doubles, part: 8 * 4096 doubles, scal: 64 doubles, status: 16 ints (host, mapped). */
+typedef struct {
+    int cap;
+    double* vec[12];
+    double* part;
+    double* scal;
+    int* status_host;
+    int* status_dev;
+} cfa_krylov_ws_t;
+
+typedef struct {
+    int iters[2];
+    int converged[2];
+    double rel_residual[2];
+} cfa_solve_result_t;
+
+/* Preconditioned CG of GpuLinearSolver::solve_pcg (src/cfa/linalg.cuh:183-256): warm start
+ * from x, stop when ||r|| < tol ||b||, x = 0 if ||b|| < 1e-14, breakdown if |p.Ap| < 1e-30.
+ * Operator: CSR val (+ oss or extra_cb); preconditioner: amg, else prec_cb, else Jacobi.
+ * Scalars stay on the device; the host only follows the status word. */
+int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double* b, double* x,
+            int max_iter, double tol, const cfa_oss_op_t* oss, cfa_op_cb extra_cb,
+            void* extra_ctx, const cfa_amg_t* amg, cfa_op_cb prec_cb, void* prec_ctx,
+            cfa_krylov_ws_t* ws, cfa_solve_result_t* result, cfa_stream_t stream);
+
+/* Multicolour ILU(0) (src/cfa/linalg.cuh:361-568). colour_start is a host array. */
+typedef struct {
+    int n, nnz, n_colours, nl, nu;
+    const int* colour_start; /* host [n_colours + 1] */
+    const int* rows;         /* [n] rows in colour order */
+    const int* colour;       /* [n] */
+    const int* diag_pos;     /* [n] CSR position of the diagonal of row i */
+    const int* row_ptr; const int* col_ind;
+    double* lu;              /* [nnz] factor in CSR layout */
+    const int* lp; const int* up;       /* [n + 1] split pattern, colour-row order */
+    const int* lc; const int* uc;       /* [nl], [nu] column of each split entry */
+    const int* l_src; const int* u_src; /* CSR position of each split entry */
+    const int* d_src;                   /* [n] CSR position of the diagonal, colour-row order */
+    double* lv; double* uv; double* dinv;
+    float* lv32; float* uv32; float* dinv32; /* FP32 copies (mixed precision); may be NULL */
+} cfa_ilu_t;
+
+/* Factor the values (linalg.cuh:531-541): lu = ILU(0) of val in colour order, then lv, uv,
+ * dinv (and the FP32 copies when present). */
+int cfa_ilu0_factor(const cfa_ilu_t* ilu, const double* val, cfa_stream_t stream);
+/* z = (LU)^{-1} r for nrhs vectors of stride n (linalg.cuh:544-555). */
+int cfa_ilu0_apply(const cfa_ilu_t* ilu, const double* r, double* z, int nrhs,
+                   cfa_stream_t stream);
+
+/* Two-component BiCGSTAB with the multicolour ILU(0) (GpuBicgstab2::solve,
+ * src/cfa/linalg.cuh:660-754): per-component coefficients and stopping rule
+ * ||r|| < tol ||r0|| or < 1e-11, breakdown guard 1e-30 ||r0||^2 + 1e-300; x_u, x_v are the
+ * initial guesses and receive the solutions. use_f32: apply the FP32 copy of the factors. */
+int cfa_bicgstab2(int n, const int* rp, const int* ci, const double* val, const double* b_u,
+                  const double* b_v, double* x_u, double* x_v, int max_iter, double tol,
+                  const cfa_ilu_t* ilu, int use_f32, cfa_krylov_ws_t* ws,
+                  cfa_solve_result_t* result, cfa_stream_t stream);
+
+/* Jacobi-preconditioned BiCGSTAB for one right-hand side (GpuLinearSolver::solve_bicgstab,
+ * src/cfa/linalg.cuh:261-347). */
+int cfa_bicgstab_jacobi(int n, const int* rp, const int* ci, const double* val, const double* b,
+                        double* x, int max_iter, double tol, cfa_krylov_ws_t* ws,
+                        cfa_solve_result_t* result, cfa_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFA_HIP_H */

@@ -1,0 +1,41 @@
+"""Copies the reference's stored run outputs used as golden data into tests/golden/.
+
+The files are data written by the reference's own runs (forces histories, MMS error tables);
+they are read here from a checkout of the reference and committed as fixtures, so the tests never
+need the reference itself. Usage: python tests/golden/make_golden.py /path/to/reference
+"""
+import os
+import shutil
+import sys
+
+RUNS = {
+    # MMS convergence runs (src/cylinder_fem_amr.cu --mms), runs/paper/mms_L*/cmd.txt
+    "mms_L1": ["cylinder_fem_amr_forces.csv", "cylinder_fem_amr_mms.csv", "cmd.txt"],
+    "mms_L2": ["cylinder_fem_amr_forces.csv", "cylinder_fem_amr_mms.csv", "cmd.txt"],
+    "mms_L3": ["cylinder_fem_amr_forces.csv", "cylinder_fem_amr_mms.csv", "cmd.txt"],
+    "mms_L4": ["cylinder_fem_amr_mms.csv", "cmd.txt"],
+    "mms_L5": ["cylinder_fem_amr_mms.csv", "cmd.txt"],
+}
+# adaptive DFG 2D-2 run (levels 2-5, dt 5e-4): only its first 2000 steps are kept
+ADAPTIVE = ("d2_A2_dt05", 2000)
+
+
+def main(ref):
+    here = os.path.dirname(os.path.abspath(__file__))
+    for run, files in RUNS.items():
+        for f in files:
+            src = os.path.join(ref, "runs", "paper", run, f)
+            dst = os.path.join(here, f"{run}_{f.replace('cylinder_fem_amr_', '')}")
+            shutil.copyfile(src, dst)
+    run, rows = ADAPTIVE
+    src = os.path.join(ref, "runs", "paper", run, "cylinder_fem_amr_forces.csv")
+    with open(src) as fi, open(os.path.join(here, f"{run}_forces_first{rows}.csv"), "w") as fo:
+        for k, line in enumerate(fi):
+            if k > rows:
+                break
+            fo.write(line)
+    shutil.copyfile(os.path.join(ref, "runs", "paper", run, "cmd.txt"), os.path.join(here, f"{run}_cmd.txt"))
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

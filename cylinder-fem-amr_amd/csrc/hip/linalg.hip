@@ -5,13 +5,17 @@
 // Krylov design: the reference copies every dot product to the host (three blocking transfers
 // per CG iteration, linalg.cuh:172-177). Here all scalars stay on the device: a one-workgroup
 // "final" kernel reduces the block partials in a fixed order and applies the scalar logic of the
-// reference (coefficients, breakdown guards, stopping rules); every vector kernel of the
-// iteration reads a device state word and returns at once when the solve has stopped. The
-// host enqueues iterations ahead and follows a progress word that the final kernels write to
-// mapped pinned memory, so the queue never drains between iterations.
+// reference (coefficients, breakdown guards, stopping rules), and the iteration counter lives on
+// the device too. Every vector kernel reads a device state word and returns at once when the
+// solve has stopped. One iteration is therefore the same kernel sequence every time: it is
+// captured once as a HIP graph and replayed (about 1.7 us per kernel node on MI355X against
+// 2.5 us per stream launch), with the host a few iterations ahead, following a progress word
+// that the final kernels write to mapped pinned memory, so the queue never drains.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <thread>
+#include <vector>
 
 #include "cfa_device.hpp"
 
@@ -21,9 +25,11 @@ namespace {
 
 constexpr int RED_BLOCKS = 1024;   // partial sums per reduction (grid-stride kernels)
 constexpr int OP_BLOCKS = 4096;    // grid cap of the fused operator kernels (partials <= workspace)
+constexpr int LOOKAHEAD = 3;       // iterations enqueued ahead of the last one known to have finished
+constexpr int GRAPH_CACHE = 4;     // captured iteration graphs kept per solver kind
 
 // Solve state (device ints after the scalar area of the workspace)
-enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_STOPPED = 2 };
+enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_STOPPED = 2, ST_TRIVIAL = 3 };
 
 __device__ __forceinline__ bool halted(const int* state) { return state && state[0] != ST_RUNNING; }
 
@@ -97,77 +103,6 @@ k_dense_matvec(int n, const double* __restrict__ A, const double* __restrict__ x
         for (int j = lane; j < n; j += WAVE) s += A[(size_t)i * n + j] * x[j];
     s = wave_sum(s);
     if (i < n && lane == 0) y[i] = s;
-}
-
-// The levels from `first` down to the coarse solve and back, inside one workgroup: every phase
-// is a pass over the rows of one level, separated by workgroup barriers. Used for the small
-// levels, where a kernel launch per phase would cost more than the work.
-__device__ __forceinline__ double row_product(const int* __restrict__ rp, const int* __restrict__ ci,
-                                              const double* __restrict__ v, const double* __restrict__ x, int row) {
-    double s = 0.0;
-    for (int k = rp[row]; k < rp[row + 1]; ++k) s += v[k] * x[ci[k]];
-    return s;
-}
-
-__global__ void __launch_bounds__(1024)
-k_vcycle_small(const cfa_amg_level_t* __restrict__ lv, int first, int n_levels, int smooth, int coarse_n,
-               const double* __restrict__ cinv, double* __restrict__ cb, double* __restrict__ cx,
-               const int* __restrict__ state) {
-    if (halted(state)) return;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    // downward sweep
-    for (int l = first; l < n_levels; ++l) {
-        const cfa_amg_level_t L = lv[l];
-        const double* b = L.b;
-        double* xa = L.x;
-        double* xb = L.ax;
-        for (int i = tid; i < L.rows; i += nt) xa[i] = L.omega * L.dinv[i] * b[i];
-        __syncthreads();
-        for (int s = 1; s < smooth; ++s) {
-            for (int i = tid; i < L.rows; i += nt)
-                xb[i] = xa[i] + L.omega * L.dinv[i] * (b[i] - row_product(L.a_rp, L.a_ci, L.a_v, xa, i));
-            __syncthreads();
-            double* t = xa; xa = xb; xb = t;
-        }
-        for (int i = tid; i < L.rows; i += nt) L.r[i] = b[i] - row_product(L.a_rp, L.a_ci, L.a_v, xa, i);
-        __syncthreads();
-        double* next_b = (l + 1 < n_levels) ? lv[l + 1].b : cb;
-        for (int i = tid; i < L.cols; i += nt) next_b[i] = row_product(L.r_rp, L.r_ci, L.r_v, L.r, i);
-        __syncthreads();
-        // keep the pre-smoothed iterate in L.x for the upward sweep
-        if (xa != L.x) {
-            for (int i = tid; i < L.rows; i += nt) L.x[i] = xa[i];
-            __syncthreads();
-        }
-    }
-    // coarse solve
-    for (int i = tid; i < coarse_n; i += nt) {
-        double s = 0.0;
-        for (int j = 0; j < coarse_n; ++j) s += cinv[(size_t)i * coarse_n + j] * cb[j];
-        cx[i] = s;
-    }
-    __syncthreads();
-    // upward sweep
-    for (int l = n_levels - 1; l >= first; --l) {
-        const cfa_amg_level_t L = lv[l];
-        const double* b = L.b;
-        const double* xc = (l + 1 < n_levels) ? lv[l + 1].x : cx;
-        double* xa = L.x;
-        double* xb = L.ax;
-        for (int i = tid; i < L.rows; i += nt) xa[i] += row_product(L.p_rp, L.p_ci, L.p_v, xc, i);
-        __syncthreads();
-        for (int s = 0; s < smooth; ++s) {
-            for (int i = tid; i < L.rows; i += nt)
-                xb[i] = xa[i] + L.omega * L.dinv[i] * (b[i] - row_product(L.a_rp, L.a_ci, L.a_v, xa, i));
-            __syncthreads();
-            double* t = xa; xa = xb; xb = t;
-        }
-        // the level's result always ends in L.x (read by the next finer level)
-        if (xa != L.x) {
-            for (int i = tid; i < L.rows; i += nt) L.x[i] = xa[i];
-            __syncthreads();
-        }
-    }
 }
 
 // ---- reductions ------------------------------------------------------------------------------------
@@ -256,15 +191,35 @@ k_spmv_multi(int n, const int* __restrict__ rp, const int* __restrict__ ci, cons
 // ======================================================================================================
 // PCG (src/cfa/linalg.cuh:183-256)
 // ======================================================================================================
-// scalar slots
 enum { S_BNORM = 0, S_RZ_OLD, S_PQ, S_ALPHA, S_RNORM, S_REL, S_BETA, S_TOL };
-// state ints: [0] state, [1] iterations
+// ints: [0] state, [1] final iteration count, [2] current iteration
+
+// ||b||; x = 0 and stop when ||b|| < 1e-14 (linalg.cuh:205-209)
+__global__ void k_pcg_init_bnorm(int nb, const double* __restrict__ part, double* __restrict__ sc,
+                                 int* __restrict__ st, int* __restrict__ status) {
+    __shared__ double lds[BLOCK / WAVE];
+    const double bb = sum_partials(part, nb, 1, lds);
+    if (threadIdx.x != 0) return;
+    const double b_norm = sqrt(bb);
+    sc[S_BNORM] = b_norm;
+    st[1] = 0;
+    st[2] = 0;
+    st[0] = b_norm < 1.0e-14 ? ST_TRIVIAL : ST_RUNNING;
+    publish(status, 0, st[0]);
+}
+
+__global__ void k_zero_if_trivial(int n, double* __restrict__ x, const int* __restrict__ st) {
+    if (st[0] != ST_TRIVIAL) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
 
 // r = b - q; partial r.r
 __global__ void __launch_bounds__(BLOCK)
 k_pcg_residual(int n, const double* __restrict__ b, const double* __restrict__ q, double* __restrict__ r,
-               double* __restrict__ part) {
+               double* __restrict__ part, const int* __restrict__ st) {
     __shared__ double lds[BLOCK / WAVE];
+    if (halted(st)) return;
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double ri = b[i] + (-1.0) * q[i];
@@ -275,9 +230,24 @@ k_pcg_residual(int n, const double* __restrict__ b, const double* __restrict__ q
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
+// initial residual test (linalg.cuh:217-218): converged with 0 iterations
+__global__ void k_pcg_init_rnorm(int nb, const double* __restrict__ part, double* __restrict__ sc,
+                                 int* __restrict__ st, int* __restrict__ status) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (st[0] != ST_RUNNING) return;
+    const double rr = sum_partials(part, nb, 1, lds);
+    if (threadIdx.x != 0) return;
+    const double r_norm = sqrt(rr);
+    if (r_norm < sc[S_TOL] * sc[S_BNORM]) {
+        st[0] = ST_CONVERGED;
+        st[1] = 0;
+        publish(status, 0, ST_CONVERGED);
+    }
+}
+
 // alpha = rz_old / (p.q); breakdown when |p.q| < 1e-30
 __global__ void k_pcg_final_alpha(int nb, const double* __restrict__ part, double* __restrict__ sc,
-                                  int* __restrict__ st, int* __restrict__ status, int it) {
+                                  int* __restrict__ st, int* __restrict__ status) {
     __shared__ double lds[BLOCK / WAVE];
     if (st[0] != ST_RUNNING) return;
     const double pq = sum_partials(part, nb, 1, lds);
@@ -285,8 +255,8 @@ __global__ void k_pcg_final_alpha(int nb, const double* __restrict__ part, doubl
     sc[S_PQ] = pq;
     if (fabs(pq) < 1.0e-30) {
         st[0] = ST_STOPPED;
-        st[1] = it;
-        publish(status, it, ST_STOPPED);
+        st[1] = st[2];
+        publish(status, st[2], ST_STOPPED);
         return;
     }
     sc[S_ALPHA] = sc[S_RZ_OLD] / pq;
@@ -312,7 +282,7 @@ k_pcg_update_xr(int n, const double* __restrict__ sc, const double* __restrict__
 
 // ||r|| < tol ||b|| -> converged after it + 1 iterations
 __global__ void k_pcg_final_rnorm(int nb, const double* __restrict__ part, double* __restrict__ sc,
-                                  int* __restrict__ st, int* __restrict__ status, int it) {
+                                  int* __restrict__ st, int* __restrict__ status) {
     __shared__ double lds[BLOCK / WAVE];
     if (st[0] != ST_RUNNING) return;
     const double rr = sum_partials(part, nb, 1, lds);
@@ -322,21 +292,24 @@ __global__ void k_pcg_final_rnorm(int nb, const double* __restrict__ part, doubl
     sc[S_REL] = r_norm / sc[S_BNORM];
     if (r_norm < sc[S_TOL] * sc[S_BNORM]) {
         st[0] = ST_CONVERGED;
-        st[1] = it + 1;
-        publish(status, it + 1, ST_CONVERGED);
+        st[1] = st[2] + 1;
+        publish(status, st[2] + 1, ST_CONVERGED);
     }
 }
 
-// beta = rz_new / (rz_old + 1e-30); rz_old = rz_new (first == 1: only rz_old, the prologue)
+// beta = rz_new / (rz_old + 1e-30); rz_old = rz_new; next iteration (first: prologue, rz only)
 __global__ void k_pcg_final_beta(int nb, const double* __restrict__ part, double* __restrict__ sc,
-                                 const int* __restrict__ st, int* __restrict__ status, int it, int first) {
+                                 int* __restrict__ st, int* __restrict__ status, int first) {
     __shared__ double lds[BLOCK / WAVE];
     if (st[0] != ST_RUNNING) return;
     const double rz = sum_partials(part, nb, 1, lds);
     if (threadIdx.x != 0) return;
-    if (!first) sc[S_BETA] = rz / (sc[S_RZ_OLD] + 1.0e-30);
+    if (!first) {
+        sc[S_BETA] = rz / (sc[S_RZ_OLD] + 1.0e-30);
+        st[2] += 1;
+    }
     sc[S_RZ_OLD] = rz;
-    publish(status, it + 1, ST_RUNNING);
+    publish(status, st[2], ST_RUNNING);
 }
 
 // p = z + beta p
@@ -416,31 +389,21 @@ k_apply_BT_dot(int n, int nnz, const int* __restrict__ rp, const int* __restrict
 // ======================================================================================================
 // multicolour ILU(0) (src/cfa/linalg.cuh:361-568)
 // ======================================================================================================
-// Rows of colour c (c >= 1): l_ik = a_ik / u_kk for the earlier-colour entries k in colour order,
-// a_ij -= l_ik u_kj on the later-colour entries of the pattern; updated in place in lu (own row).
-__global__ void k_ilu_factor_colour(int m, const int* __restrict__ rows, const int* __restrict__ rp,
-                                    const int* __restrict__ ci, const int* __restrict__ colour,
-                                    const int* __restrict__ diag_pos, double* __restrict__ lu) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m) return;
-    const int i = rows[t];
-    const int s = rp[i], len = rp[i + 1] - s;
-    const int ci_i = colour[i];
-    for (int c = 0; c < ci_i; ++c)
-        for (int q = 0; q < len; ++q) {
-            const int k = ci[s + q];
-            if (colour[k] != c) continue;
-            const double lik = lu[s + q] / lu[diag_pos[k]];
-            lu[s + q] = lik;
-            for (int mm = rp[k]; mm < rp[k + 1]; ++mm) {
-                const int j = ci[mm];
-                for (int b = 0; b < len; ++b)
-                    if (ci[s + b] == j) {
-                        if (colour[j] > c) lu[s + b] -= lik * lu[mm];
-                        break;
-                    }
-            }
-        }
+// Rows of one colour: for each elimination step of the row's schedule, l_ik = a_ik / u_kk and
+// a_ij -= l_ik u_kj for the scheduled entries, in the schedule's order (colour by colour, CSR
+// order inside a colour, then row k in column order: the order of linalg.cuh:366-392). Rows of
+// one colour do not couple and rows k of earlier colours are final, so the update is in place.
+__global__ void k_ilu_factor_rows(int m, int t0, const int* __restrict__ el_ptr, const int* __restrict__ el,
+                                  const int* __restrict__ upd, double* __restrict__ lu) {
+#pragma clang fp contract(off)   // separate multiply and subtract: the factors match a serial CPU run bit for bit
+    const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= t0 + m) return;
+    for (int e = el_ptr[t]; e < el_ptr[t + 1]; ++e) {
+        const int q = el[4 * e], kd = el[4 * e + 1], u0 = el[4 * e + 2], u1 = el[4 * e + 3];
+        const double lik = lu[q] / lu[kd];
+        lu[q] = lik;
+        for (int u = u0; u < u1; ++u) lu[upd[2 * u]] -= lik * lu[upd[2 * u + 1]];
+    }
 }
 
 __global__ void k_ilu_gather(int nl, int nu, int n, const int* __restrict__ l_src, const int* __restrict__ u_src,
@@ -508,8 +471,8 @@ enum {
     B_RN0 = 0, B_BD = 2, B_RHO = 4, B_RHO_OLD = 6, B_ALPHA = 8, B_OMEGA = 10, B_REL = 12,
     B_CB = 14, B_CO = 16, B_CA = 18, B_A1 = 20, B_A2 = 22, B_CW = 24, B_TOL = 26
 };
-// ints: [0] state (all inactive -> stopped), [2 + c] active, [4 + c] iters, [6 + c] converged,
-// [8 + c] s_done
+// ints: [0] state (stopped when both components are inactive), [2 + c] active, [4 + c] iters,
+// [6 + c] converged, [8 + c] half-step done, [10] current iteration, [11] iteration cap
 
 // Two-component partial sums: part[b * 6 + 2 q + c]
 __device__ __forceinline__ void write_partials2(double* part, int nq, double acc[][2], double* lds) {
@@ -560,7 +523,7 @@ __device__ __forceinline__ void bicg_stop(int* st, int c, int it, int ok) {
 }
 
 __global__ void k_bicg_final_init(int nb, const double* __restrict__ part, double* __restrict__ sc,
-                                  int* __restrict__ st, int* __restrict__ status) {
+                                  int* __restrict__ st, int* __restrict__ status, int max_iter) {
     __shared__ double lds[BLOCK / WAVE];
     double rr[2];
     for (int c = 0; c < 2; ++c) rr[c] = sum_partials(part + c, nb, 6, lds);
@@ -578,13 +541,16 @@ __global__ void k_bicg_final_init(int nb, const double* __restrict__ part, doubl
         st[6 + c] = 1;
         st[8 + c] = 0;
     }
-    st[0] = (st[2] || st[3]) ? ST_RUNNING : ST_STOPPED;
+    st[10] = 0;
+    st[11] = max_iter;
+    st[0] = (st[2] || st[3]) && max_iter > 0 ? ST_RUNNING : ST_STOPPED;
     publish(status, 0, st[0]);
 }
 
-// Start of iteration it: breakdown check on rho, coefficients of the p update
-__global__ void k_bicg_pre(double* __restrict__ sc, int* __restrict__ st, int* __restrict__ status, int it) {
+// Start of an iteration: breakdown check on rho, coefficients of the p update
+__global__ void k_bicg_pre(double* __restrict__ sc, int* __restrict__ st, int* __restrict__ status) {
     if (st[0] != ST_RUNNING) return;
+    const int it = st[10];
     for (int c = 0; c < 2; ++c) {
         sc[B_CB + c] = 0.0;
         sc[B_CO + c] = 0.0;
@@ -597,17 +563,17 @@ __global__ void k_bicg_pre(double* __restrict__ sc, int* __restrict__ st, int* _
     if (!(st[2] || st[3])) { st[0] = ST_STOPPED; publish(status, it, ST_STOPPED); }
 }
 
-// p = r + cb (p - co v)
+// p = r + cb (p - co v), from the second iteration on
 __global__ void k_bicg_p(int n, const double* __restrict__ r, const double* __restrict__ v,
                          const double* __restrict__ sc, double* __restrict__ p, const int* __restrict__ st) {
-    if (halted(st)) return;
+    if (halted(st) || st[10] == 0) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 2 * n) return;
     const int c = i >= n;
     p[i] = r[i] + sc[B_CB + c] * (p[i] - sc[B_CO + c] * v[i]);
 }
 
-// y = A x for both components; with dots: partial y.w0 (nq = 1) or y.y, y.w0 (nq = 2)
+// y = A x for both components; with dots: partial y.w0 (mode 1) or y.y, y.w0 (mode 2)
 template <int TPR>
 __global__ void __launch_bounds__(BLOCK)
 k_spmv2_dots(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ val,
@@ -647,12 +613,13 @@ k_spmv2_dots(int n, const int* __restrict__ rp, const int* __restrict__ ci, cons
 
 // alpha = rho / (r0 . v), breakdown guard
 __global__ void k_bicg_final_alpha(int nb, const double* __restrict__ part, double* __restrict__ sc,
-                                   int* __restrict__ st, int* __restrict__ status, int it) {
+                                   int* __restrict__ st, int* __restrict__ status) {
     __shared__ double lds[BLOCK / WAVE];
     if (st[0] != ST_RUNNING) return;
     double o[2];
     for (int c = 0; c < 2; ++c) o[c] = sum_partials(part + c, nb, 6, lds);
     if (threadIdx.x != 0) return;
+    const int it = st[10];
     for (int c = 0; c < 2; ++c) {
         sc[B_CA + c] = 0.0;
         if (st[2 + c] && fabs(o[c]) < sc[B_BD + c]) bicg_stop(st, c, it, 0);
@@ -727,9 +694,9 @@ k_bicg_xr(int n, const double* __restrict__ ph, const double* __restrict__ sh, c
     write_partials2(part, 2, acc, lds);
 }
 
-// end of iteration it: half-step stops, residual test, rho update, iteration cap
+// end of an iteration: half-step stops, residual test, rho update, iteration cap
 __global__ void k_bicg_final_post(int nb, const double* __restrict__ part, double* __restrict__ sc,
-                                  int* __restrict__ st, int* __restrict__ status, int it, int max_iter) {
+                                  int* __restrict__ st, int* __restrict__ status) {
     __shared__ double lds[BLOCK / WAVE];
     if (st[0] != ST_RUNNING) return;
     double rr[2], r0r[2];
@@ -738,27 +705,31 @@ __global__ void k_bicg_final_post(int nb, const double* __restrict__ part, doubl
         r0r[c] = sum_partials(part + 2 + c, nb, 6, lds);
     }
     if (threadIdx.x != 0) return;
+    const int it = st[10], max_iter = st[11];
     for (int c = 0; c < 2; ++c)
         if (st[8 + c]) bicg_stop(st, c, it + 1, 1);
-    if (st[2] || st[3]) {
-        for (int c = 0; c < 2; ++c) {
-            if (!st[2 + c]) continue;
-            const double r_norm = sqrt(rr[c]);
-            sc[B_REL + c] = r_norm / sc[B_RN0 + c];
-            if (r_norm < sc[B_TOL] * sc[B_RN0 + c] || r_norm < 1.0e-11) { bicg_stop(st, c, it + 1, 1); continue; }
-            sc[B_RHO_OLD + c] = sc[B_RHO + c];
-            sc[B_RHO + c] = r0r[c];
-            if (it + 1 == max_iter) bicg_stop(st, c, max_iter, 0);
-        }
+    for (int c = 0; c < 2; ++c) {
+        if (!st[2 + c]) continue;
+        const double r_norm = sqrt(rr[c]);
+        sc[B_REL + c] = r_norm / sc[B_RN0 + c];
+        if (r_norm < sc[B_TOL] * sc[B_RN0 + c] || r_norm < 1.0e-11) { bicg_stop(st, c, it + 1, 1); continue; }
+        sc[B_RHO_OLD + c] = sc[B_RHO + c];
+        sc[B_RHO + c] = r0r[c];
+        if (it + 1 == max_iter) bicg_stop(st, c, max_iter, 0);
     }
+    st[10] = it + 1;
     const int state = (st[2] || st[3]) ? ST_RUNNING : ST_STOPPED;
     st[0] = state;
     publish(status, it + 1, state);
 }
 
 // ======================================================================================================
-// Host helpers
+// Host side
 // ======================================================================================================
+hipStream_t S(cfa_stream_t s) { return as_stream(s); }
+
+#define CFA_TRY(expr) do { const hipError_t e_ = (expr); if (e_ != hipSuccess) return (int)e_; } while (0)
+
 template <int OP>
 void csr_op_launch(int tpr, int rows, const int* rp, const int* ci, const double* val, const double* x,
                    const double* b, const double* dinv, double omega, double* y, const int* st, hipStream_t s) {
@@ -788,87 +759,57 @@ void jacobi_from_zero_launch(int tpr, int rows, const int* rp, const int* ci, co
     }
 }
 
-// One V-cycle for level l with right-hand side b; the result of level 0 goes to `out` (if given).
-// The descriptors' x/ax buffers are used as a ping-pong pair; `xl` returns where the level's
-// iterate ends up.
-struct VcycleRun {
-    const cfa_amg_t* amg;
-    const int* st;
-    hipStream_t s;
-    std::vector<cfa_amg_level_t> lv;   // host copy with the current buffer roles
-
-    // returns the buffer holding level l's result
-    double* level(int l, const double* b, double* out) {
-        cfa_amg_level_t& L = lv[l];
-        const int n = L.rows;
-        const int ss = amg->smooth_steps;
-        double* xa = L.x;
-        double* xb = L.ax;
-        if (ss >= 2) {
-            jacobi_from_zero_launch(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, b, L.dinv, L.omega, xa, st, s);
-            for (int k = 2; k < ss; ++k) {
-                csr_op_launch<OP_JACOBI>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, xb, st, s);
-                std::swap(xa, xb);
-            }
-        } else {
-            hipLaunchKernelGGL(k_jacobi_init, dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, L.omega, L.dinv, b, xa, st);
-        }
-        csr_op_launch<OP_RESID>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, nullptr, 0.0, L.r, st, s);
-        const bool deeper = l + 1 < (int)lv.size();
-        double* cb = deeper ? lv[l + 1].b : amg->cb;
-        csr_op_launch<OP_SPMV>(L.tpr_r, L.cols, L.r_rp, L.r_ci, L.r_v, L.r, nullptr, nullptr, 0.0, cb, st, s);
-        const double* cx;
-        if (deeper && amg->small_rows > 0 && lv[l + 1].rows <= amg->small_rows && amg->d_levels) {
-            // all coarser levels in one workgroup; level l + 1's result ends in its x buffer
-            hipLaunchKernelGGL(k_vcycle_small, dim3(1), dim3(1024), 0, s, amg->d_levels, l + 1, (int)lv.size(),
-                               ss, amg->coarse_n, amg->coarse_inv, amg->cb, amg->cx, st);
-            cx = lv[l + 1].x;
-        } else if (deeper) {
-            cx = level(l + 1, lv[l + 1].b, nullptr);
-        } else {
-            hipLaunchKernelGGL(k_dense_matvec, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
-                               amg->coarse_n, amg->coarse_inv, amg->cb, amg->cx, st);
-            cx = amg->cx;
-        }
-        csr_op_launch<OP_ADD>(L.tpr_p, n, L.p_rp, L.p_ci, L.p_v, cx, nullptr, nullptr, 0.0, xa, st, s);
-        for (int k = 0; k < ss; ++k) {
-            double* dst = (l == 0 && out && k == ss - 1) ? out : xb;
-            csr_op_launch<OP_JACOBI>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, dst, st, s);
-            if (dst == out) { xa = out; break; }
+// One V-cycle (src/cfa/linalg.cuh:1184-1206) for level l with right-hand side b. Each level's
+// x / ax buffers form a ping-pong pair; the function returns the buffer holding the level's
+// iterate. On level 0 the last smoothing step writes straight into `out` when given.
+double* vcycle(const cfa_amg_t* amg, int l, const double* b, double* out, const int* st, hipStream_t s) {
+    const cfa_amg_level_t& L = amg->levels[l];
+    const int n = L.rows;
+    const int ss = amg->smooth_steps;
+    double* xa = L.x;
+    double* xb = L.ax;
+    if (ss >= 2) {
+        jacobi_from_zero_launch(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, b, L.dinv, L.omega, xa, st, s);
+        for (int k = 2; k < ss; ++k) {
+            csr_op_launch<OP_JACOBI>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, xb, st, s);
             std::swap(xa, xb);
         }
-        return xa;
+    } else {   // x = omega dinv b is always taken, also for fewer than one smoothing step
+        hipLaunchKernelGGL(k_jacobi_init, dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, L.omega, L.dinv, b, xa, st);
     }
-};
-
-hipStream_t S(cfa_stream_t s) { return as_stream(s); }
-
-#define CFA_TRY(expr) do { const hipError_t e_ = (expr); if (e_ != hipSuccess) return (int)e_; } while (0)
-
-// Host wait on the progress word: returns when progress >= target, the solve stopped, or the
-// stream is idle.
-void wait_progress(const cfa_krylov_ws_t* ws, int target, hipStream_t s) {
-    volatile int* st = ws->status_host;
-    for (int spin = 0;; ++spin) {
-        if (st[0] >= target || st[1] != ST_RUNNING) return;
-        if ((spin & 63) == 63) {
-            if (hipStreamQuery(s) == hipSuccess) return;
-            std::this_thread::yield();
-        }
+    csr_op_launch<OP_RESID>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, nullptr, 0.0, L.r, st, s);
+    const bool deeper = l + 1 < amg->n_levels;
+    double* cb = deeper ? amg->levels[l + 1].b : amg->cb;
+    csr_op_launch<OP_SPMV>(L.tpr_r, L.cols, L.r_rp, L.r_ci, L.r_v, L.r, nullptr, nullptr, 0.0, cb, st, s);
+    const double* cx;
+    if (deeper) {
+        cx = vcycle(amg, l + 1, amg->levels[l + 1].b, nullptr, st, s);
+    } else {
+        hipLaunchKernelGGL(k_dense_matvec, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
+                           amg->coarse_n, amg->coarse_inv, amg->cb, amg->cx, st);
+        cx = amg->cx;
     }
+    csr_op_launch<OP_ADD>(L.tpr_p, n, L.p_rp, L.p_ci, L.p_v, cx, nullptr, nullptr, 0.0, xa, st, s);
+    for (int k = 0; k < ss; ++k) {
+        double* dst = (out && k == ss - 1) ? out : xb;
+        csr_op_launch<OP_JACOBI>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, dst, st, s);
+        if (dst == out) return out;
+        std::swap(xa, xb);
+    }
+    return xa;
 }
 
-struct KrylovState {
-    double* sc;
-    int* st;
-};
-KrylovState kstate(cfa_krylov_ws_t* ws) { return {ws->scal, reinterpret_cast<int*>(ws->scal + 40)}; }
+void amg_apply_enqueue(const cfa_amg_t* amg, const double* r, double* z, const int* st, hipStream_t s) {
+    if (amg->n_levels == 0) {
+        hipLaunchKernelGGL(k_dense_matvec, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
+                           amg->coarse_n, amg->coarse_inv, r, z, st);
+        return;
+    }
+    double* res = vcycle(amg, 0, r, z, st, s);
+    if (res != z) hipLaunchKernelGGL(k_copy, dim3(blocks_for(amg->levels[0].rows)), dim3(BLOCK), 0, s,
+                                     amg->levels[0].rows, res, z, st);
+}
 
-constexpr int LOOKAHEAD = 3;   // iterations enqueued ahead of the last one known to have finished
-
-}  // namespace
-
-namespace {
 template <int NRHS, typename T>
 void ilu_apply_impl(const cfa_ilu_t* ilu, const T* lv, const T* uv, const T* dinv, const double* r, double* z,
                     const int* st, hipStream_t s) {
@@ -897,9 +838,125 @@ void ilu_apply_any(const cfa_ilu_t* ilu, int use_f32, const double* r, double* z
         else ilu_apply_impl<1, double>(ilu, ilu->lv, ilu->uv, ilu->dinv, r, z, st, s);
     }
 }
+
+// Host wait on the progress word: returns when progress >= target, the solve stopped, or the
+// stream is idle.
+void wait_progress(const cfa_krylov_ws_t* ws, int target, hipStream_t s) {
+    volatile int* st = ws->status_host;
+    for (int spin = 0;; ++spin) {
+        if (st[0] >= target || st[1] != ST_RUNNING) return;
+        if ((spin & 63) == 63) {
+            if (hipStreamQuery(s) == hipSuccess) return;
+            std::this_thread::yield();
+        }
+    }
+}
+
+int* state_words(cfa_krylov_ws_t* ws) { return reinterpret_cast<int*>(ws->scal + 40); }
+
+// Captured iteration graphs, keyed by everything the captured launches depend on
+struct IterGraph {
+    std::vector<unsigned char> key;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long used = 0;
+};
+struct WsInternal {
+    hipStream_t cap = nullptr;
+    IterGraph pcg[GRAPH_CACHE], bicg[GRAPH_CACHE];
+    unsigned long long clock = 0;
+    bool capture_broken = false;
+};
+
+WsInternal* internal_of(cfa_krylov_ws_t* ws) {
+    if (!ws->internal) ws->internal = new WsInternal();
+    return static_cast<WsInternal*>(ws->internal);
+}
+
+struct KeyBuilder {
+    std::vector<unsigned char> k;
+    template <typename T>
+    void add(const T& v) {
+        const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+        k.insert(k.end(), p, p + sizeof(T));
+    }
+    void add_bytes(const void* p, size_t n) {
+        const unsigned char* q = static_cast<const unsigned char*>(p);
+        k.insert(k.end(), q, q + n);
+    }
+};
+
+// The cached graph for `key` (least recently used slot replaced), captured from `enqueue`;
+// nullptr when capture is unavailable (the caller then launches directly).
+template <typename F>
+hipGraphExec_t iteration_graph(cfa_krylov_ws_t* ws, IterGraph* cache, const std::vector<unsigned char>& key,
+                               F enqueue) {
+    WsInternal* in = internal_of(ws);
+    if (!ws->use_graphs || in->capture_broken) return nullptr;
+    IterGraph* slot = &cache[0];
+    for (int k = 0; k < GRAPH_CACHE; ++k) {
+        if (cache[k].exec && cache[k].key == key) {
+            cache[k].used = ++in->clock;
+            return cache[k].exec;
+        }
+        if (cache[k].used < slot->used) slot = &cache[k];
+    }
+    if (slot->exec) { (void)hipGraphExecDestroy(slot->exec); slot->exec = nullptr; }
+    if (!in->cap && hipStreamCreateWithFlags(&in->cap, hipStreamNonBlocking) != hipSuccess) {
+        in->cap = nullptr;
+        in->capture_broken = true;
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    bool ok = hipStreamBeginCapture(in->cap, hipStreamCaptureModeRelaxed) == hipSuccess;
+    if (ok) {
+        enqueue(in->cap);
+        ok = hipStreamEndCapture(in->cap, &graph) == hipSuccess && graph;
+    }
+    if (ok) ok = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    if (!ok) {
+        slot->exec = nullptr;
+        in->capture_broken = true;
+        return nullptr;
+    }
+    slot->key = key;
+    slot->used = ++in->clock;
+    return slot->exec;
+}
+
+// The iteration loop shared by both solvers: enqueue iterations (graph replay or direct
+// launches), staying LOOKAHEAD iterations ahead of the progress the device has published.
+template <typename F>
+int run_iterations(cfa_krylov_ws_t* ws, hipGraphExec_t graph, int max_iter, F enqueue, hipStream_t s) {
+    for (int it = 0; it < max_iter; ++it) {
+        if (it >= LOOKAHEAD) {
+            wait_progress(ws, it - LOOKAHEAD + 1, s);
+            if (ws->status_host[1] != ST_RUNNING) break;
+        }
+        if (graph) CFA_TRY(hipGraphLaunch(graph, s));
+        else enqueue(s);
+    }
+    return launch_status();
+}
+
 }  // namespace
 
 extern "C" {
+
+int cfa_krylov_ws_release(cfa_krylov_ws_t* ws) {
+    if (!ws || !ws->internal) return CFA_OK;
+    WsInternal* in = static_cast<WsInternal*>(ws->internal);
+    for (int k = 0; k < GRAPH_CACHE; ++k) {
+        if (in->pcg[k].exec) (void)hipGraphExecDestroy(in->pcg[k].exec);
+        if (in->bicg[k].exec) (void)hipGraphExecDestroy(in->bicg[k].exec);
+    }
+    if (in->cap) (void)hipStreamDestroy(in->cap);
+    delete in;
+    ws->internal = nullptr;
+    return CFA_OK;
+}
 
 int cfa_spmv(int n, const int* rp, const int* ci, const double* val, const double* x, double* y, int nrhs,
              cfa_stream_t stream) {
@@ -919,12 +976,11 @@ int cfa_dot(int n, const double* x, const double* y, double* result, double* ws,
     const int nb = red_blocks(n);
     hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, S(stream), n, x, y, ws, nullptr);
     hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(BLOCK), 0, S(stream), nb, ws, ws + RED_BLOCKS, 0);
-    int rc = launch_status();
+    const int rc = launch_status();
     if (rc) return rc;
-    const hipError_t e = hipMemcpyAsync(result, ws + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, S(stream));
-    if (e != hipSuccess) return (int)e;
-    const hipError_t e2 = hipStreamSynchronize(S(stream));
-    return e2 == hipSuccess ? CFA_OK : (int)e2;
+    CFA_TRY(hipMemcpyAsync(result, ws + RED_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, S(stream)));
+    CFA_TRY(hipStreamSynchronize(S(stream)));
+    return CFA_OK;
 }
 
 int cfa_axpy(int n, double a, const double* x, double* y, cfa_stream_t stream) {
@@ -958,15 +1014,9 @@ int cfa_inv_diag(int n, const int* rp, const int* ci, const double* val, double 
 
 int cfa_amg_apply(const cfa_amg_t* amg, const double* r, double* z, cfa_stream_t stream) {
     if (!amg || !r || !z) return CFA_E_ARG;
-    if (amg->n_levels < 0 || (amg->n_levels > 0 && !amg->levels)) return CFA_E_ARG;
-    hipStream_t s = S(stream);
-    if (amg->n_levels == 0) {
-        hipLaunchKernelGGL(k_dense_matvec, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
-                           amg->coarse_n, amg->coarse_inv, r, z, nullptr);
-        return launch_status();
-    }
-    VcycleRun run{amg, nullptr, s, std::vector<cfa_amg_level_t>(amg->levels, amg->levels + amg->n_levels)};
-    run.level(0, r, z);
+    if (amg->n_levels < 0 || (amg->n_levels > 0 && !amg->levels) || amg->coarse_n < 0 || amg->smooth_steps < 0)
+        return CFA_E_ARG;
+    amg_apply_enqueue(amg, r, z, nullptr, S(stream));
     return launch_status();
 }
 
@@ -974,12 +1024,15 @@ int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double
             double tol, const cfa_oss_op_t* oss, cfa_op_cb extra_cb, void* extra_ctx, const cfa_amg_t* amg,
             cfa_op_cb prec_cb, void* prec_ctx, cfa_krylov_ws_t* ws, cfa_solve_result_t* result,
             cfa_stream_t stream) {
-    if (n < 0 || !rp || !ci || !val || !b || !x || !ws || !result || ws->cap < n || !ws->status_host) return CFA_E_ARG;
+    if (n < 0 || !rp || !ci || !val || !b || !x || !ws || !result || ws->cap < n || !ws->status_host || max_iter < 0)
+        return CFA_E_ARG;
     if (oss && (!oss->oss_val || !oss->inv_m || !oss->bc_tag || !oss->gx || !oss->gy)) return CFA_E_ARG;
+    if (amg && (amg->n_levels < 0 || (amg->n_levels > 0 && !amg->levels) || amg->smooth_steps < 0))
+        return CFA_E_ARG;
     hipStream_t s = S(stream);
     result->iters[0] = 0;
     result->converged[0] = 1;
-    result->rel_residual[0] = 0.0;
+    result->rel_residual[0] = -1.0;   // no residual computed (trivial or initial convergence)
     if (n == 0) return CFA_OK;
     double* r = ws->vec[0];
     double* z = ws->vec[1];
@@ -987,143 +1040,129 @@ int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double
     double* q = ws->vec[3];
     double* inv_diag = ws->vec[4];
     double* part = ws->part;
-    KrylovState ks = kstate(ws);
+    double* sc = ws->scal;
+    int* st = state_words(ws);
+    int* status = ws->status_dev;
     const int nb = red_blocks(n);
     const int blocks = blocks_for(n);
-
-    // prologue (src/cfa/linalg.cuh:205-227): ||b||, early exits, r = b - A x, z = M^-1 r, p = z
-    double host_sc[2];
-    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, s, n, b, b, part, nullptr);
-    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(BLOCK), 0, s, nb, part, ks.sc + S_BNORM, 1);
-    CFA_TRY(hipMemcpyAsync(&host_sc[0], ks.sc + S_BNORM, sizeof(double), hipMemcpyDeviceToHost, s));
-    if (hipStreamSynchronize(s) != hipSuccess) return launch_status();
-    const double b_norm = host_sc[0];
-    if (b_norm < 1.0e-14) {
-        hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(BLOCK), 0, s, n, x);
-        return launch_status();
+    constexpr int TPR = 4;
+    const int blocks_tpr = std::min(blocks_for((long long)n * TPR), OP_BLOCKS);
+    int nnz = 0;
+    if (oss) {
+        CFA_TRY(hipMemcpyAsync(&nnz, rp + n, sizeof(int), hipMemcpyDeviceToHost, s));
+        CFA_TRY(hipStreamSynchronize(s));
     }
-    if (!amg && !prec_cb) hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(BLOCK), 0, s, n, rp, ci, val, 1.0e-14, inv_diag);
-
-    int* st = ks.st;
-    // state words: running, 0 iterations; tolerance into the scalar area
-    int init_state[2] = {ST_RUNNING, 0};
-    double tol_v = tol;
-    CFA_TRY(hipMemcpyAsync(st, init_state, sizeof(init_state), hipMemcpyHostToDevice, s));
-    CFA_TRY(hipMemcpyAsync(ks.sc + S_TOL, &tol_v, sizeof(double), hipMemcpyHostToDevice, s));
+    const double init_sc[2] = {tol, -1.0};   // S_TOL, and S_REL "not computed"
+    CFA_TRY(hipMemcpyAsync(sc + S_TOL, &init_sc[0], sizeof(double), hipMemcpyHostToDevice, s));
+    CFA_TRY(hipMemcpyAsync(sc + S_REL, &init_sc[1], sizeof(double), hipMemcpyHostToDevice, s));
     ws->status_host[0] = 0;
     ws->status_host[1] = ST_RUNNING;
 
-    constexpr int TPR = 4;
-    const int blocks_tpr = std::min(blocks_for((long long)n * TPR), OP_BLOCKS);
-    // The fused operator needs nnz for the [Bx | By | BTx | BTy] layout: read it from rp[n] once.
-    int nnz = 0;
-    CFA_TRY(hipMemcpyAsync(&nnz, rp + n, sizeof(int), hipMemcpyDeviceToHost, s));
-    if (hipStreamSynchronize(s) != hipSuccess) return launch_status();
-
-    auto apply_op = [&](const double* in, double* out) {
+    auto apply_op = [&](const double* in, double* out, hipStream_t q_s) {
         if (oss) {
-            hipLaunchKernelGGL(k_apply_K_B<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, nnz, rp, ci, val,
+            hipLaunchKernelGGL(k_apply_K_B<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, q_s, n, nnz, rp, ci, val,
                                oss->oss_val, oss->inv_m, in, out, oss->gx, oss->gy, st);
         } else {
-            csr_op_launch<OP_SPMV>(TPR, n, rp, ci, val, in, nullptr, nullptr, 0.0, out, st, s);
+            csr_op_launch<OP_SPMV>(TPR, n, rp, ci, val, in, nullptr, nullptr, 0.0, out, st, q_s);
             if (extra_cb) extra_cb(in, out, extra_ctx);
         }
     };
     // second stage of the fused operator (q -= B^T g) with the p.q partials, or a plain dot
-    auto finish_op_dot = [&](const double* pv, double* qv) {
+    auto finish_op_dot = [&](const double* pv, double* qv, hipStream_t q_s) {
         if (oss) {
-            hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss->oss_val,
-                               oss->bc_tag, oss->gx, oss->gy, pv, qv, part, st);
+            hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, q_s, n, nnz, rp, ci,
+                               oss->oss_val, oss->bc_tag, oss->gx, oss->gy, pv, qv, part, st);
             return blocks_tpr;
         }
-        hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, s, n, pv, qv, part, st);
+        hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, q_s, n, pv, qv, part, st);
         return nb;
     };
-    auto apply_prec = [&](const double* rv, double* zv) {
-        if (amg) {
-            if (amg->n_levels == 0) {
-                hipLaunchKernelGGL(k_dense_matvec, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
-                                   amg->coarse_n, amg->coarse_inv, rv, zv, st);
-            } else {
-                VcycleRun run{amg, st, s, std::vector<cfa_amg_level_t>(amg->levels, amg->levels + amg->n_levels)};
-                run.level(0, rv, zv);
-            }
-        } else if (prec_cb) {
-            prec_cb(rv, zv, prec_ctx);
-        } else {
-            hipLaunchKernelGGL(k_vec_mul, dim3(blocks), dim3(BLOCK), 0, s, n, inv_diag, rv, zv, st);
-        }
+    auto apply_prec = [&](const double* rv, double* zv, hipStream_t q_s) {
+        if (amg) amg_apply_enqueue(amg, rv, zv, st, q_s);
+        else if (prec_cb) prec_cb(rv, zv, prec_ctx);
+        else hipLaunchKernelGGL(k_vec_mul, dim3(blocks), dim3(BLOCK), 0, q_s, n, inv_diag, rv, zv, st);
     };
 
-    // r = b - A x, ||r||
-    apply_op(x, q);
-    if (oss) {
-        // the BT stage without a dot: reuse the dot kernel's stage with p = q (result unused)
-        hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss->oss_val,
-                           oss->bc_tag, oss->gx, oss->gy, q, q, part, st);
-    }
-    hipLaunchKernelGGL(k_pcg_residual, dim3(nb), dim3(BLOCK), 0, s, n, b, q, r, part);
-    hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(BLOCK), 0, s, nb, part, ks.sc + S_RNORM, 1);
-    CFA_TRY(hipMemcpyAsync(&host_sc[1], ks.sc + S_RNORM, sizeof(double), hipMemcpyDeviceToHost, s));
-    if (hipStreamSynchronize(s) != hipSuccess) return launch_status();
-    if (host_sc[1] < tol * b_norm) return launch_status();
-
-    apply_prec(r, z);
+    // prologue (src/cfa/linalg.cuh:205-227), all decisions on the device
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, s, n, b, b, part, nullptr);
+    hipLaunchKernelGGL(k_pcg_init_bnorm, dim3(1), dim3(BLOCK), 0, s, nb, part, sc, st, status);
+    hipLaunchKernelGGL(k_zero_if_trivial, dim3(blocks), dim3(BLOCK), 0, s, n, x, st);
+    if (!amg && !prec_cb)
+        hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(BLOCK), 0, s, n, rp, ci, val, 1.0e-14, inv_diag);
+    apply_op(x, q, s);
+    if (oss) (void)finish_op_dot(q, q, s);   // q -= B^T g (its dot is not used)
+    hipLaunchKernelGGL(k_pcg_residual, dim3(nb), dim3(BLOCK), 0, s, n, b, q, r, part, st);
+    hipLaunchKernelGGL(k_pcg_init_rnorm, dim3(1), dim3(BLOCK), 0, s, nb, part, sc, st, status);
+    apply_prec(r, z, s);
     hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(BLOCK), 0, s, n, z, p, st);
     hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, s, n, r, z, part, st);
-    hipLaunchKernelGGL(k_pcg_final_beta, dim3(1), dim3(BLOCK), 0, s, nb, part, ks.sc, st, ws->status_dev, -1, 1);
+    hipLaunchKernelGGL(k_pcg_final_beta, dim3(1), dim3(BLOCK), 0, s, nb, part, sc, st, status, 1);
+    int rc = launch_status();
+    if (rc) return rc;
 
-    // iterations (src/cfa/linalg.cuh:231-253)
-    int it = 0;
-    for (; it < max_iter; ++it) {
-        if (it >= LOOKAHEAD) {
-            wait_progress(ws, it - LOOKAHEAD + 1, s);
-            if (ws->status_host[1] != ST_RUNNING) break;
+    // one iteration (src/cfa/linalg.cuh:231-253)
+    auto iteration = [&](hipStream_t q_s) {
+        apply_op(p, q, q_s);
+        const int npart = finish_op_dot(p, q, q_s);
+        hipLaunchKernelGGL(k_pcg_final_alpha, dim3(1), dim3(BLOCK), 0, q_s, npart, part, sc, st, status);
+        hipLaunchKernelGGL(k_pcg_update_xr, dim3(nb), dim3(BLOCK), 0, q_s, n, sc, p, q, x, r, part, st);
+        hipLaunchKernelGGL(k_pcg_final_rnorm, dim3(1), dim3(BLOCK), 0, q_s, nb, part, sc, st, status);
+        apply_prec(r, z, q_s);
+        hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, q_s, n, r, z, part, st);
+        hipLaunchKernelGGL(k_pcg_final_beta, dim3(1), dim3(BLOCK), 0, q_s, nb, part, sc, st, status, 0);
+        hipLaunchKernelGGL(k_pcg_p, dim3(blocks), dim3(BLOCK), 0, q_s, n, z, sc, p, st);
+    };
+    hipGraphExec_t graph = nullptr;
+    if (!extra_cb && !prec_cb && max_iter > 0) {
+        // graph key: every pointer, size and constant the launches of one iteration bake in
+        KeyBuilder kb;
+        kb.add(n); kb.add(nnz); kb.add(rp); kb.add(ci); kb.add(val); kb.add(x);
+        for (int k = 0; k < 5; ++k) kb.add(ws->vec[k]);
+        kb.add(part); kb.add(sc); kb.add(status);
+        kb.add(oss != nullptr);
+        if (oss) kb.add(*oss);
+        kb.add(amg != nullptr);
+        if (amg) {
+            kb.add(amg->n_levels); kb.add(amg->coarse_n); kb.add(amg->coarse_inv); kb.add(amg->cb); kb.add(amg->cx);
+            kb.add(amg->smooth_steps);
+            if (amg->n_levels > 0) kb.add_bytes(amg->levels, sizeof(cfa_amg_level_t) * (size_t)amg->n_levels);
         }
-        apply_op(p, q);
-        const int npart = finish_op_dot(p, q);
-        hipLaunchKernelGGL(k_pcg_final_alpha, dim3(1), dim3(BLOCK), 0, s, npart, part, ks.sc, st, ws->status_dev, it);
-        hipLaunchKernelGGL(k_pcg_update_xr, dim3(nb), dim3(BLOCK), 0, s, n, ks.sc, p, q, x, r, part, st);
-        hipLaunchKernelGGL(k_pcg_final_rnorm, dim3(1), dim3(BLOCK), 0, s, nb, part, ks.sc, st, ws->status_dev, it);
-        apply_prec(r, z);
-        hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, s, n, r, z, part, st);
-        hipLaunchKernelGGL(k_pcg_final_beta, dim3(1), dim3(BLOCK), 0, s, nb, part, ks.sc, st, ws->status_dev, it, 0);
-        hipLaunchKernelGGL(k_pcg_p, dim3(blocks), dim3(BLOCK), 0, s, n, z, ks.sc, p, st);
+        graph = iteration_graph(ws, internal_of(ws)->pcg, kb.k, iteration);
     }
-    int fin[2];
+    rc = run_iterations(ws, graph, max_iter, iteration, s);
+    if (rc) return rc;
+    int fin[3];
     double rel;
     CFA_TRY(hipMemcpyAsync(fin, st, sizeof(fin), hipMemcpyDeviceToHost, s));
-    CFA_TRY(hipMemcpyAsync(&rel, ks.sc + S_REL, sizeof(double), hipMemcpyDeviceToHost, s));
-    if (hipStreamSynchronize(s) != hipSuccess) return launch_status();
+    CFA_TRY(hipMemcpyAsync(&rel, sc + S_REL, sizeof(double), hipMemcpyDeviceToHost, s));
+    CFA_TRY(hipStreamSynchronize(s));
     if (fin[0] == ST_RUNNING) {   // iteration cap
         result->iters[0] = max_iter;
         result->converged[0] = 0;
     } else {
         result->iters[0] = fin[1];
-        result->converged[0] = fin[0] == ST_CONVERGED;
+        result->converged[0] = fin[0] != ST_STOPPED;
     }
     result->rel_residual[0] = rel;
     return launch_status();
 }
 
 int cfa_ilu0_factor(const cfa_ilu_t* ilu, const double* val, cfa_stream_t stream) {
-    if (!ilu || !val || !ilu->lu || !ilu->colour_start) return CFA_E_ARG;
+    if (!ilu || !val || !ilu->lu || !ilu->colour_start || !ilu->el_ptr || !ilu->el || !ilu->upd) return CFA_E_ARG;
     hipStream_t s = S(stream);
     if (ilu->n == 0) return CFA_OK;
     CFA_TRY(hipMemcpyAsync(ilu->lu, val, (size_t)ilu->nnz * sizeof(double), hipMemcpyDeviceToDevice, s));
-    for (int c = 1; c < ilu->n_colours; ++c) {
-        const int m = ilu->colour_start[c + 1] - ilu->colour_start[c];
+    for (int c = 1; c < ilu->n_colours; ++c) {   // colour 0 has no L entries
+        const int t0 = ilu->colour_start[c], m = ilu->colour_start[c + 1] - t0;
         if (m == 0) continue;
-        hipLaunchKernelGGL(k_ilu_factor_colour, dim3(blocks_for(m, 128)), dim3(128), 0, s, m,
-                           ilu->rows + ilu->colour_start[c], ilu->row_ptr, ilu->col_ind, ilu->colour, ilu->diag_pos,
-                           ilu->lu);
+        hipLaunchKernelGGL(k_ilu_factor_rows, dim3(blocks_for(m, 128)), dim3(128), 0, s, m, t0, ilu->el_ptr, ilu->el,
+                           ilu->upd, ilu->lu);
     }
     const int len = std::max(ilu->n, std::max(ilu->nl, ilu->nu));
     hipLaunchKernelGGL(k_ilu_gather, dim3(blocks_for(len)), dim3(BLOCK), 0, s, ilu->nl, ilu->nu, ilu->n, ilu->l_src,
                        ilu->u_src, ilu->d_src, ilu->lu, ilu->lv, ilu->uv, ilu->dinv, ilu->lv32, ilu->uv32, ilu->dinv32);
     return launch_status();
 }
-
 
 int cfa_ilu0_apply(const cfa_ilu_t* ilu, const double* r, double* z, int nrhs, cfa_stream_t stream) {
     if (!ilu || !r || !z || nrhs < 1 || nrhs > 2 || !ilu->colour_start) return CFA_E_ARG;
@@ -1136,7 +1175,7 @@ int cfa_bicgstab2(int n, const int* rp, const int* ci, const double* val, const 
                   double* x_u, double* x_v, int max_iter, double tol, const cfa_ilu_t* ilu, int use_f32,
                   cfa_krylov_ws_t* ws, cfa_solve_result_t* result, cfa_stream_t stream) {
     if (n < 0 || !rp || !ci || !val || !b_u || !b_v || !x_u || !x_v || !ilu || !ws || !result || ws->cap < n ||
-        !ws->status_host)
+        !ws->status_host || max_iter < 0 || !ilu->colour_start || ilu->n != n)
         return CFA_E_ARG;
     hipStream_t s = S(stream);
     for (int c = 0; c < 2; ++c) { result->iters[c] = 0; result->converged[c] = 1; result->rel_residual[c] = 0.0; }
@@ -1155,52 +1194,63 @@ int cfa_bicgstab2(int n, const int* rp, const int* ci, const double* val, const 
     double* part_a = ws->part;                 // 6 * RED_BLOCKS each
     double* part_b = ws->part + 6 * RED_BLOCKS;
     double* part_c = ws->part + 12 * RED_BLOCKS;
-    KrylovState ks = kstate(ws);
-    int* st = ks.st;
+    double* sc = ws->scal;
+    int* st = state_words(ws);
+    int* status = ws->status_dev;
     CFA_TRY(hipMemcpyAsync(x, x_u, bytes, hipMemcpyDeviceToDevice, s));
     CFA_TRY(hipMemcpyAsync(x + n, x_v, bytes, hipMemcpyDeviceToDevice, s));
     CFA_TRY(hipMemcpyAsync(bb, b_u, bytes, hipMemcpyDeviceToDevice, s));
     CFA_TRY(hipMemcpyAsync(bb + n, b_v, bytes, hipMemcpyDeviceToDevice, s));
-    double tol_v = tol;
-    CFA_TRY(hipMemcpyAsync(ks.sc + B_TOL, &tol_v, sizeof(double), hipMemcpyHostToDevice, s));
+    const double tol_v = tol;
+    CFA_TRY(hipMemcpyAsync(sc + B_TOL, &tol_v, sizeof(double), hipMemcpyHostToDevice, s));
     ws->status_host[0] = 0;
     ws->status_host[1] = ST_RUNNING;
 
     constexpr int TPR = 4;
     const int groups_per_block = BLOCK / TPR;
-    int nb_rows = (n + groups_per_block - 1) / groups_per_block;
-    if (nb_rows > RED_BLOCKS) nb_rows = RED_BLOCKS;
+    const int nb_rows = std::min((n + groups_per_block - 1) / groups_per_block, RED_BLOCKS);
     const int nb2 = red_blocks(2LL * n);
     const int blocks2 = blocks_for(2LL * n);
 
     hipLaunchKernelGGL(k_bicg_init<TPR>, dim3(nb_rows), dim3(BLOCK), 0, s, n, rp, ci, val, bb, x, r, r0, p, v, part_a);
-    hipLaunchKernelGGL(k_bicg_final_init, dim3(1), dim3(BLOCK), 0, s, nb_rows, part_a, ks.sc, st, ws->status_dev);
+    hipLaunchKernelGGL(k_bicg_final_init, dim3(1), dim3(BLOCK), 0, s, nb_rows, part_a, sc, st, status, max_iter);
+    int rc = launch_status();
+    if (rc) return rc;
 
-    for (int it = 0; it < max_iter; ++it) {
-        if (it >= LOOKAHEAD) {
-            wait_progress(ws, it - LOOKAHEAD + 1, s);
-            if (ws->status_host[1] != ST_RUNNING) break;
-        }
-        hipLaunchKernelGGL(k_bicg_pre, dim3(1), dim3(1), 0, s, ks.sc, st, ws->status_dev, it);
-        if (it > 0) hipLaunchKernelGGL(k_bicg_p, dim3(blocks2), dim3(BLOCK), 0, s, n, r, v, ks.sc, p, st);
-        ilu_apply_any(ilu, use_f32, p, ph, 2, st, s);
-        hipLaunchKernelGGL(k_spmv2_dots<TPR>, dim3(nb_rows), dim3(BLOCK), 0, s, n, rp, ci, val, ph, v, r0, 1, part_a, st);
-        hipLaunchKernelGGL(k_bicg_final_alpha, dim3(1), dim3(BLOCK), 0, s, nb_rows, part_a, ks.sc, st, ws->status_dev, it);
-        hipLaunchKernelGGL(k_bicg_s, dim3(nb2), dim3(BLOCK), 0, s, n, r, v, ks.sc, sv, part_b, st);
-        ilu_apply_any(ilu, use_f32, sv, sh, 2, st, s);
-        hipLaunchKernelGGL(k_spmv2_dots<TPR>, dim3(nb_rows), dim3(BLOCK), 0, s, n, rp, ci, val, sh, t, sv, 2, part_c, st);
-        hipLaunchKernelGGL(k_bicg_final_omega, dim3(1), dim3(BLOCK), 0, s, nb2, part_b, nb_rows, part_c, ks.sc, st);
-        hipLaunchKernelGGL(k_bicg_xr, dim3(nb2), dim3(BLOCK), 0, s, n, ph, sh, sv, t, r0, ks.sc, x, r, part_a, st);
-        hipLaunchKernelGGL(k_bicg_final_post, dim3(1), dim3(BLOCK), 0, s, nb2, part_a, ks.sc, st, ws->status_dev, it,
-                           max_iter);
+    // one iteration for both components (src/cfa/linalg.cuh:700-775)
+    auto iteration = [&](hipStream_t q_s) {
+        hipLaunchKernelGGL(k_bicg_pre, dim3(1), dim3(1), 0, q_s, sc, st, status);
+        hipLaunchKernelGGL(k_bicg_p, dim3(blocks2), dim3(BLOCK), 0, q_s, n, r, v, sc, p, st);
+        ilu_apply_any(ilu, use_f32, p, ph, 2, st, q_s);
+        hipLaunchKernelGGL(k_spmv2_dots<TPR>, dim3(nb_rows), dim3(BLOCK), 0, q_s, n, rp, ci, val, ph, v, r0, 1, part_a, st);
+        hipLaunchKernelGGL(k_bicg_final_alpha, dim3(1), dim3(BLOCK), 0, q_s, nb_rows, part_a, sc, st, status);
+        hipLaunchKernelGGL(k_bicg_s, dim3(nb2), dim3(BLOCK), 0, q_s, n, r, v, sc, sv, part_b, st);
+        ilu_apply_any(ilu, use_f32, sv, sh, 2, st, q_s);
+        hipLaunchKernelGGL(k_spmv2_dots<TPR>, dim3(nb_rows), dim3(BLOCK), 0, q_s, n, rp, ci, val, sh, t, sv, 2, part_c, st);
+        hipLaunchKernelGGL(k_bicg_final_omega, dim3(1), dim3(BLOCK), 0, q_s, nb2, part_b, nb_rows, part_c, sc, st);
+        hipLaunchKernelGGL(k_bicg_xr, dim3(nb2), dim3(BLOCK), 0, q_s, n, ph, sh, sv, t, r0, sc, x, r, part_a, st);
+        hipLaunchKernelGGL(k_bicg_final_post, dim3(1), dim3(BLOCK), 0, q_s, nb2, part_a, sc, st, status);
+    };
+    hipGraphExec_t graph = nullptr;
+    if (max_iter > 0) {
+        KeyBuilder kb;
+        kb.add(n); kb.add(rp); kb.add(ci); kb.add(val); kb.add(use_f32); kb.add(ilu->n_colours);
+        kb.add_bytes(ilu->colour_start, sizeof(int) * (size_t)(ilu->n_colours + 1));
+        kb.add(ilu->rows); kb.add(ilu->lp); kb.add(ilu->up); kb.add(ilu->lc); kb.add(ilu->uc); kb.add(ilu->lv);
+        kb.add(ilu->uv); kb.add(ilu->dinv); kb.add(ilu->lv32); kb.add(ilu->uv32); kb.add(ilu->dinv32);
+        for (int k = 0; k < 10; ++k) kb.add(ws->vec[k]);
+        kb.add(ws->part); kb.add(sc); kb.add(status);
+        graph = iteration_graph(ws, internal_of(ws)->bicg, kb.k, iteration);
     }
-    int fin[10];
+    rc = run_iterations(ws, graph, max_iter, iteration, s);
+    if (rc) return rc;
+    int fin[12];
     double rel[2];
     CFA_TRY(hipMemcpyAsync(fin, st, sizeof(fin), hipMemcpyDeviceToHost, s));
-    CFA_TRY(hipMemcpyAsync(rel, ks.sc + B_REL, sizeof(rel), hipMemcpyDeviceToHost, s));
+    CFA_TRY(hipMemcpyAsync(rel, sc + B_REL, sizeof(rel), hipMemcpyDeviceToHost, s));
     CFA_TRY(hipMemcpyAsync(x_u, x, bytes, hipMemcpyDeviceToDevice, s));
     CFA_TRY(hipMemcpyAsync(x_v, x + n, bytes, hipMemcpyDeviceToDevice, s));
-    if (hipStreamSynchronize(s) != hipSuccess) return launch_status();
+    CFA_TRY(hipStreamSynchronize(s));
     for (int c = 0; c < 2; ++c) {
         if (fin[2 + c]) {   // still active when the loop ended (cap reached without the in-kernel stop)
             result->iters[c] = max_iter;
@@ -1259,7 +1309,6 @@ int cfa_bicgstab_jacobi(int n, const int* rp, const int* ci, const double* val, 
         if (std::fabs(rho_new) < bd) break;
         if (iter > 0) {
             const double beta = (rho_new / rho_old) * (alpha / (omega + 1.0e-30));
-            // p = r + beta (p - omega v)
             hipLaunchKernelGGL(k_axpy, dim3(blocks), dim3(BLOCK), 0, s, n, -omega, v, p);
             hipLaunchKernelGGL(k_xpay, dim3(blocks), dim3(BLOCK), 0, s, n, r, beta, p);
         }
@@ -1296,8 +1345,8 @@ int cfa_bicgstab_jacobi(int n, const int* rp, const int* ci, const double* val, 
     }
     result->iters[0] = iter;
     result->converged[0] = done ? 1 : 0;
-    const hipError_t e = hipStreamSynchronize(s);
-    return e == hipSuccess ? launch_status() : (int)e;
+    CFA_TRY(hipStreamSynchronize(s));
+    return launch_status();
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #ifdef _OPENMP
@@ -20,6 +21,7 @@ namespace cfa {
 
 // ---------------------------------------------------------------------------------------------
 KrylovWorkspace::~KrylovWorkspace() {
+    cfa_krylov_ws_release(&ws_);
     if (status_host_) cfa_host_free(status_host_);
 }
 
@@ -34,6 +36,8 @@ cfa_krylov_ws_t* KrylovWorkspace::get(int n) {
         scal_.reserve(64, 0);
         ws_.part = part_.ptr;
         ws_.scal = scal_.ptr;
+        const char* env = std::getenv("CFA_KRYLOV_GRAPHS");
+        ws_.use_graphs = (env && env[0] == '0') ? 0 : 1;
     }
     if (n > ws_.cap) {
         const int cap = n + n / 2 + 256;
@@ -79,7 +83,7 @@ int GpuLinearSolver::solve_pcg(int n, const int* d_row_ptr, const int* d_col_ind
                       &res, solver_stream()));
     max_n = ws_.capacity();
     converged = res.converged[0] != 0;
-    last_rel_residual = res.rel_residual[0];
+    if (res.rel_residual[0] >= 0.0) last_rel_residual = res.rel_residual[0];
     last_iters = res.iters[0];
     return res.iters[0];
 }
@@ -92,7 +96,7 @@ int GpuLinearSolver::solve_pcg_fused(int n, const int* d_row_ptr, const int* d_c
                       amg ? amg->view() : nullptr, nullptr, nullptr, ws_.get(n), &res, solver_stream()));
     max_n = ws_.capacity();
     converged = res.converged[0] != 0;
-    last_rel_residual = res.rel_residual[0];
+    if (res.rel_residual[0] >= 0.0) last_rel_residual = res.rel_residual[0];
     last_iters = res.iters[0];
     return res.iters[0];
 }
@@ -112,6 +116,61 @@ int GpuLinearSolver::solve_bicgstab(int n, const int* d_row_ptr, const int* d_co
 // ---------------------------------------------------------------------------------------------
 // MulticolourIlu0 (src/cfa/linalg.cuh:435-568)
 // ---------------------------------------------------------------------------------------------
+// Elimination schedule of ILU(0) in colour order, row by row in colour-row order (the loop order
+// of src/cfa/linalg.cuh:375-391): for c < colour(i), the L entries a_ik of colour c in CSR order;
+// for each, the entries of row k whose column j is in row i's pattern with colour(j) > c, in row
+// k's order.
+void ilu_elimination_schedule(int n, const int* rp, const int* ci, const std::vector<int>& colour,
+                              const std::vector<int>& rows, const std::vector<int>& diag, std::vector<int>& el_ptr,
+                              std::vector<int>& el, std::vector<int>& upd) {
+    std::vector<int> up_cnt(n + 1, 0);
+    el_ptr.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i)
+        if (diag[i] < 0) fatal("[ILU] row without a diagonal entry");
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < n; ++t) {
+        const int i = rows[t];
+        int steps = 0, ups = 0;
+        for (int a = rp[i]; a < rp[i + 1]; ++a) {
+            const int k = ci[a];
+            if (k == i || colour[k] >= colour[i]) continue;
+            ++steps;
+            for (int m = rp[k]; m < rp[k + 1]; ++m) {
+                const int j = ci[m];
+                if (colour[j] <= colour[k]) continue;
+                for (int b = rp[i]; b < rp[i + 1]; ++b)
+                    if (ci[b] == j) { ++ups; break; }
+            }
+        }
+        el_ptr[t + 1] = steps;
+        up_cnt[t + 1] = ups;
+    }
+    for (int t = 0; t < n; ++t) { el_ptr[t + 1] += el_ptr[t]; up_cnt[t + 1] += up_cnt[t]; }
+    el.assign(4 * (size_t)std::max(el_ptr[n], 1), 0);
+    upd.assign(2 * (size_t)std::max(up_cnt[n], 1), 0);
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < n; ++t) {
+        const int i = rows[t];
+        int e = el_ptr[t], u = up_cnt[t];
+        for (int c = 0; c < colour[i]; ++c)
+            for (int a = rp[i]; a < rp[i + 1]; ++a) {
+                const int k = ci[a];
+                if (k == i || colour[k] != c) continue;
+                el[4 * e] = a;
+                el[4 * e + 1] = diag[k];
+                el[4 * e + 2] = u;
+                for (int m = rp[k]; m < rp[k + 1]; ++m) {
+                    const int j = ci[m];
+                    if (colour[j] <= c) continue;
+                    for (int b = rp[i]; b < rp[i + 1]; ++b)
+                        if (ci[b] == j) { upd[2 * u] = b; upd[2 * u + 1] = m; ++u; break; }
+                }
+                el[4 * e + 3] = u;
+                ++e;
+            }
+    }
+}
+
 void MulticolourIlu0::set_pattern(int n_, const std::vector<int>& rp, const std::vector<int>& ci, const int* d_rp,
                                   const int* d_ci) {
     const auto t0 = Clock::now();
@@ -170,6 +229,11 @@ void MulticolourIlu0::set_pattern(int n_, const std::vector<int>& rp, const std:
             else { uc[u] = j; u_src[u++] = a; }
         }
     }
+    std::vector<int> el_ptr, el, upd;
+    ilu_elimination_schedule(n, rp.data(), ci.data(), colour, rows, diag, el_ptr, el, upd);
+    d_el_ptr.upload(el_ptr);
+    d_el.upload(el);
+    d_upd.upload(upd);
     d_rows.upload(rows);
     d_colour.upload(colour);
     d_diag_pos.upload(diag);
@@ -219,6 +283,9 @@ void MulticolourIlu0::refresh_view() {
     view_.lv32 = keep_fp32 ? d_lv32.ptr : nullptr;
     view_.uv32 = keep_fp32 ? d_uv32.ptr : nullptr;
     view_.dinv32 = keep_fp32 ? d_dinv32.ptr : nullptr;
+    view_.el_ptr = d_el_ptr;
+    view_.el = d_el;
+    view_.upd = d_upd;
 }
 
 void MulticolourIlu0::factor(const double* d_val) { GPU_CHECK(cfa_ilu0_factor(&view_, d_val, solver_stream())); }
@@ -631,16 +698,13 @@ void AmgPreconditioner::refresh_view() {
         d.omega = L.omega;
         d.x = L.x; d.b = L.b; d.r = L.r; d.ax = L.Ax;
     }
-    if (!desc_.empty()) d_desc_.upload(desc_.data(), desc_.size());
     view_.n_levels = (int)levels.size();
     view_.levels = desc_.data();
-    view_.d_levels = desc_.empty() ? nullptr : d_desc_.ptr;
     view_.coarse_n = coarse_n;
     view_.coarse_inv = d_coarse_inv;
     view_.cb = d_cb;
     view_.cx = d_cx;
     view_.smooth_steps = smooth_steps;
-    view_.small_rows = small_rows;
 }
 
 void AmgPreconditioner::apply(const double* d_r, double* d_z) {

@@ -66,6 +66,7 @@ public:
     const int* d_row_ptr = nullptr;
     const int* d_col_ind = nullptr;
     DevArray<int> d_lp, d_up, d_lc, d_uc, d_l_src, d_u_src, d_d_src;
+    DevArray<int> d_el_ptr, d_el, d_upd;   // elimination schedule of the factorization
     DevArray<double> d_lv, d_uv, d_dinv;
     DevArray<float> d_lv32, d_uv32, d_dinv32;   // FP32 factors (mixed precision)
     bool keep_fp32 = false;
@@ -83,6 +84,11 @@ private:
     cfa_ilu_t view_{};
     void refresh_view();
 };
+
+// elimination schedule of the colour-ordered ILU(0) factorization (layout of cfa_ilu_t)
+void ilu_elimination_schedule(int n, const int* rp, const int* ci, const std::vector<int>& colour,
+                              const std::vector<int>& rows, const std::vector<int>& diag, std::vector<int>& el_ptr,
+                              std::vector<int>& el, std::vector<int>& upd);
 
 class GpuBicgstab2 {
 public:
@@ -129,7 +135,6 @@ public:
     double theta = 0.08;
     int coarse_target = 300;
     int max_levels = 15;
-    int small_rows = 2048;   // levels at most this size run in one workgroup
 
     std::vector<std::unique_ptr<Level>> levels;
     int coarse_n = 0;
@@ -160,7 +165,6 @@ public:
 private:
     std::vector<std::unique_ptr<Level>> spare_;
     std::vector<cfa_amg_level_t> desc_;
-    DevArray<cfa_amg_level_t> d_desc_;
     cfa_amg_t view_{};
     Clock::time_point phase_t_;
     void phase_start() { phase_t_ = Clock::now(); }

@@ -216,19 +216,15 @@ typedef struct {
 } cfa_amg_level_t;
 
 /* SA-AMG hierarchy for the V(ss, ss) cycle (src/cfa/linalg.cuh:1161-1206).
- * levels: host array of n_levels descriptors; d_levels: the same array in device memory
- * (read by the single-workgroup kernel that runs the small levels); coarse_inv: dense
- * inverse (coarse_n x coarse_n, row-major) of the coarsest operator; cb, cx: [coarse_n].
- * Levels whose row count is at most small_rows run inside one workgroup (0 = never). */
+ * levels: host array of n_levels descriptors; coarse_inv: dense inverse (coarse_n x coarse_n,
+ * row-major) of the coarsest operator; cb, cx: [coarse_n]. */
 typedef struct {
     int n_levels;
     const cfa_amg_level_t* levels;
-    const cfa_amg_level_t* d_levels;
     int coarse_n;
     const double* coarse_inv;
     double* cb; double* cx;
     int smooth_steps;
-    int small_rows;
 } cfa_amg_t;
 
 /* z = M^{-1} r, one V-cycle (src/cfa/linalg.cuh:1161-1169 AmgPreconditioner::apply). */
@@ -241,7 +237,10 @@ typedef void (*cfa_op_cb)(const double* in, double* out, void* ctx);
 /* Krylov workspace: device vectors of capacity >= n (x 2 for the two-component solver),
  * device scalars, and a pinned status word the kernels update so the host can follow the
  * iteration without a synchronisation per dot product. Layout sizes: vec[12] vectors of
- * 2 * cap doubles, part: 8 * 4096 doubles, scal: 64 doubles, status: 16 ints (host, mapped). */
+ * 2 * cap doubles, part: 8 * 4096 doubles, scal: 64 doubles, status: 16 ints (host, mapped).
+ * internal: NULL initially; the library keeps its captured iteration graphs there (released by
+ * cfa_krylov_ws_release). use_graphs: replay each iteration as a captured HIP graph (set to 0
+ * to launch the kernels one by one; solves with host callbacks never use graphs). */
 typedef struct {
     int cap;
     double* vec[12];
@@ -249,7 +248,12 @@ typedef struct {
     double* scal;
     int* status_host;
     int* status_dev;
+    void* internal;
+    int use_graphs;
 } cfa_krylov_ws_t;
+
+/* Frees what the library attached to ws->internal (captured graphs, its capture stream). */
+int cfa_krylov_ws_release(cfa_krylov_ws_t* ws);
 
 typedef struct {
     int iters[2];
@@ -281,6 +285,11 @@ typedef struct {
     const int* d_src;                   /* [n] CSR position of the diagonal, colour-row order */
     double* lv; double* uv; double* dinv;
     float* lv32; float* uv32; float* dinv32; /* FP32 copies (mixed precision); may be NULL */
+    /* elimination schedule of the factorization, per colour-row t (colour by colour, CSR order
+     * inside a colour, as linalg.cuh:366-392): el_ptr [n + 1]; el [4 per step] = (CSR position
+     * of l_ik, CSR position of u_kk, first and end update); upd [2 per update] = (position of
+     * a_ij, position of u_kj): a_ij -= l_ik u_kj */
+    const int* el_ptr; const int* el; const int* upd;
 } cfa_ilu_t;
 
 /* Factor the values (linalg.cuh:531-541): lu = ILU(0) of val in colour order, then lv, uv,

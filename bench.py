@@ -1,0 +1,145 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-step projection solve on one MI355X.
+
+Workload (default): the adaptive obstacle flow of the reference's runs/paper/d2_A2_dt05 —
+parabolic inflow (mean 1, Re 100), NVB levels 2-5 with coarsening and look-ahead, remesh every
+20 steps (overlapped by 10), dt 5e-4, FP64 throughout — with the momentum BiCGSTAB and pressure
+PCG run to a relative residual of 1e-8. A step is one time step (momentum predictor, pressure
+increment, correction, forces, and the remesh work that falls on it). `--warmup W` steps run
+first, untimed; then exactly `--steps K` steps are timed. The program synchronises the device
+at the end of every step (the forces are read back), and the timed region runs from the end of
+step W to the end of step W + K on the host's steady clock.
+
+After the timed run the main kernels are launched alone on the final mesh and state
+(--kernel-timing) and timed with HIP events on the solver stream; the roofline object reports
+the dominant one. The CPU baseline is the oracle (the serial FP64 CPU restatement of the same
+step) on a bounded sample, run on this host.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "cylinder-fem-amr_amd"))
+
+HBM_PEAK_GBS = 8000.0
+DOMINANT = "k_csr_op<jacobi> level 0"
+
+WORKLOADS = {
+    # runs/paper/d2_A2_dt05/cmd.txt (--steps set by the benchmark)
+    "adaptive": ["--dt", "0.0005", "--amr-interval", "20", "--amr-overlap", "10", "--amr-tol-vort", "0.05",
+                 "--amr-tol-interp", "0.005", "--max-level", "5", "--max-elems", "400000", "--min-level", "2"],
+    # runs/paper/mms_L3/cmd.txt
+    "mms_u3": ["--mms", "--uniform-level", "3", "--dt", "0.00025"],
+}
+DESCRIPTION = {
+    "adaptive": "adaptive obstacle flow, NVB levels 2-5, remesh every 20 steps, dt 5e-4, Re 100",
+    "mms_u3": "manufactured solution on uniform level 3 (29152 nodes), dt 2.5e-4",
+}
+
+
+def cpu_baseline(seconds_target: float = 15.0) -> dict:
+    """The oracle on a bounded sample: fixed uniform level-3 mesh (29,152 nodes, the size of the
+    adaptive run's early meshes), DFG inflow, dt 5e-4, tolerances 1e-8, serial FP64."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import cfa_oracle
+    steps = 5
+    rows, _, wall = cfa_oracle.run_fixed(steps, 5e-4, 3, 2, mms=False, mom_tol=1e-8, ppe_tol=1e-8)
+    per_step = wall / max(len(rows), 1)
+    steps = max(5, min(400, int(seconds_target / max(per_step, 1e-6))))
+    rows, _, wall = cfa_oracle.run_fixed(steps, 5e-4, 3, 2, mms=False, mom_tol=1e-8, ppe_tol=1e-8)
+    return {"value": round(len(rows) / wall, 3), "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (serial FP64 CPU restatement) on a fixed uniform level-3 mesh (29152 nodes), "
+                      f"DFG inflow, dt 5e-4, tol 1e-8, {len(rows)} steps in {wall:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="adaptive")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--kernel-reps", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch Krylov iterations kernel by kernel")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the path's sharding is described in DESIGN.md)", file=sys.stderr)
+        return 2
+    if args.steps < 1 or args.warmup < 0:
+        print("bench.py: --steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        return 2
+    if args.no_graphs:
+        os.environ["CFA_KRYLOV_GRAPHS"] = "0"
+    import cfa
+
+    total = args.warmup + args.steps
+    flags = WORKLOADS[args.workload] + [
+        "--steps", str(total), "--mom-tol", str(args.tol), "--ppe-tol", str(args.tol), "--precision", args.precision,
+        "--quiet", "--no-csv", "--kernel-timing", str(args.kernel_reps)]
+    rc = cfa.run(flags)
+    if rc != 0:
+        print(f"bench.py: run failed with status {rc}", file=sys.stderr)
+        return 1
+    hist = cfa.last_history()
+    col = {c: i for i, c in enumerate(cfa.HISTORY_COLUMNS)}
+    t_end = hist[total - 1, col["elapsed_s"]]
+    t_start = hist[args.warmup - 1, col["elapsed_s"]] if args.warmup > 0 else 0.0
+    timed = t_end - t_start
+    window = hist[args.warmup:total]
+    kernels = cfa.last_kernels()
+    dom = kernels.get(DOMINANT) or max(kernels.values(), key=lambda k: k["us"])
+    achieved = dom["bytes"] / (dom["us"] * 1e3)
+    line = {
+        "metric": "steps/s at 1e-8 rel. residual, FP32-precond vs FP64 GPU; adjoint/forward time",
+        "value": round(args.steps / timed, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * timed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64 (FP32 preconditioner factors)",
+        "data": "synthetic (deterministic: the reference's inflow / manufactured-solution set-up, no seeds)",
+        "config": {
+            "workload": f"{args.workload}: {DESCRIPTION[args.workload]}, tol {args.tol:g}, steps "
+                        f"{args.warmup + 1}-{total}",
+            "precision": args.precision,
+            "nodes_range": [int(window[:, col["nodes"]].min()), int(window[:, col["nodes"]].max())],
+            "elements_range": [int(window[:, col["elements"]].min()), int(window[:, col["elements"]].max())],
+            "mean_iters_u_v_p": [round(float(window[:, col[c]].mean()), 2) for c in ("it_u", "it_v", "it_p")],
+            "parallelism": "1 GPU",
+        },
+        "roofline": {
+            "kernel": dom["name"],
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": dom["bytes"],
+            "us_per_launch": round(dom["us"], 3),
+            "measured_on": "final mesh of the run, HIP events on the solver stream",
+        },
+        "kernels": {k: {"us": round(v["us"], 3), "GB/s": round(v["bytes"] / (v["us"] * 1e3), 1)}
+                    for k, v in kernels.items()},
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,197 @@
+"""ctypes binding of the gfx950 library (include/cfa_hip.h, libcfa_hip.so) and of its C++ host
+layer (include/cfa_host.h, libcfa_host.so).
+
+The host layer keeps the reference's program and solver interfaces (src/cylinder_fem_amr.cu
+main, src/cfa/*.cuh classes); this module exposes the in-process entry points a Python caller
+needs: run the program with the reference's flags, read the per-step history at full precision
+and the MMS figures, and build the reference's meshes. The libraries are built in-tree by
+`make -C cylinder-fem-amr_amd` (or __graft_entry__.build()); loading fails loudly when they are
+missing, there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+BUILD_DIR = PKG_DIR / "build"
+INCLUDE_DIR = PKG_DIR.parent / "include"
+HIP_LIB = BUILD_DIR / "libcfa_hip.so"
+HOST_LIB = BUILD_DIR / "libcfa_host.so"
+CLI = BUILD_DIR / "cylinder_fem_amr"
+TEST_BIN = BUILD_DIR / "test_cfa"
+
+# columns of cfa_last_run_history (the forces CSV columns of src/cylinder_fem_amr.cu:236, then
+# iteration counts and the elapsed time at the end of the step)
+HISTORY_COLUMNS = ("step", "time", "C_D", "C_L", "delta_p", "nodes", "elements", "p_front", "p_back",
+                   "C_D_surface", "C_L_surface", "it_u", "it_v", "it_p", "elapsed_s")
+STATS_FIELDS = ("wall_s", "momentum_s", "pressure_s", "amr_s", "amg_setup_s", "steps", "nodes", "elements",
+                "nonconverged", "it_u", "it_v", "it_p", "C_D", "C_L", "delta_p")
+MMS_FIELDS = ("err_u_L2", "err_u_H1", "err_p_L2", "norm_u_L2", "norm_u_H1", "norm_p_L2", "err_p_cyl_max",
+              "err_p_probe_max", "h_max")
+
+_hip = None
+_host = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def load():
+    """Loads libcfa_hip.so and libcfa_host.so (RTLD_GLOBAL, so the host layer resolves the
+    kernel library) and declares the signatures used here."""
+    global _hip, _host
+    if _host is not None:
+        return _hip, _host
+    for lib in (HIP_LIB, HOST_LIB):
+        if not lib.exists():
+            raise LibraryMissing(f"{lib} is missing: build the HIP library first "
+                                 f"(make -C {PKG_DIR}, or __graft_entry__.build())")
+    _hip = ctypes.CDLL(str(HIP_LIB), mode=ctypes.RTLD_GLOBAL)
+    _host = ctypes.CDLL(str(HOST_LIB), mode=ctypes.RTLD_GLOBAL)
+    c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    dptr = ctypes.POINTER(c_double)
+    iptr = ctypes.POINTER(c_int)
+    _host.cfa_main.argtypes = [c_int, ctypes.POINTER(ctypes.c_char_p)]
+    _host.cfa_main.restype = c_int
+    _host.cfa_last_run_stats.argtypes = [dptr]
+    _host.cfa_last_run_stats.restype = None
+    _host.cfa_last_run_history.argtypes = [dptr, c_int]
+    _host.cfa_last_run_history.restype = c_int
+    _host.cfa_last_run_kernels.argtypes = [ctypes.c_char_p, c_int, dptr, c_int]
+    _host.cfa_last_run_kernels.restype = c_int
+    _host.cfa_last_run_mms.argtypes = [dptr]
+    _host.cfa_last_run_mms.restype = None
+    _host.cfa_mesh_uniform.argtypes = [c_int]
+    _host.cfa_mesh_uniform.restype = c_void_p
+    _host.cfa_mesh_free.argtypes = [c_void_p]
+    _host.cfa_mesh_free.restype = None
+    _host.cfa_mesh_sizes.argtypes = [c_void_p, iptr]
+    _host.cfa_mesh_sizes.restype = None
+    _host.cfa_mesh_arrays.argtypes = [c_void_p, dptr, dptr, iptr, iptr, iptr, iptr, iptr]
+    _host.cfa_mesh_arrays.restype = None
+    _host.cfa_mesh_incidence.argtypes = [c_void_p, iptr, iptr, iptr]
+    _host.cfa_mesh_incidence.restype = None
+    _host.cfa_mesh_probes.argtypes = [c_void_p, iptr, dptr]
+    _host.cfa_mesh_probes.restype = None
+    _host.cfa_workload_pressure.argtypes = [c_int, c_int, c_double, c_int, dptr]
+    _host.cfa_workload_pressure.restype = c_int
+    _host.cfa_workload_momentum.argtypes = [c_int, c_double, c_int, dptr]
+    _host.cfa_workload_momentum.restype = c_int
+    _hip.cfa_error_string.argtypes = [c_int]
+    _hip.cfa_error_string.restype = ctypes.c_char_p
+    return _hip, _host
+
+
+def header_functions(header: str | os.PathLike) -> list[str]:
+    """Names of the functions a header declares (prototypes at the start of a line)."""
+    text = Path(header).read_text()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s+\**\s*(cfa_[A-Za-z0-9_]+)\s*\(", text,
+                       flags=re.M)
+    return sorted(set(names))
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def run(args: list[str]) -> int:
+    """Runs the program in-process with the reference's command-line flags (argv[0] added)."""
+    _, host = load()
+    argv = [b"cylinder_fem_amr"] + [str(a).encode() for a in args]
+    arr = (ctypes.c_char_p * len(argv))(*argv)
+    return host.cfa_main(len(argv), arr)
+
+
+def last_stats() -> dict:
+    _, host = load()
+    out = np.zeros(16)
+    host.cfa_last_run_stats(_dptr(out))
+    return dict(zip(STATS_FIELDS, out[:15].tolist()))
+
+
+def last_history() -> np.ndarray:
+    """Per-step history of the last run, shape (steps, 15), columns HISTORY_COLUMNS."""
+    _, host = load()
+    rows = host.cfa_last_run_history(None, 0)
+    out = np.zeros((rows, len(HISTORY_COLUMNS)))
+    if rows:
+        host.cfa_last_run_history(_dptr(out), rows)
+    return out
+
+
+def last_kernels() -> dict:
+    """--kernel-timing results of the last run: name -> {name, bytes, us, launches}."""
+    _, host = load()
+    count = host.cfa_last_run_kernels(None, 0, None, 0)
+    buf = ctypes.create_string_buffer(4096)
+    out = np.zeros((max(count, 1), 3))
+    host.cfa_last_run_kernels(buf, len(buf), _dptr(out), count)
+    names = buf.value.decode().split("\n") if count else []
+    return {nm: {"name": nm, "bytes": float(out[k, 0]), "us": float(out[k, 1]), "launches": int(out[k, 2])}
+            for k, nm in enumerate(names)}
+
+
+PRESSURE_FIELDS = ("n", "nnz", "ne", "mesh_s", "amg_setup_ms", "amg_levels", "solves", "solve_ms", "iterations",
+                   "ms_per_solve", "us_per_iteration", "max_rel_residual")
+MOMENTUM_FIELDS = ("n", "nnz", "ne", "mesh_s", "colouring_ms", "colours", "fp64_factor_ms", "fp64_solve_ms",
+                   "fp64_it_u", "fp64_it_v", "fp32_factor_ms", "fp32_solve_ms", "fp32_it_u", "fp32_it_v",
+                   "rel_diff_fp32_fp64", "us_per_fp64_iteration")
+
+
+def workload_pressure(level: int, nrhs: int = 32, tol: float = 1e-8, kernel_reps: int = 0) -> dict:
+    _, host = load()
+    out = np.zeros(12)
+    rc = host.cfa_workload_pressure(level, nrhs, tol, kernel_reps, _dptr(out))
+    if rc != 0:
+        raise RuntimeError(f"cfa_workload_pressure failed: {rc}")
+    return dict(zip(PRESSURE_FIELDS, out.tolist()))
+
+
+def workload_momentum(level: int, tol: float = 1e-8, kernel_reps: int = 0) -> dict:
+    _, host = load()
+    out = np.zeros(16)
+    rc = host.cfa_workload_momentum(level, tol, kernel_reps, _dptr(out))
+    if rc != 0:
+        raise RuntimeError(f"cfa_workload_momentum failed: {rc}")
+    return dict(zip(MOMENTUM_FIELDS, out.tolist()))
+
+
+def last_mms() -> dict:
+    _, host = load()
+    out = np.zeros(9)
+    host.cfa_last_run_mms(_dptr(out))
+    return dict(zip(MMS_FIELDS, out.tolist()))
+
+
+def uniform_mesh(levels: int) -> dict:
+    """The reference's base mesh refined `levels` times (NVB, Morton order): arrays and sizes."""
+    _, host = load()
+    h = host.cfa_mesh_uniform(levels)
+    try:
+        sizes = np.zeros(6, dtype=np.int32)
+        host.cfa_mesh_sizes(h, _iptr(sizes))
+        n, ne, nnz, ncyl = (int(v) for v in sizes[:4])
+        m = {"n": n, "ne": ne, "nnz": nnz, "n_cyl": ncyl, "max_level": int(sizes[4]),
+             "min_angle_deg": sizes[5] / 1.0e6,
+             "x": np.zeros(n), "y": np.zeros(n), "bc": np.zeros(n, np.int32), "ev": np.zeros(3 * ne, np.int32),
+             "rp": np.zeros(n + 1, np.int32), "ci": np.zeros(nnz, np.int32), "cyl": np.zeros(3 * ncyl, np.int32)}
+        host.cfa_mesh_arrays(h, _dptr(m["x"]), _dptr(m["y"]), _iptr(m["bc"]), _iptr(m["ev"]), _iptr(m["rp"]),
+                             _iptr(m["ci"]), _iptr(m["cyl"]))
+        ab = np.zeros(4, np.int32)
+        w = np.zeros(2)
+        host.cfa_mesh_probes(h, _iptr(ab), _dptr(w))
+        m["probes"] = (ab, w)
+        return m
+    finally:
+        host.cfa_mesh_free(h)

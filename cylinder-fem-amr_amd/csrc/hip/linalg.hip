@@ -105,6 +105,101 @@ k_dense_matvec(int n, const double* __restrict__ A, const double* __restrict__ x
     if (i < n && lane == 0) y[i] = s;
 }
 
+// ---- the coarse end of the V-cycle in one workgroup ---------------------------------------------------
+// Levels of at most BOTTOM_ROWS rows, down to the dense coarse solve and back up, run as one
+// workgroup with barriers between the sweeps instead of one launch per sweep: on these levels a
+// sweep is a few hundred rows, far too small to fill the chip, and each launch would cost more than
+// its work. Same operations and buffers as the per-level launches of vcycle() below.
+constexpr int BOTTOM_MAX = 6;
+constexpr int BOTTOM_ROWS = 1536;
+constexpr int BOTTOM_THREADS = 1024;
+
+struct BottomArgs {
+    int nl, ss, coarse_n;
+    const double* coarse_inv;
+    double* cb;
+    double* cx;
+    cfa_amg_level_t lv[BOTTOM_MAX];
+};
+
+// y = x + omega dinv (b - A x); with from_zero, x = omega dinv b is formed on the fly
+__device__ __forceinline__ void bt_jacobi(const cfa_amg_level_t& L, const double* __restrict__ x,
+                                          const double* __restrict__ b, double* __restrict__ y, bool from_zero) {
+    for (int i = threadIdx.x; i < L.rows; i += blockDim.x) {
+        double s = 0.0;
+        for (int k = L.a_rp[i]; k < L.a_rp[i + 1]; ++k) {
+            const int j = L.a_ci[k];
+            s += L.a_v[k] * (from_zero ? L.omega * L.dinv[j] * b[j] : x[j]);
+        }
+        const double xi = from_zero ? L.omega * L.dinv[i] * b[i] : x[i];
+        y[i] = xi + L.omega * L.dinv[i] * (b[i] - s);
+    }
+}
+
+__global__ void __launch_bounds__(BOTTOM_THREADS)
+k_vcycle_bottom(BottomArgs a, const int* __restrict__ state) {
+    if (halted(state)) return;
+    double* xa[BOTTOM_MAX];
+    double* xb[BOTTOM_MAX];
+    double* cb = a.cb;
+    for (int l = 0; l < a.nl; ++l) {   // down: pre-smoothing, residual, restriction
+        const cfa_amg_level_t& L = a.lv[l];
+        xa[l] = L.x;
+        xb[l] = L.ax;
+        if (a.ss >= 2) {
+            bt_jacobi(L, nullptr, L.b, xa[l], true);
+            __syncthreads();
+            for (int k = 2; k < a.ss; ++k) {
+                bt_jacobi(L, xa[l], L.b, xb[l], false);
+                __syncthreads();
+                double* t = xa[l]; xa[l] = xb[l]; xb[l] = t;
+            }
+        } else {
+            for (int i = threadIdx.x; i < L.rows; i += blockDim.x) xa[l][i] = L.omega * L.dinv[i] * L.b[i];
+            __syncthreads();
+        }
+        for (int i = threadIdx.x; i < L.rows; i += blockDim.x) {
+            double s = 0.0;
+            for (int k = L.a_rp[i]; k < L.a_rp[i + 1]; ++k) s += L.a_v[k] * xa[l][L.a_ci[k]];
+            L.r[i] = L.b[i] - s;
+        }
+        __syncthreads();
+        cb = l + 1 < a.nl ? a.lv[l + 1].b : a.cb;
+        for (int i = threadIdx.x; i < L.cols; i += blockDim.x) {
+            double s = 0.0;
+            for (int k = L.r_rp[i]; k < L.r_rp[i + 1]; ++k) s += L.r_v[k] * L.r[L.r_ci[k]];
+            cb[i] = s;
+        }
+        __syncthreads();
+    }
+    {   // dense coarse solve, one wavefront per row
+        const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
+        for (int i = w; i < a.coarse_n; i += nw) {
+            double s = 0.0;
+            for (int j = lane; j < a.coarse_n; j += WAVE) s += a.coarse_inv[(size_t)i * a.coarse_n + j] * a.cb[j];
+            s = wave_sum(s);
+            if (lane == 0) a.cx[i] = s;
+        }
+        __syncthreads();
+    }
+    const double* cx = a.cx;
+    for (int l = a.nl - 1; l >= 0; --l) {   // up: prolongation, post-smoothing
+        const cfa_amg_level_t& L = a.lv[l];
+        for (int i = threadIdx.x; i < L.rows; i += blockDim.x) {
+            double s = 0.0;
+            for (int k = L.p_rp[i]; k < L.p_rp[i + 1]; ++k) s += L.p_v[k] * cx[L.p_ci[k]];
+            xa[l][i] += s;
+        }
+        __syncthreads();
+        for (int k = 0; k < a.ss; ++k) {
+            bt_jacobi(L, xa[l], L.b, xb[l], false);
+            __syncthreads();
+            double* t = xa[l]; xa[l] = xb[l]; xb[l] = t;
+        }
+        cx = xa[l];
+    }
+}
+
 // ---- reductions ------------------------------------------------------------------------------------
 // partial[b] = sum over the block's grid-stride share of x . y
 __global__ void __launch_bounds__(BLOCK)
@@ -384,6 +479,229 @@ k_apply_BT_dot(int n, int nnz, const int* __restrict__ rp, const int* __restrict
     }
     contrib = block_sum(contrib, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = contrib;
+}
+
+// ======================================================================================================
+// PCG with the scalar reductions folded into the kernels that consume them
+// ======================================================================================================
+// A kernel that needs a scalar (beta, alpha, the residual test) sums its producer's block
+// partials itself, in the same fixed order in every block, so all blocks reach the same value
+// and the same decision without a separate one-workgroup kernel; only block 0 writes the state
+// words. The search direction alternates between two buffers so that the operator kernel can
+// form p_k = z_k + beta_{k-1} p_{k-1} on the fly while it applies the operator (the reference's
+// separate k_xpay, linalg.cuh:252). r.z partials alternate between two buffers as well: the
+// operator of iteration k reads r_k.z_k and r_{k-1}.z_{k-1}, while the preconditioner of the same
+// iteration writes r_{k+1}.z_{k+1} over the older one.
+enum { ST_CAPPED = 4 };
+
+struct PcgArgs {
+    int n, nnz, max_iter;
+    const int* rp; const int* ci; const double* val;
+    const double* oss; const double* inv_m; const int* tag; double* gx; double* gy;
+    double* x; double* r; double* z; double* q; double* p[2];
+    double* prz[2]; double* ppq; double* prr;
+    int nb_rz, nb_rr;
+    const double* inv_diag;
+    double* sc; int* st; int* status;
+};
+
+// state word read once per block (block-uniform early exits in kernels with barriers)
+__device__ __forceinline__ int block_state(const int* st, int* it_out) {
+    __shared__ int s_state, s_it;
+    if (threadIdx.x == 0) { s_state = st[0]; s_it = st[2]; }
+    __syncthreads();
+    if (it_out) *it_out = s_it;
+    return s_state;
+}
+
+// Operator of iteration k (parity h): p_k formed on the fly and stored to p[h]; q = Kp p_k, and
+// with OSS g = M^-1 B p_k for the second stage, else the p.q partials here. Iteration cap check.
+template <int TPR, bool OSS>
+__global__ void __launch_bounds__(BLOCK)
+k_fp_operator(PcgArgs a, int h) {
+    __shared__ double lds[BLOCK / WAVE];
+    int it;
+    if (block_state(a.st, &it) != ST_RUNNING) return;
+    if (it >= a.max_iter) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { a.st[0] = ST_CAPPED; publish(a.status, it, ST_CAPPED); }
+        return;
+    }
+    const bool first = it == 0;
+    double beta = 0.0;
+    if (!first) {
+        const double rz_new = sum_partials(a.prz[h], a.nb_rz, 1, lds);
+        const double rz_old = sum_partials(a.prz[1 - h], a.nb_rz, 1, lds);
+        beta = rz_new / (rz_old + 1.0e-30);
+    }
+    const double* __restrict__ z = a.z;
+    const double* __restrict__ po = a.p[1 - h];
+    double* __restrict__ pn = a.p[h];
+    const int* __restrict__ rp = a.rp;
+    const int* __restrict__ ci = a.ci;
+    const double* __restrict__ val = a.val;
+    const int groups = gridDim.x * (blockDim.x / TPR);
+    const int lane = threadIdx.x % TPR;
+    double contrib = 0.0;
+    for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < a.n; row += groups) {
+        double s = 0.0, sx = 0.0, sy = 0.0;
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
+            const int j = ci[k];
+            const double pj = first ? z[j] : fma(beta, po[j], z[j]);
+            s += val[k] * pj;
+            if (OSS) {
+                sx += a.oss[k] * pj;
+                sy += a.oss[a.nnz + k] * pj;
+            }
+        }
+        s = group_sum<TPR>(s);
+        if (OSS) {
+            sx = group_sum<TPR>(sx);
+            sy = group_sum<TPR>(sy);
+        }
+        if (lane == 0) {
+            const double pi = first ? z[row] : fma(beta, po[row], z[row]);
+            pn[row] = pi;
+            a.q[row] = s;
+            if (OSS) {
+                a.gx[row] = sx * a.inv_m[row];
+                a.gy[row] = sy * a.inv_m[row];
+            } else {
+                contrib += pi * s;
+            }
+        }
+    }
+    if (!OSS) {
+        contrib = block_sum(contrib, lds);
+        if (threadIdx.x == 0) a.ppq[blockIdx.x] = contrib;
+    }
+}
+
+// alpha = r_k.z_k / (p_k.q) with the breakdown guard, x += alpha p, r -= alpha q, r.r partials;
+// the completed-iteration counter advances here.
+__global__ void __launch_bounds__(BLOCK)
+k_fp_update(PcgArgs a, int h, int nb_pq) {
+    __shared__ double lds[BLOCK / WAVE];
+    int it;
+    if (block_state(a.st, &it) != ST_RUNNING) return;
+    const double pq = sum_partials(a.ppq, nb_pq, 1, lds);
+    if (fabs(pq) < 1.0e-30) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.st[0] = ST_STOPPED;
+            a.st[1] = it;
+            publish(a.status, it, ST_STOPPED);
+        }
+        return;
+    }
+    const double alpha = sum_partials(a.prz[h], a.nb_rz, 1, lds) / pq;
+    const double* __restrict__ p = a.p[h];
+    const double* __restrict__ q = a.q;
+    double* __restrict__ x = a.x;
+    double* __restrict__ r = a.r;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] + (-alpha) * q[i];
+        r[i] = ri;
+        acc += ri * ri;
+    }
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) {
+        a.prr[blockIdx.x] = acc;
+        if (blockIdx.x == 0) a.st[2] = it + 1;
+    }
+}
+
+// Residual test of the iteration (||r|| < tol ||b||, linalg.cuh:241-247) from the r.r
+// partials; returns true (in every thread of every block) when the solve goes on.
+__device__ __forceinline__ bool fp_residual_test(const double* prr, int nb_rr, double* sc, int* st, int* status,
+                                                 double* lds) {
+    const double r_norm = sqrt(sum_partials(prr, nb_rr, 1, lds));
+    const bool done = r_norm < sc[S_TOL] * sc[S_BNORM];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc[S_RNORM] = r_norm;
+        sc[S_REL] = r_norm / sc[S_BNORM];
+        const int it = st[2];
+        if (done) {
+            st[0] = ST_CONVERGED;
+            st[1] = it;
+        }
+        publish(status, it, done ? ST_CONVERGED : ST_RUNNING);
+    }
+    return !done;
+}
+
+// One-workgroup residual test, for preconditioners whose first kernel cannot take it
+__global__ void k_fp_check(const double* prr, int nb_rr, double* sc, int* st, int* status) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (block_state(st, nullptr) != ST_RUNNING) return;
+    (void)fp_residual_test(prr, nb_rr, sc, st, status, lds);
+}
+
+// First two V-cycle sweeps from x = 0 on level 0 (as k_jacobi_from_zero), preceded by the
+// residual test of the iteration.
+template <int TPR>
+__global__ void __launch_bounds__(BLOCK)
+k_jacobi_from_zero_check(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
+                         const double* __restrict__ val, const double* __restrict__ b,
+                         const double* __restrict__ dinv, double omega, double* __restrict__ y,
+                         const double* __restrict__ prr, int nb_rr, double* sc, int* st, int* status) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (block_state(st, nullptr) != ST_RUNNING) return;
+    if (!fp_residual_test(prr, nb_rr, sc, st, status, lds)) return;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = g / TPR, lane = g % TPR;
+    double s = 0.0;
+    if (row < rows)
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
+            const int j = ci[k];
+            s += val[k] * (omega * dinv[j] * b[j]);
+        }
+    s = group_sum<TPR>(s);
+    if (row >= rows || lane != 0) return;
+    const double x0 = omega * dinv[row] * b[row];
+    y[row] = x0 + omega * dinv[row] * (b[row] - s);
+}
+
+// Last V-cycle sweep on level 0 with the r.z partials of the new preconditioned residual:
+// y = x + omega dinv (b - A x), partial sum of b . y (b = r, y = z). Grid-stride, capped grid.
+template <int TPR>
+__global__ void __launch_bounds__(BLOCK)
+k_jacobi_dot(int rows, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ val,
+             const double* __restrict__ x, const double* __restrict__ b, const double* __restrict__ dinv,
+             double omega, double* __restrict__ y, double* __restrict__ part, const int* __restrict__ st) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (st && block_state(st, nullptr) != ST_RUNNING) return;
+    const int groups = gridDim.x * (blockDim.x / TPR);
+    const int lane = threadIdx.x % TPR;
+    double contrib = 0.0;
+    for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < rows; row += groups) {
+        double s = 0.0;
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) s += val[k] * x[ci[k]];
+        s = group_sum<TPR>(s);
+        if (lane == 0) {
+            const double yi = x[row] + omega * dinv[row] * (b[row] - s);
+            y[row] = yi;
+            contrib += b[row] * yi;
+        }
+    }
+    contrib = block_sum(contrib, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = contrib;
+}
+
+// Jacobi preconditioner z = D^-1 r with the r.z partials, optionally after the residual test
+__global__ void __launch_bounds__(BLOCK)
+k_fp_jacobi_prec(PcgArgs a, int h_next, int check) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (block_state(a.st, nullptr) != ST_RUNNING) return;
+    if (check && !fp_residual_test(a.prr, a.nb_rr, a.sc, a.st, a.status, lds)) return;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        const double zi = a.inv_diag[i] * a.r[i];
+        a.z[i] = zi;
+        acc += a.r[i] * zi;
+    }
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) a.prz[h_next][blockIdx.x] = acc;
 }
 
 // ======================================================================================================
@@ -759,22 +1077,99 @@ void jacobi_from_zero_launch(int tpr, int rows, const int* rp, const int* ci, co
     }
 }
 
+// Hooks of the fused PCG into the preconditioner: the iteration's residual test runs in the first
+// level-0 kernel, and the last level-0 sweep emits the r.z partials.
+struct PrecHooks {
+    const double* prr = nullptr;   // r.r partials of the iteration (residual test when set)
+    int nb_rr = 0;
+    double* sc = nullptr;
+    int* st_w = nullptr;
+    int* status = nullptr;
+    double* dot_part = nullptr;    // r.z partials out (when set)
+    int dot_blocks = 0;            // set: number of r.z partials written
+};
+
+template <int TPR>
+void jacobi_dot_launch_t(int rows, const int* rp, const int* ci, const double* val, const double* x, const double* b,
+                         const double* dinv, double omega, double* y, double* part, const int* st, hipStream_t s,
+                         int& nb) {
+    nb = std::min(blocks_for((long long)rows * TPR), RED_BLOCKS);
+    hipLaunchKernelGGL(k_jacobi_dot<TPR>, dim3(nb), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, part, st);
+}
+
+int jacobi_dot_launch(int tpr, int rows, const int* rp, const int* ci, const double* val, const double* x,
+                      const double* b, const double* dinv, double omega, double* y, double* part, const int* st,
+                      hipStream_t s) {
+    int nb = 0;
+    switch (tpr) {
+        case 1: jacobi_dot_launch_t<1>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
+        case 2: jacobi_dot_launch_t<2>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
+        case 4: jacobi_dot_launch_t<4>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
+        case 8: jacobi_dot_launch_t<8>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
+        case 16: jacobi_dot_launch_t<16>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
+        case 32: jacobi_dot_launch_t<32>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
+        default: jacobi_dot_launch_t<64>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
+    }
+    return nb;
+}
+
+void jacobi_from_zero_check_launch(int tpr, int rows, const int* rp, const int* ci, const double* val,
+                                   const double* b, const double* dinv, double omega, double* y,
+                                   const PrecHooks& hk, hipStream_t s) {
+    const int blocks = blocks_for((long long)rows * tpr);
+#define CFA_JFZC(T) hipLaunchKernelGGL(k_jacobi_from_zero_check<T>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, \
+                                       b, dinv, omega, y, hk.prr, hk.nb_rr, hk.sc, hk.st_w, hk.status)
+    switch (tpr) {
+        case 1: CFA_JFZC(1); break;
+        case 2: CFA_JFZC(2); break;
+        case 4: CFA_JFZC(4); break;
+        case 8: CFA_JFZC(8); break;
+        case 16: CFA_JFZC(16); break;
+        case 32: CFA_JFZC(32); break;
+        default: CFA_JFZC(64); break;
+    }
+#undef CFA_JFZC
+}
+
 // One V-cycle (src/cfa/linalg.cuh:1184-1206) for level l with right-hand side b. Each level's
 // x / ax buffers form a ping-pong pair; the function returns the buffer holding the level's
 // iterate. On level 0 the last smoothing step writes straight into `out` when given.
-double* vcycle(const cfa_amg_t* amg, int l, const double* b, double* out, const int* st, hipStream_t s) {
+double* vcycle(const cfa_amg_t* amg, int l, const double* b, double* out, const int* st, hipStream_t s,
+               PrecHooks* hk = nullptr) {
     const cfa_amg_level_t& L = amg->levels[l];
     const int n = L.rows;
     const int ss = amg->smooth_steps;
     double* xa = L.x;
     double* xb = L.ax;
+    if (l >= 1 && n <= BOTTOM_ROWS && amg->n_levels - l <= BOTTOM_MAX) {
+        BottomArgs ba;
+        std::memset(&ba, 0, sizeof(ba));
+        ba.nl = amg->n_levels - l;
+        ba.ss = ss;
+        ba.coarse_n = amg->coarse_n;
+        ba.coarse_inv = amg->coarse_inv;
+        ba.cb = amg->cb;
+        ba.cx = amg->cx;
+        for (int k = 0; k < ba.nl; ++k) ba.lv[k] = amg->levels[l + k];
+        hipLaunchKernelGGL(k_vcycle_bottom, dim3(1), dim3(BOTTOM_THREADS), 0, s, ba, st);
+        // the buffer that holds the level's iterate after the same sequence of swaps
+        if (ss >= 2)
+            for (int k = 2; k < ss; ++k) std::swap(xa, xb);
+        for (int k = 0; k < ss; ++k) std::swap(xa, xb);
+        return xa;
+    }
+    const bool check_here = hk && hk->prr && l == 0;
     if (ss >= 2) {
-        jacobi_from_zero_launch(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, b, L.dinv, L.omega, xa, st, s);
+        if (check_here)
+            jacobi_from_zero_check_launch(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, b, L.dinv, L.omega, xa, *hk, s);
+        else
+            jacobi_from_zero_launch(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, b, L.dinv, L.omega, xa, st, s);
         for (int k = 2; k < ss; ++k) {
             csr_op_launch<OP_JACOBI>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, xb, st, s);
             std::swap(xa, xb);
         }
     } else {   // x = omega dinv b is always taken, also for fewer than one smoothing step
+        if (check_here) hipLaunchKernelGGL(k_fp_check, dim3(1), dim3(BLOCK), 0, s, hk->prr, hk->nb_rr, hk->sc, hk->st_w, hk->status);
         hipLaunchKernelGGL(k_jacobi_init, dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, L.omega, L.dinv, b, xa, st);
     }
     csr_op_launch<OP_RESID>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, nullptr, 0.0, L.r, st, s);
@@ -792,22 +1187,35 @@ double* vcycle(const cfa_amg_t* amg, int l, const double* b, double* out, const 
     csr_op_launch<OP_ADD>(L.tpr_p, n, L.p_rp, L.p_ci, L.p_v, cx, nullptr, nullptr, 0.0, xa, st, s);
     for (int k = 0; k < ss; ++k) {
         double* dst = (out && k == ss - 1) ? out : xb;
-        csr_op_launch<OP_JACOBI>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, dst, st, s);
+        if (dst == out && hk && hk->dot_part && l == 0)
+            hk->dot_blocks = jacobi_dot_launch(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, dst,
+                                               hk->dot_part, st, s);
+        else
+            csr_op_launch<OP_JACOBI>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, dst, st, s);
         if (dst == out) return out;
         std::swap(xa, xb);
     }
     return xa;
 }
 
-void amg_apply_enqueue(const cfa_amg_t* amg, const double* r, double* z, const int* st, hipStream_t s) {
+// z = V-cycle(r); with hooks, the residual test first and the r.z partials last
+void amg_apply_enqueue(const cfa_amg_t* amg, const double* r, double* z, const int* st, hipStream_t s,
+                       PrecHooks* hk = nullptr) {
+    double* res = z;
     if (amg->n_levels == 0) {
+        if (hk && hk->prr)
+            hipLaunchKernelGGL(k_fp_check, dim3(1), dim3(BLOCK), 0, s, hk->prr, hk->nb_rr, hk->sc, hk->st_w, hk->status);
         hipLaunchKernelGGL(k_dense_matvec, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
                            amg->coarse_n, amg->coarse_inv, r, z, st);
-        return;
+    } else {
+        res = vcycle(amg, 0, r, z, st, s, hk);
     }
-    double* res = vcycle(amg, 0, r, z, st, s);
-    if (res != z) hipLaunchKernelGGL(k_copy, dim3(blocks_for(amg->levels[0].rows)), dim3(BLOCK), 0, s,
-                                     amg->levels[0].rows, res, z, st);
+    const int n = amg->n_levels ? amg->levels[0].rows : amg->coarse_n;
+    if (res != z) hipLaunchKernelGGL(k_copy, dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, res, z, st);
+    if (hk && hk->dot_part && (amg->n_levels == 0 || res != z || amg->smooth_steps < 1)) {
+        hk->dot_blocks = red_blocks(n);
+        hipLaunchKernelGGL(k_dot_partial, dim3(hk->dot_blocks), dim3(BLOCK), 0, s, n, r, z, hk->dot_part, st);
+    }
 }
 
 template <int NRHS, typename T>
@@ -1020,6 +1428,167 @@ int cfa_amg_apply(const cfa_amg_t* amg, const double* r, double* z, cfa_stream_t
     return launch_status();
 }
 
+}  // extern "C"
+
+namespace {
+
+// PCG with the reductions folded into the consuming kernels (k_fp_*), two iterations per
+// captured graph (the search direction and the r.z partials alternate between two buffers).
+// Same algorithm, guards and stopping rules as the reference (src/cfa/linalg.cuh:183-256).
+int pcg_fused(int n, const int* rp, const int* ci, const double* val, const double* b, double* x, int max_iter,
+              double tol, const cfa_oss_op_t* oss, const cfa_amg_t* amg, cfa_krylov_ws_t* ws,
+              cfa_solve_result_t* result, hipStream_t s) {
+    constexpr int TPR = 4;
+    PcgArgs a;
+    std::memset(&a, 0, sizeof(a));   // the graph key compares the bytes, padding included
+    a.n = n;
+    a.max_iter = max_iter;
+    a.rp = rp; a.ci = ci; a.val = val;
+    a.r = ws->vec[0]; a.z = ws->vec[1]; a.p[0] = ws->vec[2]; a.q = ws->vec[3]; a.p[1] = ws->vec[5];
+    double* inv_diag = ws->vec[4];
+    a.inv_diag = inv_diag;
+    a.x = x;
+    a.prz[0] = ws->part; a.prz[1] = ws->part + RED_BLOCKS; a.ppq = ws->part + 2 * RED_BLOCKS;
+    a.prr = ws->part + 3 * RED_BLOCKS;
+    double* scratch = ws->part + 4 * RED_BLOCKS;   // prologue partials (up to OP_BLOCKS)
+    a.sc = ws->scal; a.st = state_words(ws); a.status = ws->status_dev;
+    if (oss) {
+        CFA_TRY(hipMemcpyAsync(&a.nnz, rp + n, sizeof(int), hipMemcpyDeviceToHost, s));
+        CFA_TRY(hipStreamSynchronize(s));
+        a.oss = oss->oss_val; a.inv_m = oss->inv_m; a.tag = oss->bc_tag; a.gx = oss->gx; a.gy = oss->gy;
+    }
+    const double init_sc[2] = {tol, -1.0};   // S_TOL, and S_REL "not computed"
+    CFA_TRY(hipMemcpyAsync(a.sc + S_TOL, &init_sc[0], sizeof(double), hipMemcpyHostToDevice, s));
+    CFA_TRY(hipMemcpyAsync(a.sc + S_REL, &init_sc[1], sizeof(double), hipMemcpyHostToDevice, s));
+    ws->status_host[0] = 0;
+    ws->status_host[1] = ST_RUNNING;
+    const int nb = red_blocks(n);
+    const int blocks = blocks_for(n);
+    const int blocks_tpr = std::min(blocks_for((long long)n * TPR), OP_BLOCKS);
+    const int nb_op = std::min(blocks_for((long long)n * TPR), RED_BLOCKS);
+    const int* st_r = a.st;
+
+    // prologue (linalg.cuh:205-227): ||b||, x = 0 when trivial, r = b - A x, initial test, z = M^-1 r
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, s, n, b, b, scratch, nullptr);
+    hipLaunchKernelGGL(k_pcg_init_bnorm, dim3(1), dim3(BLOCK), 0, s, nb, scratch, a.sc, a.st, a.status);
+    hipLaunchKernelGGL(k_zero_if_trivial, dim3(blocks), dim3(BLOCK), 0, s, n, x, st_r);
+    if (!amg) hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(BLOCK), 0, s, n, rp, ci, val, 1.0e-14, inv_diag);
+    if (oss) {
+        hipLaunchKernelGGL(k_apply_K_B<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, a.nnz, rp, ci, val, a.oss, a.inv_m,
+                           x, a.q, a.gx, a.gy, st_r);
+        hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, a.nnz, rp, ci, a.oss, a.tag,
+                           a.gx, a.gy, a.q, a.q, scratch, st_r);
+    } else {
+        csr_op_launch<OP_SPMV>(TPR, n, rp, ci, val, x, nullptr, nullptr, 0.0, a.q, st_r, s);
+    }
+    hipLaunchKernelGGL(k_pcg_residual, dim3(nb), dim3(BLOCK), 0, s, n, b, a.q, a.r, scratch, st_r);
+    hipLaunchKernelGGL(k_pcg_init_rnorm, dim3(1), dim3(BLOCK), 0, s, nb, scratch, a.sc, a.st, a.status);
+    auto apply_prec = [&](int h_next, bool check, hipStream_t q_s) {
+        if (amg) {
+            PrecHooks hk;
+            if (check) {
+                hk.prr = a.prr; hk.nb_rr = nb; hk.sc = a.sc; hk.st_w = a.st; hk.status = a.status;
+            }
+            hk.dot_part = a.prz[h_next];
+            amg_apply_enqueue(amg, a.r, a.z, st_r, q_s, &hk);
+            return hk.dot_blocks;
+        }
+        hipLaunchKernelGGL(k_fp_jacobi_prec, dim3(nb), dim3(BLOCK), 0, q_s, a, h_next, check ? 1 : 0);
+        return nb;
+    };
+    a.nb_rz = apply_prec(0, false, s);
+    a.nb_rr = nb;
+    int rc = launch_status();
+    if (rc) return rc;
+
+    // iteration k with parity h (linalg.cuh:231-253)
+    auto iteration = [&](int h, hipStream_t q_s) {
+        if (oss) {
+            hipLaunchKernelGGL((k_fp_operator<TPR, true>), dim3(nb_op), dim3(BLOCK), 0, q_s, a, h);
+            hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(nb_op), dim3(BLOCK), 0, q_s, n, a.nnz, rp, ci, a.oss, a.tag,
+                               a.gx, a.gy, a.p[h], a.q, a.ppq, st_r);
+        } else {
+            hipLaunchKernelGGL((k_fp_operator<TPR, false>), dim3(nb_op), dim3(BLOCK), 0, q_s, a, h);
+        }
+        hipLaunchKernelGGL(k_fp_update, dim3(nb), dim3(BLOCK), 0, q_s, a, h, nb_op);
+        (void)apply_prec(1 - h, true, q_s);
+    };
+    auto two_iterations = [&](hipStream_t q_s) {
+        iteration(0, q_s);
+        iteration(1, q_s);
+    };
+    hipGraphExec_t graph = nullptr;
+    if (max_iter > 0) {
+        // graph key: every pointer, size and constant the launches bake in
+        KeyBuilder kb;
+        kb.add(a);
+        kb.add(oss != nullptr);
+        kb.add(amg != nullptr);
+        if (amg) {
+            kb.add(amg->n_levels); kb.add(amg->coarse_n); kb.add(amg->coarse_inv); kb.add(amg->cb); kb.add(amg->cx);
+            kb.add(amg->smooth_steps);
+            if (amg->n_levels > 0) kb.add_bytes(amg->levels, sizeof(cfa_amg_level_t) * (size_t)amg->n_levels);
+        }
+        graph = iteration_graph(ws, internal_of(ws)->pcg, kb.k, two_iterations);
+    }
+    for (int g = 0; 2 * g < max_iter; ++g) {
+        if (g >= 2) {
+            wait_progress(ws, 2 * g - 3, s);
+            if (ws->status_host[1] != ST_RUNNING) break;
+        }
+        if (graph) CFA_TRY(hipGraphLaunch(graph, s));
+        else two_iterations(s);
+    }
+    rc = launch_status();
+    if (rc) return rc;
+    int fin[3];
+    double rel;
+    CFA_TRY(hipMemcpyAsync(fin, a.st, sizeof(fin), hipMemcpyDeviceToHost, s));
+    CFA_TRY(hipMemcpyAsync(&rel, a.sc + S_REL, sizeof(double), hipMemcpyDeviceToHost, s));
+    CFA_TRY(hipStreamSynchronize(s));
+    if (fin[0] == ST_RUNNING || fin[0] == ST_CAPPED) {   // iteration cap
+        result->iters[0] = max_iter;
+        result->converged[0] = 0;
+    } else {
+        result->iters[0] = fin[1];
+        result->converged[0] = fin[0] != ST_STOPPED;
+    }
+    result->rel_residual[0] = rel;
+    return launch_status();
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfa_amg_smooth(const cfa_amg_t* amg, int level, const double* x, const double* b, double* y,
+                   cfa_stream_t stream) {
+    if (!amg || !x || !b || !y || level < 0 || level >= amg->n_levels || !amg->levels) return CFA_E_ARG;
+    const cfa_amg_level_t& L = amg->levels[level];
+    csr_op_launch<OP_JACOBI>(L.tpr_a, L.rows, L.a_rp, L.a_ci, L.a_v, x, b, L.dinv, L.omega, y, nullptr, S(stream));
+    return launch_status();
+}
+
+int cfa_pcg_operator(int n, const int* rp, const int* ci, const double* val, const cfa_oss_op_t* oss,
+                     const double* in, double* out, double* part, int stage, cfa_stream_t stream) {
+    if (n < 0 || !rp || !ci || !val || !oss || !in || !out || !part || stage < 0 || stage > 2) return CFA_E_ARG;
+    if (!oss->oss_val || !oss->inv_m || !oss->bc_tag || !oss->gx || !oss->gy) return CFA_E_ARG;
+    if (n == 0) return CFA_OK;
+    hipStream_t s = S(stream);
+    int nnz = 0;
+    CFA_TRY(hipMemcpyAsync(&nnz, rp + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    CFA_TRY(hipStreamSynchronize(s));
+    constexpr int TPR = 4;
+    const int blocks_tpr = std::min(blocks_for((long long)n * TPR), OP_BLOCKS);
+    if (stage != 2)
+        hipLaunchKernelGGL(k_apply_K_B<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, nnz, rp, ci, val, oss->oss_val,
+                           oss->inv_m, in, out, oss->gx, oss->gy, nullptr);
+    if (stage != 1)
+        hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss->oss_val,
+                           oss->bc_tag, oss->gx, oss->gy, in, out, part, nullptr);
+    return launch_status();
+}
+
 int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double* b, double* x, int max_iter,
             double tol, const cfa_oss_op_t* oss, cfa_op_cb extra_cb, void* extra_ctx, const cfa_amg_t* amg,
             cfa_op_cb prec_cb, void* prec_ctx, cfa_krylov_ws_t* ws, cfa_solve_result_t* result,
@@ -1034,6 +1603,8 @@ int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double
     result->converged[0] = 1;
     result->rel_residual[0] = -1.0;   // no residual computed (trivial or initial convergence)
     if (n == 0) return CFA_OK;
+    if (!extra_cb && !prec_cb) return pcg_fused(n, rp, ci, val, b, x, max_iter, tol, oss, amg, ws, result, s);
+    // solves with host callbacks: the iteration as separate kernels with one-workgroup finals
     double* r = ws->vec[0];
     double* z = ws->vec[1];
     double* p = ws->vec[2];
@@ -1112,24 +1683,7 @@ int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double
         hipLaunchKernelGGL(k_pcg_final_beta, dim3(1), dim3(BLOCK), 0, q_s, nb, part, sc, st, status, 0);
         hipLaunchKernelGGL(k_pcg_p, dim3(blocks), dim3(BLOCK), 0, q_s, n, z, sc, p, st);
     };
-    hipGraphExec_t graph = nullptr;
-    if (!extra_cb && !prec_cb && max_iter > 0) {
-        // graph key: every pointer, size and constant the launches of one iteration bake in
-        KeyBuilder kb;
-        kb.add(n); kb.add(nnz); kb.add(rp); kb.add(ci); kb.add(val); kb.add(x);
-        for (int k = 0; k < 5; ++k) kb.add(ws->vec[k]);
-        kb.add(part); kb.add(sc); kb.add(status);
-        kb.add(oss != nullptr);
-        if (oss) kb.add(*oss);
-        kb.add(amg != nullptr);
-        if (amg) {
-            kb.add(amg->n_levels); kb.add(amg->coarse_n); kb.add(amg->coarse_inv); kb.add(amg->cb); kb.add(amg->cx);
-            kb.add(amg->smooth_steps);
-            if (amg->n_levels > 0) kb.add_bytes(amg->levels, sizeof(cfa_amg_level_t) * (size_t)amg->n_levels);
-        }
-        graph = iteration_graph(ws, internal_of(ws)->pcg, kb.k, iteration);
-    }
-    rc = run_iterations(ws, graph, max_iter, iteration, s);
+    rc = run_iterations(ws, nullptr, max_iter, iteration, s);
     if (rc) return rc;
     int fin[3];
     double rel;

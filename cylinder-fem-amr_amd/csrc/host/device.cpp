@@ -11,6 +11,11 @@
 
 namespace cfa {
 
+std::vector<KernelTime>& kernel_timings() {
+    static std::vector<KernelTime> k;
+    return k;
+}
+
 namespace {
 double wrap_angle(double a) {
     while (a <= -Sim::PI) a += 2.0 * Sim::PI;

@@ -6,6 +6,7 @@
 #pragma once
 
 #include <functional>
+#include <string>
 
 #include "amr.hpp"
 #include "gpu.hpp"
@@ -64,6 +65,35 @@ void set_initial_condition(const Mesh2D& mesh, std::vector<double>& h_u, std::ve
                            std::vector<double>& h_p, double inflow_peak = Sim::U_MAX);
 void print_amr_report(const char* tag, const AmrStats& st, const Mesh2D& mesh, const MeshQuality& q,
                       int proj_iters);
+
+// Kernel timings of the last --kernel-timing run or solver workload: name, algorithmic bytes per
+// launch, mean duration (us), launches timed.
+struct KernelTime {
+    std::string name;
+    double bytes, us;
+    int launches;
+};
+std::vector<KernelTime>& kernel_timings();
+
+// Mean duration of `launch` over `reps` launches between two HIP events on the solver stream,
+// after three untimed launches; microseconds.
+template <typename F>
+double time_launches(int reps, F launch) {
+    const cfa_stream_t s = solver_stream();
+    for (int k = 0; k < 3; ++k) launch();
+    void *t0 = nullptr, *t1 = nullptr;
+    GPU_CHECK(cfa_timer_create(&t0));
+    GPU_CHECK(cfa_timer_create(&t1));
+    GPU_CHECK(cfa_timer_record(t0, s));
+    for (int k = 0; k < reps; ++k) launch();
+    GPU_CHECK(cfa_timer_record(t1, s));
+    GPU_CHECK(cfa_stream_sync(s));
+    float ms = 0.0f;
+    GPU_CHECK(cfa_timer_elapsed_ms(t0, t1, &ms));
+    GPU_CHECK(cfa_timer_destroy(t0));
+    GPU_CHECK(cfa_timer_destroy(t1));
+    return 1000.0 * ms / reps;
+}
 
 struct RemeshProfile {
     int count = 0;

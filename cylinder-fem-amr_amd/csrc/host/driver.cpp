@@ -2,6 +2,7 @@
 // Steps A-D of the BDF2 rotational incremental projection, remeshing (synchronous or
 // overlapped with a host worker thread), outputs and the [Timing] report. All device work goes
 // through the C ABI of include/cfa_hip.h.
+#include <algorithm>
 #include <fstream>
 #include <future>
 #include <iomanip>
@@ -23,6 +24,11 @@ struct RunStats {
     int steps = 0, nodes = 0, elems = 0, nonconverged = 0;
     long long it_u = 0, it_v = 0, it_p = 0;
     double last_cd = 0, last_cl = 0, last_dp = 0;
+    double mms[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // sqrt of the six MMS integrals, cylinder/probe max, h_max
+    // one row per step: step, time, C_D, C_L, delta_p, nodes, elements, p_front, p_back,
+    // C_D_surface, C_L_surface, it_u, it_v, it_p, seconds since the loop started (end of the step)
+    static constexpr int HIST_COLS = 15;
+    std::vector<double> hist;
 };
 static RunStats g_stats;
 
@@ -85,6 +91,7 @@ int run_driver(int argc, char** argv) {
     RunOptions opt;
     if (!parse_options(argc, argv, opt)) return 0;
     g_stats = RunStats();
+    kernel_timings().clear();
     if (opt.outlet_dirichlet) {
         std::fprintf(stderr, "--outlet dirichlet is not available in this build\n");
         return EXIT_FAILURE;
@@ -346,6 +353,11 @@ int run_driver(int argc, char** argv) {
         g_stats.last_cd = C_D;
         g_stats.last_cl = C_L;
         g_stats.last_dp = delta_p;
+        const size_t hist_row = g_stats.hist.size();
+        for (double val : {(double)step, t_now, C_D, C_L, delta_p, (double)n_nodes, (double)n_elems, diag[2], diag[3],
+                           surface_scale * diag[4], surface_scale * diag[5], (double)it_u, (double)it_v, (double)it_p,
+                           0.0})
+            g_stats.hist.push_back(val);
         if (csv.is_open())
             csv << step << "," << t_now << "," << C_D << "," << C_L << "," << delta_p << "," << n_nodes << ","
                 << n_elems << "," << diag[2] << "," << diag[3] << "," << surface_scale * diag[4] << ","
@@ -519,6 +531,7 @@ int run_driver(int argc, char** argv) {
             pend.join_step = -1;
             finish_remesh(pend.st, pend.q, pend.old_vids, old_x, old_y);
         }
+        g_stats.hist[hist_row + RunStats::HIST_COLS - 1] = ms_since(wall_start) / 1000.0;
     }
     if (csv.is_open()) csv.close();
     dev_sync();
@@ -555,6 +568,10 @@ int run_driver(int argc, char** argv) {
                   << "[MMS] max |p-p_h| on cylinder nodes = " << cyl_max << " | at the DFG probes = " << probe_max
                   << "\n"
                   << std::fixed;
+        for (int k = 0; k < 6; ++k) g_stats.mms[k] = std::sqrt(e2[k]);
+        g_stats.mms[6] = cyl_max;
+        g_stats.mms[7] = probe_max;
+        g_stats.mms[8] = h_max;
         if (opt.write_csv) {
             std::ofstream f(opt.prefix + "_mms.csv");
             f << "nodes,elements,h_max,dt,t,err_u_L2,err_u_H1,err_p_L2,norm_u_L2,norm_u_H1,norm_p_L2,err_p_cyl_max,"
@@ -565,6 +582,55 @@ int run_driver(int argc, char** argv) {
             for (int k = 0; k < 6; ++k) f << "," << std::sqrt(e2[k]);
             f << "," << cyl_max << "," << probe_max << "\n";
         }
+    }
+    if (opt.kernel_timing > 0) {
+        // The main kernels of the step on the final mesh and state, each launched alone. Bytes
+        // count every array a kernel reads or writes once (n rows, nnz entries, ne elements).
+        const int R = opt.kernel_timing;
+        const double n = g.n_nodes, nnz = g.nnz, ne = g.n_elems;
+        DevArray<double> t_in, t_out, t_part;
+        t_in.reserve(g.n_nodes, 0);
+        t_out.reserve(g.n_nodes, 0);
+        t_part.reserve(4096, 0);
+        dev_copy(t_in, g.d_dp, g.n_nodes);
+        auto add = [](const char* name, double bytes, double us, int launches) {
+            kernel_timings().push_back({name, bytes, us, launches});
+        };
+        if (opt.pressure_stab > 0.0) {
+            const cfa_oss_op_t op = g.oss_op();
+            add("k_apply_K_B", 28.0 * nnz + 44.0 * n + 4.0, time_launches(R, [&] {
+                    GPU_CHECK(cfa_pcg_operator(g.n_nodes, g.d_row_ptr, g.d_col_ind, g.d_Kp_val, &op, t_in, t_out,
+                                               t_part, 1, s));
+                }), R);
+            add("k_apply_BT_dot", 20.0 * nnz + 48.0 * n + 4.0, time_launches(R, [&] {
+                    GPU_CHECK(cfa_pcg_operator(g.n_nodes, g.d_row_ptr, g.d_col_ind, g.d_Kp_val, &op, t_in, t_out,
+                                               t_part, 2, s));
+                }), R);
+        }
+        if (use_amg && !amg.levels.empty()) {
+            const double n0 = amg.levels[0]->A.rows, nnz0 = amg.levels[0]->A.nnz;
+            add("k_csr_op<jacobi> level 0", 12.0 * nnz0 + 36.0 * n0 + 4.0, time_launches(R, [&] {
+                    GPU_CHECK(cfa_amg_smooth(amg.view(), 0, t_in, g.d_rhs_p, t_out, s));
+                }), R);
+        }
+        const double t_end = opt.num_steps * dt;
+        const bool bdf = opt.bdf2 && opt.num_steps > 1;
+        add("k_momentum_rows", 12.0 * nnz + 92.0 * n + 60.0 * ne + (opt.mms ? 16.0 * n : 0.0), time_launches(R, [&] {
+                GPU_CHECK(cfa_assemble_momentum(g.view(), g.d_u_old, g.d_v_old, g.d_u_prev, g.d_v_prev, g.d_p, Sim::NU,
+                                                Sim::RHO, bdf ? 1.5 / dt : 1.0 / dt, bdf ? 2.0 / dt : 1.0 / dt,
+                                                bdf ? -0.5 / dt : 0.0, bdf ? 2.0 : 1.0, bdf ? -1.0 : 0.0,
+                                                opt.supg_tau_time > 0.0 ? opt.supg_tau_time : dt,
+                                                opt.skew_convection ? 0.5 : 0.0, opt.mms ? g.d_grad_dp_x.ptr : nullptr,
+                                                opt.mms ? g.d_grad_dp_y.ptr : nullptr, inflow_peak(t_end, opt.case_2d3),
+                                                opt.mms ? t_end : -1.0, g.d_A_val, g.d_rhs_u, g.d_rhs_v,
+                                                g.d_mass_lumped, g.d_u, g.d_v, s));
+            }), R);
+        std::cout << "[Timing] kernels on the final mesh (" << g.n_nodes << " nodes, nnz " << g.nnz << ", " << R
+                  << " launches each):";
+        for (const auto& k : kernel_timings())
+            std::cout << " " << k.name << " " << std::setprecision(2) << k.us << " us = " << std::setprecision(0)
+                      << k.bytes / (k.us * 1.0e3) << " GB/s;";
+        std::cout << "\n";
     }
     std::cout << std::fixed << std::setprecision(2) << "[Timing] " << wall_s << " s wall, " << amr_ms_total / 1000.0
               << " s in AMR (incl. transfer) | PPE AMG setup " << amg_setup_ms_total / 1000.0 << " s | AMG remaps "
@@ -619,6 +685,46 @@ void cfa_last_run_stats(double* out) {
                           (double)r.it_v, (double)r.it_p, r.last_cd, r.last_cl, r.last_dp};
     for (int k = 0; k < 15; ++k) out[k] = v[k];
     out[15] = 0.0;
+}
+
+// Per-step history of the last run (rows of 15 values, see RunStats::hist); copies at most
+// max_rows rows into out (may be NULL) and returns the number of rows recorded.
+int cfa_last_run_history(double* out, int max_rows) {
+    const auto& h = cfa::g_stats.hist;
+    const int rows = (int)(h.size() / cfa::RunStats::HIST_COLS);
+    if (out && max_rows > 0) {
+        const int k = std::min(rows, max_rows);
+        std::copy(h.begin(), h.begin() + (size_t)k * cfa::RunStats::HIST_COLS, out);
+    }
+    return rows;
+}
+
+// --kernel-timing figures of the last run: names (joined by '\n') into names[cap] and rows of
+// (algorithmic bytes per launch, mean microseconds, launches) into out; returns the kernel count.
+int cfa_last_run_kernels(char* names, int cap, double* out, int max_rows) {
+    const auto& ks = cfa::kernel_timings();
+    std::string joined;
+    for (size_t k = 0; k < ks.size(); ++k) {
+        if (k) joined += '\n';
+        joined += ks[k].name;
+        if ((int)k < max_rows && out) {
+            out[3 * k] = ks[k].bytes;
+            out[3 * k + 1] = ks[k].us;
+            out[3 * k + 2] = ks[k].launches;
+        }
+    }
+    if (names && cap > 0) {
+        const size_t len = std::min(joined.size(), (size_t)cap - 1);
+        std::copy(joined.begin(), joined.begin() + len, names);
+        names[len] = '\0';
+    }
+    return (int)ks.size();
+}
+
+// MMS figures of the last --mms run: out[9] = ||u-u_h||_L2, |u-u_h|_H1, ||p-p_h||_L2, ||u||_L2,
+// |u|_H1, ||p||_L2, max |p-p_h| on the cylinder, at the probes, h_max
+void cfa_last_run_mms(double* out) {
+    for (int k = 0; k < 9; ++k) out[k] = cfa::g_stats.mms[k];
 }
 
 }  // extern "C"

@@ -65,7 +65,9 @@ void print_usage(const char* prog) {
         << "  --ppe-tol T         pressure PCG relative tolerance               (" << d.ppe_tol << ")\n"
         << "  --proj-tol T        re-projection PCG relative tolerance          (" << d.proj_tol << ")\n"
         << "  --precision fp64|mixed                  momentum ILU factors in FP64 or FP32 (fp64)\n"
-        << "  --quiet             no per-step console lines\n";
+        << "  --quiet             no per-step console lines\n"
+        << "  --no-csv            write no CSV files (the history stays available in-process)\n"
+        << "  --kernel-timing R   after the run, time the main kernels R launches each (HIP events)\n";
 }
 
 static double parse_double(const char* s, const char* name) {
@@ -165,6 +167,8 @@ bool parse_options(int argc, char** argv, RunOptions& opt) {
         else if (a == "--proj-tol") opt.proj_tol = parse_double(value(), "--proj-tol");
         else if (a == "--precision") choice("--precision", "mixed", "fp64", opt.mixed);
         else if (a == "--quiet") opt.quiet = true;
+        else if (a == "--no-csv") opt.write_csv = false;
+        else if (a == "--kernel-timing") opt.kernel_timing = parse_int(value(), "--kernel-timing");
         else if (a.rfind("--", 0) == 0) {
             std::fprintf(stderr, "Unknown option %s (see --help)\n", a.c_str());
             std::exit(EXIT_FAILURE);
@@ -188,6 +192,7 @@ bool parse_options(int argc, char** argv, RunOptions& opt) {
     };
     require(opt.num_steps > 0, "--steps must be > 0");
     require(opt.dt > 0.0, "--dt must be > 0");
+    require(opt.kernel_timing >= 0, "--kernel-timing must be >= 0");
     require(opt.amr_interval >= 0 && opt.init_amr >= 0, "AMR intervals must be >= 0");
     require(opt.amr.max_level >= 0 && opt.amr.max_level <= 12, "--max-level must be in [0, 12]");
     require(opt.amr.tol_vort > 0.0 || opt.amr.tol_interp > 0.0,

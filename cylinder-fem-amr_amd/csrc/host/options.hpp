@@ -47,6 +47,7 @@ struct RunOptions {
     bool mixed = false;         // FP32 ILU factors inside the FP64 momentum BiCGSTAB
     bool quiet = false;         // no per-step console lines
     bool write_csv = true;
+    int kernel_timing = 0;      // > 0: time the main kernels this many launches each on the final state
 };
 
 void print_usage(const char* prog);

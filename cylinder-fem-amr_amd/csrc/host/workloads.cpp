@@ -1,0 +1,216 @@
+// Isolated solver workloads on a frozen uniform mesh: the pressure Poisson system with SA-AMG PCG
+// and the coupled momentum system with multicolour-ILU(0) BiCGSTAB (FP64 and FP32 factors). The
+// systems are assembled by the step's own kernels (src/cfa/fem_kernels.cuh
+// k_assemble_pressure_poisson with stab = 0 and apply_bc_pressure; k_assemble_momentum_fem with
+// k_apply_bc_momentum) and solved by the step's own solver classes, so timings per right-hand
+// side and per Krylov iteration measure exactly the path's solvers at a chosen mesh size.
+#include <cmath>
+#include <random>
+
+#include "device.hpp"
+
+namespace cfa {
+namespace {
+
+Mesh2D uniform_mesh(int levels) {
+    Mesh2D m;
+    generate_cylinder_mesh(m, 8, 7, 28);
+    finalize_mesh(m);
+    std::vector<double> z(m.num_nodes, 0.0), u = z, v = z, p = z, uo = z, vo = z;
+    AdaptiveMesh amr(m, z, z, z);
+    amr.extract(m, u, v, p, uo, vo);
+    finalize_mesh(m);
+    if (levels > 0) {
+        amr.refine_uniform(levels);
+        amr.extract(m, u, v, p, uo, vo);
+        finalize_mesh(m);
+    }
+    return m;
+}
+
+double host_ms(Clock::time_point t0) {
+    dev_sync();
+    return ms_since(t0);
+}
+
+}  // namespace
+}  // namespace cfa
+
+using namespace cfa;
+
+extern "C" {
+
+// Pressure workload: P1 Laplacian of the PPE (stab = 0, outlet rows Dirichlet) on the uniform
+// level-`level` mesh, one SA-AMG set-up, then `nrhs` solves from x = 0 to relative residual `tol`
+// with right-hand sides drawn from mt19937_64(1000 + k) N(0,1), mean over the free rows removed,
+// Dirichlet rows zero. out[12] = n, nnz, ne, mesh s, AMG set-up ms, AMG levels, solves,
+// solve ms total, iterations total, ms per solve, us per iteration, max final relative residual.
+// With kernel_reps > 0 the SpMV and level-0 smoother are timed afterwards (kernel_timings()).
+int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, double* out) {
+    if (level < 0 || level > 7 || nrhs < 1 || !(tol > 0.0) || !out) return CFA_E_ARG;
+    GPU_CHECK(cfa_init(0));
+    kernel_timings().clear();
+    auto t0 = Clock::now();
+    const Mesh2D mesh = uniform_mesh(level);
+    const double mesh_s = ms_since(t0) / 1000.0;
+    const int n = mesh.num_nodes;
+    std::vector<double> zero(n, 0.0);
+    DeviceMeshBuffers g;
+    g.sync_from_host(mesh, zero, zero, zero, zero, zero);
+    const cfa_stream_t s = solver_stream();
+    GPU_CHECK(cfa_lumped_mass(g.view(), g.d_mass_lumped, s));
+    GPU_CHECK(cfa_assemble_pressure(g.view(), g.d_u, g.d_v, g.d_p, g.d_grad_dp_x, g.d_grad_dp_y, g.d_mass_lumped,
+                                    g.d_elem_s, Sim::DT_DEFAULT, Sim::RHO, 0.0, g.d_Kp_val, g.d_rhs_p, g.d_div_num,
+                                    g.d_dp, nullptr, g.d_inv_m, s));
+    AmgPreconditioner amg;
+    t0 = Clock::now();
+    amg.setup(n, g.nnz, g.d_row_ptr, g.d_col_ind, g.d_Kp_val);
+    const double setup_ms = host_ms(t0);
+    GpuLinearSolver solver;
+    double solve_ms = 0.0, worst = 0.0;
+    long long iters = 0;
+    std::vector<double> b(n);
+    for (int k = 0; k < nrhs; ++k) {
+        std::mt19937_64 rng(1000 + k);
+        std::normal_distribution<double> N(0.0, 1.0);
+        double mean = 0.0;
+        int free_rows = 0;
+        for (int i = 0; i < n; ++i) {
+            b[i] = N(rng);
+            if (mesh.bc_tag[i] != BC_OUTLET) { mean += b[i]; ++free_rows; }
+        }
+        mean /= std::max(free_rows, 1);
+        for (int i = 0; i < n; ++i) b[i] = mesh.bc_tag[i] == BC_OUTLET ? 0.0 : b[i] - mean;
+        g.d_rhs_p.upload(b);
+        dev_zero(g.d_dp, n * sizeof(double));
+        dev_sync();
+        t0 = Clock::now();
+        iters += solver.solve_pcg_fused(n, g.d_row_ptr, g.d_col_ind, g.d_Kp_val, g.d_rhs_p, g.d_dp, 4000, tol, nullptr,
+                                        &amg);
+        solve_ms += host_ms(t0);
+        worst = std::max(worst, solver.last_rel_residual);
+    }
+    if (kernel_reps > 0) {
+        DevArray<double> t_out;
+        t_out.reserve(n);
+        const double nnz = g.nnz, nn = n;
+        kernel_timings().push_back({"k_spmv_multi<8,1>", 12.0 * nnz + 20.0 * nn + 4.0, time_launches(kernel_reps, [&] {
+            GPU_CHECK(cfa_spmv(n, g.d_row_ptr, g.d_col_ind, g.d_Kp_val, g.d_dp, t_out, 1, s));
+        }), kernel_reps});
+        if (!amg.levels.empty()) {
+            const double n0 = amg.levels[0]->A.rows, nnz0 = amg.levels[0]->A.nnz;
+            kernel_timings().push_back({"k_csr_op<jacobi> level 0", 12.0 * nnz0 + 36.0 * n0 + 4.0,
+                                        time_launches(kernel_reps, [&] {
+                GPU_CHECK(cfa_amg_smooth(amg.view(), 0, g.d_dp, g.d_rhs_p, t_out, s));
+            }), kernel_reps});
+        }
+    }
+    const double v[12] = {(double)n, (double)g.nnz, (double)g.n_elems, mesh_s, setup_ms, (double)amg.num_levels(),
+                          (double)nrhs, solve_ms, (double)iters, solve_ms / nrhs,
+                          iters ? 1000.0 * solve_ms / iters : 0.0, worst};
+    for (int k = 0; k < 12; ++k) out[k] = v[k];
+    return CFA_OK;
+}
+
+// Momentum workload: the coupled BDF2 momentum matrix (dt 1e-3, nu 1e-3, SUPG time scale 1e-3,
+// skew 0.5, u^n = u^{n-1} = w, p = 0, inflow peak 1.5) for the smooth divergence-free field
+// w = curl psi, psi = sum_{k<16} a_k sin(pi m_k x / L) sin(pi l_k y / H) with m_k = 1 + k mod 4,
+// l_k = 1 + k / 4 and a_k ~ N(0,1) from mt19937_64(42), scaled to max |w| = 1.5 over the nodes.
+// BiCGSTAB2 from x = w to `tol`, once with FP64 and once with FP32 ILU(0) factors.
+// out[16] = n, nnz, ne, mesh s, colouring ms, colours, FP64: factor ms, solve ms, it_u, it_v;
+// FP32: factor ms, solve ms, it_u, it_v; max |x32 - x64| / max |x64|, us per FP64 iteration.
+int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out) {
+    if (level < 0 || level > 7 || !(tol > 0.0) || !out) return CFA_E_ARG;
+    GPU_CHECK(cfa_init(0));
+    kernel_timings().clear();
+    auto t0 = Clock::now();
+    const Mesh2D mesh = uniform_mesh(level);
+    const double mesh_s = ms_since(t0) / 1000.0;
+    const int n = mesh.num_nodes;
+    std::mt19937_64 rng(42);
+    std::normal_distribution<double> N(0.0, 1.0);
+    double amp[16];
+    for (double& a : amp) a = N(rng);
+    std::vector<double> wu(n), wv(n), zero(n, 0.0);
+    double wmax = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double dpx = 0.0, dpy = 0.0;
+        for (int k = 0; k < 16; ++k) {
+            const double kx = Sim::PI * (1 + k % 4) / Sim::L_CHANNEL, ky = Sim::PI * (1 + k / 4) / Sim::H_CHANNEL;
+            dpx += amp[k] * kx * std::cos(kx * mesh.x[i]) * std::sin(ky * mesh.y[i]);
+            dpy += amp[k] * ky * std::sin(kx * mesh.x[i]) * std::cos(ky * mesh.y[i]);
+        }
+        wu[i] = dpy;
+        wv[i] = -dpx;
+        wmax = std::max(wmax, std::hypot(wu[i], wv[i]));
+    }
+    for (int i = 0; i < n; ++i) {
+        wu[i] *= 1.5 / wmax;
+        wv[i] *= 1.5 / wmax;
+    }
+    DeviceMeshBuffers g;
+    g.sync_from_host(mesh, wu, wv, zero, wu, wv);
+    const cfa_stream_t s = solver_stream();
+    const double dt = Sim::DT_DEFAULT;
+    GPU_CHECK(cfa_assemble_momentum(g.view(), g.d_u_old, g.d_v_old, g.d_u_prev, g.d_v_prev, g.d_p, Sim::NU, Sim::RHO,
+                                    1.5 / dt, 2.0 / dt, -0.5 / dt, 2.0, -1.0, 1.0e-3, 0.5, nullptr, nullptr, Sim::U_MAX,
+                                    -1.0, g.d_A_val, g.d_rhs_u, g.d_rhs_v, g.d_mass_lumped, g.d_u, g.d_v, s));
+    DevArray<double> x0u, x0v;
+    x0u.reserve(n);
+    x0v.reserve(n);
+    dev_copy(x0u, g.d_u, n);
+    dev_copy(x0v, g.d_v, n);
+    MulticolourIlu0 ilu;
+    ilu.keep_fp32 = true;
+    t0 = Clock::now();
+    ilu.set_pattern(n, mesh.csr_row_ptr, mesh.csr_col_ind, g.d_row_ptr, g.d_col_ind);
+    const double colour_ms = host_ms(t0);
+    double res[2][4];
+    std::vector<double> x64, x32;
+    for (int prec = 0; prec < 2; ++prec) {
+        dev_copy(g.d_u, x0u, n);
+        dev_copy(g.d_v, x0v, n);
+        t0 = Clock::now();
+        ilu.factor(g.d_A_val);
+        const double factor_ms = host_ms(t0);
+        GpuBicgstab2 sv;
+        sv.precond_fp32 = prec == 1;
+        t0 = Clock::now();
+        sv.solve(n, g.d_row_ptr, g.d_col_ind, g.d_A_val, g.d_rhs_u, g.d_rhs_v, g.d_u, g.d_v, 150, tol, ilu);
+        const double solve_ms = host_ms(t0);
+        res[prec][0] = factor_ms;
+        res[prec][1] = solve_ms;
+        res[prec][2] = sv.iters[0];
+        res[prec][3] = sv.iters[1];
+        std::vector<double>& xs = prec ? x32 : x64;
+        xs = g.d_u.to_host(n);
+        const std::vector<double> xv = g.d_v.to_host(n);
+        xs.insert(xs.end(), xv.begin(), xv.end());
+    }
+    double diff = 0.0, scale = 0.0;
+    for (size_t i = 0; i < x64.size(); ++i) {
+        diff = std::max(diff, std::fabs(x32[i] - x64[i]));
+        scale = std::max(scale, std::fabs(x64[i]));
+    }
+    if (kernel_reps > 0) {
+        DevArray<double> t_in, t_out;
+        t_in.reserve(2 * (size_t)n);
+        t_out.reserve(2 * (size_t)n);
+        dev_copy(t_in, g.d_rhs_u, n);
+        dev_copy(t_in + n, g.d_rhs_v, n);
+        const double nnz = g.nnz, nn = n;
+        kernel_timings().push_back({"k_spmv_multi<8,2>", 12.0 * nnz + 36.0 * nn + 4.0, time_launches(kernel_reps, [&] {
+            GPU_CHECK(cfa_spmv(n, g.d_row_ptr, g.d_col_ind, g.d_A_val, t_in, t_out, 2, s));
+        }), kernel_reps});
+        kernel_timings().push_back({"ilu0 apply, 2 components (all colours)", 12.0 * nnz + 100.0 * nn,
+                                    time_launches(kernel_reps, [&] { ilu.apply(t_in, t_out, 2); }), kernel_reps});
+    }
+    const double its64 = res[0][2] + res[0][3];
+    const double v[16] = {(double)n, (double)g.nnz, (double)g.n_elems, mesh_s, colour_ms, (double)ilu.n_colours,
+                          res[0][0], res[0][1], res[0][2], res[0][3], res[1][0], res[1][1], res[1][2], res[1][3],
+                          scale > 0.0 ? diff / scale : 0.0, its64 > 0 ? 1000.0 * res[0][1] / std::max(res[0][2], res[0][3]) : 0.0};
+    for (int k = 0; k < 16; ++k) out[k] = v[k];
+    return CFA_OK;
+}
+
+}  // extern "C"

@@ -230,6 +230,11 @@ typedef struct {
 /* z = M^{-1} r, one V-cycle (src/cfa/linalg.cuh:1161-1169 AmgPreconditioner::apply). */
 int cfa_amg_apply(const cfa_amg_t* amg, const double* r, double* z, cfa_stream_t stream);
 
+/* One damped Jacobi step on level `level`: y = x + omega D^{-1} (b - A x), the smoother of the
+ * V-cycle (src/cfa/linalg.cuh:1177-1181 jacobi_step); one kernel. */
+int cfa_amg_smooth(const cfa_amg_t* amg, int level, const double* x, const double* b, double* y,
+                   cfa_stream_t stream);
+
 /* Host callback that enqueues out = op(in) (or out += op(in) for an extra operator) on the
  * stream; the generic form of the std::function arguments of GpuLinearSolver::solve_pcg. */
 typedef void (*cfa_op_cb)(const double* in, double* out, void* ctx);
@@ -269,6 +274,14 @@ int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double
             int max_iter, double tol, const cfa_oss_op_t* oss, cfa_op_cb extra_cb,
             void* extra_ctx, const cfa_amg_t* amg, cfa_op_cb prec_cb, void* prec_ctx,
             cfa_krylov_ws_t* ws, cfa_solve_result_t* result, cfa_stream_t stream);
+
+/* out = (Kp + OSS) in, the operator of the fused PCG iteration (CSR SpMV of
+ * src/cfa/linalg.cuh:199-203 plus apply_oss_projection of src/cfa/device.cuh:310-323) in its
+ * two kernels: stage 1 computes out = Kp in and g = M^-1 B in (oss->gx, oss->gy), stage 2
+ * out -= B^T g on non-outlet rows and writes in.out block partials to part [4096];
+ * stage 0 runs both. */
+int cfa_pcg_operator(int n, const int* rp, const int* ci, const double* val, const cfa_oss_op_t* oss,
+                     const double* in, double* out, double* part, int stage, cfa_stream_t stream);
 
 /* Multicolour ILU(0) (src/cfa/linalg.cuh:361-568). colour_start is a host array. */
 typedef struct {

@@ -1,0 +1,89 @@
+/*
+ * C entry points of the host layer (libcfa_host.so): the command-line program run in-process,
+ * the figures and per-step history of the last run, and the host mesh / NVB / AMG set-up
+ * records for FFI callers (Python ctypes, tests, benchmarks).
+ *
+ * The reference reaches the same code directly from its main() and its tests
+ * (src/cylinder_fem_amr.cu:119-819, tests/test_cfa.cu); these functions are the C-ABI form of
+ * those call sites. None of them is on the per-iteration path.
+ */
+#ifndef CFA_HOST_H
+#define CFA_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Runs the command-line program in-process (src/cylinder_fem_amr.cu:119 main); argv[0] is the
+ * program name, the flags are those of src/cfa/options.cuh:54-289 plus --mom-tol, --ppe-tol,
+ * --proj-tol, --precision fp64|mixed, --quiet, --no-csv. Returns the exit status. */
+int cfa_main(int argc, char** argv);
+
+/* out[16] = wall s, momentum s, pressure s, AMR s, AMG set-up s, steps, nodes, elements,
+ * non-converged solves, sum of momentum u / v and pressure iterations, last C_D, C_L, delta_p, 0
+ * (the figures of the [Timing] lines, src/cylinder_fem_amr.cu:729-783). */
+void cfa_last_run_stats(double* out);
+
+/* Per-step history of the last run at full precision: rows of 15 values = step, time, C_D, C_L,
+ * delta_p, nodes, elements, p_front, p_back, C_D_surface, C_L_surface (the columns of the
+ * forces CSV, src/cylinder_fem_amr.cu:236,526-528), it_u, it_v, it_p, seconds since the time
+ * loop started at the end of the step. Copies at most max_rows rows (out may be NULL) and
+ * returns the number of rows recorded. */
+int cfa_last_run_history(double* out, int max_rows);
+
+/* Kernel timings of the last run with --kernel-timing R (or of the last solver workload): the
+ * main kernels launched alone R times each on the final mesh and state, timed with HIP events on
+ * the solver stream.
+ * Names joined by '\n' into names[cap]; rows of (algorithmic bytes per launch, mean
+ * microseconds per launch, launches timed) into out (at most max_rows rows). Returns the number
+ * of kernels timed. */
+int cfa_last_run_kernels(char* names, int cap, double* out, int max_rows);
+
+/* MMS figures of the last --mms run (src/cylinder_fem_amr.cu:731-783): out[9] = ||u-u_h||_L2,
+ * |u-u_h|_H1, ||p-p_h||_L2, ||u||_L2, |u|_H1, ||p||_L2, max |p-p_h| on the cylinder, at the
+ * probes, h_max. */
+void cfa_last_run_mms(double* out);
+
+/* Base mesh (src/cfa/mesh.cuh:83-193 with n_quarter 8, n_radial 7, n_wake_x 28) refined
+ * uniformly `levels` times by NVB (src/cfa/amr.cuh:461-471 refine_uniform), extracted in
+ * Morton order (amr.cuh:104-228) and finalised (CSR graph, cylinder edges, validation,
+ * src/cfa/mesh.cuh:198-376). Returns an opaque handle for the functions below. */
+void* cfa_mesh_uniform(int levels);
+void cfa_mesh_free(void* mesh);
+/* sizes[6] = nodes, elements, nnz, cylinder edges, max level, (int)(1e6 * min angle in degrees) */
+void cfa_mesh_sizes(void* mesh, int* sizes);
+/* Copies the arrays; any pointer may be NULL. x, y [n]; bc [n]; ev [3 ne]; rp [n + 1]; ci [nnz];
+ * cyl [3 n_cyl] = (a, b, owning element) per cylinder edge. */
+void cfa_mesh_arrays(void* mesh, double* x, double* y, int* bc, int* ev, int* rp, int* ci, int* cyl);
+/* Owner-computes node -> element table: inc_ptr [n + 1], inc_elem [3 ne], inc_pos [9 ne]. */
+void cfa_mesh_incidence(void* mesh, int* inc_ptr, int* inc_elem, int* inc_pos);
+/* Cylinder pressure probes at theta = pi and 0 (src/cfa/diagnostics.cuh:23-45):
+ * ab = {a_front, b_front, a_back, b_back}, w = {w_front, w_back}. */
+void cfa_mesh_probes(void* mesh, int* ab, double* w);
+
+/* Isolated solver workloads on a frozen uniform mesh (the reference's solvers at a chosen size;
+ * its momentum study harness scripts/momentum_study does the same on dumped systems).
+ * Pressure: the PPE Laplacian (src/cfa/fem_kernels.cuh:201-282 with stab = 0, :460-531 BCs), one
+ * SA-AMG set-up, nrhs seeded zero-mean right-hand sides (mt19937_64(1000 + k), N(0,1)) solved
+ * from 0 to tol. out[12] = n, nnz, ne, mesh s, AMG set-up ms, AMG levels, solves, solve ms,
+ * iterations, ms per solve, us per iteration, max final relative residual.
+ * Momentum: the BDF2 momentum matrix (fem_kernels.cuh:40-181, :411-453) for a seeded smooth
+ * divergence-free field, BiCGSTAB2 with FP64 and with FP32 ILU(0) factors to tol. out[16] = n,
+ * nnz, ne, mesh s, colouring ms, colours, FP64 factor ms, solve ms, it_u, it_v, FP32 factor ms,
+ * solve ms, it_u, it_v, max |x32 - x64| / max |x64|, us per FP64 iteration.
+ * kernel_reps > 0 also times the main kernels (read with cfa_last_run_kernels). */
+int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, double* out);
+int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out);
+
+/* Host SA-AMG set-up (src/cfa/linalg.cuh:995-1060, 1276-1431) of a host CSR matrix: per level
+ * rows, (nnz A, nnz P), Jacobi weight, the coarse size and the level-0 aggregates [n].
+ * Returns the number of smoothed levels (<= 16 reported). Needs a GPU (the hierarchy is
+ * uploaded as in the solver). */
+int cfa_amg_setup_record(int n, const int* rp, const int* ci, const double* val, int* level_rows,
+                         int* level_nnz, double* level_omega, int* coarse_n, int* agg0);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFA_HOST_H */

@@ -1,0 +1,23 @@
+"""Kernel-group parity through the C ABI against the oracle (tests/native/parity.cpp): assembly,
+BCs, OSS operator, correction, indicators bit for bit; forces, MMS integrals, SpMV, ILU sweeps,
+V-cycle, PCG and BiCGSTAB2 to the relative tolerances printed by the program. Meshes: uniform
+levels 1 and 3 of the reference's generator; fields seeded."""
+import subprocess
+from pathlib import Path
+
+import pytest
+
+BIN = Path(__file__).resolve().parent / "native" / "build" / "parity"
+CASES = ["momentum", "pressure", "correction", "forces", "indicator", "mms", "oss", "ilu", "spmv", "amg", "pcg",
+         "bicgstab"]
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("level", [1, 3])
+@pytest.mark.parametrize("case", CASES)
+def test_parity(case, level):
+    print(f"[case] parity {case} level {level}", flush=True)
+    r = subprocess.run([str(BIN), case, str(level)], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and "ALL PASSED" in r.stdout, r.stdout[-3000:] + r.stderr[-2000:]

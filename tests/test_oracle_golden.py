@@ -1,0 +1,59 @@
+"""The oracle (CPU restatement of the reference's step) against the reference's stored runs.
+
+Fixtures: runs/paper/mms_L1, mms_L2 of the reference (forces histories written with %.10g and
+the MMS error table), copied by tests/golden/make_golden.py. Tolerance per value: one unit of
+the tenth printed digit, or 1e-10 of the column's largest magnitude, whichever is larger (the
+path's tolerance is 1e-10 relative over the history; the reference's own run-to-run noise is
+1e-11 absolute on these columns, runs/review/mms_L3_dt2, which near a zero crossing exceeds the
+print resolution).
+"""
+import numpy as np
+import pytest
+
+import cfa_oracle
+from conftest import read_csv
+
+COLS = ("C_D", "C_L", "delta_p", "p_front", "p_back", "C_D_surface", "C_L_surface")
+
+
+def print_quantum(v):
+    """One unit in the tenth significant digit of a value printed with %.10g."""
+    a = np.abs(np.asarray(v, dtype=float))
+    e = np.floor(np.log10(np.where(a > 0, a, 1e-300)))
+    return np.where(a > 0, 10.0 ** (e - 9), 1e-300)
+
+
+def assert_matches_printed(ours, printed, what, rel=1e-10):
+    ours = np.asarray(ours, float)
+    printed = np.asarray(printed, float)
+    d = np.abs(ours - printed)
+    bound = np.maximum(print_quantum(printed), rel * np.max(np.abs(printed)))
+    bad = np.nonzero(d > bound)[0]
+    assert bad.size == 0, f"{what}: {bad.size} values off, first at {bad[0]}: {ours[bad[0]]!r} vs {printed[bad[0]]!r}"
+
+
+@pytest.mark.parametrize("level,steps,dt,min_level", [(1, 100, 1e-3, 0), (2, 200, 5e-4, 2)])
+def test_oracle_mms_history(golden, level, steps, dt, min_level):
+    rows, mms, _ = cfa_oracle.run_fixed(steps, dt, level, min_level, mms=True)
+    ref = read_csv(golden / f"mms_L{level}_forces.csv")
+    assert len(rows) == len(ref) == steps
+    idx = {c: i for i, c in enumerate(cfa_oracle.ROW_COLUMNS)}
+    np.testing.assert_array_equal(rows[:, idx["nodes"]], ref["nodes"])
+    np.testing.assert_array_equal(rows[:, idx["elements"]], ref["elements"])
+    for c in COLS:
+        assert_matches_printed(rows[:, idx[c]], ref[c], f"L{level} {c}")
+    table = read_csv(golden / f"mms_L{level}_mms.csv")
+    names = ("err_u_L2", "err_u_H1", "err_p_L2", "norm_u_L2", "norm_u_H1", "norm_p_L2", "err_p_cyl_max",
+             "err_p_probe_max", "h_max")
+    for k, c in enumerate(names):
+        assert_matches_printed([mms[k]], [float(table[c])], f"L{level} MMS {c}")
+
+
+def test_oracle_plants_are_caught(golden):
+    """A history with one drag value moved by 1e-8 of the column scale fails the comparison above."""
+    rows, _, _ = cfa_oracle.run_fixed(100, 1e-3, 1, 0, mms=True)
+    ref = read_csv(golden / "mms_L1_forces.csv")
+    bad = rows[:, 2].copy()
+    bad[37] += 1e-8 * np.max(np.abs(bad))
+    with pytest.raises(AssertionError):
+        assert_matches_printed(bad, ref["C_D"], "planted")

@@ -59,6 +59,26 @@ def cpu_baseline(seconds_target: float = 15.0) -> dict:
                       f"DFG inflow, dt 5e-4, tol 1e-8, {len(rows)} steps in {wall:.1f} s"}
 
 
+def trace_summary(path: str, reps: int, top: int = 25) -> int:
+    """Per-kernel dispatch statistics of a rocprofv3 kernel trace (CSV), and the mean duration of
+    the last `reps` dispatches of each kernel (the --kernel-timing pass, which launches each timed
+    kernel alone `reps` times after the run) for comparison with the HIP-event figures."""
+    import csv
+    from collections import defaultdict
+    durs = defaultdict(list)
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            durs[row["Kernel_Name"]].append(int(row["End_Timestamp"]) - int(row["Start_Timestamp"]))
+    total = sum(sum(v) for v in durs.values())
+    print(f"kernel dispatches {sum(len(v) for v in durs.values())}, busy {total / 1e6:.3f} ms")
+    print(f"{'total ms':>10} {'share':>6} {'calls':>8} {'mean us':>9} {'last-' + str(reps) + ' mean us':>16}  kernel")
+    for name, v in sorted(durs.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        tail = v[-reps:] if reps > 0 else v
+        print(f"{sum(v) / 1e6:10.3f} {100.0 * sum(v) / total:5.1f}% {len(v):8d} {sum(v) / len(v) / 1e3:9.2f} "
+              f"{sum(tail) / len(tail) / 1e3:16.2f}  {name[:120]}")
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,10 +90,12 @@ def main() -> int:
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch Krylov iterations kernel by kernel")
+    ap.add_argument("--trace-summary", metavar="CSV", help="summarise a rocprofv3 kernel trace and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        return trace_summary(args.trace_summary, args.kernel_reps)
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the path's sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
     if args.steps < 1 or args.warmup < 0:
         print("bench.py: --steps must be >= 1 and --warmup >= 0", file=sys.stderr)

@@ -68,6 +68,13 @@ __device__ __forceinline__ double group_sum(double v) {
     return v;
 }
 
+template <int TPR>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (int o = TPR / 2; o > 0; o >>= 1) v += __shfl_down(v, o, TPR);
+    return v;
+}
+
 // Row product sum_k val[k] x[ci[k]] over [s, e) with TPR lanes (lane = rank inside the group)
 template <int TPR>
 __device__ __forceinline__ double row_dot(const int* __restrict__ ci, const double* __restrict__ val,

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "cfa_device.hpp"
@@ -42,16 +43,17 @@ __device__ __forceinline__ void publish(int* status, int progress, int state) {
 // ---- CSR products --------------------------------------------------------------------------------
 enum CsrOp { OP_SPMV, OP_ADD, OP_RESID, OP_JACOBI };
 
-// y = A x | y += A x | y = b - A x | y = x + omega dinv (b - A x)   (src/cfa/linalg.cuh:884-901)
-template <int TPR, int OP>
+// y = A x | y += A x | y = b - A x | y = x + omega dinv (b - A x)   (src/cfa/linalg.cuh:884-901).
+// T = double, or float for the mixed-precision cycle (values, vectors and arithmetic in FP32).
+template <int TPR, int OP, typename T = double>
 __global__ void __launch_bounds__(BLOCK)
-k_csr_op(int rows, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ val,
-         const double* __restrict__ x, const double* __restrict__ b, const double* __restrict__ dinv,
-         double omega, double* __restrict__ y, const int* __restrict__ state) {
+k_csr_op(int rows, const int* __restrict__ rp, const int* __restrict__ ci, const T* __restrict__ val,
+         const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* __restrict__ y,
+         const int* __restrict__ state) {
     if (halted(state)) return;
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = g / TPR, lane = g % TPR;
-    double s = 0.0;
+    T s = 0;
     if (row < rows) {
         for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) s += val[k] * x[ci[k]];
     }
@@ -65,15 +67,15 @@ k_csr_op(int rows, const int* __restrict__ rp, const int* __restrict__ ci, const
 
 // First two steps of the V-cycle smoother fused: x0 = omega dinv b (src/cfa/linalg.cuh:864-868),
 // y = x0 + omega dinv (b - A x0), with x0 formed on the fly from b and dinv at the neighbours.
-template <int TPR>
+template <int TPR, typename T = double>
 __global__ void __launch_bounds__(BLOCK)
-k_jacobi_from_zero(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
-                   const double* __restrict__ val, const double* __restrict__ b, const double* __restrict__ dinv,
-                   double omega, double* __restrict__ y, const int* __restrict__ state) {
+k_jacobi_from_zero(int rows, const int* __restrict__ rp, const int* __restrict__ ci, const T* __restrict__ val,
+                   const T* __restrict__ b, const T* __restrict__ dinv, T omega, T* __restrict__ y,
+                   const int* __restrict__ state) {
     if (halted(state)) return;
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = g / TPR, lane = g % TPR;
-    double s = 0.0;
+    T s = 0;
     if (row < rows)
         for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
             const int j = ci[k];
@@ -81,29 +83,59 @@ k_jacobi_from_zero(int rows, const int* __restrict__ rp, const int* __restrict__
         }
     s = group_sum<TPR>(s);
     if (row >= rows || lane != 0) return;
-    const double x0 = omega * dinv[row] * b[row];
+    const T x0 = omega * dinv[row] * b[row];
     y[row] = x0 + omega * dinv[row] * (b[row] - s);
 }
 
-__global__ void k_jacobi_init(int n, double omega, const double* __restrict__ dinv, const double* __restrict__ b,
-                              double* __restrict__ x, const int* __restrict__ state) {
+template <typename T = double>
+__global__ void k_jacobi_init(int n, T omega, const T* __restrict__ dinv, const T* __restrict__ b, T* __restrict__ x,
+                              const int* __restrict__ state) {
     if (halted(state)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] = omega * dinv[i] * b[i];
 }
 
-// y = Ainv x with the dense coarse inverse, one wavefront per row (src/cfa/linalg.cuh:929-937)
+// y = Ainv x with the dense coarse inverse, one wavefront per row (src/cfa/linalg.cuh:929-937);
+// FP64 arithmetic also when x and y are FP32 (the coarse solve stays FP64 in the mixed cycle)
+template <typename T = double>
 __global__ void __launch_bounds__(BLOCK)
-k_dense_matvec(int n, const double* __restrict__ A, const double* __restrict__ x, double* __restrict__ y,
+k_dense_matvec(int n, const double* __restrict__ A, const T* __restrict__ x, T* __restrict__ y,
                const int* __restrict__ state) {
     if (halted(state)) return;
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
     double s = 0.0;
     if (i < n)
-        for (int j = lane; j < n; j += WAVE) s += A[(size_t)i * n + j] * x[j];
+        for (int j = lane; j < n; j += WAVE) s += A[(size_t)i * n + j] * (double)x[j];
     s = wave_sum(s);
-    if (i < n && lane == 0) y[i] = s;
+    if (i < n && lane == 0) y[i] = (T)s;
 }
+
+// Level data of the double or the float cycle
+template <typename T> struct Lv;
+template <> struct Lv<double> {
+    __host__ __device__ static const double* a(const cfa_amg_level_t& L) { return L.a_v; }
+    __host__ __device__ static const double* p(const cfa_amg_level_t& L) { return L.p_v; }
+    __host__ __device__ static const double* r(const cfa_amg_level_t& L) { return L.r_v; }
+    __host__ __device__ static const double* dinv(const cfa_amg_level_t& L) { return L.dinv; }
+    __host__ __device__ static double* x(const cfa_amg_level_t& L) { return L.x; }
+    __host__ __device__ static double* b(const cfa_amg_level_t& L) { return L.b; }
+    __host__ __device__ static double* res(const cfa_amg_level_t& L) { return L.r; }
+    __host__ __device__ static double* ax(const cfa_amg_level_t& L) { return L.ax; }
+    static double* cb(const cfa_amg_t* a) { return a->cb; }
+    static double* cx(const cfa_amg_t* a) { return a->cx; }
+};
+template <> struct Lv<float> {
+    __host__ __device__ static const float* a(const cfa_amg_level_t& L) { return L.a_v32; }
+    __host__ __device__ static const float* p(const cfa_amg_level_t& L) { return L.p_v32; }
+    __host__ __device__ static const float* r(const cfa_amg_level_t& L) { return L.r_v32; }
+    __host__ __device__ static const float* dinv(const cfa_amg_level_t& L) { return L.dinv32; }
+    __host__ __device__ static float* x(const cfa_amg_level_t& L) { return L.x32; }
+    __host__ __device__ static float* b(const cfa_amg_level_t& L) { return L.b32; }
+    __host__ __device__ static float* res(const cfa_amg_level_t& L) { return L.r32; }
+    __host__ __device__ static float* ax(const cfa_amg_level_t& L) { return L.ax32; }
+    static float* cb(const cfa_amg_t* a) { return a->cb32; }
+    static float* cx(const cfa_amg_t* a) { return a->cx32; }
+};
 
 // ---- the coarse end of the V-cycle in one workgroup ---------------------------------------------------
 // Levels of at most BOTTOM_ROWS rows, down to the dense coarse solve and back up, run as one
@@ -114,87 +146,103 @@ constexpr int BOTTOM_MAX = 6;
 constexpr int BOTTOM_ROWS = 1536;
 constexpr int BOTTOM_THREADS = 1024;
 
+template <typename T>
 struct BottomArgs {
     int nl, ss, coarse_n;
     const double* coarse_inv;
-    double* cb;
-    double* cx;
+    T* cb;
+    T* cx;
     cfa_amg_level_t lv[BOTTOM_MAX];
 };
 
 // y = x + omega dinv (b - A x); with from_zero, x = omega dinv b is formed on the fly
-__device__ __forceinline__ void bt_jacobi(const cfa_amg_level_t& L, const double* __restrict__ x,
-                                          const double* __restrict__ b, double* __restrict__ y, bool from_zero) {
+template <typename T>
+__device__ __forceinline__ void bt_jacobi(const cfa_amg_level_t& L, const T* __restrict__ x, const T* __restrict__ b,
+                                          T* __restrict__ y, bool from_zero) {
+    const int* __restrict__ rp = L.a_rp;
+    const int* __restrict__ ci = L.a_ci;
+    const T* __restrict__ av = Lv<T>::a(L);
+    const T* __restrict__ d = Lv<T>::dinv(L);
+    const T om = (T)L.omega;
     for (int i = threadIdx.x; i < L.rows; i += blockDim.x) {
-        double s = 0.0;
-        for (int k = L.a_rp[i]; k < L.a_rp[i + 1]; ++k) {
-            const int j = L.a_ci[k];
-            s += L.a_v[k] * (from_zero ? L.omega * L.dinv[j] * b[j] : x[j]);
+        T s = 0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int j = ci[k];
+            s += av[k] * (from_zero ? om * d[j] * b[j] : x[j]);
         }
-        const double xi = from_zero ? L.omega * L.dinv[i] * b[i] : x[i];
-        y[i] = xi + L.omega * L.dinv[i] * (b[i] - s);
+        const T xi = from_zero ? om * d[i] * b[i] : x[i];
+        y[i] = xi + om * d[i] * (b[i] - s);
     }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(BOTTOM_THREADS)
-k_vcycle_bottom(BottomArgs a, const int* __restrict__ state) {
+k_vcycle_bottom(BottomArgs<T> a, const int* __restrict__ state) {
     if (halted(state)) return;
-    double* xa[BOTTOM_MAX];
-    double* xb[BOTTOM_MAX];
-    double* cb = a.cb;
+    T* xa[BOTTOM_MAX];
+    T* xb[BOTTOM_MAX];
+    T* cb = a.cb;
     for (int l = 0; l < a.nl; ++l) {   // down: pre-smoothing, residual, restriction
         const cfa_amg_level_t& L = a.lv[l];
-        xa[l] = L.x;
-        xb[l] = L.ax;
+        const T* b = Lv<T>::b(L);
+        T* res = Lv<T>::res(L);
+        xa[l] = Lv<T>::x(L);
+        xb[l] = Lv<T>::ax(L);
         if (a.ss >= 2) {
-            bt_jacobi(L, nullptr, L.b, xa[l], true);
+            bt_jacobi<T>(L, nullptr, b, xa[l], true);
             __syncthreads();
             for (int k = 2; k < a.ss; ++k) {
-                bt_jacobi(L, xa[l], L.b, xb[l], false);
+                bt_jacobi<T>(L, xa[l], b, xb[l], false);
                 __syncthreads();
-                double* t = xa[l]; xa[l] = xb[l]; xb[l] = t;
+                T* t = xa[l]; xa[l] = xb[l]; xb[l] = t;
             }
         } else {
-            for (int i = threadIdx.x; i < L.rows; i += blockDim.x) xa[l][i] = L.omega * L.dinv[i] * L.b[i];
+            const T* d = Lv<T>::dinv(L);
+            for (int i = threadIdx.x; i < L.rows; i += blockDim.x) xa[l][i] = (T)L.omega * d[i] * b[i];
             __syncthreads();
         }
+        const T* av = Lv<T>::a(L);
         for (int i = threadIdx.x; i < L.rows; i += blockDim.x) {
-            double s = 0.0;
-            for (int k = L.a_rp[i]; k < L.a_rp[i + 1]; ++k) s += L.a_v[k] * xa[l][L.a_ci[k]];
-            L.r[i] = L.b[i] - s;
+            T s = 0;
+            for (int k = L.a_rp[i]; k < L.a_rp[i + 1]; ++k) s += av[k] * xa[l][L.a_ci[k]];
+            res[i] = b[i] - s;
         }
         __syncthreads();
-        cb = l + 1 < a.nl ? a.lv[l + 1].b : a.cb;
+        cb = l + 1 < a.nl ? Lv<T>::b(a.lv[l + 1]) : a.cb;
+        const T* rv = Lv<T>::r(L);
         for (int i = threadIdx.x; i < L.cols; i += blockDim.x) {
-            double s = 0.0;
-            for (int k = L.r_rp[i]; k < L.r_rp[i + 1]; ++k) s += L.r_v[k] * L.r[L.r_ci[k]];
+            T s = 0;
+            for (int k = L.r_rp[i]; k < L.r_rp[i + 1]; ++k) s += rv[k] * res[L.r_ci[k]];
             cb[i] = s;
         }
         __syncthreads();
     }
-    {   // dense coarse solve, one wavefront per row
+    {   // dense coarse solve in FP64, one wavefront per row
         const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
         for (int i = w; i < a.coarse_n; i += nw) {
             double s = 0.0;
-            for (int j = lane; j < a.coarse_n; j += WAVE) s += a.coarse_inv[(size_t)i * a.coarse_n + j] * a.cb[j];
+            for (int j = lane; j < a.coarse_n; j += WAVE)
+                s += a.coarse_inv[(size_t)i * a.coarse_n + j] * (double)a.cb[j];
             s = wave_sum(s);
-            if (lane == 0) a.cx[i] = s;
+            if (lane == 0) a.cx[i] = (T)s;
         }
         __syncthreads();
     }
-    const double* cx = a.cx;
+    const T* cx = a.cx;
     for (int l = a.nl - 1; l >= 0; --l) {   // up: prolongation, post-smoothing
         const cfa_amg_level_t& L = a.lv[l];
+        const T* pv = Lv<T>::p(L);
+        const T* b = Lv<T>::b(L);
         for (int i = threadIdx.x; i < L.rows; i += blockDim.x) {
-            double s = 0.0;
-            for (int k = L.p_rp[i]; k < L.p_rp[i + 1]; ++k) s += L.p_v[k] * cx[L.p_ci[k]];
+            T s = 0;
+            for (int k = L.p_rp[i]; k < L.p_rp[i + 1]; ++k) s += pv[k] * cx[L.p_ci[k]];
             xa[l][i] += s;
         }
         __syncthreads();
         for (int k = 0; k < a.ss; ++k) {
-            bt_jacobi(L, xa[l], L.b, xb[l], false);
+            bt_jacobi<T>(L, xa[l], b, xb[l], false);
             __syncthreads();
-            double* t = xa[l]; xa[l] = xb[l]; xb[l] = t;
+            T* t = xa[l]; xa[l] = xb[l]; xb[l] = t;
         }
         cx = xa[l];
     }
@@ -662,30 +710,50 @@ k_jacobi_from_zero_check(int rows, const int* __restrict__ rp, const int* __rest
     y[row] = x0 + omega * dinv[row] * (b[row] - s);
 }
 
-// Last V-cycle sweep on level 0 with the r.z partials of the new preconditioned residual:
-// y = x + omega dinv (b - A x), partial sum of b . y (b = r, y = z). Grid-stride, capped grid.
-template <int TPR>
+// Last V-cycle sweep on level 0: y = x + omega dinv (b - A x) written as FP64 (the
+// preconditioned residual z), with the r.z block partials when part is set (rdot = the FP64
+// residual r; b itself in the FP64 cycle). Grid-stride, capped grid.
+template <int TPR, typename T = double>
 __global__ void __launch_bounds__(BLOCK)
-k_jacobi_dot(int rows, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ val,
-             const double* __restrict__ x, const double* __restrict__ b, const double* __restrict__ dinv,
-             double omega, double* __restrict__ y, double* __restrict__ part, const int* __restrict__ st) {
+k_jacobi_dot(int rows, const int* __restrict__ rp, const int* __restrict__ ci, const T* __restrict__ val,
+             const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ dinv, T omega,
+             double* __restrict__ y, const double* __restrict__ rdot, double* __restrict__ part,
+             const int* __restrict__ st) {
     __shared__ double lds[BLOCK / WAVE];
     if (st && block_state(st, nullptr) != ST_RUNNING) return;
     const int groups = gridDim.x * (blockDim.x / TPR);
     const int lane = threadIdx.x % TPR;
     double contrib = 0.0;
     for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < rows; row += groups) {
-        double s = 0.0;
+        T s = 0;
         for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) s += val[k] * x[ci[k]];
         s = group_sum<TPR>(s);
         if (lane == 0) {
-            const double yi = x[row] + omega * dinv[row] * (b[row] - s);
+            const double yi = (double)(x[row] + omega * dinv[row] * (b[row] - s));
             y[row] = yi;
-            contrib += b[row] * yi;
+            if (part) contrib += rdot[row] * yi;
         }
     }
-    contrib = block_sum(contrib, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = contrib;
+    if (part) {
+        contrib = block_sum(contrib, lds);
+        if (threadIdx.x == 0) part[blockIdx.x] = contrib;
+    }
+}
+
+// Entry of the FP32 cycle: b32 = (float) r, after the iteration's residual test when prr is set
+__global__ void k_to_f32_check(int n, const double* __restrict__ r, float* __restrict__ b32,
+                               const double* __restrict__ prr, int nb_rr, double* sc, int* st, int* status) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (st) {
+        if (block_state(st, nullptr) != ST_RUNNING) return;
+        if (prr && !fp_residual_test(prr, nb_rr, sc, st, status, lds)) return;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b32[i] = (float)r[i];
+}
+
+__global__ void k_f64_to_f32(size_t n, const double* __restrict__ in, float* __restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
 }
 
 // Jacobi preconditioner z = D^-1 r with the r.z partials, optionally after the residual test
@@ -866,7 +934,7 @@ __global__ void k_bicg_final_init(int nb, const double* __restrict__ part, doubl
 }
 
 // Start of an iteration: breakdown check on rho, coefficients of the p update
-__global__ void k_bicg_pre(double* __restrict__ sc, int* __restrict__ st, int* __restrict__ status) {
+__device__ void bicg_pre_logic(double* __restrict__ sc, int* __restrict__ st, int* __restrict__ status) {
     if (st[0] != ST_RUNNING) return;
     const int it = st[10];
     for (int c = 0; c < 2; ++c) {
@@ -879,6 +947,10 @@ __global__ void k_bicg_pre(double* __restrict__ sc, int* __restrict__ st, int* _
         }
     }
     if (!(st[2] || st[3])) { st[0] = ST_STOPPED; publish(status, it, ST_STOPPED); }
+}
+
+__global__ void k_bicg_pre(double* __restrict__ sc, int* __restrict__ st, int* __restrict__ status) {
+    bicg_pre_logic(sc, st, status);
 }
 
 // p = r + cb (p - co v), from the second iteration on
@@ -1012,7 +1084,8 @@ k_bicg_xr(int n, const double* __restrict__ ph, const double* __restrict__ sh, c
     write_partials2(part, 2, acc, lds);
 }
 
-// end of an iteration: half-step stops, residual test, rho update, iteration cap
+// end of an iteration: half-step stops, residual test, rho update, iteration cap; then the start
+// of the next one (breakdown check on rho, p-update coefficients)
 __global__ void k_bicg_final_post(int nb, const double* __restrict__ part, double* __restrict__ sc,
                                   int* __restrict__ st, int* __restrict__ status) {
     __shared__ double lds[BLOCK / WAVE];
@@ -1039,6 +1112,7 @@ __global__ void k_bicg_final_post(int nb, const double* __restrict__ part, doubl
     const int state = (st[2] || st[3]) ? ST_RUNNING : ST_STOPPED;
     st[0] = state;
     publish(status, it + 1, state);
+    bicg_pre_logic(sc, st, status);   // start of the next iteration
 }
 
 // ======================================================================================================
@@ -1048,33 +1122,40 @@ hipStream_t S(cfa_stream_t s) { return as_stream(s); }
 
 #define CFA_TRY(expr) do { const hipError_t e_ = (expr); if (e_ != hipSuccess) return (int)e_; } while (0)
 
-template <int OP>
-void csr_op_launch(int tpr, int rows, const int* rp, const int* ci, const double* val, const double* x,
-                   const double* b, const double* dinv, double omega, double* y, const int* st, hipStream_t s) {
-    const int blocks = blocks_for((long long)rows * tpr);
+// Calls f with std::integral_constant<int, TPR> for the lanes-per-row value tpr (1..64)
+template <typename F>
+void tpr_dispatch(int tpr, F f) {
     switch (tpr) {
-        case 1: hipLaunchKernelGGL((k_csr_op<1, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
-        case 2: hipLaunchKernelGGL((k_csr_op<2, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
-        case 4: hipLaunchKernelGGL((k_csr_op<4, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
-        case 8: hipLaunchKernelGGL((k_csr_op<8, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
-        case 16: hipLaunchKernelGGL((k_csr_op<16, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
-        case 32: hipLaunchKernelGGL((k_csr_op<32, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
-        default: hipLaunchKernelGGL((k_csr_op<64, OP>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, st); break;
+        case 1: f(std::integral_constant<int, 1>{}); break;
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        case 16: f(std::integral_constant<int, 16>{}); break;
+        case 32: f(std::integral_constant<int, 32>{}); break;
+        default: f(std::integral_constant<int, 64>{}); break;
     }
 }
 
-void jacobi_from_zero_launch(int tpr, int rows, const int* rp, const int* ci, const double* val, const double* b,
-                             const double* dinv, double omega, double* y, const int* st, hipStream_t s) {
+template <int OP, typename T = double>
+void csr_op_launch(int tpr, int rows, const int* rp, const int* ci, const T* val, const T* x, const T* b,
+                   const T* dinv, double omega, T* y, const int* st, hipStream_t s) {
     const int blocks = blocks_for((long long)rows * tpr);
-    switch (tpr) {
-        case 1: hipLaunchKernelGGL(k_jacobi_from_zero<1>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
-        case 2: hipLaunchKernelGGL(k_jacobi_from_zero<2>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
-        case 4: hipLaunchKernelGGL(k_jacobi_from_zero<4>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
-        case 8: hipLaunchKernelGGL(k_jacobi_from_zero<8>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
-        case 16: hipLaunchKernelGGL(k_jacobi_from_zero<16>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
-        case 32: hipLaunchKernelGGL(k_jacobi_from_zero<32>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
-        default: hipLaunchKernelGGL(k_jacobi_from_zero<64>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv, omega, y, st); break;
-    }
+    tpr_dispatch(tpr, [&](auto c) {
+        constexpr int TPR = decltype(c)::value;
+        hipLaunchKernelGGL((k_csr_op<TPR, OP, T>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv,
+                           (T)omega, y, st);
+    });
+}
+
+template <typename T = double>
+void jacobi_from_zero_launch(int tpr, int rows, const int* rp, const int* ci, const T* val, const T* b, const T* dinv,
+                             double omega, T* y, const int* st, hipStream_t s) {
+    const int blocks = blocks_for((long long)rows * tpr);
+    tpr_dispatch(tpr, [&](auto c) {
+        constexpr int TPR = decltype(c)::value;
+        hipLaunchKernelGGL((k_jacobi_from_zero<TPR, T>), dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv,
+                           (T)omega, y, st);
+    });
 }
 
 // Hooks of the fused PCG into the preconditioner: the iteration's residual test runs in the first
@@ -1089,27 +1170,19 @@ struct PrecHooks {
     int dot_blocks = 0;            // set: number of r.z partials written
 };
 
-template <int TPR>
-void jacobi_dot_launch_t(int rows, const int* rp, const int* ci, const double* val, const double* x, const double* b,
-                         const double* dinv, double omega, double* y, double* part, const int* st, hipStream_t s,
-                         int& nb) {
-    nb = std::min(blocks_for((long long)rows * TPR), RED_BLOCKS);
-    hipLaunchKernelGGL(k_jacobi_dot<TPR>, dim3(nb), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv, omega, y, part, st);
-}
-
-int jacobi_dot_launch(int tpr, int rows, const int* rp, const int* ci, const double* val, const double* x,
-                      const double* b, const double* dinv, double omega, double* y, double* part, const int* st,
+// last level-0 sweep into the FP64 output, with the r.z partials when part is set; returns the
+// number of partials
+template <typename T = double>
+int jacobi_dot_launch(int tpr, int rows, const int* rp, const int* ci, const T* val, const T* x, const T* b,
+                      const T* dinv, double omega, double* y, const double* rdot, double* part, const int* st,
                       hipStream_t s) {
     int nb = 0;
-    switch (tpr) {
-        case 1: jacobi_dot_launch_t<1>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
-        case 2: jacobi_dot_launch_t<2>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
-        case 4: jacobi_dot_launch_t<4>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
-        case 8: jacobi_dot_launch_t<8>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
-        case 16: jacobi_dot_launch_t<16>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
-        case 32: jacobi_dot_launch_t<32>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
-        default: jacobi_dot_launch_t<64>(rows, rp, ci, val, x, b, dinv, omega, y, part, st, s, nb); break;
-    }
+    tpr_dispatch(tpr, [&](auto c) {
+        constexpr int TPR = decltype(c)::value;
+        nb = std::min(blocks_for((long long)rows * TPR), RED_BLOCKS);
+        hipLaunchKernelGGL((k_jacobi_dot<TPR, T>), dim3(nb), dim3(BLOCK), 0, s, rows, rp, ci, val, x, b, dinv,
+                           (T)omega, y, rdot, part, st);
+    });
     return nb;
 }
 
@@ -1117,102 +1190,130 @@ void jacobi_from_zero_check_launch(int tpr, int rows, const int* rp, const int* 
                                    const double* b, const double* dinv, double omega, double* y,
                                    const PrecHooks& hk, hipStream_t s) {
     const int blocks = blocks_for((long long)rows * tpr);
-#define CFA_JFZC(T) hipLaunchKernelGGL(k_jacobi_from_zero_check<T>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, \
-                                       b, dinv, omega, y, hk.prr, hk.nb_rr, hk.sc, hk.st_w, hk.status)
-    switch (tpr) {
-        case 1: CFA_JFZC(1); break;
-        case 2: CFA_JFZC(2); break;
-        case 4: CFA_JFZC(4); break;
-        case 8: CFA_JFZC(8); break;
-        case 16: CFA_JFZC(16); break;
-        case 32: CFA_JFZC(32); break;
-        default: CFA_JFZC(64); break;
-    }
-#undef CFA_JFZC
+    tpr_dispatch(tpr, [&](auto c) {
+        constexpr int TPR = decltype(c)::value;
+        hipLaunchKernelGGL(k_jacobi_from_zero_check<TPR>, dim3(blocks), dim3(BLOCK), 0, s, rows, rp, ci, val, b, dinv,
+                           omega, y, hk.prr, hk.nb_rr, hk.sc, hk.st_w, hk.status);
+    });
 }
 
-// One V-cycle (src/cfa/linalg.cuh:1184-1206) for level l with right-hand side b. Each level's
-// x / ax buffers form a ping-pong pair; the function returns the buffer holding the level's
-// iterate. On level 0 the last smoothing step writes straight into `out` when given.
-double* vcycle(const cfa_amg_t* amg, int l, const double* b, double* out, const int* st, hipStream_t s,
-               PrecHooks* hk = nullptr) {
+// One V-cycle (src/cfa/linalg.cuh:1184-1206) for level l with right-hand side b, in FP64 or FP32
+// (T). Each level's x / ax buffers form a ping-pong pair; the function returns the buffer holding
+// the level's iterate. On level 0 the last smoothing step writes straight into the FP64 `out`
+// when given (rdot: the FP64 residual for the r.z partials).
+template <typename T>
+T* vcycle(const cfa_amg_t* amg, int l, const T* b, double* out, const double* rdot, const int* st, hipStream_t s,
+          PrecHooks* hk = nullptr) {
     const cfa_amg_level_t& L = amg->levels[l];
     const int n = L.rows;
     const int ss = amg->smooth_steps;
-    double* xa = L.x;
-    double* xb = L.ax;
+    T* xa = Lv<T>::x(L);
+    T* xb = Lv<T>::ax(L);
+    const T* av = Lv<T>::a(L);
+    const T* dv = Lv<T>::dinv(L);
     if (l >= 1 && n <= BOTTOM_ROWS && amg->n_levels - l <= BOTTOM_MAX) {
-        BottomArgs ba;
+        BottomArgs<T> ba;
         std::memset(&ba, 0, sizeof(ba));
         ba.nl = amg->n_levels - l;
         ba.ss = ss;
         ba.coarse_n = amg->coarse_n;
         ba.coarse_inv = amg->coarse_inv;
-        ba.cb = amg->cb;
-        ba.cx = amg->cx;
+        ba.cb = Lv<T>::cb(amg);
+        ba.cx = Lv<T>::cx(amg);
         for (int k = 0; k < ba.nl; ++k) ba.lv[k] = amg->levels[l + k];
-        hipLaunchKernelGGL(k_vcycle_bottom, dim3(1), dim3(BOTTOM_THREADS), 0, s, ba, st);
+        hipLaunchKernelGGL(k_vcycle_bottom<T>, dim3(1), dim3(BOTTOM_THREADS), 0, s, ba, st);
         // the buffer that holds the level's iterate after the same sequence of swaps
         if (ss >= 2)
             for (int k = 2; k < ss; ++k) std::swap(xa, xb);
         for (int k = 0; k < ss; ++k) std::swap(xa, xb);
         return xa;
     }
-    const bool check_here = hk && hk->prr && l == 0;
+    // the residual test of a fused PCG iteration (the FP32 cycle takes it in its entry kernel)
+    const bool check_here = std::is_same<T, double>::value && hk && hk->prr && l == 0;
     if (ss >= 2) {
-        if (check_here)
-            jacobi_from_zero_check_launch(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, b, L.dinv, L.omega, xa, *hk, s);
-        else
-            jacobi_from_zero_launch(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, b, L.dinv, L.omega, xa, st, s);
+        bool done = false;
+        if constexpr (std::is_same<T, double>::value) {
+            if (check_here) {
+                jacobi_from_zero_check_launch(L.tpr_a, n, L.a_rp, L.a_ci, av, b, dv, L.omega, xa, *hk, s);
+                done = true;
+            }
+        }
+        if (!done) jacobi_from_zero_launch<T>(L.tpr_a, n, L.a_rp, L.a_ci, av, b, dv, L.omega, xa, st, s);
         for (int k = 2; k < ss; ++k) {
-            csr_op_launch<OP_JACOBI>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, xb, st, s);
+            csr_op_launch<OP_JACOBI, T>(L.tpr_a, n, L.a_rp, L.a_ci, av, xa, b, dv, L.omega, xb, st, s);
             std::swap(xa, xb);
         }
     } else {   // x = omega dinv b is always taken, also for fewer than one smoothing step
-        if (check_here) hipLaunchKernelGGL(k_fp_check, dim3(1), dim3(BLOCK), 0, s, hk->prr, hk->nb_rr, hk->sc, hk->st_w, hk->status);
-        hipLaunchKernelGGL(k_jacobi_init, dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, L.omega, L.dinv, b, xa, st);
+        if (check_here)
+            hipLaunchKernelGGL(k_fp_check, dim3(1), dim3(BLOCK), 0, s, hk->prr, hk->nb_rr, hk->sc, hk->st_w, hk->status);
+        hipLaunchKernelGGL(k_jacobi_init<T>, dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, (T)L.omega, dv, b, xa, st);
     }
-    csr_op_launch<OP_RESID>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, nullptr, 0.0, L.r, st, s);
+    csr_op_launch<OP_RESID, T>(L.tpr_a, n, L.a_rp, L.a_ci, av, xa, b, nullptr, 0.0, Lv<T>::res(L), st, s);
     const bool deeper = l + 1 < amg->n_levels;
-    double* cb = deeper ? amg->levels[l + 1].b : amg->cb;
-    csr_op_launch<OP_SPMV>(L.tpr_r, L.cols, L.r_rp, L.r_ci, L.r_v, L.r, nullptr, nullptr, 0.0, cb, st, s);
-    const double* cx;
+    T* cb = deeper ? Lv<T>::b(amg->levels[l + 1]) : Lv<T>::cb(amg);
+    csr_op_launch<OP_SPMV, T>(L.tpr_r, L.cols, L.r_rp, L.r_ci, Lv<T>::r(L), Lv<T>::res(L), nullptr, nullptr, 0.0, cb,
+                              st, s);
+    const T* cx;
     if (deeper) {
-        cx = vcycle(amg, l + 1, amg->levels[l + 1].b, nullptr, st, s);
+        cx = vcycle<T>(amg, l + 1, Lv<T>::b(amg->levels[l + 1]), nullptr, nullptr, st, s);
     } else {
-        hipLaunchKernelGGL(k_dense_matvec, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
-                           amg->coarse_n, amg->coarse_inv, amg->cb, amg->cx, st);
-        cx = amg->cx;
+        hipLaunchKernelGGL(k_dense_matvec<T>, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
+                           amg->coarse_n, amg->coarse_inv, Lv<T>::cb(amg), Lv<T>::cx(amg), st);
+        cx = Lv<T>::cx(amg);
     }
-    csr_op_launch<OP_ADD>(L.tpr_p, n, L.p_rp, L.p_ci, L.p_v, cx, nullptr, nullptr, 0.0, xa, st, s);
+    csr_op_launch<OP_ADD, T>(L.tpr_p, n, L.p_rp, L.p_ci, Lv<T>::p(L), cx, nullptr, nullptr, 0.0, xa, st, s);
     for (int k = 0; k < ss; ++k) {
-        double* dst = (out && k == ss - 1) ? out : xb;
-        if (dst == out && hk && hk->dot_part && l == 0)
-            hk->dot_blocks = jacobi_dot_launch(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, dst,
-                                               hk->dot_part, st, s);
-        else
-            csr_op_launch<OP_JACOBI>(L.tpr_a, n, L.a_rp, L.a_ci, L.a_v, xa, b, L.dinv, L.omega, dst, st, s);
-        if (dst == out) return out;
+        if (out && k == ss - 1) {
+            const int nb = jacobi_dot_launch<T>(L.tpr_a, n, L.a_rp, L.a_ci, av, xa, b, dv, L.omega, out, rdot,
+                                                hk ? hk->dot_part : nullptr, st, s);
+            if (hk && hk->dot_part) hk->dot_blocks = nb;
+            return nullptr;   // the result is in out
+        }
+        csr_op_launch<OP_JACOBI, T>(L.tpr_a, n, L.a_rp, L.a_ci, av, xa, b, dv, L.omega, xb, st, s);
         std::swap(xa, xb);
     }
     return xa;
 }
 
-// z = V-cycle(r); with hooks, the residual test first and the r.z partials last
+// A hierarchy descriptor the V-cycle can run: sizes, and the FP32 arrays when use_f32 is set
+bool amg_valid(const cfa_amg_t* amg) {
+    if (amg->n_levels < 0 || (amg->n_levels > 0 && !amg->levels) || amg->coarse_n < 0 || amg->smooth_steps < 0)
+        return false;
+    if (amg->use_f32) {
+        if (amg->n_levels > 0 && (!amg->cb32 || !amg->cx32)) return false;
+        for (int l = 0; l < amg->n_levels; ++l) {
+            const cfa_amg_level_t& L = amg->levels[l];
+            if (!L.a_v32 || !L.p_v32 || !L.r_v32 || !L.dinv32 || !L.x32 || !L.b32 || !L.r32 || !L.ax32) return false;
+        }
+    }
+    return true;
+}
+
+// z = V-cycle(r); with hooks, the residual test first and the r.z partials last. The FP32 cycle
+// (amg->use_f32) reads r through b32 and writes z in FP64.
 void amg_apply_enqueue(const cfa_amg_t* amg, const double* r, double* z, const int* st, hipStream_t s,
                        PrecHooks* hk = nullptr) {
-    double* res = z;
+    const int n = amg->n_levels ? amg->levels[0].rows : amg->coarse_n;
+    const bool f32 = amg->use_f32 && amg->n_levels > 0 && amg->smooth_steps >= 1;
+    bool wrote_z = false;
     if (amg->n_levels == 0) {
         if (hk && hk->prr)
             hipLaunchKernelGGL(k_fp_check, dim3(1), dim3(BLOCK), 0, s, hk->prr, hk->nb_rr, hk->sc, hk->st_w, hk->status);
-        hipLaunchKernelGGL(k_dense_matvec, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
+        hipLaunchKernelGGL(k_dense_matvec<double>, dim3(blocks_for((long long)amg->coarse_n * WAVE)), dim3(BLOCK), 0, s,
                            amg->coarse_n, amg->coarse_inv, r, z, st);
+    } else if (f32) {
+        float* b32 = amg->levels[0].b32;
+        hipLaunchKernelGGL(k_to_f32_check, dim3(red_blocks(n)), dim3(BLOCK), 0, s, n, r, b32,
+                           hk ? hk->prr : nullptr, hk ? hk->nb_rr : 0, hk ? hk->sc : nullptr,
+                           hk && hk->prr ? hk->st_w : const_cast<int*>(st), hk ? hk->status : nullptr);
+        (void)vcycle<float>(amg, 0, b32, z, r, st, s, hk);
+        wrote_z = true;
     } else {
-        res = vcycle(amg, 0, r, z, st, s, hk);
+        double* res = vcycle<double>(amg, 0, r, z, r, st, s, hk);
+        wrote_z = res == nullptr;
+        if (!wrote_z) hipLaunchKernelGGL(k_copy, dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, res, z, st);
     }
-    const int n = amg->n_levels ? amg->levels[0].rows : amg->coarse_n;
-    if (res != z) hipLaunchKernelGGL(k_copy, dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, res, z, st);
-    if (hk && hk->dot_part && (amg->n_levels == 0 || res != z || amg->smooth_steps < 1)) {
+    if (hk && hk->dot_part && !wrote_z) {
         hk->dot_blocks = red_blocks(n);
         hipLaunchKernelGGL(k_dot_partial, dim3(hk->dot_blocks), dim3(BLOCK), 0, s, n, r, z, hk->dot_part, st);
     }
@@ -1421,9 +1522,7 @@ int cfa_inv_diag(int n, const int* rp, const int* ci, const double* val, double 
 }
 
 int cfa_amg_apply(const cfa_amg_t* amg, const double* r, double* z, cfa_stream_t stream) {
-    if (!amg || !r || !z) return CFA_E_ARG;
-    if (amg->n_levels < 0 || (amg->n_levels > 0 && !amg->levels) || amg->coarse_n < 0 || amg->smooth_steps < 0)
-        return CFA_E_ARG;
+    if (!amg || !r || !z || !amg_valid(amg)) return CFA_E_ARG;
     amg_apply_enqueue(amg, r, z, nullptr, S(stream));
     return launch_status();
 }
@@ -1479,7 +1578,7 @@ int pcg_fused(int n, const int* rp, const int* ci, const double* val, const doub
         hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, a.nnz, rp, ci, a.oss, a.tag,
                            a.gx, a.gy, a.q, a.q, scratch, st_r);
     } else {
-        csr_op_launch<OP_SPMV>(TPR, n, rp, ci, val, x, nullptr, nullptr, 0.0, a.q, st_r, s);
+        csr_op_launch<OP_SPMV, double>(TPR, n, rp, ci, val, x, nullptr, nullptr, 0.0, a.q, st_r, s);
     }
     hipLaunchKernelGGL(k_pcg_residual, dim3(nb), dim3(BLOCK), 0, s, n, b, a.q, a.r, scratch, st_r);
     hipLaunchKernelGGL(k_pcg_init_rnorm, dim3(1), dim3(BLOCK), 0, s, nb, scratch, a.sc, a.st, a.status);
@@ -1561,11 +1660,20 @@ int pcg_fused(int n, const int* rp, const int* ci, const double* val, const doub
 
 extern "C" {
 
+int cfa_to_f32(size_t n, const double* in, float* out, cfa_stream_t stream) {
+    if ((n > 0 && (!in || !out))) return CFA_E_ARG;
+    if (n == 0) return CFA_OK;
+    const int blocks = (int)std::min<size_t>((n + BLOCK - 1) / BLOCK, 4096);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks), dim3(BLOCK), 0, S(stream), n, in, out);
+    return launch_status();
+}
+
 int cfa_amg_smooth(const cfa_amg_t* amg, int level, const double* x, const double* b, double* y,
                    cfa_stream_t stream) {
     if (!amg || !x || !b || !y || level < 0 || level >= amg->n_levels || !amg->levels) return CFA_E_ARG;
     const cfa_amg_level_t& L = amg->levels[level];
-    csr_op_launch<OP_JACOBI>(L.tpr_a, L.rows, L.a_rp, L.a_ci, L.a_v, x, b, L.dinv, L.omega, y, nullptr, S(stream));
+    csr_op_launch<OP_JACOBI, double>(L.tpr_a, L.rows, L.a_rp, L.a_ci, L.a_v, x, b, L.dinv, L.omega, y, nullptr,
+                                      S(stream));
     return launch_status();
 }
 
@@ -1596,8 +1704,7 @@ int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double
     if (n < 0 || !rp || !ci || !val || !b || !x || !ws || !result || ws->cap < n || !ws->status_host || max_iter < 0)
         return CFA_E_ARG;
     if (oss && (!oss->oss_val || !oss->inv_m || !oss->bc_tag || !oss->gx || !oss->gy)) return CFA_E_ARG;
-    if (amg && (amg->n_levels < 0 || (amg->n_levels > 0 && !amg->levels) || amg->smooth_steps < 0))
-        return CFA_E_ARG;
+    if (amg && !amg_valid(amg)) return CFA_E_ARG;
     hipStream_t s = S(stream);
     result->iters[0] = 0;
     result->converged[0] = 1;
@@ -1634,7 +1741,7 @@ int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double
             hipLaunchKernelGGL(k_apply_K_B<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, q_s, n, nnz, rp, ci, val,
                                oss->oss_val, oss->inv_m, in, out, oss->gx, oss->gy, st);
         } else {
-            csr_op_launch<OP_SPMV>(TPR, n, rp, ci, val, in, nullptr, nullptr, 0.0, out, st, q_s);
+            csr_op_launch<OP_SPMV, double>(TPR, n, rp, ci, val, in, nullptr, nullptr, 0.0, out, st, q_s);
             if (extra_cb) extra_cb(in, out, extra_ctx);
         }
     };
@@ -1768,12 +1875,12 @@ int cfa_bicgstab2(int n, const int* rp, const int* ci, const double* val, const 
 
     hipLaunchKernelGGL(k_bicg_init<TPR>, dim3(nb_rows), dim3(BLOCK), 0, s, n, rp, ci, val, bb, x, r, r0, p, v, part_a);
     hipLaunchKernelGGL(k_bicg_final_init, dim3(1), dim3(BLOCK), 0, s, nb_rows, part_a, sc, st, status, max_iter);
+    hipLaunchKernelGGL(k_bicg_pre, dim3(1), dim3(1), 0, s, sc, st, status);
     int rc = launch_status();
     if (rc) return rc;
 
     // one iteration for both components (src/cfa/linalg.cuh:700-775)
     auto iteration = [&](hipStream_t q_s) {
-        hipLaunchKernelGGL(k_bicg_pre, dim3(1), dim3(1), 0, q_s, sc, st, status);
         hipLaunchKernelGGL(k_bicg_p, dim3(blocks2), dim3(BLOCK), 0, q_s, n, r, v, sc, p, st);
         ilu_apply_any(ilu, use_f32, p, ph, 2, st, q_s);
         hipLaunchKernelGGL(k_spmv2_dots<TPR>, dim3(nb_rows), dim3(BLOCK), 0, q_s, n, rp, ci, val, ph, v, r0, 1, part_a, st);

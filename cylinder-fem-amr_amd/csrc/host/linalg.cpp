@@ -697,7 +697,32 @@ void AmgPreconditioner::refresh_view() {
         d.dinv = L.dinv;
         d.omega = L.omega;
         d.x = L.x; d.b = L.b; d.r = L.r; d.ax = L.Ax;
+        d.a_v32 = d.p_v32 = d.r_v32 = d.dinv32 = nullptr;
+        d.x32 = d.b32 = d.r32 = d.ax32 = nullptr;
     }
+    if (fp32) {   // FP32 copies of the values and FP32 work vectors
+        const cfa_stream_t s = solver_stream();
+        for (size_t l = 0; l < levels.size(); ++l) {
+            Level& L = *levels[l];
+            cfa_amg_level_t& d = desc_[l];
+            const int n = L.A.rows;
+            L.A32.reserve(std::max(L.A.nnz, 1), 256);
+            L.P32.reserve(std::max(L.P.nnz, 1), 256);
+            L.R32.reserve(std::max(L.R.nnz, 1), 256);
+            for (DevArray<float>* v : {&L.dinv32, &L.x32, &L.b32, &L.r32, &L.Ax32}) v->reserve(n, 16);
+            GPU_CHECK(cfa_to_f32(L.A.nnz, L.A.v, L.A32, s));
+            GPU_CHECK(cfa_to_f32(L.P.nnz, L.P.v, L.P32, s));
+            GPU_CHECK(cfa_to_f32(L.R.nnz, L.R.v, L.R32, s));
+            GPU_CHECK(cfa_to_f32(n, L.dinv, L.dinv32, s));
+            d.a_v32 = L.A32; d.p_v32 = L.P32; d.r_v32 = L.R32; d.dinv32 = L.dinv32;
+            d.x32 = L.x32; d.b32 = L.b32; d.r32 = L.r32; d.ax32 = L.Ax32;
+        }
+        d_cx32.reserve(std::max(coarse_n, 1), 16);
+        d_cb32.reserve(std::max(coarse_n, 1), 16);
+    }
+    view_.use_f32 = fp32 ? 1 : 0;
+    view_.cb32 = fp32 ? d_cb32.ptr : nullptr;
+    view_.cx32 = fp32 ? d_cx32.ptr : nullptr;
     view_.n_levels = (int)levels.size();
     view_.levels = desc_.data();
     view_.coarse_n = coarse_n;

@@ -129,16 +129,20 @@ public:
         DevArray<double> dinv;
         double omega = 0.0;
         DevArray<double> x, b, r, Ax;
+        // FP32 copies and work vectors of the mixed-precision cycle
+        DevArray<float> A32, P32, R32, dinv32, x32, b32, r32, Ax32;
     };
 
     int smooth_steps = 2;
     double theta = 0.08;
     int coarse_target = 300;
     int max_levels = 15;
+    bool fp32 = false;   // run the cycle in FP32 (coarse solve in FP64); FP32 copies made at set-up
 
     std::vector<std::unique_ptr<Level>> levels;
     int coarse_n = 0;
     DevArray<double> d_coarse_inv, d_cx, d_cb;
+    DevArray<float> d_cx32, d_cb32;
     double setup_ms = 0.0;
     enum { PH_DOWNLOAD, PH_RHO, PH_AGG, PH_SMOOTH, PH_TRANSPOSE, PH_GALERKIN, PH_UPLOAD, PH_COARSE, PH_REMAP, PH_COUNT };
     HostCsr P0_host;

@@ -64,7 +64,7 @@ void print_usage(const char* prog) {
         << "  --mom-tol T         momentum BiCGSTAB relative tolerance          (" << d.mom_tol << ")\n"
         << "  --ppe-tol T         pressure PCG relative tolerance               (" << d.ppe_tol << ")\n"
         << "  --proj-tol T        re-projection PCG relative tolerance          (" << d.proj_tol << ")\n"
-        << "  --precision fp64|mixed                  momentum ILU factors in FP64 or FP32 (fp64)\n"
+        << "  --precision fp64|mixed                  ILU factors and AMG cycle in FP64 or FP32 (fp64)\n"
         << "  --quiet             no per-step console lines\n"
         << "  --no-csv            write no CSV files (the history stays available in-process)\n"
         << "  --kernel-timing R   after the run, time the main kernels R launches each (HIP events)\n";

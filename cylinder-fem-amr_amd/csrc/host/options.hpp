@@ -40,11 +40,11 @@ struct RunOptions {
     int dump_systems = 0;
     std::string prefix = "cylinder_fem_amr";
     AmrParams amr;
-    // additions of this build (defaults = the reference's hard-coded behaviour)
+    // options beyond the reference's flags (defaults = its hard-coded behaviour)
     double mom_tol = 1.0e-7;    // momentum BiCGSTAB (src/cylinder_fem_amr.cu:358)
     double ppe_tol = 1.0e-7;    // pressure PCG (src/cylinder_fem_amr.cu:457)
     double proj_tol = 1.0e-8;   // re-projection PCG (src/cfa/device.cuh:388,449)
-    bool mixed = false;         // FP32 ILU factors inside the FP64 momentum BiCGSTAB
+    bool mixed = false;         // FP32 ILU factors and AMG cycle inside the FP64 Krylov solvers
     bool quiet = false;         // no per-step console lines
     bool write_csv = true;
     int kernel_timing = 0;      // > 0: time the main kernels this many launches each on the final state

@@ -213,11 +213,17 @@ typedef struct {
     const double* dinv;
     double omega;
     double* x; double* b; double* r; double* ax;
+    /* FP32 copies of the values and FP32 work vectors for the mixed-precision cycle
+     * (cfa_amg_t.use_f32); NULL when the hierarchy has none */
+    const float* a_v32; const float* p_v32; const float* r_v32; const float* dinv32;
+    float* x32; float* b32; float* r32; float* ax32;
 } cfa_amg_level_t;
 
 /* SA-AMG hierarchy for the V(ss, ss) cycle (src/cfa/linalg.cuh:1161-1206).
  * levels: host array of n_levels descriptors; coarse_inv: dense inverse (coarse_n x coarse_n,
- * row-major) of the coarsest operator; cb, cx: [coarse_n]. */
+ * row-major) of the coarsest operator; cb, cx: [coarse_n]. use_f32: run the cycle on the FP32
+ * copies (smoothers, residuals, transfers in FP32; the coarse solve in FP64; r in and z out stay
+ * FP64), with cb32, cx32 its coarse vectors. */
 typedef struct {
     int n_levels;
     const cfa_amg_level_t* levels;
@@ -225,10 +231,15 @@ typedef struct {
     const double* coarse_inv;
     double* cb; double* cx;
     int smooth_steps;
+    int use_f32;
+    float* cb32; float* cx32;
 } cfa_amg_t;
 
 /* z = M^{-1} r, one V-cycle (src/cfa/linalg.cuh:1161-1169 AmgPreconditioner::apply). */
 int cfa_amg_apply(const cfa_amg_t* amg, const double* r, double* z, cfa_stream_t stream);
+
+/* out[i] = (float) in[i], i < n: the FP32 copies of a hierarchy. */
+int cfa_to_f32(size_t n, const double* in, float* out, cfa_stream_t stream);
 
 /* One damped Jacobi step on level `level`: y = x + omega D^{-1} (b - A x), the smoother of the
  * V-cycle (src/cfa/linalg.cuh:1177-1181 jacobi_step); one kernel. */

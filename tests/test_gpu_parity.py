@@ -9,7 +9,7 @@ import pytest
 
 BIN = Path(__file__).resolve().parent / "native" / "build" / "parity"
 CASES = ["momentum", "pressure", "correction", "forces", "indicator", "mms", "oss", "ilu", "spmv", "amg", "pcg",
-         "bicgstab"]
+         "bicgstab", "mixed"]
 
 pytestmark = pytest.mark.gpu
 

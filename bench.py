@@ -79,6 +79,42 @@ def trace_summary(path: str, reps: int, top: int = 25) -> int:
     return 0
 
 
+def counter_summary(path: str, reps: int, top: int = 25) -> int:
+    """Mean value per dispatch of each counter in a rocprofv3 --pmc CSV, over all dispatches of a
+    kernel and over its last `reps` dispatches (the --kernel-timing pass)."""
+    import csv
+    from collections import defaultdict
+    vals = defaultdict(lambda: defaultdict(list))
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            vals[row["Counter_Name"]][row["Kernel_Name"]].append(float(row["Counter_Value"]))
+    for counter, per_kernel in vals.items():
+        print(f"counter {counter}")
+        print(f"{'calls':>8} {'mean':>14} {'last-' + str(reps) + ' mean':>16}  kernel")
+        for name, v in sorted(per_kernel.items(), key=lambda kv: -sum(kv[1]))[:top]:
+            tail = v[-reps:] if reps > 0 else v
+            print(f"{len(v):8d} {sum(v) / len(v):14.1f} {sum(tail) / len(tail):16.1f}  {name[:120]}")
+    return 0
+
+
+def solver_workloads(level: int, nrhs: int, tol: float, reps: int) -> int:
+    """The isolated solver workloads on the uniform level-`level` mesh: SA-AMG PCG on the pressure
+    Laplacian (nrhs seeded right-hand sides) and ILU-BiCGSTAB2 on the momentum system with FP64 and
+    FP32 factors; one JSON line with times per solve and per iteration and the kernel timings."""
+    import cfa
+    p = cfa.workload_pressure(level, nrhs, tol, reps)
+    pk = cfa.last_kernels()
+    m = cfa.workload_momentum(level, tol, reps)
+    mk = cfa.last_kernels()
+    ks = {}
+    for k, v in list(pk.items()) + list(mk.items()):
+        gbs = v["bytes"] / (v["us"] * 1e3)
+        ks[k] = {"us": round(v["us"], 3), "bytes": v["bytes"], "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    print(json.dumps({"workload": f"frozen uniform level-{level} mesh, tol {tol:g}", "pressure": p, "momentum": m,
+                      "kernels": ks}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,9 +127,17 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch Krylov iterations kernel by kernel")
     ap.add_argument("--trace-summary", metavar="CSV", help="summarise a rocprofv3 kernel trace and exit")
+    ap.add_argument("--counter-summary", metavar="CSV", help="summarise a rocprofv3 --pmc CSV and exit")
+    ap.add_argument("--solver-workloads", type=int, metavar="LEVEL",
+                    help="run the isolated solver workloads on a uniform mesh of this level and exit")
+    ap.add_argument("--nrhs", type=int, default=32)
     args = ap.parse_args()
     if args.trace_summary:
         return trace_summary(args.trace_summary, args.kernel_reps)
+    if args.counter_summary:
+        return counter_summary(args.counter_summary, args.kernel_reps)
+    if args.solver_workloads is not None:
+        return solver_workloads(args.solver_workloads, args.nrhs, args.tol, args.kernel_reps)
     if args.gpus != 1:
         print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         return 2

@@ -137,117 +137,6 @@ template <> struct Lv<float> {
     static float* cx(const cfa_amg_t* a) { return a->cx32; }
 };
 
-// ---- the coarse end of the V-cycle in one workgroup ---------------------------------------------------
-// Levels of at most BOTTOM_ROWS rows, down to the dense coarse solve and back up, run as one
-// workgroup with barriers between the sweeps instead of one launch per sweep: on these levels a
-// sweep is a few hundred rows, far too small to fill the chip, and each launch would cost more than
-// its work. Same operations and buffers as the per-level launches of vcycle() below.
-constexpr int BOTTOM_MAX = 6;
-constexpr int BOTTOM_ROWS = 1536;
-constexpr int BOTTOM_THREADS = 1024;
-
-template <typename T>
-struct BottomArgs {
-    int nl, ss, coarse_n;
-    const double* coarse_inv;
-    T* cb;
-    T* cx;
-    cfa_amg_level_t lv[BOTTOM_MAX];
-};
-
-// y = x + omega dinv (b - A x); with from_zero, x = omega dinv b is formed on the fly
-template <typename T>
-__device__ __forceinline__ void bt_jacobi(const cfa_amg_level_t& L, const T* __restrict__ x, const T* __restrict__ b,
-                                          T* __restrict__ y, bool from_zero) {
-    const int* __restrict__ rp = L.a_rp;
-    const int* __restrict__ ci = L.a_ci;
-    const T* __restrict__ av = Lv<T>::a(L);
-    const T* __restrict__ d = Lv<T>::dinv(L);
-    const T om = (T)L.omega;
-    for (int i = threadIdx.x; i < L.rows; i += blockDim.x) {
-        T s = 0;
-        for (int k = rp[i]; k < rp[i + 1]; ++k) {
-            const int j = ci[k];
-            s += av[k] * (from_zero ? om * d[j] * b[j] : x[j]);
-        }
-        const T xi = from_zero ? om * d[i] * b[i] : x[i];
-        y[i] = xi + om * d[i] * (b[i] - s);
-    }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(BOTTOM_THREADS)
-k_vcycle_bottom(BottomArgs<T> a, const int* __restrict__ state) {
-    if (halted(state)) return;
-    T* xa[BOTTOM_MAX];
-    T* xb[BOTTOM_MAX];
-    T* cb = a.cb;
-    for (int l = 0; l < a.nl; ++l) {   // down: pre-smoothing, residual, restriction
-        const cfa_amg_level_t& L = a.lv[l];
-        const T* b = Lv<T>::b(L);
-        T* res = Lv<T>::res(L);
-        xa[l] = Lv<T>::x(L);
-        xb[l] = Lv<T>::ax(L);
-        if (a.ss >= 2) {
-            bt_jacobi<T>(L, nullptr, b, xa[l], true);
-            __syncthreads();
-            for (int k = 2; k < a.ss; ++k) {
-                bt_jacobi<T>(L, xa[l], b, xb[l], false);
-                __syncthreads();
-                T* t = xa[l]; xa[l] = xb[l]; xb[l] = t;
-            }
-        } else {
-            const T* d = Lv<T>::dinv(L);
-            for (int i = threadIdx.x; i < L.rows; i += blockDim.x) xa[l][i] = (T)L.omega * d[i] * b[i];
-            __syncthreads();
-        }
-        const T* av = Lv<T>::a(L);
-        for (int i = threadIdx.x; i < L.rows; i += blockDim.x) {
-            T s = 0;
-            for (int k = L.a_rp[i]; k < L.a_rp[i + 1]; ++k) s += av[k] * xa[l][L.a_ci[k]];
-            res[i] = b[i] - s;
-        }
-        __syncthreads();
-        cb = l + 1 < a.nl ? Lv<T>::b(a.lv[l + 1]) : a.cb;
-        const T* rv = Lv<T>::r(L);
-        for (int i = threadIdx.x; i < L.cols; i += blockDim.x) {
-            T s = 0;
-            for (int k = L.r_rp[i]; k < L.r_rp[i + 1]; ++k) s += rv[k] * res[L.r_ci[k]];
-            cb[i] = s;
-        }
-        __syncthreads();
-    }
-    {   // dense coarse solve in FP64, one wavefront per row
-        const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE, nw = blockDim.x / WAVE;
-        for (int i = w; i < a.coarse_n; i += nw) {
-            double s = 0.0;
-            for (int j = lane; j < a.coarse_n; j += WAVE)
-                s += a.coarse_inv[(size_t)i * a.coarse_n + j] * (double)a.cb[j];
-            s = wave_sum(s);
-            if (lane == 0) a.cx[i] = (T)s;
-        }
-        __syncthreads();
-    }
-    const T* cx = a.cx;
-    for (int l = a.nl - 1; l >= 0; --l) {   // up: prolongation, post-smoothing
-        const cfa_amg_level_t& L = a.lv[l];
-        const T* pv = Lv<T>::p(L);
-        const T* b = Lv<T>::b(L);
-        for (int i = threadIdx.x; i < L.rows; i += blockDim.x) {
-            T s = 0;
-            for (int k = L.p_rp[i]; k < L.p_rp[i + 1]; ++k) s += pv[k] * cx[L.p_ci[k]];
-            xa[l][i] += s;
-        }
-        __syncthreads();
-        for (int k = 0; k < a.ss; ++k) {
-            bt_jacobi<T>(L, xa[l], b, xb[l], false);
-            __syncthreads();
-            T* t = xa[l]; xa[l] = xb[l]; xb[l] = t;
-        }
-        cx = xa[l];
-    }
-}
-
 // ---- reductions ------------------------------------------------------------------------------------
 // partial[b] = sum over the block's grid-stride share of x . y
 __global__ void __launch_bounds__(BLOCK)
@@ -1211,23 +1100,6 @@ T* vcycle(const cfa_amg_t* amg, int l, const T* b, double* out, const double* rd
     T* xb = Lv<T>::ax(L);
     const T* av = Lv<T>::a(L);
     const T* dv = Lv<T>::dinv(L);
-    if (l >= 1 && n <= BOTTOM_ROWS && amg->n_levels - l <= BOTTOM_MAX) {
-        BottomArgs<T> ba;
-        std::memset(&ba, 0, sizeof(ba));
-        ba.nl = amg->n_levels - l;
-        ba.ss = ss;
-        ba.coarse_n = amg->coarse_n;
-        ba.coarse_inv = amg->coarse_inv;
-        ba.cb = Lv<T>::cb(amg);
-        ba.cx = Lv<T>::cx(amg);
-        for (int k = 0; k < ba.nl; ++k) ba.lv[k] = amg->levels[l + k];
-        hipLaunchKernelGGL(k_vcycle_bottom<T>, dim3(1), dim3(BOTTOM_THREADS), 0, s, ba, st);
-        // the buffer that holds the level's iterate after the same sequence of swaps
-        if (ss >= 2)
-            for (int k = 2; k < ss; ++k) std::swap(xa, xb);
-        for (int k = 0; k < ss; ++k) std::swap(xa, xb);
-        return xa;
-    }
     // the residual test of a fused PCG iteration (the FP32 cycle takes it in its entry kernel)
     const bool check_here = std::is_same<T, double>::value && hk && hk->prr && l == 0;
     if (ss >= 2) {

@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "device.hpp"
+#include "outputs.hpp"
 
 using namespace cfa;
 
@@ -102,6 +103,23 @@ int cfa_amg_setup_record(int n, const int* rp, const int* ci, const double* val,
     if (agg0 && !amg.last_aggregates.empty())
         std::memcpy(agg0, amg.last_aggregates.data(), amg.last_aggregates.size() * sizeof(int));
     return nl;
+}
+
+// DFG 2D-2 statistics of a force history (the program's end-of-run report):
+// out[13] = lift periods, 6 mean values, 6 spreads (C_D max/min, C_L max/min, St, dp half period).
+int cfa_force_statistics(int n, const double* t, const double* cd, const double* cl, const double* dp, double window,
+                         double* out) {
+    if (n < 0 || !out || (n > 0 && (!t || !cd || !cl || !dp))) return CFA_E_ARG;
+    ForceHistory h;
+    for (int i = 0; i < n; ++i) h.add(t[i], cd[i], cl[i], dp[i]);
+    size_t first = 0;
+    const ForceHistory::Stats S = h.statistics(window, first);
+    out[0] = S.periods;
+    for (int k = 0; k < 6; ++k) {
+        out[1 + k] = S.value[k];
+        out[7 + k] = S.spread[k];
+    }
+    return CFA_OK;
 }
 
 }  // extern "C"

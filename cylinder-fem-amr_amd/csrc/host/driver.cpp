@@ -11,6 +11,7 @@
 
 #include "device.hpp"
 #include "options.hpp"
+#include "outputs.hpp"
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -192,6 +193,7 @@ int run_driver(int argc, char** argv) {
             << std::setprecision(10);
         csv.unsetf(std::ios::floatfield);
     }
+    ForceHistory history;
     int nonconverged_solves = 0;
     auto note_nonconverged = [&nonconverged_solves](const GpuLinearSolver& sv, int at_step, const char* what, int iters) {
         if (sv.converged) return;
@@ -369,6 +371,16 @@ int run_driver(int argc, char** argv) {
                       << " | Iters(u,v,p): (" << it_u << "," << it_v << "," << it_p << ") | C_D = " << C_D
                       << " | C_L = " << C_L << " | dP = " << delta_p << std::endl;
 
+        history.add(t_now, C_D, C_L, delta_p);
+        // Step E: VTK snapshots (src/cylinder_fem_amr.cu:540-547)
+        if (opt.write_csv && opt.vtk_interval > 0 && step % opt.vtk_interval == 0 && step < opt.num_steps) {
+            compute_error_indicator(g);
+            g.copy_solution_to_host(h_u, h_v, h_p, h_uo, h_vo, h_vort, h_elem_err);
+            std::ostringstream name;
+            name << opt.prefix << "_" << std::setw(6) << std::setfill('0') << step << ".vtk";
+            write_vtk(name.str(), mesh, h_u, h_v, h_p, h_vort, h_elem_err, opt.amr);
+        }
+
         // Step F: remeshing (src/cylinder_fem_amr.cu:549-725)
         const bool overlap = opt.amr_overlap > 0;
         const bool remesh_due = opt.amr_interval > 0 && step % opt.amr_interval == 0 && step < opt.num_steps &&
@@ -538,6 +550,16 @@ int run_driver(int argc, char** argv) {
     dev_sync();
     const double wall_s = ms_since(wall_start) / 1000.0;
 
+    if (opt.write_csv) {   // final solution and mesh (src/cylinder_fem_amr.cu:731-735)
+        compute_error_indicator(g);
+        g.copy_solution_to_host(h_u, h_v, h_p, h_uo, h_vo, h_vort, h_elem_err);
+        write_vtk(opt.prefix + "_final.vtk", mesh, h_u, h_v, h_p, h_vort, h_elem_err, opt.amr);
+    }
+    if (!opt.mms) {   // DFG statistics of the force history (src/cylinder_fem_amr.cu:784-788)
+        const std::string stats_csv = opt.write_csv ? opt.prefix + "_stats.csv" : std::string();
+        if (opt.case_2d3) history.report_2d3(stats_csv);
+        else history.report(opt.stats_window, stats_csv);
+    }
     if (opt.mms) {
         double e2[6];
         DevArray<double> d_err;

@@ -75,6 +75,13 @@ void cfa_mesh_probes(void* mesh, int* ab, double* w);
 int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, double* out);
 int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out);
 
+/* DFG 2D-2 statistics of a force history, as the program reports them at the end of a run
+ * (src/cfa/diagnostics.cuh:47-121 ForceHistory::compute; window = --stats-window):
+ * out[13] = number of lift periods, the 6 period means (C_D max, C_D min, C_L max, C_L min,
+ * Strouhal number, delta_p half a period after the lift maximum), then their 6 spreads. */
+int cfa_force_statistics(int n, const double* t, const double* cd, const double* cl, const double* dp,
+                         double window, double* out);
+
 /* Host SA-AMG set-up (src/cfa/linalg.cuh:995-1060, 1276-1431) of a host CSR matrix: per level
  * rows, (nnz A, nnz P), Jacobi weight, the coarse size and the level-0 aggregates [n].
  * Returns the number of smoothed levels (<= 16 reported). Needs a GPU (the hierarchy is

@@ -18,6 +18,9 @@ RUNS = {
 }
 # adaptive DFG 2D-2 run (levels 2-5, dt 5e-4): only its first 2000 steps are kept
 ADAPTIVE = ("d2_A2_dt05", 2000)
+# uniform level-3 DFG 2D-2 run to t = 8 s (runs/v14/timing/U3_r1): the force history from
+# t >= 5.9 s (its statistics window was the last 1.5 s) and the statistics the run wrote
+STATS_RUN = (os.path.join("v14", "timing", "U3_r1"), 5.9)
 
 
 def main(ref):
@@ -35,6 +38,20 @@ def main(ref):
                 break
             fo.write(line)
     shutil.copyfile(os.path.join(ref, "runs", "paper", run, "cmd.txt"), os.path.join(here, f"{run}_cmd.txt"))
+    # the same MMS U3 case run by a second binary of the reference (its run-to-run noise)
+    shutil.copyfile(os.path.join(ref, "runs", "review", "mms_L3_dt2", "cylinder_fem_amr_mms.csv"),
+                    os.path.join(here, "mms_L3_dt2_mms.csv"))
+    run, t_from = STATS_RUN
+    src = os.path.join(ref, "runs", run, "cylinder_fem_amr_forces.csv")
+    with open(src) as fi, open(os.path.join(here, "U3_r1_forces_tail.csv"), "w") as fo:
+        header = fi.readline().strip().split(",")
+        cols = [header.index(c) for c in ("time", "C_D", "C_L", "delta_p")]
+        fo.write("time,C_D,C_L,delta_p\n")
+        for line in fi:
+            v = line.strip().split(",")
+            if float(v[1]) >= t_from:
+                fo.write(",".join(v[c] for c in cols) + "\n")
+    shutil.copyfile(os.path.join(ref, "runs", run, "cylinder_fem_amr_stats.csv"), os.path.join(here, "U3_r1_stats.csv"))
 
 
 if __name__ == "__main__":

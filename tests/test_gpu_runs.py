@@ -4,8 +4,9 @@ reference's stored runs.
 * MMS L1 (uniform level 1, dt 1e-3, 100 steps): the GPU history at full precision against the
   oracle's, relative 1e-10 per column (the tolerance of the path), and against the reference's
   CSV to its print resolution.
-* MMS L3, 400 steps (the reference's published U3 case, runs/paper/mms_L3): forces history and
-  MMS error table against the reference to one unit of the tenth printed digit.
+* MMS L3, 400 steps (the reference's published U3 case, runs/paper/mms_L3): forces history
+  against the reference to one unit of the tenth printed digit or 1e-10 of the column scale; the
+  MMS error table within twice the spread between the reference's own two binaries.
 * Adaptive flow (levels 2-5, dt 5e-4, remesh every 20 steps, runs/paper/d2_A2_dt05): the mesh
   trajectory (nodes, elements per step) is integer work and must match exactly; forces follow the
   reference to 1e-8 relative over the first 200 steps (remesh decisions are sensitive to
@@ -43,7 +44,7 @@ def test_mms_L1_gpu_vs_oracle_and_reference(golden, tmp_path):
     ref = read_csv(golden / "mms_L1_forces.csv")
     for c in COLS:
         assert_matches_printed(hist[:, H[c]], ref[c], f"GPU L1 {c}")
-    for k, name in enumerate(cfa.cfa.MMS_FIELDS):
+    for k, name in enumerate(cfa.MMS_FIELDS):
         assert abs(mms[name] - omms[k]) <= 1e-10 * abs(omms[k]), name
 
 
@@ -57,10 +58,16 @@ def test_mms_L3_400_steps_vs_reference(golden):
     np.testing.assert_array_equal(hist[:, H["nodes"]], ref["nodes"])
     for c in COLS:
         assert_matches_printed(hist[:, H[c]], ref[c], f"GPU L3 {c}")
+    # MMS error table: the reference's two binaries (runs/paper/mms_L3, runs/review/mms_L3_dt2)
+    # differ by up to 1.5e-9 relative in these figures (pressure solves stop at 1e-7, so the
+    # rounding order of the solver shows); the GPU run must stay within twice that spread
     table = read_csv(golden / "mms_L3_mms.csv")
+    table2 = read_csv(golden / "mms_L3_dt2_mms.csv")
     mms = cfa.last_mms()
-    for name in cfa.cfa.MMS_FIELDS:
-        assert_matches_printed([mms[name]], [float(table[name])], f"L3 MMS {name}")
+    for name in cfa.MMS_FIELDS:
+        a, b = float(table[name]), float(table2[name])
+        noise = abs(a - b) / abs(a)
+        assert abs(mms[name] - a) <= max(2.0 * noise, 1e-10) * abs(a) + 1e-13, (name, mms[name], a, b)
     # iteration counts of the reference's log (runs/paper/mms_L3/run.log:12-52): (3, 3, 29-39)
     assert set(hist[:, H["it_u"]].astype(int)) <= {2, 3, 4}
     assert 25 <= hist[:, H["it_p"]].min() and hist[:, H["it_p"]].max() <= 42

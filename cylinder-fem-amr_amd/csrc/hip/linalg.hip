@@ -890,82 +890,93 @@ k_spmv2_dots(int n, const int* __restrict__ rp, const int* __restrict__ ci, cons
     write_partials2(part, mode == 1 ? 1 : 2, acc, lds);
 }
 
-// alpha = rho / (r0 . v), breakdown guard
-__global__ void k_bicg_final_alpha(int nb, const double* __restrict__ part, double* __restrict__ sc,
-                                   int* __restrict__ st, int* __restrict__ status) {
-    __shared__ double lds[BLOCK / WAVE];
-    if (st[0] != ST_RUNNING) return;
-    double o[2];
-    for (int c = 0; c < 2; ++c) o[c] = sum_partials(part + c, nb, 6, lds);
-    if (threadIdx.x != 0) return;
-    const int it = st[10];
-    for (int c = 0; c < 2; ++c) {
-        sc[B_CA + c] = 0.0;
-        if (st[2 + c] && fabs(o[c]) < sc[B_BD + c]) bicg_stop(st, c, it, 0);
-        if (st[2 + c]) {
-            sc[B_ALPHA + c] = sc[B_RHO + c] / o[c];
-            sc[B_CA + c] = sc[B_ALPHA + c];
-        }
-    }
-    if (!(st[2] || st[3])) { st[0] = ST_STOPPED; publish(status, it, ST_STOPPED); }
-}
+// The scalar steps between the vector kernels are folded into the kernels that consume them:
+// every block sums the producer's partials in the same fixed order and takes the same decision;
+// block 0 alone writes the state. A component stopped here is treated by a late block exactly
+// as an already inactive one (zero coefficient), so a block that reads the state before or after
+// block 0's write computes the same values.
 
-// s = r - ca v; partial s.s
+// alpha = rho / (r0 . v) with the breakdown guard (from the r0.v partials), then s = r - alpha v
+// and the s.s partials
 __global__ void __launch_bounds__(BLOCK)
-k_bicg_s(int n, const double* __restrict__ r, const double* __restrict__ v, const double* __restrict__ sc,
-         double* __restrict__ s, double* __restrict__ part, const int* __restrict__ st) {
+k_bicg_s(int n, const double* __restrict__ r, const double* __restrict__ v, double* __restrict__ sc,
+         double* __restrict__ s, double* __restrict__ part, int nb_a, const double* __restrict__ part_a,
+         int* __restrict__ st, int* __restrict__ status) {
     __shared__ double lds[BLOCK / WAVE];
-    if (halted(st)) return;
+    __shared__ int s_state, s_act[2];
+    if (threadIdx.x == 0) { s_state = st[0]; s_act[0] = st[2]; s_act[1] = st[3]; }
+    __syncthreads();
+    if (s_state != ST_RUNNING) return;
+    double ca[2];
+    bool act[2];
+    for (int c = 0; c < 2; ++c) {
+        const double o = sum_partials(part_a + c, nb_a, 6, lds);
+        act[c] = s_act[c] && !(fabs(o) < sc[B_BD + c]);
+        ca[c] = act[c] ? sc[B_RHO + c] / o : 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int it = st[10];
+        for (int c = 0; c < 2; ++c) {
+            sc[B_CA + c] = ca[c];
+            if (s_act[c] && !act[c]) bicg_stop(st, c, it, 0);
+            if (act[c]) sc[B_ALPHA + c] = ca[c];
+        }
+        if (!(act[0] || act[1])) { st[0] = ST_STOPPED; publish(status, it, ST_STOPPED); }
+    }
+    if (!(act[0] || act[1])) return;
     double acc[1][2] = {{0.0, 0.0}};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += gridDim.x * blockDim.x) {
         const int c = i >= n;
-        const double si = r[i] - sc[B_CA + c] * v[i];
+        const double si = r[i] - ca[c] * v[i];
         s[i] = si;
         acc[0][c] += si * si;
     }
     write_partials2(part, 1, acc, lds);
 }
 
-// omega = (t . s) / (t . t) unless the half step already converged
-__global__ void k_bicg_final_omega(int nb_s, const double* __restrict__ part_s, int nb_t,
-                                   const double* __restrict__ part_t, double* __restrict__ sc,
-                                   int* __restrict__ st) {
-    __shared__ double lds[BLOCK / WAVE];
-    if (st[0] != ST_RUNNING) return;
-    double ss[2], tt[2], ts[2];
-    for (int c = 0; c < 2; ++c) {
-        ss[c] = sum_partials(part_s + c, nb_s, 6, lds);
-        tt[c] = sum_partials(part_t + c, nb_t, 6, lds);
-        ts[c] = sum_partials(part_t + 2 + c, nb_t, 6, lds);
-    }
-    if (threadIdx.x != 0) return;
-    for (int c = 0; c < 2; ++c) {
-        sc[B_A1 + c] = 0.0;
-        sc[B_A2 + c] = 0.0;
-        sc[B_CW + c] = 0.0;
-        st[8 + c] = 0;
-        if (!st[2 + c]) continue;
-        sc[B_A1 + c] = sc[B_ALPHA + c];
-        const double s_norm = sqrt(ss[c]);
-        if (s_norm < sc[B_TOL] * sc[B_RN0 + c] || s_norm < 1.0e-11) { st[8 + c] = 1; continue; }
-        sc[B_OMEGA + c] = (tt[c] > sc[B_BD + c]) ? (ts[c] / tt[c]) : 0.0;
-        sc[B_A2 + c] = sc[B_OMEGA + c];
-        sc[B_CW + c] = sc[B_OMEGA + c];
-    }
-}
-
-// x += a1 phat + a2 shat; r = s - cw t; partial r.r, r0.r
+// omega = (t . s) / (t . t) unless the half step already converged (from the s.s and t.t, t.s
+// partials), then x += alpha phat + omega shat, r = s - omega t and the r.r, r0.r partials
 __global__ void __launch_bounds__(BLOCK)
 k_bicg_xr(int n, const double* __restrict__ ph, const double* __restrict__ sh, const double* __restrict__ s,
-          const double* __restrict__ t, const double* __restrict__ r0, const double* __restrict__ sc,
-          double* __restrict__ x, double* __restrict__ r, double* __restrict__ part, const int* __restrict__ st) {
+          const double* __restrict__ t, const double* __restrict__ r0, double* __restrict__ sc, double* __restrict__ x,
+          double* __restrict__ r, double* __restrict__ part, int nb_s, const double* __restrict__ part_s, int nb_t,
+          const double* __restrict__ part_t, int* __restrict__ st) {
     __shared__ double lds[BLOCK / WAVE];
-    if (halted(st)) return;
+    __shared__ int s_state, s_act[2];
+    if (threadIdx.x == 0) { s_state = st[0]; s_act[0] = st[2]; s_act[1] = st[3]; }
+    __syncthreads();
+    if (s_state != ST_RUNNING) return;
+    double a1[2], a2[2], om[2];
+    int half[2];
+    for (int c = 0; c < 2; ++c) {
+        const double ss = sum_partials(part_s + c, nb_s, 6, lds);
+        const double tt = sum_partials(part_t + c, nb_t, 6, lds);
+        const double ts = sum_partials(part_t + 2 + c, nb_t, 6, lds);
+        a1[c] = a2[c] = om[c] = 0.0;
+        half[c] = 0;
+        if (!s_act[c]) continue;
+        a1[c] = sc[B_ALPHA + c];
+        const double s_norm = sqrt(ss);
+        if (s_norm < sc[B_TOL] * sc[B_RN0 + c] || s_norm < 1.0e-11) {
+            half[c] = 1;
+            continue;
+        }
+        om[c] = (tt > sc[B_BD + c]) ? (ts / tt) : 0.0;
+        a2[c] = om[c];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int c = 0; c < 2; ++c) {
+            sc[B_A1 + c] = a1[c];
+            sc[B_A2 + c] = a2[c];
+            sc[B_CW + c] = a2[c];
+            st[8 + c] = half[c];
+            if (s_act[c] && !half[c]) sc[B_OMEGA + c] = om[c];
+        }
     double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += gridDim.x * blockDim.x) {
         const int c = i >= n;
-        x[i] += sc[B_A1 + c] * ph[i] + sc[B_A2 + c] * sh[i];
-        const double ri = s[i] - sc[B_CW + c] * t[i];
+        x[i] += a1[c] * ph[i] + a2[c] * sh[i];
+        const double ri = s[i] - a2[c] * t[i];
         r[i] = ri;
         acc[0][c] += ri * ri;
         acc[1][c] += r0[i] * ri;
@@ -1727,6 +1738,7 @@ int cfa_bicgstab2(int n, const int* rp, const int* ci, const double* val, const 
     double* part_a = ws->part;                 // 6 * RED_BLOCKS each
     double* part_b = ws->part + 6 * RED_BLOCKS;
     double* part_c = ws->part + 12 * RED_BLOCKS;
+    double* part_d = ws->part + 18 * RED_BLOCKS;
     double* sc = ws->scal;
     int* st = state_words(ws);
     int* status = ws->status_dev;
@@ -1756,13 +1768,13 @@ int cfa_bicgstab2(int n, const int* rp, const int* ci, const double* val, const 
         hipLaunchKernelGGL(k_bicg_p, dim3(blocks2), dim3(BLOCK), 0, q_s, n, r, v, sc, p, st);
         ilu_apply_any(ilu, use_f32, p, ph, 2, st, q_s);
         hipLaunchKernelGGL(k_spmv2_dots<TPR>, dim3(nb_rows), dim3(BLOCK), 0, q_s, n, rp, ci, val, ph, v, r0, 1, part_a, st);
-        hipLaunchKernelGGL(k_bicg_final_alpha, dim3(1), dim3(BLOCK), 0, q_s, nb_rows, part_a, sc, st, status);
-        hipLaunchKernelGGL(k_bicg_s, dim3(nb2), dim3(BLOCK), 0, q_s, n, r, v, sc, sv, part_b, st);
+        hipLaunchKernelGGL(k_bicg_s, dim3(nb2), dim3(BLOCK), 0, q_s, n, r, v, sc, sv, part_b, nb_rows, part_a, st,
+                           status);
         ilu_apply_any(ilu, use_f32, sv, sh, 2, st, q_s);
         hipLaunchKernelGGL(k_spmv2_dots<TPR>, dim3(nb_rows), dim3(BLOCK), 0, q_s, n, rp, ci, val, sh, t, sv, 2, part_c, st);
-        hipLaunchKernelGGL(k_bicg_final_omega, dim3(1), dim3(BLOCK), 0, q_s, nb2, part_b, nb_rows, part_c, sc, st);
-        hipLaunchKernelGGL(k_bicg_xr, dim3(nb2), dim3(BLOCK), 0, q_s, n, ph, sh, sv, t, r0, sc, x, r, part_a, st);
-        hipLaunchKernelGGL(k_bicg_final_post, dim3(1), dim3(BLOCK), 0, q_s, nb2, part_a, sc, st, status);
+        hipLaunchKernelGGL(k_bicg_xr, dim3(nb2), dim3(BLOCK), 0, q_s, n, ph, sh, sv, t, r0, sc, x, r, part_d, nb2,
+                           part_b, nb_rows, part_c, st);
+        hipLaunchKernelGGL(k_bicg_final_post, dim3(1), dim3(BLOCK), 0, q_s, nb2, part_d, sc, st, status);
     };
     hipGraphExec_t graph = nullptr;
     if (max_iter > 0) {

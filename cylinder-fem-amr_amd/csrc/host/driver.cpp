@@ -78,6 +78,8 @@ struct PendingRemesh {
     std::vector<double> u, v, p, uo, vo;
     std::vector<int> old_vids;
     double extract_ms = 0.0, csr_ms = 0.0, validate_ms = 0.0;
+    MulticolourIlu0::Pattern ilu;   // prepared by the worker when the momentum ILU is on
+    bool has_ilu = false;
 };
 
 void dump_device(std::ofstream& f, const void* d, size_t bytes) {
@@ -98,6 +100,8 @@ int run_driver(int argc, char** argv) {
         return EXIT_FAILURE;
     }
     GPU_CHECK(cfa_init(0));
+    std::cout.unsetf(std::ios::floatfield);   // a previous in-process run may have left fixed format
+    std::cout << std::setprecision(6);
     const double dt = opt.dt;
     opt.amr.lookahead_time = opt.lookahead * opt.amr_interval * dt;
     const double rot_coeff = opt.rotational ? Sim::RHO * Sim::NU : 0.0;
@@ -423,8 +427,15 @@ int run_driver(int argc, char** argv) {
             g.sync_from_host(mesh, h_u, h_v, h_p, h_uo, h_vo);
             dev_sync();
             remesh_prof.upload += lap();
-            if (opt.momentum_ilu)
-                mom_ilu.set_pattern(mesh.num_nodes, mesh.csr_row_ptr, mesh.csr_col_ind, g.d_row_ptr, g.d_col_ind);
+            if (opt.momentum_ilu) {
+                if (pend.has_ilu) {   // colouring done by the remesh worker, only the upload here
+                    mom_ilu.set_pattern(pend.ilu, g.d_row_ptr, g.d_col_ind);
+                    pend.ilu = MulticolourIlu0::Pattern();
+                    pend.has_ilu = false;
+                } else {
+                    mom_ilu.set_pattern(mesh.num_nodes, mesh.csr_row_ptr, mesh.csr_col_ind, g.d_row_ptr, g.d_col_ind);
+                }
+            }
             remesh_prof.colouring += lap();
             std::function<void()> build_amg;
             if (amg_remap_now)
@@ -496,7 +507,8 @@ int run_driver(int argc, char** argv) {
                 pend.old_vids = amr.node_vertices();
                 pend.join_step = step + opt.amr_overlap;
                 // the worker owns the forest and pend until the join; it never touches the GPU
-                pend.job = std::async(std::launch::async, [&amr, &pend, prm_now, err = h_elem_err]() {
+                const bool want_ilu = opt.momentum_ilu;
+                pend.job = std::async(std::launch::async, [&amr, &pend, prm_now, want_ilu, err = h_elem_err]() {
                     auto tw = Clock::now();
                     pend.st = amr.adapt(err, prm_now);
                     tw = Clock::now();
@@ -508,6 +520,11 @@ int run_driver(int argc, char** argv) {
                     tw = Clock::now();
                     const bool ok = validate_mesh(pend.mesh, pend.q);
                     pend.validate_ms = ms_since(tw);
+                    if (ok && want_ilu) {
+                        pend.ilu = MulticolourIlu0::build_pattern(pend.mesh.num_nodes, pend.mesh.csr_row_ptr,
+                                                                  pend.mesh.csr_col_ind);
+                        pend.has_ilu = true;
+                    }
                     return ok;
                 });
                 amr_ms_total += ms_since(t0);
@@ -527,7 +544,7 @@ int run_driver(int argc, char** argv) {
             remesh_prof.validate += pend.validate_ms;
             remesh_prof.overlapped += pend.st.ms_lookahead + pend.st.ms_mark + pend.st.ms_neighbours +
                                       pend.st.ms_coarsen + pend.st.ms_refine + pend.extract_ms + pend.csr_ms +
-                                      pend.validate_ms;
+                                      pend.validate_ms + (pend.has_ilu ? pend.ilu.build_ms : 0.0);
             g.copy_solution_to_host(h_u, h_v, h_p, h_uo, h_vo, h_vort, h_elem_err);
             amr.inlet_umax = inflow_peak(step * dt, opt.case_2d3);
             std::vector<double> old_x, old_y;

@@ -171,16 +171,16 @@ void ilu_elimination_schedule(int n, const int* rp, const int* ci, const std::ve
     }
 }
 
-void MulticolourIlu0::set_pattern(int n_, const std::vector<int>& rp, const std::vector<int>& ci, const int* d_rp,
-                                  const int* d_ci) {
+MulticolourIlu0::Pattern MulticolourIlu0::build_pattern(int n, const std::vector<int>& rp, const std::vector<int>& ci) {
     const auto t0 = Clock::now();
-    n = n_;
-    nnz = rp[n];
-    d_row_ptr = d_rp;
-    d_col_ind = d_ci;
+    Pattern P;
+    P.n = n;
+    P.nnz = rp[n];
     // greedy colouring in node order: the smallest colour not used by an earlier neighbour
-    std::vector<int> colour(n, -1), diag(n, -1), mark;
-    n_colours = 0;
+    P.colour.assign(n, -1);
+    P.diag.assign(n, -1);
+    std::vector<int> mark;
+    std::vector<int>& colour = P.colour;
     for (int i = 0; i < n; ++i) {
         if (rp[i + 1] - rp[i] > 32) {
             std::fprintf(stderr, "[ILU] row %d has %d entries (limit %d)\n", i, rp[i + 1] - rp[i], 32);
@@ -189,61 +189,85 @@ void MulticolourIlu0::set_pattern(int n_, const std::vector<int>& rp, const std:
         for (int a = rp[i]; a < rp[i + 1]; ++a) {
             const int c = colour[ci[a]];
             if (c >= 0) mark[c] = i;
-            if (ci[a] == i) diag[i] = a;
+            if (ci[a] == i) P.diag[i] = a;
         }
         int c = 0;
-        while (c < n_colours && mark[c] == i) ++c;
-        if (c == n_colours) { ++n_colours; mark.push_back(-1); }
+        while (c < P.n_colours && mark[c] == i) ++c;
+        if (c == P.n_colours) { ++P.n_colours; mark.push_back(-1); }
         colour[i] = c;
     }
-    colour_start.assign(n_colours + 1, 0);
-    for (int i = 0; i < n; ++i) colour_start[colour[i] + 1]++;
-    for (int c = 0; c < n_colours; ++c) colour_start[c + 1] += colour_start[c];
-    std::vector<int> rows(n), fill(colour_start.begin(), colour_start.end() - 1);
-    for (int i = 0; i < n; ++i) rows[fill[colour[i]]++] = i;
+    P.colour_start.assign(P.n_colours + 1, 0);
+    for (int i = 0; i < n; ++i) P.colour_start[colour[i] + 1]++;
+    for (int c = 0; c < P.n_colours; ++c) P.colour_start[c + 1] += P.colour_start[c];
+    P.rows.resize(n);
+    std::vector<int> fill(P.colour_start.begin(), P.colour_start.end() - 1);
+    for (int i = 0; i < n; ++i) P.rows[fill[colour[i]]++] = i;
     // split pattern in colour-row order: L = earlier colours, U = later colours
-    std::vector<int> lp(n + 1, 0), up(n + 1, 0);
+    P.lp.assign(n + 1, 0);
+    P.up.assign(n + 1, 0);
 #pragma omp parallel for schedule(static)
     for (int t = 0; t < n; ++t) {
-        const int i = rows[t];
+        const int i = P.rows[t];
         int l = 0, u = 0;
         for (int a = rp[i]; a < rp[i + 1]; ++a) {
             if (ci[a] == i) continue;
             if (colour[ci[a]] < colour[i]) ++l; else ++u;
         }
-        lp[t + 1] = l;
-        up[t + 1] = u;
+        P.lp[t + 1] = l;
+        P.up[t + 1] = u;
     }
-    for (int t = 0; t < n; ++t) { lp[t + 1] += lp[t]; up[t + 1] += up[t]; }
-    nl = lp[n];
-    nu = up[n];
-    std::vector<int> lc(nl), uc(nu), l_src(nl), u_src(nu), d_src(n);
+    for (int t = 0; t < n; ++t) { P.lp[t + 1] += P.lp[t]; P.up[t + 1] += P.up[t]; }
+    P.nl = P.lp[n];
+    P.nu = P.up[n];
+    P.lc.resize(P.nl);
+    P.uc.resize(P.nu);
+    P.l_src.resize(P.nl);
+    P.u_src.resize(P.nu);
+    P.d_src.resize(n);
 #pragma omp parallel for schedule(static)
     for (int t = 0; t < n; ++t) {
-        const int i = rows[t];
-        int l = lp[t], u = up[t];
+        const int i = P.rows[t];
+        int l = P.lp[t], u = P.up[t];
         for (int a = rp[i]; a < rp[i + 1]; ++a) {
             const int j = ci[a];
-            if (j == i) d_src[t] = a;
-            else if (colour[j] < colour[i]) { lc[l] = j; l_src[l++] = a; }
-            else { uc[u] = j; u_src[u++] = a; }
+            if (j == i) P.d_src[t] = a;
+            else if (colour[j] < colour[i]) { P.lc[l] = j; P.l_src[l++] = a; }
+            else { P.uc[u] = j; P.u_src[u++] = a; }
         }
     }
-    std::vector<int> el_ptr, el, upd;
-    ilu_elimination_schedule(n, rp.data(), ci.data(), colour, rows, diag, el_ptr, el, upd);
-    d_el_ptr.upload(el_ptr);
-    d_el.upload(el);
-    d_upd.upload(upd);
-    d_rows.upload(rows);
-    d_colour.upload(colour);
-    d_diag_pos.upload(diag);
-    d_lp.upload(lp);
-    d_up.upload(up);
-    d_lc.upload(lc);
-    d_uc.upload(uc);
-    d_l_src.upload(l_src);
-    d_u_src.upload(u_src);
-    d_d_src.upload(d_src);
+    ilu_elimination_schedule(n, rp.data(), ci.data(), colour, P.rows, P.diag, P.el_ptr, P.el, P.upd);
+    P.build_ms = ms_since(t0);
+    return P;
+}
+
+void MulticolourIlu0::set_pattern(int n_, const std::vector<int>& rp, const std::vector<int>& ci, const int* d_rp,
+                                  const int* d_ci) {
+    set_pattern(build_pattern(n_, rp, ci), d_rp, d_ci);
+}
+
+void MulticolourIlu0::set_pattern(const Pattern& P, const int* d_rp, const int* d_ci) {
+    const auto t0 = Clock::now();
+    n = P.n;
+    nnz = P.nnz;
+    n_colours = P.n_colours;
+    nl = P.nl;
+    nu = P.nu;
+    colour_start = P.colour_start;
+    d_row_ptr = d_rp;
+    d_col_ind = d_ci;
+    d_el_ptr.upload(P.el_ptr);
+    d_el.upload(P.el);
+    d_upd.upload(P.upd);
+    d_rows.upload(P.rows);
+    d_colour.upload(P.colour);
+    d_diag_pos.upload(P.diag);
+    d_lp.upload(P.lp);
+    d_up.upload(P.up);
+    d_lc.upload(P.lc);
+    d_uc.upload(P.uc);
+    d_l_src.upload(P.l_src);
+    d_u_src.upload(P.u_src);
+    d_d_src.upload(P.d_src);
     d_lu.reserve(nnz, 256);
     d_lv.reserve(std::max(nl, 1), 256);
     d_uv.reserve(std::max(nu, 1), 256);
@@ -254,7 +278,7 @@ void MulticolourIlu0::set_pattern(int n_, const std::vector<int>& rp, const std:
         d_dinv32.reserve(n, 64);
     }
     refresh_view();
-    colour_ms += ms_since(t0);
+    colour_ms += P.build_ms + ms_since(t0);
 }
 
 void MulticolourIlu0::refresh_view() {

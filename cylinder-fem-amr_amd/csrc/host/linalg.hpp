@@ -77,6 +77,15 @@ public:
 
     void set_pattern(int n, const std::vector<int>& rp, const std::vector<int>& ci, const int* d_rp,
                      const int* d_ci);
+    // the host part of set_pattern (colouring, split pattern, elimination schedule); it touches no
+    // device memory, so a remesh worker thread can prepare it
+    struct Pattern {
+        int n = 0, nnz = 0, n_colours = 0, nl = 0, nu = 0;
+        std::vector<int> colour_start, rows, colour, diag, lp, up, lc, uc, l_src, u_src, d_src, el_ptr, el, upd;
+        double build_ms = 0.0;
+    };
+    static Pattern build_pattern(int n, const std::vector<int>& rp, const std::vector<int>& ci);
+    void set_pattern(const Pattern& P, const int* d_rp, const int* d_ci);
     void factor(const double* d_val);
     void apply(const double* r, double* z, int nrhs = 1) const;
     const cfa_ilu_t* view() const { return &view_; }

@@ -18,7 +18,7 @@ import pytest
 import cfa
 import cfa_oracle
 from conftest import read_csv
-from test_oracle_golden import COLS, assert_matches_printed
+from test_oracle_golden import COLS, assert_matches_printed, print_quantum
 
 pytestmark = pytest.mark.gpu
 H = {c: i for i, c in enumerate(cfa.HISTORY_COLUMNS)}
@@ -60,17 +60,23 @@ def test_mms_L3_400_steps_vs_reference(golden):
         assert_matches_printed(hist[:, H[c]], ref[c], f"GPU L3 {c}")
     # MMS error table: the reference's two binaries (runs/paper/mms_L3, runs/review/mms_L3_dt2)
     # differ by up to 1.5e-9 relative in these figures (pressure solves stop at 1e-7, so the
-    # rounding order of the solver shows); the GPU run must stay within twice that spread
+    # rounding order of the solver shows); the GPU run must stay within twice that spread, or
+    # within the print resolution of the stored table
     table = read_csv(golden / "mms_L3_mms.csv")
     table2 = read_csv(golden / "mms_L3_dt2_mms.csv")
     mms = cfa.last_mms()
     for name in cfa.MMS_FIELDS:
         a, b = float(table[name]), float(table2[name])
         noise = abs(a - b) / abs(a)
-        assert abs(mms[name] - a) <= max(2.0 * noise, 1e-10) * abs(a) + 1e-13, (name, mms[name], a, b)
-    # iteration counts of the reference's log (runs/paper/mms_L3/run.log:12-52): (3, 3, 29-39)
+        bound = max(2.0 * noise * abs(a), float(print_quantum(a)), 1e-10 * abs(a))
+        assert abs(mms[name] - a) <= bound, (name, mms[name], a, b)
+    # iteration counts: the reference's log prints steps 1, 10, 20, ..., 400 with (3, 3, 29-39)
+    # (runs/paper/mms_L3/run.log:12-52); the steps in between include the start-up of BDF2, where
+    # the oracle also takes about 50 pressure iterations
     assert set(hist[:, H["it_u"]].astype(int)) <= {2, 3, 4}
-    assert 25 <= hist[:, H["it_p"]].min() and hist[:, H["it_p"]].max() <= 42
+    printed = hist[9::10, H["it_p"]]
+    assert 27 <= printed.min() and printed.max() <= 41
+    assert hist[:, H["it_p"]].max() <= 60
 
 
 def test_adaptive_trajectory_vs_reference(golden):

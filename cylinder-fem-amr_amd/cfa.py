@@ -167,6 +167,93 @@ def workload_momentum(level: int, tol: float = 1e-8, kernel_reps: int = 0) -> di
     return dict(zip(MOMENTUM_FIELDS, out.tolist()))
 
 
+DUMP_MOMENTUM_FIELDS = ("n", "nnz", "colours", "colouring_ms", "factor_ms", "solve_ms", "it_u", "it_v",
+                        "converged_u", "converged_v", "rel_dev_from_dumped")
+DUMP_PRESSURE_FIELDS = ("n", "nnz", "amg_levels", "amg_setup_ms", "solve_ms", "iterations", "converged",
+                        "rel_residual")
+
+
+def read_system(path: str | os.PathLike) -> dict:
+    """Parses a --dump-systems file on the host (the layout of src/cylinder_fem_amr.cu:331-352,
+    375-378 for <prefix>_momentum.bin and :426-438 for <prefix>_pressure.bin; the reference reads
+    it the same way in scripts/momentum_study/study.py load()). Keys: n, nnz, rp, ci, val, then
+    for momentum b_u, b_v, u0, v0, x, y, bc_tag and, when present, u, v; for pressure b, x0."""
+    raw = Path(path).read_bytes()
+    n, nnz = (int(k) for k in np.frombuffer(raw, np.int32, 2, 0))
+    base = 8 + 4 * (n + 1) + 12 * nnz
+    momentum = len(raw) in (base + 52 * n, base + 68 * n)
+    if n <= 0 or nnz <= 0 or not (momentum or len(raw) == base + 16 * n):
+        raise ValueError(f"{path}: size {len(raw)} fits neither dump layout for n={n}, nnz={nnz}")
+    off = 8
+    out = {"n": n, "nnz": nnz}
+
+    def take(key, dtype, count):
+        nonlocal off
+        a = np.frombuffer(raw, dtype, count, off).copy()
+        off += a.nbytes
+        out[key] = a
+
+    take("rp", np.int32, n + 1)
+    take("ci", np.int32, nnz)
+    take("val", np.float64, nnz)
+    if momentum:
+        for key in ("b_u", "b_v", "u0", "v0", "x", "y"):
+            take(key, np.float64, n)
+        take("bc_tag", np.int32, n)
+        if off < len(raw):
+            take("u", np.float64, n)
+            take("v", np.float64, n)
+    else:
+        take("b", np.float64, n)
+        take("x0", np.float64, n)
+    return out
+
+
+def _declare_dump_functions(host):
+    c_int, c_double, c_char_p = ctypes.c_int, ctypes.c_double, ctypes.c_char_p
+    dptr = ctypes.POINTER(c_double)
+    host.cfa_dumped_momentum_solve.argtypes = [c_char_p, c_double, c_int, c_int, c_int, dptr, dptr, dptr]
+    host.cfa_dumped_momentum_solve.restype = c_int
+    host.cfa_dumped_pressure_solve.argtypes = [c_char_p, c_double, c_int, c_int, c_int, c_int, dptr, dptr]
+    host.cfa_dumped_pressure_solve.restype = c_int
+    host.cfa_dump_header.argtypes = [c_char_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
+    host.cfa_dump_header.restype = c_int
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        hip, _ = load()
+        raise RuntimeError(f"{what} failed: {hip.cfa_error_string(rc).decode()} ({rc})")
+
+
+def solve_dumped_momentum(path: str | os.PathLike, tol: float = 1e-7, max_iter: int = 150, precision: str = "fp64",
+                          kernel_reps: int = 0):
+    """BiCGSTAB2 with multicolour ILU(0) on a dumped momentum system, on the GPU (the library's own
+    solver classes). Returns (stats dict, u, v)."""
+    _, host = load()
+    _declare_dump_functions(host)
+    n, nnz = ctypes.c_int(0), ctypes.c_int(0)
+    _check(host.cfa_dump_header(str(path).encode(), ctypes.byref(n), ctypes.byref(nnz)), "cfa_dump_header")
+    u, v, out = np.zeros(n.value), np.zeros(n.value), np.zeros(11)
+    _check(host.cfa_dumped_momentum_solve(str(path).encode(), tol, max_iter, {"fp64": 0, "mixed": 1}[precision],
+                                          kernel_reps, _dptr(u), _dptr(v), _dptr(out)), "cfa_dumped_momentum_solve")
+    return dict(zip(DUMP_MOMENTUM_FIELDS, out.tolist())), u, v
+
+
+def solve_dumped_pressure(path: str | os.PathLike, tol: float = 1e-7, max_iter: int = 4000, precision: str = "fp64",
+                          from_zero: bool = False, kernel_reps: int = 0):
+    """SA-AMG PCG on K_p of a dumped pressure system, on the GPU. Returns (stats dict, x)."""
+    _, host = load()
+    _declare_dump_functions(host)
+    n, nnz = ctypes.c_int(0), ctypes.c_int(0)
+    _check(host.cfa_dump_header(str(path).encode(), ctypes.byref(n), ctypes.byref(nnz)), "cfa_dump_header")
+    x, out = np.zeros(n.value), np.zeros(8)
+    _check(host.cfa_dumped_pressure_solve(str(path).encode(), tol, max_iter, {"fp64": 0, "mixed": 1}[precision],
+                                          1 if from_zero else 0, kernel_reps, _dptr(x), _dptr(out)),
+           "cfa_dumped_pressure_solve")
+    return dict(zip(DUMP_PRESSURE_FIELDS, out.tolist())), x
+
+
 def last_mms() -> dict:
     _, host = load()
     out = np.zeros(9)

@@ -33,6 +33,7 @@ const char* cfa_error_string(int code) {
         case CFA_E_ARG: return "invalid argument";
         case CFA_E_ROWLEN: return "CSR row too long for the kernel";
         case CFA_E_NOTREADY: return "object not set up";
+        case CFA_E_IO: return "file missing or not in the expected layout";
         default: return code > 0 ? hipGetErrorString((hipError_t)code) : "unknown error";
     }
 }

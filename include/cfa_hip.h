@@ -32,7 +32,8 @@ enum {
     CFA_OK = 0,
     CFA_E_ARG = -1,      /* invalid argument (null pointer, negative size, ...) */
     CFA_E_ROWLEN = -2,   /* a CSR row longer than the kernel limit (ILU: 32, as linalg.cuh:361) */
-    CFA_E_NOTREADY = -3  /* object not set up */
+    CFA_E_NOTREADY = -3, /* object not set up */
+    CFA_E_IO = -4        /* file missing or not in the expected layout */
 };
 
 /* ---------------------------------------------------------------------------------------------

@@ -75,6 +75,29 @@ void cfa_mesh_probes(void* mesh, int* ab, double* w);
 int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, double* out);
 int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out);
 
+/* Solver-study dumps written by --dump-systems N (src/cylinder_fem_amr.cu:331-352 and :375-378:
+ * <prefix>_momentum.bin = int32 n, nnz; row_ptr [n + 1], col_ind [nnz] int32; values [nnz],
+ * rhs_u, rhs_v, u0, v0, x, y [n] float64; bc_tag [n] int32; then the solutions u, v [n];
+ * :426-438: <prefix>_pressure.bin = int32 n, nnz; row_ptr, col_ind; K_p values, rhs, initial
+ * guess). The reference reads them in scripts/momentum_study (study.py load(), mb.cu, mb_amg.cu).
+ * All return CFA_E_IO for a missing file or one whose size or CSR pattern does not fit.
+ * cfa_dump_header: n and nnz of either file.
+ * cfa_dumped_momentum_solve: BiCGSTAB2 with multicolour ILU(0) (precision 0: FP64 factors,
+ * 1: FP32 factors) from the dumped u0, v0 to tol; x_u, x_v [n] (host, may be NULL) receive the
+ * solution. out[11] = n, nnz, colours, colouring ms, factor ms, solve ms, it_u, it_v,
+ * converged_u, converged_v, max relative deviation from the dumped solution (-1 if the file has
+ * none).
+ * cfa_dumped_pressure_solve: PCG with SA-AMG (precision 1: FP32 cycle, FP64 coarse solve) on K_p
+ * alone (the files hold no OSS operator) from the dumped guess, or from 0 with from_zero = 1 (as
+ * mb_amg.cu); x [n] may be NULL. out[8] = n, nnz, AMG levels, set-up ms, solve ms, iterations,
+ * converged, final relative residual.
+ * kernel_reps > 0 also times the SpMV, the ILU apply or the V-cycle (cfa_last_run_kernels). */
+int cfa_dump_header(const char* path, int* n, int* nnz);
+int cfa_dumped_momentum_solve(const char* path, double tol, int max_iter, int precision, int kernel_reps,
+                              double* x_u, double* x_v, double* out);
+int cfa_dumped_pressure_solve(const char* path, double tol, int max_iter, int precision, int from_zero,
+                              int kernel_reps, double* x, double* out);
+
 /* DFG 2D-2 statistics of a force history, as the program reports them at the end of a run
  * (src/cfa/diagnostics.cuh:47-121 ForceHistory::compute; window = --stats-window):
  * out[13] = number of lift periods, the 6 period means (C_D max, C_D min, C_L max, C_L min,

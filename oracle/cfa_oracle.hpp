@@ -155,6 +155,7 @@ struct RunConfig {
     int print_interval = 10;
     bool quiet = false;
     std::string prefix;   // empty: no files
+    int dump_systems = 0;  // > 0 (with a prefix): write the momentum and pressure systems of this step
 };
 struct StepRow {
     int step;

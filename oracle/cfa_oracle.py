@@ -26,20 +26,21 @@ def load():
         _lib = ctypes.CDLL(str(LIB))
         d = ctypes.POINTER(ctypes.c_double)
         _lib.oracle_run_fixed.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                          ctypes.c_double, ctypes.c_double, ctypes.c_char_p, d, d, d]
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_char_p, ctypes.c_int, d, d, d]
         _lib.oracle_run_fixed.restype = ctypes.c_int
     return _lib
 
 
 def run_fixed(steps: int, dt: float, uniform_level: int, min_level: int = 2, mms: bool = False,
-              mom_tol: float = 1e-7, ppe_tol: float = 1e-7):
+              mom_tol: float = 1e-7, ppe_tol: float = 1e-7, prefix: str | None = None, dump_systems: int = 0):
     """Fixed-mesh run of the restated time loop. Returns (rows (steps x 13), mms[9], wall seconds);
-    it_uv = it_u + 1000 it_v."""
+    it_uv = it_u + 1000 it_v. With a prefix the CSV files are written, and with dump_systems = N
+    the momentum and pressure systems of step N (the reference's --dump-systems layout)."""
     lib = load()
     rows = np.zeros((steps, len(ROW_COLUMNS)))
     mms_out = np.zeros(9)
     wall = np.zeros(1)
     d = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
-    k = lib.oracle_run_fixed(steps, dt, uniform_level, min_level, 1 if mms else 0, mom_tol, ppe_tol, None,
-                             d(rows), d(mms_out), d(wall))
+    k = lib.oracle_run_fixed(steps, dt, uniform_level, min_level, 1 if mms else 0, mom_tol, ppe_tol,
+                             prefix.encode() if prefix else None, dump_systems, d(rows), d(mms_out), d(wall))
     return rows[:k], mms_out, float(wall[0])

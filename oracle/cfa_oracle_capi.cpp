@@ -10,9 +10,11 @@ extern "C" {
 
 // Fixed-mesh run. rows: steps x 13 doubles (step, t, C_D, C_L, dP, nodes, elements, p_front,
 // p_back, C_D_surface, C_L_surface, it_u + 1000 it_v, it_p); mms: 9 doubles (6 norms, cyl, probe,
-// h_max). Returns the number of rows written.
+// h_max). With a prefix the forces/MMS CSV files are written, and with dump_systems > 0 the
+// systems of that step (<prefix>_momentum.bin, <prefix>_pressure.bin). Returns the number of rows.
 int oracle_run_fixed(int steps, double dt, int uniform_level, int min_level, int mms, double mom_tol,
-                     double ppe_tol, const char* prefix, double* rows, double* mms_out, double* wall_s) {
+                     double ppe_tol, const char* prefix, int dump_systems, double* rows, double* mms_out,
+                     double* wall_s) {
     RunConfig c;
     c.steps = steps;
     c.dt = dt;
@@ -23,6 +25,7 @@ int oracle_run_fixed(int steps, double dt, int uniform_level, int min_level, int
     c.ppe_tol = ppe_tol;
     c.quiet = true;
     if (prefix) c.prefix = prefix;
+    c.dump_systems = dump_systems;
     RunResult r = run_fixed(c);
     for (size_t k = 0; k < r.rows.size(); ++k) {
         const StepRow& s = r.rows[k];

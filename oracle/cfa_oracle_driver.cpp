@@ -56,6 +56,13 @@ Probe find_probe(const Mesh2D& m, double theta) {
     }
     return best;
 }
+// Solver-study dump of one step's systems, the layout written by src/cylinder_fem_amr.cu:331-352
+// (momentum: int32 n, nnz; row_ptr, col_ind int32; values, rhs_u, rhs_v, u0, v0, x, y float64;
+// bc_tag int32; the solutions u, v appended after the solve, :375-378) and :426-438 (pressure:
+// int32 n, nnz; row_ptr, col_ind; K_p values, rhs, initial guess).
+void dump_raw(std::FILE* f, const void* data, size_t bytes) {
+    if (f && bytes) std::fwrite(data, 1, bytes, f);
+}
 }  // namespace
 
 RunResult run_fixed(const RunConfig& cfg) {
@@ -144,9 +151,27 @@ RunResult run_fixed(const RunConfig& cfg) {
                           cfg.mms ? gy.data() : nullptr);
         apply_bc_momentum(n, tag, x, y, cfa::inflow_peak(step * dt, cfg.case_2d3), cfg.mms ? step * dt : -1.0,
                           rp, ci, A.data(), rhs_u.data(), rhs_v.data(), u.data(), v.data());
+        const bool dump_now = step == cfg.dump_systems && !cfg.prefix.empty();
+        std::FILE* dump = dump_now ? std::fopen((cfg.prefix + "_momentum.bin").c_str(), "wb") : nullptr;
+        if (dump) {
+            const int hdr[2] = {n, nnz};
+            dump_raw(dump, hdr, sizeof(hdr));
+            dump_raw(dump, rp, (n + 1) * sizeof(int));
+            dump_raw(dump, ci, nnz * sizeof(int));
+            dump_raw(dump, A.data(), nnz * sizeof(double));
+            for (const double* a : {(const double*)rhs_u.data(), (const double*)rhs_v.data(),
+                                    (const double*)u.data(), (const double*)v.data(), x, y})
+                dump_raw(dump, a, n * sizeof(double));
+            dump_raw(dump, tag, n * sizeof(int));
+        }
         ilu.factor(A.data());
         const Bicg2Info bi = bicgstab2(n, rp, ci, A.data(), rhs_u.data(), rhs_v.data(), u.data(), v.data(),
                                        150, cfg.mom_tol, ilu);
+        if (dump) {
+            dump_raw(dump, u.data(), n * sizeof(double));
+            dump_raw(dump, v.data(), n * sizeof(double));
+            std::fclose(dump);
+        }
         res.nonconverged += (bi.converged[0] ? 0 : 1) + (bi.converged[1] ? 0 : 1);
 
         // Step B: pressure
@@ -165,6 +190,18 @@ RunResult run_fixed(const RunConfig& cfg) {
                                   mass.data(), elem_s.data(), dt_proj, Sim::RHO, cfg.pressure_stab, rp, ci,
                                   Kp.data(), rhs_p.data(), div_num.data());
         apply_bc_pressure(n, tag, rp, ci, Kp.data(), rhs_p.data(), dp.data());
+        if (dump_now) {
+            if (std::FILE* f = std::fopen((cfg.prefix + "_pressure.bin").c_str(), "wb")) {
+                const int hdr[2] = {n, nnz};
+                dump_raw(f, hdr, sizeof(hdr));
+                dump_raw(f, rp, (n + 1) * sizeof(int));
+                dump_raw(f, ci, nnz * sizeof(int));
+                dump_raw(f, Kp.data(), nnz * sizeof(double));
+                dump_raw(f, rhs_p.data(), n * sizeof(double));
+                dump_raw(f, dp.data(), n * sizeof(double));
+                std::fclose(f);
+            }
+        }
         if (oss) {
             std::fill(oss_val.begin(), oss_val.end(), 0.0);
             assemble_oss_operator(ne, ev.data(), x, y, elem_s.data(), rp, ci, nnz, oss_val.data());

@@ -254,6 +254,95 @@ def solve_dumped_pressure(path: str | os.PathLike, tol: float = 1e-7, max_iter: 
     return dict(zip(DUMP_PRESSURE_FIELDS, out.tolist())), x
 
 
+ADJOINT_STATS = ("forward_ms", "reverse_ms", "recomputed_steps", "adjoint_pcg_iterations",
+                 "adjoint_bicgstab2_iterations")
+
+
+class Adjoint:
+    """Frozen uniform-mesh window of the program's time step with its discrete adjoint (C ABI
+    cfa_adjoint_*, cfa_step_fwd, cfa_window_forward, cfa_window_adjoint). theta = (nu, U_max);
+    a state is 6 n doubles [u | v | u_old | v_old | p | dp]; the window functional is the mean of
+    weights[0] C_D + weights[1] C_L over the window's steps."""
+
+    def __init__(self, level: int, dt: float = 1e-3, tol: float = 1e-12):
+        _, host = load()
+        c_int, c_double, vp = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+        dptr, iptr = ctypes.POINTER(c_double), ctypes.POINTER(c_int)
+        host.cfa_adjoint_create.argtypes = [c_int, c_double, c_double]
+        host.cfa_adjoint_create.restype = vp
+        host.cfa_adjoint_destroy.argtypes = [vp]
+        host.cfa_adjoint_destroy.restype = None
+        host.cfa_adjoint_sizes.argtypes = [vp, iptr, iptr, iptr]
+        host.cfa_adjoint_sizes.restype = c_int
+        host.cfa_adjoint_initial_state.argtypes = [vp, dptr]
+        host.cfa_adjoint_initial_state.restype = c_int
+        host.cfa_step_fwd.argtypes = [vp, dptr, c_int, dptr, dptr, dptr]
+        host.cfa_step_fwd.restype = c_int
+        host.cfa_window_forward.argtypes = [vp, dptr, c_int, c_int, dptr, dptr, dptr, dptr, dptr]
+        host.cfa_window_forward.restype = c_int
+        host.cfa_window_adjoint.argtypes = [vp, dptr, c_int, c_int, c_int, dptr, dptr, dptr, dptr, dptr, dptr]
+        host.cfa_window_adjoint.restype = c_int
+        self._host = host
+        self._ctx = host.cfa_adjoint_create(level, dt, tol)
+        if not self._ctx:
+            raise RuntimeError(f"cfa_adjoint_create(level={level}) failed")
+        sizes = np.zeros(3, dtype=np.int32)
+        host.cfa_adjoint_sizes(self._ctx, _iptr(sizes[0:1]), _iptr(sizes[1:2]), _iptr(sizes[2:3]))
+        self.n, self.ne, self.nnz = (int(k) for k in sizes)
+
+    def close(self):
+        if self._ctx:
+            self._host.cfa_adjoint_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _vec(a, size=None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if size is not None and a.size != size:
+            raise ValueError(f"expected {size} values, got {a.size}")
+        return a
+
+    def initial_state(self) -> np.ndarray:
+        out = np.zeros(6 * self.n)
+        _check(self._host.cfa_adjoint_initial_state(self._ctx, _dptr(out)), "cfa_adjoint_initial_state")
+        return out
+
+    def step(self, theta, step: int, state):
+        th = self._vec(theta, 2)
+        s_in = self._vec(state, 6 * self.n)
+        s_out, forces = np.zeros(6 * self.n), np.zeros(2)
+        _check(self._host.cfa_step_fwd(self._ctx, _dptr(th), step, _dptr(s_in), _dptr(s_out), _dptr(forces)),
+               "cfa_step_fwd")
+        return s_out, forces
+
+    def forward(self, theta, first_step: int, steps: int, state0, weights=(1.0, 0.0)):
+        """Returns (J, final state, history (steps x 2) of C_D, C_L)."""
+        th, w = self._vec(theta, 2), self._vec(weights, 2)
+        s0 = self._vec(state0, 6 * self.n)
+        s_out, J, hist = np.zeros(6 * self.n), np.zeros(1), np.zeros((steps, 2))
+        _check(self._host.cfa_window_forward(self._ctx, _dptr(th), first_step, steps, _dptr(w), _dptr(s0),
+                                             _dptr(s_out), _dptr(J), _dptr(hist)), "cfa_window_forward")
+        return float(J[0]), s_out, hist
+
+    def gradient(self, theta, first_step: int, steps: int, state0, checkpoint_every: int = 20,
+                 weights=(1.0, 0.0), state_gradient: bool = False):
+        """Returns (J, dJ/dtheta [2], dJ/dstate0 [6 n] or None, stats dict)."""
+        th, w = self._vec(theta, 2), self._vec(weights, 2)
+        s0 = self._vec(state0, 6 * self.n)
+        J, g, st = np.zeros(1), np.zeros(2), np.zeros(5)
+        gs = np.zeros(6 * self.n) if state_gradient else None
+        _check(self._host.cfa_window_adjoint(self._ctx, _dptr(th), first_step, steps, checkpoint_every, _dptr(w),
+                                             _dptr(s0), _dptr(J), _dptr(g), _dptr(gs) if gs is not None else None,
+                                             _dptr(st)), "cfa_window_adjoint")
+        return float(J[0]), g, gs, dict(zip(ADJOINT_STATS, st.tolist()))
+
+
 def last_mms() -> dict:
     _, host = load()
     out = np.zeros(9)

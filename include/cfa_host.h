@@ -112,6 +112,39 @@ int cfa_force_statistics(int n, const double* t, const double* cd, const double*
 int cfa_amg_setup_record(int n, const int* rp, const int* ci, const double* val, int* level_rows,
                          int* level_nnz, double* level_omega, int* coarse_n, int* agg0);
 
+/* Discrete adjoint of the projection step on a frozen uniform mesh (no counterpart in the
+ * reference, which has no derivative code). The step is the program's time step
+ * (src/cylinder_fem_amr.cu:277-538, default options: BDF2 after step 1, rotational projection,
+ * OSS 0.3, SUPG time scale 1e-3, ILU(0)-BiCGSTAB2 momentum, SA-AMG PCG pressure rebuilt every 50
+ * steps, do-nothing outlet) with parameters theta = {viscosity nu, inflow peak velocity U_max}
+ * (the reference's constants are nu = 1e-3, U_max = 1.5) and both solvers run to relative
+ * tolerance tol. A state is 6 n doubles [u | v | u_old | v_old | p | dp] (host arrays; dp is the
+ * pressure solver's warm start). `step` numbers are the program's (step 1 is the BDF1 start).
+ * The window functional is J = (1/W) sum over W steps of (weights[0] C_D + weights[1] C_L), the
+ * volume-integral force coefficients (src/cfa/fem_kernels.cuh:1046-1123).
+ * cfa_adjoint_create: level 0..6 uniform NVB refinements of the base mesh; NULL on failure.
+ * cfa_adjoint_initial_state: the DFG 2D-2 initial condition (src/cfa/device.cuh:466-486).
+ * cfa_step_fwd: one step, forces[2] = (C_D, C_L) of the new state (may be NULL).
+ * cfa_window_forward: W steps from state0; state_out (may be NULL), J, history [2 W] (may be
+ * NULL) = (C_D, C_L) per step.
+ * cfa_window_adjoint: J and dJ/dtheta [2] by the discrete adjoint with checkpoints every
+ * checkpoint_every steps (0: one segment) and recomputation in between; dJ_dstate0 [6 n] (may
+ * be NULL) = dJ/dstate0 (zero in the dp slot); stats [5] (may be NULL) = forward ms,
+ * recompute + reverse ms, recomputed steps, adjoint PCG iterations, adjoint BiCGSTAB2
+ * iterations. */
+typedef struct cfa_adjoint cfa_adjoint_t;
+cfa_adjoint_t* cfa_adjoint_create(int level, double dt, double tol);
+void cfa_adjoint_destroy(cfa_adjoint_t* ctx);
+int cfa_adjoint_sizes(const cfa_adjoint_t* ctx, int* n, int* ne, int* nnz);
+int cfa_adjoint_initial_state(const cfa_adjoint_t* ctx, double* state);
+int cfa_step_fwd(cfa_adjoint_t* ctx, const double* theta, int step, const double* state_in, double* state_out,
+                 double* forces);
+int cfa_window_forward(cfa_adjoint_t* ctx, const double* theta, int first_step, int steps, const double* weights,
+                       const double* state0, double* state_out, double* J, double* history);
+int cfa_window_adjoint(cfa_adjoint_t* ctx, const double* theta, int first_step, int steps, int checkpoint_every,
+                       const double* weights, const double* state0, double* J, double* dJ_dtheta,
+                       double* dJ_dstate0, double* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -156,6 +156,8 @@ struct RunConfig {
     bool quiet = false;
     std::string prefix;   // empty: no files
     int dump_systems = 0;  // > 0 (with a prefix): write the momentum and pressure systems of this step
+    double nu = cfa::Sim::NU;        // viscosity of the step (the reference's constant by default)
+    double u_max = -1.0;             // > 0: inflow peak of the Dirichlet data instead of inflow_peak(t)
 };
 struct StepRow {
     int step;

@@ -28,6 +28,9 @@ def load():
         _lib.oracle_run_fixed.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_char_p, ctypes.c_int, d, d, d]
         _lib.oracle_run_fixed.restype = ctypes.c_int
+        _lib.oracle_run_params.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, d]
+        _lib.oracle_run_params.restype = ctypes.c_int
     return _lib
 
 
@@ -44,3 +47,13 @@ def run_fixed(steps: int, dt: float, uniform_level: int, min_level: int = 2, mms
     k = lib.oracle_run_fixed(steps, dt, uniform_level, min_level, 1 if mms else 0, mom_tol, ppe_tol,
                              prefix.encode() if prefix else None, dump_systems, d(rows), d(mms_out), d(wall))
     return rows[:k], mms_out, float(wall[0])
+
+
+def run_params(steps: int, dt: float, uniform_level: int, nu: float, u_max: float, tol: float = 1e-12):
+    """DFG flow on a fixed uniform mesh from the initial condition with viscosity nu and inflow
+    peak u_max, solvers at relative tolerance tol. Returns rows (steps x 13)."""
+    lib = load()
+    rows = np.zeros((steps, len(ROW_COLUMNS)))
+    k = lib.oracle_run_params(steps, dt, uniform_level, nu, u_max, tol,
+                              rows.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return rows[:k]

@@ -69,7 +69,7 @@ RunResult run_fixed(const RunConfig& cfg) {
     RunResult res;
     const auto t_wall0 = cfa::Clock::now();
     const double dt = cfg.dt;
-    const double rot_coeff = cfg.rotational ? Sim::RHO * Sim::NU : 0.0;
+    const double rot_coeff = cfg.rotational ? Sim::RHO * cfg.nu : 0.0;
     const double force_scale = -2.0 / (Sim::U_MEAN * Sim::U_MEAN * 2.0 * Sim::R_CYL);
     const double surface_scale = -force_scale;
 
@@ -145,11 +145,12 @@ RunResult run_fixed(const RunConfig& cfg) {
         std::fill(mass.begin(), mass.end(), 0.0);
         if (cfg.mms) mms_forcing(n, x, y, step * dt, gx.data(), gy.data());
         assemble_momentum(ne, ev.data(), x, y, u.data(), v.data(), u_prev.data(), v_prev.data(), p.data(),
-                          Sim::NU, Sim::RHO, a0, a1, a2, c1, c2,
+                          cfg.nu, Sim::RHO, a0, a1, a2, c1, c2,
                           cfg.supg_tau_time > 0.0 ? cfg.supg_tau_time : dt, 0.5, rp, ci, A.data(),
                           rhs_u.data(), rhs_v.data(), mass.data(), cfg.mms ? gx.data() : nullptr,
                           cfg.mms ? gy.data() : nullptr);
-        apply_bc_momentum(n, tag, x, y, cfa::inflow_peak(step * dt, cfg.case_2d3), cfg.mms ? step * dt : -1.0,
+        apply_bc_momentum(n, tag, x, y, cfg.u_max > 0.0 ? cfg.u_max : cfa::inflow_peak(step * dt, cfg.case_2d3),
+                          cfg.mms ? step * dt : -1.0,
                           rp, ci, A.data(), rhs_u.data(), rhs_v.data(), u.data(), v.data());
         const bool dump_now = step == cfg.dump_systems && !cfg.prefix.empty();
         std::FILE* dump = dump_now ? std::fopen((cfg.prefix + "_momentum.bin").c_str(), "wb") : nullptr;
@@ -182,7 +183,7 @@ RunResult run_fixed(const RunConfig& cfg) {
         std::fill(gy.begin(), gy.end(), 0.0);
         const bool oss = cfg.pressure_stab > 0.0;
         if (oss) {
-            compute_oss_weights(ne, ev.data(), x, y, u.data(), v.data(), dt_proj, Sim::RHO, Sim::NU,
+            compute_oss_weights(ne, ev.data(), x, y, u.data(), v.data(), dt_proj, Sim::RHO, cfg.nu,
                                 cfg.pressure_stab, elem_s.data());
             accumulate_scalar_gradient(ne, ev.data(), x, y, p.data(), elem_s.data(), gx.data(), gy.data());
         }
@@ -242,11 +243,11 @@ RunResult run_fixed(const RunConfig& cfg) {
         // Step D: forces and probes
         double diag[6] = {0, 0, 0, 0, 0, 0};
         volume_forces(ne, ev.data(), tag, x, y, u.data(), v.data(), u_old.data(), v_old.data(), u_prev.data(),
-                      v_prev.data(), p.data(), a0, -a1, -a2, Sim::NU, Sim::RHO, diag);
+                      v_prev.data(), p.data(), a0, -a1, -a2, cfg.nu, Sim::RHO, diag);
         diag[2] = (1.0 - pf.w) * p[pf.a] + pf.w * p[pf.b];
         diag[3] = (1.0 - pb.w) * p[pb.a] + pb.w * p[pb.b];
         surface_forces((int)mesh.cylinder_edges.size(), cyl.data(), ev.data(), x, y, u.data(), v.data(), p.data(),
-                       Sim::NU, Sim::RHO, Sim::X_CYL, Sim::Y_CYL, diag + 4);
+                       cfg.nu, Sim::RHO, Sim::X_CYL, Sim::Y_CYL, diag + 4);
         StepRow row;
         row.step = step;
         row.t = step * dt;

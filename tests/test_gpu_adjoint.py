@@ -11,12 +11,17 @@ cfa_window_adjoint), on the frozen uniform level-1 mesh (1912 nodes), dt 1e-3.
 * The gradient does not depend on the checkpoint spacing.
 * A torch.autograd.Function (this harness only) whose forward is cfa_window_forward and whose
   backward is cfa_window_adjoint gives the same gradient through theta.grad.
+* On the level-2 mesh (7408 nodes), J = mean C_D over the first 12 steps from the initial
+  condition (theta acting on every step): the GPU window equals the oracle's run (the serial
+  FP64 CPU restatement) to 1e-9, and the GPU adjoint gradient matches central differences of the
+  oracle's J to 1e-5 relative.
 """
 import numpy as np
 import pytest
 import torch
 
 import cfa
+import cfa_oracle
 
 pytestmark = pytest.mark.gpu
 THETA0 = np.array([1.0e-3, 1.5])
@@ -123,3 +128,27 @@ def test_torch_autograd_function(adj, spun):
     _, g, _, _ = adj.gradient(THETA0, FIRST, W, spun, EVERY)
     np.testing.assert_array_equal(theta.grad.numpy(), g)
     assert abs(J.item() - adj.forward(THETA0, FIRST, W, spun)[0]) == 0.0
+
+
+def test_level2_gradient_vs_oracle_central_differences():
+    print("[case] level 2, 12 steps from the initial condition: GPU adjoint vs oracle differences", flush=True)
+    steps = 12
+    a = cfa.Adjoint(2, dt=1e-3, tol=1e-12)
+    J, g, _, _ = a.gradient(THETA0, 1, steps, a.initial_state(), 4)
+    a.close()
+
+    def oracle_J(theta):
+        return float(np.mean(cfa_oracle.run_params(steps, 1e-3, 2, theta[0], theta[1], 1e-12)[:, 2]))
+
+    J_o = oracle_J(THETA0)
+    assert abs(J - J_o) <= 1e-9 * abs(J_o), (J, J_o)
+    fd = np.zeros(2)
+    for k in range(2):
+        h = 1e-6 * THETA0[k]
+        tp, tm = THETA0.copy(), THETA0.copy()
+        tp[k] += h
+        tm[k] -= h
+        fd[k] = (oracle_J(tp) - oracle_J(tm)) / (2 * h)
+    print(f"  J {J:.12f}  adjoint {g}  oracle central differences {fd}", flush=True)
+    for k in range(2):
+        assert abs(g[k] - fd[k]) <= 1e-5 * abs(fd[k]), (k, g[k], fd[k])

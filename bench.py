@@ -115,6 +115,67 @@ def solver_workloads(level: int, nrhs: int, tol: float, reps: int) -> int:
     return 0
 
 
+def mixed_comparison(flags: list[str], hist64, warmup: int, total: int) -> dict:
+    """The same workload with FP32 ILU factors and FP32 SA-AMG cycle inside the FP64 Krylov solvers:
+    steps/s over the same timed steps and the force deviation from the FP64 run, step by step
+    while the two runs still have the same mesh (remesh decisions are made from the solution, so
+    the adaptive trajectories may part)."""
+    import cfa
+    import numpy as np
+    mixed_flags = [f if f != "fp64" else "mixed" for f in flags]
+    if cfa.run(mixed_flags) != 0:
+        return {"error": "mixed-precision run failed"}
+    h = cfa.last_history()
+    col = {c: i for i, c in enumerate(cfa.HISTORY_COLUMNS)}
+    t0 = h[warmup - 1, col["elapsed_s"]] if warmup > 0 else 0.0
+    timed = h[total - 1, col["elapsed_s"]] - t0
+    same = (h[:, col["nodes"]] == hist64[:, col["nodes"]]) & (h[:, col["elements"]] == hist64[:, col["elements"]])
+    upto = int(np.argmin(same)) if not same.all() else len(same)
+    d_cd = np.abs(h[:upto, col["C_D"]] - hist64[:upto, col["C_D"]])
+    d_cl = np.abs(h[:upto, col["C_L"]] - hist64[:upto, col["C_L"]])
+    return {
+        "value": round((total - warmup) / timed, 3),
+        "unit": "steps/s",
+        "ms_per_step": round(1000.0 * timed / (total - warmup), 4),
+        "steps_on_identical_meshes": upto,
+        "max_abs_dC_D": float(d_cd.max()) if upto else None,
+        "max_abs_dC_L": float(d_cl.max()) if upto else None,
+        "mean_iters_u_v_p": [round(float(h[warmup:total, col[c]].mean()), 2) for c in ("it_u", "it_v", "it_p")],
+    }
+
+
+def adjoint_cost(level: int, spinup: int, window: int, every: int, tol: float) -> dict:
+    """Discrete adjoint of the window functional J = mean C_D over `window` steps on the frozen
+    uniform level-`level` mesh after a `spinup`-step FP64 run from the DFG initial condition,
+    theta = (nu, U_max): time of the forward window, time of the gradient (forward pass with
+    checkpoints every `every` steps, segment recomputation and reverse sweep), and their ratio."""
+    import time
+    import cfa
+    theta = [1.0e-3, 1.5]
+    a = cfa.Adjoint(level, dt=1e-3, tol=tol)
+    t = time.perf_counter()
+    _, state, _ = a.forward(theta, 1, spinup, a.initial_state())
+    spin_s = time.perf_counter() - t
+    t = time.perf_counter()
+    J, _, _ = a.forward(theta, spinup + 1, window, state)
+    fwd_s = time.perf_counter() - t
+    t = time.perf_counter()
+    J2, g, _, st = a.gradient(theta, spinup + 1, window, state, every)
+    grad_s = time.perf_counter() - t
+    a.close()
+    return {
+        "workload": f"uniform level-{level} mesh ({a.n} nodes, {a.ne} elements), {spinup}-step spin-up, "
+                    f"{window}-step window, checkpoints every {every}, tol {tol:g}",
+        "J": J, "dJ_dnu": float(g[0]), "dJ_dUmax": float(g[1]),
+        "forward_s": round(fwd_s, 4), "gradient_s": round(grad_s, 4),
+        "adjoint_over_forward": round(grad_s / fwd_s, 3),
+        "reverse_over_forward": round(st["reverse_ms"] / 1000.0 / fwd_s, 3),
+        "spinup_s": round(spin_s, 3),
+        "adjoint_iterations_pcg_bicgstab2": [int(st["adjoint_pcg_iterations"]), int(st["adjoint_bicgstab2_iterations"])],
+        "J_check": abs(J2 - J) <= 1e-12 * abs(J),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +192,12 @@ def main() -> int:
     ap.add_argument("--solver-workloads", type=int, metavar="LEVEL",
                     help="run the isolated solver workloads on a uniform mesh of this level and exit")
     ap.add_argument("--nrhs", type=int, default=32)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run of the same workload")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
+    ap.add_argument("--adjoint-level", type=int, default=4)
+    ap.add_argument("--adjoint-spinup", type=int, default=1000)
+    ap.add_argument("--adjoint-window", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=20)
     args = ap.parse_args()
     if args.trace_summary:
         return trace_summary(args.trace_summary, args.kernel_reps)
@@ -203,6 +270,13 @@ def main() -> int:
                     for k, v in kernels.items()},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(),
     }
+    if not args.no_mixed and args.precision == "fp64":
+        line["mixed_precision"] = mixed_comparison(flags, hist, args.warmup, total)
+        if "value" in line["mixed_precision"]:
+            line["mixed_precision"]["speedup_vs_fp64"] = round(line["mixed_precision"]["value"] / line["value"], 3)
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_cost(args.adjoint_level, args.adjoint_spinup, args.adjoint_window,
+                                       args.adjoint_every, args.tol)
     print(json.dumps(line), flush=True)
     return 0
 

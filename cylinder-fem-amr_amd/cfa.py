@@ -134,7 +134,7 @@ def last_kernels() -> dict:
     """--kernel-timing results of the last run: name -> {name, bytes, us, launches}."""
     _, host = load()
     count = host.cfa_last_run_kernels(None, 0, None, 0)
-    buf = ctypes.create_string_buffer(4096)
+    buf = ctypes.create_string_buffer(65536)
     out = np.zeros((max(count, 1), 3))
     host.cfa_last_run_kernels(buf, len(buf), _dptr(out), count)
     names = buf.value.decode().split("\n") if count else []

@@ -103,6 +103,25 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
                                         time_launches(kernel_reps, [&] {
                 GPU_CHECK(cfa_amg_smooth(amg.view(), 0, g.d_dp, g.d_rhs_p, t_out, s));
             }), kernel_reps});
+            // one smoothing sweep per level with each lanes-per-row choice (the set-up picks one
+            // by the mean row length; these timings are what that rule is checked against)
+            cfa_amg_t v = *amg.view();
+            std::vector<cfa_amg_level_t> lv(v.levels, v.levels + v.n_levels);
+            v.levels = lv.data();
+            for (int l = 0; l < v.n_levels; ++l) {
+                const cfa_amg_level_t& L = amg.view()->levels[l];
+                const double nl = L.rows, nnzl = L.a_nnz;
+                for (int tpr = 1; tpr <= 32; tpr *= 2) {
+                    lv[l].tpr_a = tpr;
+                    const std::string name = "jacobi level " + std::to_string(l) + " rows " + std::to_string(L.rows) +
+                                             " nnz/row " + std::to_string((int)std::lround(nnzl / nl)) + " tpr " +
+                                             std::to_string(tpr) + (tpr == L.tpr_a ? " (chosen)" : "");
+                    kernel_timings().push_back({name, 12.0 * nnzl + 36.0 * nl + 4.0, time_launches(kernel_reps, [&] {
+                        GPU_CHECK(cfa_amg_smooth(&v, l, L.x, L.b, L.ax, s));
+                    }), kernel_reps});
+                }
+                lv[l].tpr_a = L.tpr_a;
+            }
         }
     }
     const double v[12] = {(double)n, (double)g.nnz, (double)g.n_elems, mesh_s, setup_ms, (double)amg.num_levels(),

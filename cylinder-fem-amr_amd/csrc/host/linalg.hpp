@@ -193,8 +193,9 @@ private:
     static void ensure_level_vectors(Level& L, int n);
 };
 
-// lanes per row of the V-cycle kernels from the mean row length (wave64 form of
-// src/cfa/linalg.cuh:919-926)
+// lanes per row of the V-cycle kernels from the mean row length and the level size (the
+// reference picks them by row length alone, src/cfa/linalg.cuh:919-926; this rule is the
+// MI355X measurement's)
 int csr_lanes_per_row(int rows, long long nnz);
 
 }  // namespace cfa

@@ -11,9 +11,12 @@ at the end of every step (the forces are read back), and the timed region runs f
 step W to the end of step W + K on the host's steady clock.
 
 After the timed run the main kernels are launched alone on the final mesh and state
-(--kernel-timing) and timed with HIP events on the solver stream; the roofline object reports
-the dominant one. The CPU baseline is the oracle (the serial FP64 CPU restatement of the same
-step) on a bounded sample, run on this host.
+(--kernel-timing) and timed with HIP events on the solver stream ("kernels"). The roofline object
+reports the dominant kernel family's level-0 member, the damped-Jacobi sweep of the SA-AMG
+V-cycle, measured the same way on a fixed shape (the pressure matrix of the uniform level-4
+mesh, 115,648 rows, the size of the adaptive run's late meshes), so that the hardware-counter
+passes can profile exactly that launch (--roofline-only). The CPU baseline is the oracle (the
+serial FP64 CPU restatement of the same step) on a bounded sample, run on this host.
 
 Prints one JSON line.
 """
@@ -30,6 +33,8 @@ sys.path.insert(0, str(ROOT / "cylinder-fem-amr_amd"))
 
 HBM_PEAK_GBS = 8000.0
 DOMINANT = "k_csr_op<jacobi> level 0"
+ROOFLINE_LEVEL = 4
+TRAFFIC_FILE = ROOT / "profiles" / "jacobi_level0_uniform_L4_counters.txt"
 
 WORKLOADS = {
     # runs/paper/d2_A2_dt05/cmd.txt (--steps set by the benchmark)
@@ -115,6 +120,56 @@ def solver_workloads(level: int, nrhs: int, tol: float, reps: int) -> int:
     return 0
 
 
+def read_traffic() -> dict | None:
+    """HBM bytes per launch of the roofline kernel from the committed counter summary (two
+    rocprofv3 --pmc passes of `bench.py --roofline-only`: FETCH_SIZE and WRITE_SIZE in KiB per
+    dispatch, each scaled by the same counter over a 64 MiB device copy)."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    vals = {}
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except (ValueError, IndexError):
+                pass
+    need = ("kernel_fetch_kib", "kernel_write_kib", "copy_fetch_kib", "copy_write_kib", "copy_bytes")
+    if not all(k in vals for k in need):
+        return None
+    half = vals["copy_bytes"] / 2.0   # the copy reads and writes half of its bytes each
+    f_scale = half / (1024.0 * vals["copy_fetch_kib"])
+    w_scale = half / (1024.0 * vals["copy_write_kib"])
+    return {"bytes": 1024.0 * (vals["kernel_fetch_kib"] * f_scale + vals["kernel_write_kib"] * w_scale),
+            "fetch_scale": f_scale, "write_scale": w_scale}
+
+
+def roofline_measurement(reps: int, tol: float) -> tuple[dict, dict]:
+    """The level-0 smoother of the SA-AMG V-cycle on the uniform level-4 pressure matrix, launched
+    `reps` times between HIP events on the solver stream (cfa_workload_pressure timings)."""
+    import cfa
+    cfa.workload_pressure(ROOFLINE_LEVEL, 1, tol, max(reps, 1))
+    k = cfa.last_kernels()
+    return k[DOMINANT], k
+
+
+def roofline_object(dom: dict) -> dict:
+    achieved = dom["bytes"] / (dom["us"] * 1e3)
+    tr = read_traffic()
+    return {
+        "kernel": f"{dom['name']} (uniform level-{ROOFLINE_LEVEL} pressure hierarchy)",
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": round(tr["bytes"]) if tr else None,
+        "bytes_per_launch": dom["bytes"],
+        "us_per_launch": round(dom["us"], 3),
+        "measured_on": "HIP events on the solver stream, the kernel launched alone",
+    }
+
+
 def mixed_comparison(flags: list[str], hist64, warmup: int, total: int) -> dict:
     """The same workload with FP32 ILU factors and FP32 SA-AMG cycle inside the FP64 Krylov solvers:
     steps/s over the same timed steps and the force deviation from the FP64 run, step by step
@@ -198,6 +253,8 @@ def main() -> int:
     ap.add_argument("--adjoint-spinup", type=int, default=1000)
     ap.add_argument("--adjoint-window", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=20)
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="time the roofline kernel alone (the launch the counter passes profile) and exit")
     args = ap.parse_args()
     if args.trace_summary:
         return trace_summary(args.trace_summary, args.kernel_reps)
@@ -205,6 +262,12 @@ def main() -> int:
         return counter_summary(args.counter_summary, args.kernel_reps)
     if args.solver_workloads is not None:
         return solver_workloads(args.solver_workloads, args.nrhs, args.tol, args.kernel_reps)
+    if args.roofline_only:
+        dom, ks = roofline_measurement(args.kernel_reps, args.tol)
+        out = {"roofline": roofline_object(dom),
+               "kernels": {k: {"us": round(v["us"], 3), "bytes": v["bytes"]} for k, v in ks.items()}}
+        print(json.dumps(out), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
@@ -230,8 +293,6 @@ def main() -> int:
     timed = t_end - t_start
     window = hist[args.warmup:total]
     kernels = cfa.last_kernels()
-    dom = kernels.get(DOMINANT) or max(kernels.values(), key=lambda k: k["us"])
-    achieved = dom["bytes"] / (dom["us"] * 1e3)
     line = {
         "metric": "steps/s at 1e-8 rel. residual, FP32-precond vs FP64 GPU; adjoint/forward time",
         "value": round(args.steps / timed, 3),
@@ -254,22 +315,13 @@ def main() -> int:
             "mean_iters_u_v_p": [round(float(window[:, col[c]].mean()), 2) for c in ("it_u", "it_v", "it_p")],
             "parallelism": "1 GPU",
         },
-        "roofline": {
-            "kernel": dom["name"],
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "bytes_per_launch": dom["bytes"],
-            "us_per_launch": round(dom["us"], 3),
-            "measured_on": "final mesh of the run, HIP events on the solver stream",
-        },
+        "roofline": None,
         "kernels": {k: {"us": round(v["us"], 3), "GB/s": round(v["bytes"] / (v["us"] * 1e3), 1)}
                     for k, v in kernels.items()},
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(),
     }
+    dom, _ = roofline_measurement(args.kernel_reps, args.tol)
+    line["roofline"] = roofline_object(dom)
     if not args.no_mixed and args.precision == "fp64":
         line["mixed_precision"] = mixed_comparison(flags, hist, args.warmup, total)
         if "value" in line["mixed_precision"]:

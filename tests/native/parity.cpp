@@ -570,6 +570,7 @@ void case_bicgstab(Fixture& f) {
 }  // namespace
 
 int main(int argc, char** argv) {
+    std::setvbuf(stdout, nullptr, _IOLBF, 0);   // each line reaches a pipe at once
     const std::string what = argc > 1 ? argv[1] : "all";
     const int level = argc > 2 ? std::atoi(argv[2]) : 2;
     GPU_CHECK(cfa_init(0));

@@ -1,0 +1,403 @@
+// SA-AMG set-up on gfx950: the numeric phases of src/cfa/linalg.cuh:995-1060 and 1276-1431
+// (AmgPreconditioner::setup, spectral_radius_DinvA, smoothed_prolongator, csr_multiply,
+// csr_transpose), which the reference runs on the host with OpenMP. The greedy aggregation stays
+// on the host (it is sequential in row order); everything else runs here and reproduces the
+// host's floating-point results bit for bit:
+//   * the power iteration keeps the host's chunking (rows in chunks of 4096, each chunk summed in
+//     row order, the chunk sums added in chunk order);
+//   * a sparse product accumulates every output entry in the host's encounter order (row entries
+//     of the left factor in order, each right row in order), starting from 0.0;
+//   * contraction is off, so every product and sum is the host's own operation.
+// There are no atomics: products and P are built row by row, the transpose column by column from
+// the aggregates' member lists.
+#include "cfa_device.hpp"
+
+using namespace cfa_dev;
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int CHUNK = 4096;          // host chunk of spectral_radius_DinvA
+constexpr int SPGEMM_CAP = 4096;     // products per output row held in LDS
+constexpr int SPGEMM_THREADS = 256;
+
+// inverse diagonal (1 where |a_ii| <= 1e-300, the AMG rule of linalg.cuh:871-878)
+__global__ void k_amg_dinv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ v, double* __restrict__ dinv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = 0.0;
+    for (int a = rp[i]; a < rp[i + 1]; ++a)
+        if (ci[a] == i) d = v[a];
+    dinv[i] = fabs(d) > 1.0e-300 ? 1.0 / d : 1.0;
+}
+
+// y = D^-1 A x, one thread per row, entries in row order
+__global__ void k_power_rows(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                             const double* __restrict__ v, const double* __restrict__ dinv,
+                             const double* __restrict__ x, double* __restrict__ y) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int a = rp[i]; a < rp[i + 1]; ++a) s += v[a] * x[ci[a]];
+    y[i] = dinv[i] * s;
+}
+
+// per chunk of 4096 rows: sum y_i^2 and x_i^2 in row order. One wavefront per chunk: the lanes
+// form 64 products at a time, lane 0 adds them in order.
+__global__ void __launch_bounds__(WAVE) k_power_chunks(int n, const double* __restrict__ x,
+                                                       const double* __restrict__ y,
+                                                       double* __restrict__ py, double* __restrict__ px) {
+    __shared__ double sy2[WAVE], sx2[WAVE];
+    const int c = blockIdx.x, lane = threadIdx.x;
+    const int lo = c * CHUNK, hi = min(n, lo + CHUNK);
+    double sy = 0.0, sx = 0.0;
+    for (int base = lo; base < hi; base += WAVE) {
+        const int i = base + lane;
+        const double yi = i < hi ? y[i] : 0.0, xi = i < hi ? x[i] : 0.0;
+        sy2[lane] = yi * yi;
+        sx2[lane] = xi * xi;
+        __syncthreads();
+        if (lane == 0) {
+            const int cnt = min(WAVE, hi - base);
+            for (int k = 0; k < cnt; ++k) {
+                sy += sy2[k];
+                sx += sx2[k];
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        py[c] = sy;
+        px[c] = sx;
+    }
+}
+
+// nrm = sqrt(sum of chunk sums), lambda = nrm / sqrt(sum of x^2), in chunk order; out[0] = nrm,
+// out[1] = lambda
+__global__ void k_power_final(int nch, const double* __restrict__ py, const double* __restrict__ px,
+                              double* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double nrm = 0.0, xn = 0.0;
+    for (int c = 0; c < nch; ++c) {
+        nrm += py[c];
+        xn += px[c];
+    }
+    nrm = sqrt(nrm);
+    out[0] = nrm;
+    out[1] = nrm / sqrt(xn);
+}
+
+__global__ void k_power_scale(int n, const double* __restrict__ y, const double* __restrict__ nrm,
+                              double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = y[i] / nrm[0];
+}
+
+// ---- sparse product C = A B, host csr_multiply semantics ------------------------------------------
+// One workgroup per row of A. The products a_ik b_kj are listed in encounter order (entries of
+// A's row in order, each B row in order) into LDS with key (j, position); a bitonic sort by key
+// groups equal columns with positions ascending; each column is then summed from 0.0 in that
+// order. Pass 0 writes the number of distinct columns of each row to cnt[row] (and sets *overflow
+// when a row has more than SPGEMM_CAP products); pass 1 writes columns and values at c_rp[row].
+__device__ __forceinline__ int block_exclusive_scan(int v, int* s_tmp, int& total) {
+    // simple two-level scan over the workgroup (SPGEMM_THREADS lanes)
+    const int t = threadIdx.x;
+    s_tmp[t] = v;
+    __syncthreads();
+    for (int off = 1; off < SPGEMM_THREADS; off <<= 1) {
+        const int add = t >= off ? s_tmp[t - off] : 0;
+        __syncthreads();
+        s_tmp[t] += add;
+        __syncthreads();
+    }
+    const int incl = s_tmp[t];
+    total = s_tmp[SPGEMM_THREADS - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(SPGEMM_THREADS)
+k_spgemm_rows(int rows, const int* __restrict__ a_rp, const int* __restrict__ a_ci, const double* __restrict__ a_v,
+              const int* __restrict__ b_rp, const int* __restrict__ b_ci, const double* __restrict__ b_v, int pass,
+              int* __restrict__ cnt, const int* __restrict__ c_rp, int* __restrict__ c_ci, double* __restrict__ c_v,
+              int* __restrict__ overflow) {
+    __shared__ unsigned long long key[SPGEMM_CAP];
+    __shared__ double val[SPGEMM_CAP];
+    __shared__ int s_tmp[SPGEMM_THREADS];
+    const int row = blockIdx.x;
+    if (row >= rows) return;
+    const int t = threadIdx.x;
+    const int a0 = a_rp[row], a1 = a_rp[row + 1];
+    // list the products in encounter order
+    int filled = 0;
+    bool over = false;
+    for (int base = a0; base < a1; base += SPGEMM_THREADS) {
+        const int a = base + t;
+        const int k = a < a1 ? a_ci[a] : 0;
+        const int len = a < a1 ? b_rp[k + 1] - b_rp[k] : 0;
+        int total = 0;
+        const int off = block_exclusive_scan(len, s_tmp, total);
+        if (filled + total > SPGEMM_CAP) {
+            over = true;
+            break;
+        }
+        if (a < a1) {
+            const double av = a_v[a];
+            const int b0 = b_rp[k];
+            for (int q = 0; q < len; ++q) {
+                const int pos = filled + off + q;
+                key[pos] = ((unsigned long long)(unsigned)b_ci[b0 + q] << 32) | (unsigned)pos;
+                val[pos] = av * b_v[b0 + q];
+            }
+        }
+        filled += total;
+        __syncthreads();
+    }
+    if (over) {
+        if (t == 0) {
+            overflow[0] = 1;
+            if (pass == 0) cnt[row] = 0;
+        }
+        return;
+    }
+    // bitonic sort of the keys (padded to a power of two with the largest key)
+    int size = 1;
+    while (size < filled) size <<= 1;
+    for (int p = filled + t; p < size; p += SPGEMM_THREADS) key[p] = ~0ull;
+    __syncthreads();
+    for (int k = 2; k <= size; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int p = t; p < size; p += SPGEMM_THREADS) {
+                const int q = p ^ j;
+                if (q > p) {
+                    const unsigned long long kp = key[p], kq = key[q];
+                    const bool up = (p & k) == 0;
+                    if ((kp > kq) == up) {
+                        key[p] = kq;
+                        key[q] = kp;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    // segment heads: first entry of each distinct column
+    int distinct_before = 0;
+    int n_distinct = 0;
+    for (int base = 0; base < filled; base += SPGEMM_THREADS) {
+        const int p = base + t;
+        const int head = p < filled && (p == 0 || (key[p] >> 32) != (key[p - 1] >> 32)) ? 1 : 0;
+        int total = 0;
+        const int off = block_exclusive_scan(head, s_tmp, total);
+        if (pass == 1 && head) {
+            // sum this column's products in encounter order (positions ascending)
+            const unsigned col = (unsigned)(key[p] >> 32);
+            double acc = 0.0;
+            for (int r = p; r < filled && (unsigned)(key[r] >> 32) == col; ++r)
+                acc += val[(unsigned)(key[r] & 0xffffffffull)];
+            const int o = c_rp[row] + distinct_before + off;
+            c_ci[o] = (int)col;
+            c_v[o] = acc;
+        }
+        distinct_before += total;
+        n_distinct += total;
+    }
+    if (pass == 0 && t == 0) cnt[row] = n_distinct;
+}
+
+// exclusive scan of cnt[0..n) into out[0..n] (out[n] = total), one workgroup, fixed order
+__global__ void __launch_bounds__(1024) k_scan_counts(int n, const int* __restrict__ cnt, int* __restrict__ out) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int lo = min(n, t * per), hi = min(n, lo + per);
+    int s = 0;
+    for (int i = lo; i < hi; ++i) s += cnt[i];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int k = 0; k < 1024; ++k) {
+            const int v = part[k];
+            part[k] = acc;
+            acc += v;
+        }
+        out[n] = acc;
+    }
+    __syncthreads();
+    int acc = part[t];
+    for (int i = lo; i < hi; ++i) {
+        out[i] = acc;
+        acc += cnt[i];
+    }
+}
+
+// ---- smoothed prolongator P = P0 - omega D^-1 A P0 (linalg.cuh smoothed_prolongator): per row,
+// AP0's entries in column order, the row's own aggregate merged in, exact zeros dropped
+__global__ void k_prolongator_rows(int n, const int* __restrict__ ap_rp, const int* __restrict__ ap_ci,
+                                   const double* __restrict__ ap_v, const int* __restrict__ agg,
+                                   const double* __restrict__ p0v, const double* __restrict__ dinv, double omega,
+                                   int pass, int* __restrict__ cnt, const int* __restrict__ p_rp,
+                                   int* __restrict__ p_ci, double* __restrict__ p_v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int out = 0;
+    bool own = false;
+    const int oc = agg[i];
+    const double own_v = p0v[i];
+    const int o0 = pass == 1 ? p_rp[i] : 0;
+    for (int a = ap_rp[i]; a < ap_rp[i + 1]; ++a) {
+        const int j = ap_ci[a];
+        if (!own && j > oc) {
+            if (pass == 1) { p_ci[o0 + out] = oc; p_v[o0 + out] = own_v; }
+            ++out;
+            own = true;
+        }
+        double val = -omega * dinv[i] * ap_v[a];
+        if (j == oc) { val += own_v; own = true; }
+        if (val != 0.0) {
+            if (pass == 1) { p_ci[o0 + out] = j; p_v[o0 + out] = val; }
+            ++out;
+        }
+    }
+    if (!own) {
+        if (pass == 1) { p_ci[o0 + out] = oc; p_v[o0 + out] = own_v; }
+        ++out;
+    }
+    if (pass == 0) cnt[i] = out;
+}
+
+// ---- R = P^T (csr_transpose: entries of each column in ascending row order) ----------------------
+// Column c of P can only be non-zero in the rows of aggregate c and their A-neighbours. One
+// thread per column walks those candidate rows in ascending order (a merge of the members' sorted
+// neighbour lists) and looks the column up in each candidate's P row.
+__device__ __forceinline__ int find_col(const int* __restrict__ ci, int lo, int hi, int col) {
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ci[mid] < col) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_transpose_cols(int n_agg, const int* __restrict__ mem_ptr, const int* __restrict__ mem_rows,
+                                 const int* __restrict__ a_rp, const int* __restrict__ a_ci,
+                                 const int* __restrict__ p_rp, const int* __restrict__ p_ci,
+                                 const double* __restrict__ p_v, int pass, int* __restrict__ cnt,
+                                 const int* __restrict__ r_rp, int* __restrict__ r_ci, double* __restrict__ r_v) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_agg) return;
+    const int m0 = mem_ptr[c], m1 = mem_ptr[c + 1];
+    int out = 0;
+    const int o0 = pass == 1 ? r_rp[c] : 0;
+    int last = -1;
+    // repeatedly take the smallest candidate row above `last` over the members' (sorted) rows of
+    // A; each member's row of A contains the member itself (the diagonal)
+    while (true) {
+        int next = 0x7fffffff;
+        for (int m = m0; m < m1; ++m) {
+            const int r = mem_rows[m];
+            const int lo = a_rp[r], hi = a_rp[r + 1];
+            const int pos = find_col(a_ci, lo, hi, last + 1);
+            if (pos < hi && a_ci[pos] < next) next = a_ci[pos];
+        }
+        if (next == 0x7fffffff) break;
+        const int lo = p_rp[next], hi = p_rp[next + 1];
+        const int pos = find_col(p_ci, lo, hi, c);
+        if (pos < hi && p_ci[pos] == c) {
+            if (pass == 1) {
+                r_ci[o0 + out] = next;
+                r_v[o0 + out] = p_v[pos];
+            }
+            ++out;
+        }
+        last = next;
+    }
+    if (pass == 0) cnt[c] = out;
+}
+
+int blocks(int n) { return blocks_for(n > 0 ? n : 1); }
+
+}  // namespace
+
+extern "C" {
+
+int cfa_amg_dinv(int n, const int* rp, const int* ci, const double* val, double* dinv, cfa_stream_t stream) {
+    if (n < 0 || (n > 0 && (!rp || !ci || !val || !dinv))) return CFA_E_ARG;
+    if (n == 0) return CFA_OK;
+    hipLaunchKernelGGL(k_amg_dinv, dim3(blocks(n)), dim3(BLOCK), 0, as_stream(stream), n, rp, ci, val, dinv);
+    return launch_status();
+}
+
+size_t cfa_amg_power_ws_doubles(int n) {
+    return n < 0 ? 0 : 2 * (size_t)n + 2 * (size_t)((n + CHUNK - 1) / CHUNK) + 8;
+}
+
+int cfa_amg_power(int n, const int* rp, const int* ci, const double* val, const double* dinv, const double* x0,
+                  int iterations, double* ws, double* lambda, cfa_stream_t stream) {
+    if (n <= 0 || !rp || !ci || !val || !dinv || !x0 || !ws || !lambda || iterations < 1) return CFA_E_ARG;
+    const hipStream_t s = as_stream(stream);
+    const int nch = (n + CHUNK - 1) / CHUNK;
+    double* x = ws;
+    double* y = ws + n;
+    double* py = y + n;
+    double* px = py + nch;
+    double* out = px + nch;
+    hipError_t e = hipMemcpyAsync(x, x0, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return (int)e;
+    for (int it = 0; it < iterations; ++it) {
+        hipLaunchKernelGGL(k_power_rows, dim3(blocks(n)), dim3(BLOCK), 0, s, n, rp, ci, val, dinv, x, y);
+        hipLaunchKernelGGL(k_power_chunks, dim3(nch), dim3(WAVE), 0, s, n, x, y, py, px);
+        hipLaunchKernelGGL(k_power_final, dim3(1), dim3(WAVE), 0, s, nch, py, px, out);
+        hipLaunchKernelGGL(k_power_scale, dim3(blocks(n)), dim3(BLOCK), 0, s, n, y, out, x);
+    }
+    e = hipMemcpyAsync(lambda, out + 1, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return (int)e;
+    return launch_status();
+}
+
+int cfa_spgemm(int rows, const int* a_rp, const int* a_ci, const double* a_v, const int* b_rp, const int* b_ci,
+               const double* b_v, int pass, int* cnt, const int* c_rp, int* c_ci, double* c_v, int* overflow,
+               cfa_stream_t stream) {
+    if (rows < 0 || (pass != 0 && pass != 1) || !overflow) return CFA_E_ARG;
+    if (rows == 0) return CFA_OK;
+    if (!a_rp || !a_ci || !a_v || !b_rp || !b_ci || !b_v) return CFA_E_ARG;
+    if ((pass == 0 && !cnt) || (pass == 1 && (!c_rp || !c_ci || !c_v))) return CFA_E_ARG;
+    hipLaunchKernelGGL(k_spgemm_rows, dim3(rows), dim3(SPGEMM_THREADS), 0, as_stream(stream), rows, a_rp, a_ci, a_v,
+                       b_rp, b_ci, b_v, pass, cnt, c_rp, c_ci, c_v, overflow);
+    return launch_status();
+}
+
+int cfa_scan_counts(int n, const int* cnt, int* out, cfa_stream_t stream) {
+    if (n < 0 || !out || (n > 0 && !cnt)) return CFA_E_ARG;
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, as_stream(stream), n, cnt, out);
+    return launch_status();
+}
+
+int cfa_amg_prolongator(int n, const int* ap_rp, const int* ap_ci, const double* ap_v, const int* agg,
+                        const double* p0v, const double* dinv, double omega, int pass, int* cnt, const int* p_rp,
+                        int* p_ci, double* p_v, cfa_stream_t stream) {
+    if (n < 0 || (pass != 0 && pass != 1)) return CFA_E_ARG;
+    if (n == 0) return CFA_OK;
+    if (!ap_rp || !ap_ci || !ap_v || !agg || !p0v || !dinv) return CFA_E_ARG;
+    if ((pass == 0 && !cnt) || (pass == 1 && (!p_rp || !p_ci || !p_v))) return CFA_E_ARG;
+    hipLaunchKernelGGL(k_prolongator_rows, dim3(blocks(n)), dim3(BLOCK), 0, as_stream(stream), n, ap_rp, ap_ci, ap_v,
+                       agg, p0v, dinv, omega, pass, cnt, p_rp, p_ci, p_v);
+    return launch_status();
+}
+
+int cfa_amg_transpose(int n_agg, const int* mem_ptr, const int* mem_rows, const int* a_rp, const int* a_ci,
+                      const int* p_rp, const int* p_ci, const double* p_v, int pass, int* cnt, const int* r_rp,
+                      int* r_ci, double* r_v, cfa_stream_t stream) {
+    if (n_agg < 0 || (pass != 0 && pass != 1)) return CFA_E_ARG;
+    if (n_agg == 0) return CFA_OK;
+    if (!mem_ptr || !mem_rows || !a_rp || !a_ci || !p_rp || !p_ci || !p_v) return CFA_E_ARG;
+    if ((pass == 0 && !cnt) || (pass == 1 && (!r_rp || !r_ci || !r_v))) return CFA_E_ARG;
+    hipLaunchKernelGGL(k_transpose_cols, dim3(blocks(n_agg)), dim3(BLOCK), 0, as_stream(stream), n_agg, mem_ptr,
+                       mem_rows, a_rp, a_ci, p_rp, p_ci, p_v, pass, cnt, r_rp, r_ci, r_v);
+    return launch_status();
+}
+
+}  // extern "C"

@@ -208,6 +208,7 @@ int run_driver(int argc, char** argv) {
     const bool use_amg = opt.amg;
     AmgPreconditioner amg;
     amg.fp32 = opt.mixed;
+    amg.keep_p0_host = opt.amg_remap > 0;   // remap() is the only reader of the host prolongator
     bool amg_dirty = true, phi_pending = false;
     int amg_built_step = 0, amg_remaps_in_row = 0;
     PendingRemesh pend;

@@ -448,8 +448,7 @@ double AmgPreconditioner::spectral_radius_DinvA(const HostCsr& A, const std::vec
     return std::max(lambda, 1.0e-12) * 1.05;
 }
 
-HostCsr AmgPreconditioner::smoothed_prolongator(const HostCsr& A, const std::vector<double>& dinv, double omega,
-                                                std::vector<int>* agg_out) const {
+std::vector<int> AmgPreconditioner::aggregate(const HostCsr& A, const std::vector<double>& dinv, int& n_agg) const {
     const int n = A.rows;
     std::vector<double> diag(n);
 #pragma omp parallel for schedule(static)
@@ -492,6 +491,15 @@ HostCsr AmgPreconditioner::smoothed_prolongator(const HostCsr& A, const std::vec
             if (strong[a] && agg[A.ci[a]] < 0) agg[A.ci[a]] = na;
         ++na;
     }
+    n_agg = na;
+    return agg;
+}
+
+HostCsr AmgPreconditioner::smoothed_prolongator(const HostCsr& A, const std::vector<double>& dinv, double omega,
+                                                std::vector<int>* agg_out) const {
+    const int n = A.rows;
+    int na = 0;
+    const std::vector<int> agg = aggregate(A, dinv, na);
     if (agg_out) *agg_out = agg;
     std::vector<int> size(na, 0);
     for (int i = 0; i < n; ++i) size[agg[i]]++;
@@ -628,6 +636,174 @@ void AmgPreconditioner::release() {
     coarse_n = 0;
 }
 
+// C = A B on the device (host csr_multiply results, bit for bit); false when a row has more
+// products than the device kernel holds
+bool AmgPreconditioner::device_product(int rows, const int* arp, const int* aci, const double* av, const int* brp,
+                                       const int* bci, const double* bv, DevCsr& C) {
+    const cfa_stream_t s = solver_stream();
+    d_cnt_.reserve(std::max(rows, 1));
+    d_flag_.reserve(1);
+    dev_zero(d_flag_, sizeof(int));
+    GPU_CHECK(cfa_spgemm(rows, arp, aci, av, brp, bci, bv, 0, d_cnt_, nullptr, nullptr, nullptr, d_flag_, s));
+    int flag = 0;
+    d_flag_.download(&flag, 1);
+    if (flag) return false;
+    C.rp.reserve(rows + 1);
+    GPU_CHECK(cfa_scan_counts(rows, d_cnt_, C.rp, s));
+    int total = 0;
+    GPU_CHECK(cfa_memcpy(&total, C.rp.ptr + rows, sizeof(int), 1, s));
+    C.ci.reserve(std::max(total, 1));
+    C.v.reserve(std::max(total, 1));
+    GPU_CHECK(cfa_spgemm(rows, arp, aci, av, brp, bci, bv, 1, nullptr, C.rp, C.ci, C.v, d_flag_, s));
+    C.rows = rows;
+    C.nnz = total;
+    C.tpr = csr_lanes_per_row(rows, total);
+    return true;
+}
+
+int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, const double* av,
+                                    std::vector<std::unique_ptr<Level>>& pool) {
+    const cfa_stream_t s = solver_stream();
+    const int n = A.rows, nnz = (int)A.ci.size();
+    const bool first = levels.empty();
+    phase_start();
+    // D^-1 (host copy for the strength test, device copy for the level), spectral radius
+    std::vector<double> dinv(n);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+        double d = 0.0;
+        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
+            if (A.ci[a] == i) d = A.v[a];
+        dinv[i] = (std::fabs(d) > 1.0e-300) ? 1.0 / d : 1.0;
+    }
+    std::unique_ptr<Level> L;
+    if (!pool.empty()) {
+        L = std::move(pool.front());
+        pool.erase(pool.begin());
+    } else {
+        L = std::make_unique<Level>();
+    }
+    auto give_back = [&]() { pool.insert(pool.begin(), std::move(L)); };
+    ensure_level_vectors(*L, n);
+    GPU_CHECK(cfa_amg_dinv(n, arp, aci, av, L->dinv, s));
+    if (x0_dev_n_ < n) {   // power-iteration start vector of spectral_radius_DinvA
+        x0_host_.resize(n);
+        for (int i = 0; i < n; ++i) x0_host_[i] = 1.0 + 0.1 * std::sin(1.0 + i);
+        d_x0_.upload(x0_host_);
+        x0_dev_n_ = n;
+    }
+    d_pw_.reserve(cfa_amg_power_ws_doubles(n));
+    double lambda = 0.0;
+    GPU_CHECK(cfa_amg_power(n, arp, aci, av, L->dinv, d_x0_, 20, d_pw_, &lambda, s));
+    const double omega = 4.0 / (3.0 * (std::max(lambda, 1.0e-12) * 1.05));
+    phase_end(PH_RHO);
+    int na = 0;
+    const std::vector<int> agg = aggregate(A, dinv, na);
+    phase_end(PH_AGG);
+    if (na >= (int)(0.85 * n) || na == 0) {
+        give_back();
+        return 0;
+    }
+    if (first) last_aggregates = agg;
+    // P0 values, aggregate member lists (rows ascending), P0 = (iota, agg, p0v)
+    std::vector<int> size(na, 0);
+    for (int i = 0; i < n; ++i) size[agg[i]]++;
+    std::vector<double> p0v(n);
+    for (int i = 0; i < n; ++i) p0v[i] = 1.0 / std::sqrt((double)size[agg[i]]);
+    std::vector<int> mem_ptr(na + 1, 0), mem_rows(n);
+    for (int c = 0; c < na; ++c) mem_ptr[c + 1] = mem_ptr[c] + size[c];
+    {
+        std::vector<int> fill(mem_ptr.begin(), mem_ptr.end() - 1);
+        for (int i = 0; i < n; ++i) mem_rows[fill[agg[i]]++] = i;
+    }
+    if (iota_dev_n_ < n + 1) {
+        std::vector<int> iota(n + 1);
+        for (int i = 0; i <= n; ++i) iota[i] = i;
+        d_iota_.upload(iota);
+        iota_dev_n_ = n + 1;
+    }
+    d_agg_.upload(agg);
+    d_p0v_.upload(p0v);
+    d_mem_ptr_.upload(mem_ptr);
+    d_mem_rows_.upload(mem_rows);
+    phase_end(PH_UPLOAD);
+    // P = P0 - omega D^-1 A P0
+    if (!device_product(n, arp, aci, av, d_iota_, d_agg_, d_p0v_, dev_tmp_)) {
+        give_back();
+        return -1;
+    }
+    d_cnt_.reserve(std::max(std::max(n, na), 1));
+    GPU_CHECK(cfa_amg_prolongator(n, dev_tmp_.rp, dev_tmp_.ci, dev_tmp_.v, d_agg_, d_p0v_, L->dinv, omega, 0, d_cnt_,
+                                  nullptr, nullptr, nullptr, s));
+    L->P.rp.reserve(n + 1);
+    GPU_CHECK(cfa_scan_counts(n, d_cnt_, L->P.rp, s));
+    int p_nnz = 0;
+    GPU_CHECK(cfa_memcpy(&p_nnz, L->P.rp.ptr + n, sizeof(int), 1, s));
+    L->P.ci.reserve(std::max(p_nnz, 1));
+    L->P.v.reserve(std::max(p_nnz, 1));
+    GPU_CHECK(cfa_amg_prolongator(n, dev_tmp_.rp, dev_tmp_.ci, dev_tmp_.v, d_agg_, d_p0v_, L->dinv, omega, 1, nullptr,
+                                  L->P.rp, L->P.ci, L->P.v, s));
+    L->P.rows = n;
+    L->P.nnz = p_nnz;
+    L->P.tpr = csr_lanes_per_row(n, p_nnz);
+    phase_end(PH_SMOOTH);
+    // R = P^T
+    GPU_CHECK(cfa_amg_transpose(na, d_mem_ptr_, d_mem_rows_, arp, aci, L->P.rp, L->P.ci, L->P.v, 0, d_cnt_, nullptr,
+                                nullptr, nullptr, s));
+    L->R.rp.reserve(na + 1);
+    GPU_CHECK(cfa_scan_counts(na, d_cnt_, L->R.rp, s));
+    int r_nnz = 0;
+    GPU_CHECK(cfa_memcpy(&r_nnz, L->R.rp.ptr + na, sizeof(int), 1, s));
+    if (r_nnz != p_nnz) {   // every entry of P lies in a member's neighbourhood; anything else is a bug
+        std::fprintf(stderr, "[AMG] device transpose found %d of %d entries\n", r_nnz, p_nnz);
+        std::exit(EXIT_FAILURE);
+    }
+    L->R.ci.reserve(std::max(r_nnz, 1));
+    L->R.v.reserve(std::max(r_nnz, 1));
+    GPU_CHECK(cfa_amg_transpose(na, d_mem_ptr_, d_mem_rows_, arp, aci, L->P.rp, L->P.ci, L->P.v, 1, nullptr, L->R.rp,
+                                L->R.ci, L->R.v, s));
+    L->R.rows = na;
+    L->R.nnz = r_nnz;
+    L->R.tpr = csr_lanes_per_row(na, r_nnz);
+    phase_end(PH_TRANSPOSE);
+    // Galerkin product R (A P)
+    if (!device_product(n, arp, aci, av, L->P.rp, L->P.ci, L->P.v, dev_tmp_) ||
+        !device_product(na, L->R.rp, L->R.ci, L->R.v, dev_tmp_.rp, dev_tmp_.ci, dev_tmp_.v, dev_next2_)) {
+        give_back();
+        return -1;
+    }
+    phase_end(PH_GALERKIN);
+    // the level's operator: the caller's matrix on level 0, the previous product after that
+    if (first) upload_csr_from_device(n, nnz, arp, aci, av, L->A);
+    else std::swap(L->A, dev_next_);
+    L->A.tpr = csr_lanes_per_row(n, L->A.nnz);
+    L->omega = omega;
+    // host copy of the coarse operator for the next aggregation (and the coarse inverse)
+    HostCsr Ac;
+    Ac.rows = Ac.cols = na;
+    Ac.rp.resize(na + 1);
+    dev_next2_.rp.download(Ac.rp.data(), na + 1);
+    Ac.ci.resize(dev_next2_.nnz);
+    Ac.v.resize(dev_next2_.nnz);
+    dev_next2_.ci.download(Ac.ci.data(), dev_next2_.nnz);
+    dev_next2_.v.download(Ac.v.data(), dev_next2_.nnz);
+    if (first && keep_p0_host) {
+        P0_host.rows = n;
+        P0_host.cols = na;
+        P0_host.rp.resize(n + 1);
+        P0_host.ci.resize(p_nnz);
+        P0_host.v.resize(p_nnz);
+        L->P.rp.download(P0_host.rp.data(), n + 1);
+        L->P.ci.download(P0_host.ci.data(), p_nnz);
+        L->P.v.download(P0_host.v.data(), p_nnz);
+    }
+    std::swap(dev_next_, dev_next2_);
+    levels.push_back(std::move(L));
+    A = std::move(Ac);
+    phase_end(PH_DOWNLOAD);
+    return 1;
+}
+
 void AmgPreconditioner::setup(int n, int nnz, const int* d_row_ptr, const int* d_col_ind, const double* d_val) {
     const auto t0 = Clock::now();
     // keep the buffers of the previous hierarchy for reuse, finest first
@@ -654,8 +830,27 @@ void AmgPreconditioner::setup(int n, int nnz, const int* d_row_ptr, const int* d
     GPU_CHECK(cfa_memcpy(A.v.data(), d_val, nnz * sizeof(double), 1, solver_stream()));
     phase_end(PH_DOWNLOAD);
     last_aggregates.clear();
+    device_levels = 0;
+    bool on_device = device_setup;
+    // device copy of the current level's operator: the caller's matrix, then the products
+    const int* arp = d_row_ptr;
+    const int* aci = d_col_ind;
+    const double* av = d_val;
 
     while (A.rows > coarse_target && (int)levels.size() < max_levels - 1) {
+        if (on_device) {
+            const int r = device_level(A, arp, aci, av, old);
+            if (r == 1) {
+                ++device_levels;
+                arp = dev_next_.rp;
+                aci = dev_next_.ci;
+                av = dev_next_.v;
+                continue;
+            }
+            if (r == 0) break;
+            on_device = false;
+            phase_start();
+        }
         std::vector<double> dinv(A.rows);
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < A.rows; ++i) {

@@ -147,6 +147,12 @@ public:
     int coarse_target = 300;
     int max_levels = 15;
     bool fp32 = false;   // run the cycle in FP32 (coarse solve in FP64); FP32 copies made at set-up
+    // numeric set-up phases (D^-1, spectral radius, P smoothing, transpose, Galerkin product) on
+    // the device, bit-identical to the host path; the aggregation stays on the host. A level
+    // whose products exceed the device kernel's row capacity falls back to the host path.
+    bool device_setup = true;
+    bool keep_p0_host = true;   // download the level-0 prolongator for remap()
+    int device_levels = 0;      // levels the last set-up built on the device
 
     std::vector<std::unique_ptr<Level>> levels;
     int coarse_n = 0;
@@ -174,8 +180,21 @@ public:
     static double spectral_radius_DinvA(const HostCsr& A, const std::vector<double>& dinv);
     HostCsr smoothed_prolongator(const HostCsr& A, const std::vector<double>& dinv, double omega,
                                  std::vector<int>* agg_out) const;
+    // the three-phase greedy aggregation of smoothed_prolongator (row order, sequential)
+    std::vector<int> aggregate(const HostCsr& A, const std::vector<double>& dinv, int& n_agg) const;
     static std::vector<double> dense_spd_inverse(std::vector<double> M, int n);
 private:
+    DevCsr dev_next_, dev_tmp_, dev_next2_;    // device set-up: next operator, products
+    DevArray<int> d_cnt_, d_flag_, d_agg_, d_iota_, d_mem_ptr_, d_mem_rows_;
+    DevArray<double> d_p0v_, d_x0_, d_pw_;
+    std::vector<double> x0_host_;
+    int x0_dev_n_ = 0, iota_dev_n_ = 0;
+    // one level on the device: 1 built (A becomes the coarse operator), 0 stop (coarsening
+    // stalled), -1 fall back to the host for this and the following levels
+    int device_level(HostCsr& A, const int* arp, const int* aci, const double* av,
+                     std::vector<std::unique_ptr<Level>>& pool);
+    bool device_product(int rows, const int* arp, const int* aci, const double* av, const int* brp, const int* bci,
+                        const double* bv, DevCsr& C);
     std::vector<std::unique_ptr<Level>> spare_;
     std::vector<cfa_amg_level_t> desc_;
     cfa_amg_t view_{};

@@ -339,6 +339,42 @@ int cfa_bicgstab_jacobi(int n, const int* rp, const int* ci, const double* val, 
                         double* x, int max_iter, double tol, cfa_krylov_ws_t* ws,
                         cfa_solve_result_t* result, cfa_stream_t stream);
 
+/* ---- SA-AMG set-up on the device (src/cfa/linalg.cuh:995-1060, 1276-1431, host code in the
+ * reference) -----------------------------------------------------------------------------------
+ * Bit-identical to the reference's host set-up: the same operations in the same order
+ * (contraction off). The greedy aggregation is not here (it stays on the host).
+ * cfa_amg_dinv: dinv = 1 / a_ii, or 1 where |a_ii| <= 1e-300 (linalg.cuh:1003-1008).
+ * cfa_amg_power: spectral_radius_DinvA (linalg.cuh:1276-1300): `iterations` power steps of
+ * D^-1 A from the device vector x0 (the reference starts from 1 + 0.1 sin(1 + i)), norms summed
+ * per chunk of 4096 rows in row order and over the chunks in order; *lambda (host) = the last
+ * ||D^-1 A x|| / ||x||. ws: cfa_amg_power_ws_doubles(n) doubles.
+ * cfa_spgemm: C = A B in two passes (csr_multiply, linalg.cuh:1336-1380): pass 0 writes the
+ * number of distinct columns of each row to cnt and sets *overflow (device int, cleared by the
+ * caller) when a row has more than 4096 products; pass 1 writes the sorted columns and the
+ * values, each summed from 0 in the order the products are met, at c_rp (from
+ * cfa_scan_counts). One workgroup per row.
+ * cfa_scan_counts: out[0..n] = exclusive prefix sums of cnt (out[n] = total), one workgroup.
+ * cfa_amg_prolongator: P = P0 - omega D^-1 A P0 from AP0 = A P0 (smoothed_prolongator,
+ * linalg.cuh:1400-1431: the row's own aggregate merged in column order, exact zeros dropped);
+ * pass 0 counts per row, pass 1 fills at p_rp.
+ * cfa_amg_transpose: R = P^T (csr_transpose, entries of a column in ascending row order) from
+ * the aggregates' member lists (mem_ptr [n_agg + 1], mem_rows ascending) and the pattern of A;
+ * pass 0 counts per column, pass 1 fills at r_rp. No atomics anywhere. */
+int cfa_amg_dinv(int n, const int* rp, const int* ci, const double* val, double* dinv, cfa_stream_t stream);
+size_t cfa_amg_power_ws_doubles(int n);
+int cfa_amg_power(int n, const int* rp, const int* ci, const double* val, const double* dinv, const double* x0,
+                  int iterations, double* ws, double* lambda, cfa_stream_t stream);
+int cfa_spgemm(int rows, const int* a_rp, const int* a_ci, const double* a_v, const int* b_rp, const int* b_ci,
+               const double* b_v, int pass, int* cnt, const int* c_rp, int* c_ci, double* c_v, int* overflow,
+               cfa_stream_t stream);
+int cfa_scan_counts(int n, const int* cnt, int* out, cfa_stream_t stream);
+int cfa_amg_prolongator(int n, const int* ap_rp, const int* ap_ci, const double* ap_v, const int* agg,
+                        const double* p0v, const double* dinv, double omega, int pass, int* cnt, const int* p_rp,
+                        int* p_ci, double* p_v, cfa_stream_t stream);
+int cfa_amg_transpose(int n_agg, const int* mem_ptr, const int* mem_rows, const int* a_rp, const int* a_ci,
+                      const int* p_rp, const int* p_ci, const double* p_v, int pass, int* cnt, const int* r_rp,
+                      int* r_ci, double* r_v, cfa_stream_t stream);
+
 /* ---- discrete adjoint of the step (no counterpart in the reference) --------------------------
  * Vector-Jacobian products of the step's assemblies and updates (the forward operations are
  * those of src/cylinder_fem_amr.cu:277-538; the reference has no derivative code). A linear

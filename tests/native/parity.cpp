@@ -91,6 +91,7 @@ Vec smooth_field(const Mesh2D& m, std::mt19937_64& rng, double scale) {
 }
 
 Vec down(const DevArray<double>& d, size_t n) { return d.to_host(n); }
+std::vector<int> down_int(const DevArray<int>& d, size_t n) { return d.to_host(n); }
 
 struct Fixture {
     Mesh2D mesh;
@@ -450,6 +451,34 @@ void case_amg(Fixture& f) {
     same_ints("level nnz (A, P)", nnz_g, nnz_o);
     same_bits("Jacobi weights omega", om_g, om_o);
     expect(amg.coarse_n == oa.coarse_n, "coarse size");
+    std::printf("  levels built on the device: %d of %zu\n", amg.device_levels, amg.levels.size());
+    // every level's operator, prolongator, restriction and inverse diagonal, bit for bit
+    if (amg.levels.size() == oa.levels.size()) {
+        std::vector<int> pat_g, pat_o;
+        Vec val_g, val_o;
+        auto add = [&](const AmgPreconditioner::DevCsr& D, const oracle::Csr& C) {
+            const std::vector<int> rp = down_int(D.rp, D.rows + 1), ci = down_int(D.ci, D.nnz);
+            pat_g.insert(pat_g.end(), rp.begin(), rp.end());
+            pat_g.insert(pat_g.end(), ci.begin(), ci.end());
+            pat_o.insert(pat_o.end(), C.rp.begin(), C.rp.end());
+            pat_o.insert(pat_o.end(), C.ci.begin(), C.ci.end());
+            const Vec v = down(D.v, D.nnz);
+            val_g.insert(val_g.end(), v.begin(), v.end());
+            val_o.insert(val_o.end(), C.v.begin(), C.v.end());
+        };
+        for (size_t l = 0; l < amg.levels.size(); ++l) {
+            const auto& L = *amg.levels[l];
+            const auto& O = oa.levels[l];
+            add(L.A, O.A);
+            add(L.P, O.P);
+            add(L.R, O.R);
+            const Vec d = down(L.dinv, L.A.rows);
+            val_g.insert(val_g.end(), d.begin(), d.end());
+            val_o.insert(val_o.end(), O.dinv.begin(), O.dinv.end());
+        }
+        same_ints("hierarchy patterns (A, P, R per level)", pat_g, pat_o);
+        same_bits("hierarchy values (A, P, R, D^-1 per level)", val_g, val_o);
+    }
     if (!oa.levels.empty()) {
         std::vector<int> agg;
         oracle::smoothed_prolongator(oa.levels[0].A, oa.levels[0].dinv, oa.levels[0].omega, 0.08, &agg);

@@ -108,6 +108,7 @@ def solver_workloads(level: int, nrhs: int, tol: float, reps: int) -> int:
     FP32 factors; one JSON line with times per solve and per iteration and the kernel timings."""
     import cfa
     p = cfa.workload_pressure(level, nrhs, tol, reps)
+    p["amg_phases_ms"] = cfa.workload_amg_phases()
     pk = cfa.last_kernels()
     m = cfa.workload_momentum(level, tol, reps)
     mk = cfa.last_kernels()

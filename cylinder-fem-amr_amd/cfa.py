@@ -158,6 +158,20 @@ def workload_pressure(level: int, nrhs: int = 32, tol: float = 1e-8, kernel_reps
     return dict(zip(PRESSURE_FIELDS, out.tolist()))
 
 
+AMG_PHASES = ("download", "dinv_rho", "aggregation", "p_smoothing", "transpose", "galerkin", "upload", "coarse_inverse",
+              "remap", "device_levels")
+
+
+def workload_amg_phases() -> dict:
+    """AMG set-up phase times (ms) of the last pressure workload and the levels built on the device."""
+    _, host = load()
+    host.cfa_workload_amg_phases.argtypes = [ctypes.POINTER(ctypes.c_double)]
+    host.cfa_workload_amg_phases.restype = ctypes.c_int
+    out = np.zeros(len(AMG_PHASES))
+    host.cfa_workload_amg_phases(_dptr(out))
+    return dict(zip(AMG_PHASES, out.tolist()))
+
+
 def workload_momentum(level: int, tol: float = 1e-8, kernel_reps: int = 0) -> dict:
     _, host = load()
     out = np.zeros(16)

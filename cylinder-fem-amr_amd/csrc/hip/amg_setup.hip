@@ -44,29 +44,31 @@ __global__ void k_power_rows(int n, const int* __restrict__ rp, const int* __res
     y[i] = dinv[i] * s;
 }
 
+// value of lane k (wave-uniform k) of a double held by every lane
+__device__ __forceinline__ double lane_value(double v, int k) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+    return __hiloint2double(hi, lo);
+}
+
 // per chunk of 4096 rows: sum y_i^2 and x_i^2 in row order. One wavefront per chunk: the lanes
-// form 64 products at a time, lane 0 adds them in order.
+// form 64 squares at a time, then every lane adds them in lane order (the same sums in every
+// lane; lane 0 writes).
 __global__ void __launch_bounds__(WAVE) k_power_chunks(int n, const double* __restrict__ x,
                                                        const double* __restrict__ y,
                                                        double* __restrict__ py, double* __restrict__ px) {
-    __shared__ double sy2[WAVE], sx2[WAVE];
     const int c = blockIdx.x, lane = threadIdx.x;
     const int lo = c * CHUNK, hi = min(n, lo + CHUNK);
     double sy = 0.0, sx = 0.0;
     for (int base = lo; base < hi; base += WAVE) {
         const int i = base + lane;
         const double yi = i < hi ? y[i] : 0.0, xi = i < hi ? x[i] : 0.0;
-        sy2[lane] = yi * yi;
-        sx2[lane] = xi * xi;
-        __syncthreads();
-        if (lane == 0) {
-            const int cnt = min(WAVE, hi - base);
-            for (int k = 0; k < cnt; ++k) {
-                sy += sy2[k];
-                sx += sx2[k];
-            }
+        const double y2 = yi * yi, x2 = xi * xi;
+        const int cnt = min(WAVE, hi - base);
+        for (int k = 0; k < cnt; ++k) {
+            sy += lane_value(y2, k);
+            sx += lane_value(x2, k);
         }
-        __syncthreads();
     }
     if (lane == 0) {
         py[c] = sy;
@@ -96,7 +98,56 @@ __global__ void k_power_scale(int n, const double* __restrict__ y, const double*
 }
 
 // ---- sparse product C = A B, host csr_multiply semantics ------------------------------------------
-// One workgroup per row of A. The products a_ik b_kj are listed in encounter order (entries of
+// Every row of B has sorted, unique columns, so the host's encounter order for an output column
+// j is the order of the entries of A's row: c_ij = sum over a of a_ia b_(k_a) j, in a's order,
+// from 0. One thread per row merges the B rows of its A entries (one cursor each) in ascending
+// column order. Rows of A longer than MERGE_MAX entries set *overflow = 1 (the caller then uses
+// k_spgemm_rows).
+constexpr int MERGE_MAX = 64;
+
+__global__ void k_spgemm_merge(int rows, const int* __restrict__ a_rp, const int* __restrict__ a_ci,
+                               const double* __restrict__ a_v, const int* __restrict__ b_rp,
+                               const int* __restrict__ b_ci, const double* __restrict__ b_v, int pass,
+                               int* __restrict__ cnt, const int* __restrict__ c_rp, int* __restrict__ c_ci,
+                               double* __restrict__ c_v, int* __restrict__ overflow) {
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    const int a0 = a_rp[row], len = a_rp[row + 1] - a0;
+    if (len > MERGE_MAX) {
+        overflow[0] = 1;
+        if (pass == 0) cnt[row] = 0;
+        return;
+    }
+    int cur[MERGE_MAX], end[MERGE_MAX];
+    for (int t = 0; t < len; ++t) {
+        const int k = a_ci[a0 + t];
+        cur[t] = b_rp[k];
+        end[t] = b_rp[k + 1];
+    }
+    int out = 0;
+    const int o0 = pass == 1 ? c_rp[row] : 0;
+    while (true) {
+        int j = 0x7fffffff;
+        for (int t = 0; t < len; ++t)
+            if (cur[t] < end[t]) j = min(j, b_ci[cur[t]]);
+        if (j == 0x7fffffff) break;
+        double acc = 0.0;
+        for (int t = 0; t < len; ++t)
+            if (cur[t] < end[t] && b_ci[cur[t]] == j) {
+                const double prod = a_v[a0 + t] * b_v[cur[t]];
+                acc += prod;
+                ++cur[t];
+            }
+        if (pass == 1) {
+            c_ci[o0 + out] = j;
+            c_v[o0 + out] = acc;
+        }
+        ++out;
+    }
+    if (pass == 0) cnt[row] = out;
+}
+
+// Products of long rows: one workgroup per row of A. The products a_ik b_kj are listed in encounter order (entries of
 // A's row in order, each B row in order) into LDS with key (j, position); a bitonic sort by key
 // groups equal columns with positions ascending; each column is then summed from 0.0 in that
 // order. Pass 0 writes the number of distinct columns of each row to cnt[row] (and sets *overflow
@@ -359,14 +410,18 @@ int cfa_amg_power(int n, const int* rp, const int* ci, const double* val, const 
 }
 
 int cfa_spgemm(int rows, const int* a_rp, const int* a_ci, const double* a_v, const int* b_rp, const int* b_ci,
-               const double* b_v, int pass, int* cnt, const int* c_rp, int* c_ci, double* c_v, int* overflow,
-               cfa_stream_t stream) {
-    if (rows < 0 || (pass != 0 && pass != 1) || !overflow) return CFA_E_ARG;
+               const double* b_v, int pass, int method, int* cnt, const int* c_rp, int* c_ci, double* c_v,
+               int* overflow, cfa_stream_t stream) {
+    if (rows < 0 || (pass != 0 && pass != 1) || (method != 0 && method != 1) || !overflow) return CFA_E_ARG;
     if (rows == 0) return CFA_OK;
     if (!a_rp || !a_ci || !a_v || !b_rp || !b_ci || !b_v) return CFA_E_ARG;
     if ((pass == 0 && !cnt) || (pass == 1 && (!c_rp || !c_ci || !c_v))) return CFA_E_ARG;
-    hipLaunchKernelGGL(k_spgemm_rows, dim3(rows), dim3(SPGEMM_THREADS), 0, as_stream(stream), rows, a_rp, a_ci, a_v,
-                       b_rp, b_ci, b_v, pass, cnt, c_rp, c_ci, c_v, overflow);
+    if (method == 0)
+        hipLaunchKernelGGL(k_spgemm_merge, dim3(blocks(rows)), dim3(BLOCK), 0, as_stream(stream), rows, a_rp, a_ci,
+                           a_v, b_rp, b_ci, b_v, pass, cnt, c_rp, c_ci, c_v, overflow);
+    else
+        hipLaunchKernelGGL(k_spgemm_rows, dim3(rows), dim3(SPGEMM_THREADS), 0, as_stream(stream), rows, a_rp, a_ci,
+                           a_v, b_rp, b_ci, b_v, pass, cnt, c_rp, c_ci, c_v, overflow);
     return launch_status();
 }
 

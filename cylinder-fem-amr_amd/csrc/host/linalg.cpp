@@ -402,6 +402,14 @@ HostCsr csr_transpose(const HostCsr& A) {
     return T;
 }
 
+bool amg_device_setup_default() {
+    static const bool on = [] {
+        const char* e = std::getenv("CFA_AMG_HOST_SETUP");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
+
 int csr_lanes_per_row(int rows, long long nnz) {
     // Measured on MI355X (one Jacobi sweep per level, lanes 1..32, uniform level-4 and level-5
     // pressure hierarchies): rows of ~7 entries run fastest with 2-4 lanes (one lane is 2.2x
@@ -643,22 +651,28 @@ bool AmgPreconditioner::device_product(int rows, const int* arp, const int* aci,
     const cfa_stream_t s = solver_stream();
     d_cnt_.reserve(std::max(rows, 1));
     d_flag_.reserve(1);
-    dev_zero(d_flag_, sizeof(int));
-    GPU_CHECK(cfa_spgemm(rows, arp, aci, av, brp, bci, bv, 0, d_cnt_, nullptr, nullptr, nullptr, d_flag_, s));
-    int flag = 0;
-    d_flag_.download(&flag, 1);
-    if (flag) return false;
-    C.rp.reserve(rows + 1);
-    GPU_CHECK(cfa_scan_counts(rows, d_cnt_, C.rp, s));
-    int total = 0;
-    GPU_CHECK(cfa_memcpy(&total, C.rp.ptr + rows, sizeof(int), 1, s));
-    C.ci.reserve(std::max(total, 1));
-    C.v.reserve(std::max(total, 1));
-    GPU_CHECK(cfa_spgemm(rows, arp, aci, av, brp, bci, bv, 1, nullptr, C.rp, C.ci, C.v, d_flag_, s));
-    C.rows = rows;
-    C.nnz = total;
-    C.tpr = csr_lanes_per_row(rows, total);
-    return true;
+    // method 0: one thread per row merging the right rows (rows of A up to 64 entries);
+    // method 1: one workgroup per row sorting the products in LDS (up to 4096 per row)
+    for (int method = 0; method < 2; ++method) {
+        dev_zero(d_flag_, sizeof(int));
+        GPU_CHECK(cfa_spgemm(rows, arp, aci, av, brp, bci, bv, 0, method, d_cnt_, nullptr, nullptr, nullptr, d_flag_,
+                             s));
+        int flag = 0;
+        d_flag_.download(&flag, 1);
+        if (flag) continue;
+        C.rp.reserve(rows + 1);
+        GPU_CHECK(cfa_scan_counts(rows, d_cnt_, C.rp, s));
+        int total = 0;
+        GPU_CHECK(cfa_memcpy(&total, C.rp.ptr + rows, sizeof(int), 1, s));
+        C.ci.reserve(std::max(total, 1));
+        C.v.reserve(std::max(total, 1));
+        GPU_CHECK(cfa_spgemm(rows, arp, aci, av, brp, bci, bv, 1, method, nullptr, C.rp, C.ci, C.v, d_flag_, s));
+        C.rows = rows;
+        C.nnz = total;
+        C.tpr = csr_lanes_per_row(rows, total);
+        return true;
+    }
+    return false;
 }
 
 int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, const double* av,

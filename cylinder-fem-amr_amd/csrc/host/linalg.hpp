@@ -126,6 +126,8 @@ struct HostCsr {
 HostCsr csr_multiply(const HostCsr& A, const HostCsr& B);
 HostCsr csr_transpose(const HostCsr& A);
 
+bool amg_device_setup_default();
+
 class AmgPreconditioner {
 public:
     struct DevCsr {
@@ -150,7 +152,8 @@ public:
     // numeric set-up phases (D^-1, spectral radius, P smoothing, transpose, Galerkin product) on
     // the device, bit-identical to the host path; the aggregation stays on the host. A level
     // whose products exceed the device kernel's row capacity falls back to the host path.
-    bool device_setup = true;
+    // (CFA_AMG_HOST_SETUP=1 in the environment selects the host path, for comparisons)
+    bool device_setup = amg_device_setup_default();
     bool keep_p0_host = true;   // download the level-0 prolongator for remap()
     int device_levels = 0;      // levels the last set-up built on the device
 

@@ -28,6 +28,9 @@ Mesh2D uniform_mesh(int levels) {
     return m;
 }
 
+// phase times (ms) of the last pressure workload's AMG set-up, then the levels built on the device
+double g_amg_phases[AmgPreconditioner::PH_COUNT + 1] = {};
+
 double host_ms(Clock::time_point t0) {
     dev_sync();
     return ms_since(t0);
@@ -68,6 +71,8 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
     t0 = Clock::now();
     amg.setup(n, g.nnz, g.d_row_ptr, g.d_col_ind, g.d_Kp_val);
     const double setup_ms = host_ms(t0);
+    for (int k = 0; k < AmgPreconditioner::PH_COUNT; ++k) g_amg_phases[k] = amg.phase_ms[k];
+    g_amg_phases[AmgPreconditioner::PH_COUNT] = amg.device_levels;
     GpuLinearSolver solver;
     double solve_ms = 0.0, worst = 0.0;
     long long iters = 0;
@@ -156,6 +161,12 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
 // BiCGSTAB2 from x = w to `tol`, once with FP64 and once with FP32 ILU(0) factors.
 // out[16] = n, nnz, ne, mesh s, colouring ms, colours, FP64: factor ms, solve ms, it_u, it_v;
 // FP32: factor ms, solve ms, it_u, it_v; max |x32 - x64| / max |x64|, us per FP64 iteration.
+int cfa_workload_amg_phases(double* out) {
+    if (!out) return CFA_E_ARG;
+    for (int k = 0; k <= AmgPreconditioner::PH_COUNT; ++k) out[k] = g_amg_phases[k];
+    return CFA_OK;
+}
+
 int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out) {
     if (level < 0 || level > 7 || !(tol > 0.0) || !out) return CFA_E_ARG;
     GPU_CHECK(cfa_init(0));

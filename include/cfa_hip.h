@@ -350,9 +350,11 @@ int cfa_bicgstab_jacobi(int n, const int* rp, const int* ci, const double* val, 
  * ||D^-1 A x|| / ||x||. ws: cfa_amg_power_ws_doubles(n) doubles.
  * cfa_spgemm: C = A B in two passes (csr_multiply, linalg.cuh:1336-1380): pass 0 writes the
  * number of distinct columns of each row to cnt and sets *overflow (device int, cleared by the
- * caller) when a row has more than 4096 products; pass 1 writes the sorted columns and the
+ * caller) when a row is beyond the method's limit; pass 1 writes the sorted columns and the
  * values, each summed from 0 in the order the products are met, at c_rp (from
- * cfa_scan_counts). One workgroup per row.
+ * cfa_scan_counts). method 0: one thread per row merging the right rows (rows of A with at
+ * most 64 entries); method 1: one workgroup per row sorting up to 4096 products in LDS. Rows of
+ * B must have sorted, unique columns.
  * cfa_scan_counts: out[0..n] = exclusive prefix sums of cnt (out[n] = total), one workgroup.
  * cfa_amg_prolongator: P = P0 - omega D^-1 A P0 from AP0 = A P0 (smoothed_prolongator,
  * linalg.cuh:1400-1431: the row's own aggregate merged in column order, exact zeros dropped);
@@ -365,8 +367,8 @@ size_t cfa_amg_power_ws_doubles(int n);
 int cfa_amg_power(int n, const int* rp, const int* ci, const double* val, const double* dinv, const double* x0,
                   int iterations, double* ws, double* lambda, cfa_stream_t stream);
 int cfa_spgemm(int rows, const int* a_rp, const int* a_ci, const double* a_v, const int* b_rp, const int* b_ci,
-               const double* b_v, int pass, int* cnt, const int* c_rp, int* c_ci, double* c_v, int* overflow,
-               cfa_stream_t stream);
+               const double* b_v, int pass, int method, int* cnt, const int* c_rp, int* c_ci, double* c_v,
+               int* overflow, cfa_stream_t stream);
 int cfa_scan_counts(int n, const int* cnt, int* out, cfa_stream_t stream);
 int cfa_amg_prolongator(int n, const int* ap_rp, const int* ap_ci, const double* ap_v, const int* agg,
                         const double* p0v, const double* dinv, double omega, int pass, int* cnt, const int* p_rp,

@@ -74,6 +74,10 @@ void cfa_mesh_probes(void* mesh, int* ab, double* w);
  * kernel_reps > 0 also times the main kernels (read with cfa_last_run_kernels). */
 int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, double* out);
 int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out);
+/* AMG set-up of the last pressure workload: out[10] = ms in download, D^-1 + spectral radius,
+ * aggregation, P smoothing, transpose, Galerkin product, upload, coarse inverse, remap, then the
+ * number of levels built on the device. */
+int cfa_workload_amg_phases(double* out);
 
 /* Solver-study dumps written by --dump-systems N (src/cylinder_fem_amr.cu:331-352 and :375-378:
  * <prefix>_momentum.bin = int32 n, nnz; row_ptr [n + 1], col_ind [nnz] int32; values [nnz],

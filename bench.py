@@ -42,10 +42,15 @@ WORKLOADS = {
                  "--amr-tol-interp", "0.005", "--max-level", "5", "--max-elems", "400000", "--min-level", "2"],
     # runs/paper/mms_L3/cmd.txt
     "mms_u3": ["--mms", "--uniform-level", "3", "--dt", "0.00025"],
+    # the adaptive flow above with refinement capped at level 6 (300k-500k triangles)
+    "adaptive_l6": ["--dt", "0.0005", "--amr-interval", "20", "--amr-overlap", "10", "--amr-tol-vort", "0.05",
+                    "--amr-tol-interp", "0.005", "--max-level", "6", "--max-elems", "600000", "--min-level", "2"],
 }
 DESCRIPTION = {
     "adaptive": "adaptive obstacle flow, NVB levels 2-5, remesh every 20 steps, dt 5e-4, Re 100",
     "mms_u3": "manufactured solution on uniform level 3 (29152 nodes), dt 2.5e-4",
+    "adaptive_l6": "adaptive obstacle flow, NVB levels 2-6 (element budget 600k), remesh every 20 steps, dt 5e-4, "
+                   "Re 100",
 }
 
 

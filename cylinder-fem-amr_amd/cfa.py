@@ -294,8 +294,10 @@ class Adjoint:
         host.cfa_step_fwd.restype = c_int
         host.cfa_window_forward.argtypes = [vp, dptr, c_int, c_int, dptr, dptr, dptr, dptr, dptr]
         host.cfa_window_forward.restype = c_int
-        host.cfa_window_adjoint.argtypes = [vp, dptr, c_int, c_int, c_int, dptr, dptr, dptr, dptr, dptr, dptr]
+        host.cfa_window_adjoint.argtypes = [vp, dptr, c_int, c_int, c_int, dptr, dptr, dptr, dptr, dptr, dptr, dptr]
         host.cfa_window_adjoint.restype = c_int
+        host.cfa_adjoint_set_shape.argtypes = [vp, dptr, c_int]
+        host.cfa_adjoint_set_shape.restype = c_int
         self._host = host
         self._ctx = host.cfa_adjoint_create(level, dt, tol)
         if not self._ctx:
@@ -344,17 +346,31 @@ class Adjoint:
                                              _dptr(s_out), _dptr(J), _dptr(hist)), "cfa_window_forward")
         return float(J[0]), s_out, hist
 
+    N_MODES = 64
+
+    def set_shape(self, eps):
+        """Radial modes of the hole boundary (64 values; zeros = the reference geometry)."""
+        e = self._vec(eps, self.N_MODES)
+        _check(self._host.cfa_adjoint_set_shape(self._ctx, _dptr(e), self.N_MODES), "cfa_adjoint_set_shape")
+
     def gradient(self, theta, first_step: int, steps: int, state0, checkpoint_every: int = 20,
-                 weights=(1.0, 0.0), state_gradient: bool = False):
-        """Returns (J, dJ/dtheta [2], dJ/dstate0 [6 n] or None, stats dict)."""
+                 weights=(1.0, 0.0), state_gradient: bool = False, shape_gradient: bool = False):
+        """Returns (J, dJ/dtheta [2], dJ/dstate0 [6 n] or None, stats dict); with shape_gradient the
+        dict also holds dJ_dshape [64]."""
         th, w = self._vec(theta, 2), self._vec(weights, 2)
         s0 = self._vec(state0, 6 * self.n)
         J, g, st = np.zeros(1), np.zeros(2), np.zeros(5)
         gs = np.zeros(6 * self.n) if state_gradient else None
+        gsh = np.zeros(self.N_MODES) if shape_gradient else None
         _check(self._host.cfa_window_adjoint(self._ctx, _dptr(th), first_step, steps, checkpoint_every, _dptr(w),
-                                             _dptr(s0), _dptr(J), _dptr(g), _dptr(gs) if gs is not None else None,
-                                             _dptr(st)), "cfa_window_adjoint")
-        return float(J[0]), g, gs, dict(zip(ADJOINT_STATS, st.tolist()))
+                                             _dptr(s0), _dptr(J), _dptr(g),
+                                             _dptr(gsh) if gsh is not None else None,
+                                             _dptr(gs) if gs is not None else None, _dptr(st)),
+               "cfa_window_adjoint")
+        stats = dict(zip(ADJOINT_STATS, st.tolist()))
+        if gsh is not None:
+            stats["dJ_dshape"] = gsh
+        return float(J[0]), g, gs, stats
 
 
 def last_mms() -> dict:

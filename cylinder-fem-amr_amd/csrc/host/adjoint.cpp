@@ -35,6 +35,25 @@ constexpr double kTauTime = 1.0e-3;   // SUPG time scale (src/cfa/options.cuh:19
 constexpr double kSkew = 0.5;
 constexpr double kStab = 0.3;         // OSS pressure stabilisation (--pressure-stab default)
 constexpr int kAmgRebuild = 50;
+constexpr int kModes = 64;            // radial modes of the hole boundary
+
+// Radial deformation of the hole boundary r(phi) = R (1 + sum_k eps_k g_k(phi)), g_k Gaussians
+// centred at 2 pi k / 64 with width 2 pi / 64 (periodic), carried into the mesh by the smooth
+// radial cut-off chi(r) = 1 - t^2 (3 - 2 t), t = (r - R) / (2 R) clamped to [0, 1]: nodes at
+// r >= 3 R (the channel walls and the inlet) do not move. The displacement is linear in eps;
+// mode k moves node (x, y) by R g_k(phi) chi(r) (x - xc, y - yc) / r.
+double mode_weight(int k, double dx, double dy) {
+    const double r = std::hypot(dx, dy);
+    const double R = Sim::R_CYL, R2 = 3.0 * Sim::R_CYL;
+    if (r >= R2 || r <= 0.0) return 0.0;
+    const double t = std::min(1.0, std::max(0.0, (r - R) / (R2 - R)));
+    const double chi = 1.0 - t * t * (3.0 - 2.0 * t);
+    const double phi = std::atan2(dy, dx);
+    const double sigma = 2.0 * Sim::PI / kModes;
+    double d = phi - 2.0 * Sim::PI * k / kModes;
+    d = std::remainder(d, 2.0 * Sim::PI);
+    return R * std::exp(-0.5 * (d / sigma) * (d / sigma)) * chi / r;   // times (dx, dy)
+}
 
 struct Coeffs {
     double dt_proj, a0, a1, a2, c1, c2;
@@ -93,6 +112,8 @@ public:
         uo = u;
         vo = v;
         g_.sync_from_host(mesh_, u, v, p, uo, vo);
+        x0_ = mesh_.x;
+        y0_ = mesh_.y;
         n = g_.n_nodes;
         ne = g_.n_elems;
         nnz = g_.nnz;
@@ -106,6 +127,8 @@ public:
         d_state_.reserve(6 * (size_t)n);
         for (auto* a : {&ustar_, &lam_m_, &lam_p_, &dpb_, &usb_, &vec_tmp_}) a->reserve(2 * (size_t)n);
         for (auto* a : {&adj_a_, &adj_b_}) a->reserve(5 * (size_t)n);
+        d_xb_.reserve(n);
+        d_yb_.reserve(n);
         initial_[0] = u;
         initial_[1] = v;
         initial_[2] = uo;
@@ -117,6 +140,36 @@ public:
     void initial_state(double* out) const {
         for (int f = 0; f < 5; ++f) std::memcpy(out + (size_t)f * n, initial_[f].data(), n * sizeof(double));
         std::memset(out + 5 * (size_t)n, 0, n * sizeof(double));
+    }
+
+    // deform the mesh by the radial modes eps[64] (from the undeformed coordinates)
+    void set_shape(const double* eps) {
+        std::vector<double> x = x0_, y = y0_;
+        for (int k = 0; k < kModes; ++k) eps_[k] = eps[k];
+        for (int i = 0; i < n; ++i) {
+            const double dx = x0_[i] - Sim::X_CYL, dy = y0_[i] - Sim::Y_CYL;
+            double w = 0.0;
+            for (int k = 0; k < kModes; ++k)
+                if (eps_[k] != 0.0) w += eps_[k] * mode_weight(k, dx, dy);
+            x[i] += w * dx;
+            y[i] += w * dy;
+        }
+        g_.d_x.upload(x);
+        g_.d_y.upload(y);
+        dev_sync();
+    }
+    // dJ/deps_k = sum_i x_bar_i dx_i/deps_k + y_bar_i dy_i/deps_k
+    void shape_gradient(double* out) const {
+        const std::vector<double> xb = d_xb_.to_host(n), yb = d_yb_.to_host(n);
+        for (int k = 0; k < kModes; ++k) {
+            double acc = 0.0;
+            for (int i = 0; i < n; ++i) {
+                const double dx = x0_[i] - Sim::X_CYL, dy = y0_[i] - Sim::Y_CYL;
+                const double w = mode_weight(k, dx, dy);
+                if (w != 0.0) acc += w * (xb[i] * dx + yb[i] * dy);
+            }
+            out[k] = acc;
+        }
     }
 
     // device state [u | v | u_old | v_old | p | dp] <-> the working arrays
@@ -197,7 +250,8 @@ public:
     // Gradient of the window functional: theta_bar[2] and, when state_bar is given, dJ/dS^0
     // (device, 6n; the dp slot is zero).
     double gradient(const double* theta, int first_step, int steps, int every, const double* w,
-                    const double* state0, double* theta_bar, double* state_bar) {
+                    const double* state0, double* theta_bar, double* state_bar, double* shape_bar = nullptr) {
+        shape_grad_ = shape_bar != nullptr;
         if (every <= 0) every = steps;
         const int n_seg = (steps + every - 1) / every;
         std::vector<DevArray<double>> cps(n_seg);
@@ -211,6 +265,10 @@ public:
         seg_states.reserve((size_t)(every + 1) * 6 * n);
         seg_ustar.reserve((size_t)every * 2 * n);
         dev_zero(d_theta_bar_, 2 * sizeof(double));
+        if (shape_grad_) {
+            dev_zero(d_xb_, n * sizeof(double));
+            dev_zero(d_yb_, n * sizeof(double));
+        }
         dev_zero(adj_a_, 5 * (size_t)n * sizeof(double));   // adjoint of S^W
         dev_zero(lam_m_, 2 * (size_t)n * sizeof(double));
         dev_zero(lam_p_, n * sizeof(double));
@@ -237,6 +295,7 @@ public:
             }
         }
         d_theta_bar_.download(theta_bar, 2);
+        if (shape_grad_) shape_gradient(shape_bar);
         if (state_bar) {
             dev_copy(state_bar, adj_next, 5 * (size_t)n);
             dev_zero(state_bar + 5 * (size_t)n, n * sizeof(double));
@@ -256,8 +315,11 @@ private:
     int amg_built_ = -1000000;
     DevArray<int> d_perm_;
     DevArray<double> d_AT_, d_ws_, d_theta_bar_, d_state_, ustar_, lam_m_, lam_p_, dpb_, usb_, vec_tmp_;
-    DevArray<double> adj_a_, adj_b_;
+    DevArray<double> adj_a_, adj_b_, d_xb_, d_yb_;
     std::vector<double> initial_[5];
+    std::vector<double> x0_, y0_;       // undeformed node coordinates
+    double eps_[kModes] = {};            // radial boundary modes in use
+    bool shape_grad_ = false;
 
     void maybe_rebuild_amg(int step_index) {
         if (std::abs(step_index - amg_built_) >= kAmgRebuild) {
@@ -302,11 +364,13 @@ private:
         maybe_rebuild_amg(step_index);
 
         // forces of this step
+        double* xb = shape_grad_ ? d_xb_.ptr : nullptr;
+        double* yb = shape_grad_ ? d_yb_.ptr : nullptr;
         GPU_CHECK(cfa_adj_forces(m, u1, v1, u0, v0, uo0, vo0, p1, c.a0, -c.a1, -c.a2, nu, Sim::RHO, wx, wy, d_ws_,
-                                 au1, av1, auo1, avo1, auo0, avo0, ap1, d_theta_bar_, s));
+                                 au1, av1, auo1, avo1, auo0, avo0, ap1, xb, yb, d_theta_bar_, s));
         // rotational pressure update and velocity correction
-        GPU_CHECK(cfa_adj_correction(m, c.dt_proj, Sim::RHO, nu, g_.d_div_num, g_.d_mass_lumped, ap1, au1, av1, d_ws_,
-                                     dpb_, d_theta_bar_, s));
+        GPU_CHECK(cfa_adj_correction(m, c.dt_proj, Sim::RHO, nu, g_.d_div_num, g_.d_mass_lumped, dp, ap1, au1, av1,
+                                     d_ws_, dpb_, xb, yb, d_theta_bar_, s));
         GPU_CHECK(cfa_axpy(n, 1.0, ap1, ap0, s));
         double* usb = usb_;
         double* vsb = usb_ + N;
@@ -322,7 +386,7 @@ private:
                                                    &op, &amg_);
             GPU_CHECK(cfa_scale(n, sp, lam_p_, s));
             GPU_CHECK(cfa_adj_pressure(m, g_.d_elem_s, us, vs, p0, dp, lam_p_, g_.d_mass_lumped, nu, Sim::RHO,
-                                       c.dt_proj, kStab, d_ws_, usb, vsb, ap0, d_theta_bar_, s));
+                                       c.dt_proj, kStab, d_ws_, usb, vsb, ap0, xb, yb, d_theta_bar_, s));
         } else {
             dev_zero(lam_p_, N * sizeof(double));
         }
@@ -343,7 +407,7 @@ private:
             GPU_CHECK(cfa_scale(n, fu, lu, s));
             GPU_CHECK(cfa_scale(n, fv, lv, s));
             GPU_CHECK(cfa_adj_momentum(m, u0, v0, uo0, vo0, p0, us, vs, lu, lv, nu, Sim::RHO, c.a0, c.a1, c.a2, c.c1,
-                                       c.c2, kTauTime, kSkew, Sim::H_CHANNEL, d_ws_, au0, av0, auo0, avo0, ap0,
+                                       c.c2, kTauTime, kSkew, Sim::H_CHANNEL, d_ws_, au0, av0, auo0, avo0, ap0, xb, yb,
                                        d_theta_bar_, s));
         } else {
             dev_zero(lam_m_, 2 * N * sizeof(double));
@@ -427,9 +491,15 @@ int cfa_window_forward(cfa_adjoint* ctx, const double* theta, int first_step, in
     return CFA_OK;
 }
 
+int cfa_adjoint_set_shape(cfa_adjoint* ctx, const double* eps, int n_modes) {
+    if (!ctx || !eps || n_modes != kModes) return CFA_E_ARG;
+    ctx->w.set_shape(eps);
+    return CFA_OK;
+}
+
 int cfa_window_adjoint(cfa_adjoint* ctx, const double* theta, int first_step, int steps, int checkpoint_every,
                        const double* weights, const double* state0, double* J, double* dJ_dtheta,
-                       double* dJ_dstate0, double* stats) {
+                       double* dJ_dshape, double* dJ_dstate0, double* stats) {
     if (!ctx || !theta || first_step < 1 || steps < 1 || checkpoint_every < 0 || !weights || !state0 || !J ||
         !dJ_dtheta)
         return CFA_E_ARG;
@@ -440,7 +510,7 @@ int cfa_window_adjoint(cfa_adjoint* ctx, const double* theta, int first_step, in
     GPU_CHECK(cfa_memcpy(d0, state0, len * sizeof(double), 0, solver_stream()));
     if (dJ_dstate0) dbar.reserve(len);
     *J = w.gradient(theta, first_step, steps, checkpoint_every, weights, d0, dJ_dtheta,
-                    dJ_dstate0 ? dbar.ptr : nullptr);
+                    dJ_dstate0 ? dbar.ptr : nullptr, dJ_dshape);
     if (dJ_dstate0) {
         GPU_CHECK(cfa_memcpy(dJ_dstate0, dbar, len * sizeof(double), 1, solver_stream()));
         dev_sync();

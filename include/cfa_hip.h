@@ -382,40 +382,44 @@ int cfa_amg_transpose(int n_agg, const int* mem_ptr, const int* mem_rows, const 
  * those of src/cylinder_fem_amr.cu:277-538; the reference has no derivative code). A linear
  * solve x = A(q)^-1 b(q) is differentiated as q_bar = d/dq [lambda^T (b(q) - A(q) x)] with
  * A^T lambda = x_bar solved by the caller. All results are ADDED to the *_bar outputs and to
- * theta_bar[0] (viscosity) and theta_bar[1] (inflow peak velocity). ws: device workspace of
- * cfa_adj_workspace_doubles(n, ne) doubles. Deterministic: element contributions are summed per
- * node in element order. */
+ * theta_bar[0] (viscosity) and theta_bar[1] (inflow peak velocity). x_bar, y_bar (node
+ * coordinates, shape sensitivities) may both be NULL, which skips that part. ws: device
+ * workspace of cfa_adj_workspace_doubles(n, ne) doubles. Deterministic: element contributions
+ * are summed per node in element order. */
 size_t cfa_adj_workspace_doubles(int n, int ne);
 /* Momentum system (fem_kernels.cuh:40-181, :411-453) at x = (u_star, v_star) with adjoint
  * solutions lam_u, lam_v of A^T: sensitivities to u^n, v^n (u_n), u^{n-1}, v^{n-1} (u_prev),
- * p^n, viscosity and the inflow peak (Dirichlet inlet data 4 u_max y (H - y) / H^2). */
+ * p^n, the coordinates, viscosity and the inflow peak (inlet data 4 u_max y (H - y) / H^2). */
 int cfa_adj_momentum(const cfa_mesh_t* m, const double* u_n, const double* v_n, const double* u_prev,
                      const double* v_prev, const double* p, const double* u_star, const double* v_star,
                      const double* lam_u, const double* lam_v, double nu, double rho, double a0, double a1,
                      double a2, double c1, double c2, double tau_time, double skew, double h_channel,
                      double* ws, double* u_n_bar, double* v_n_bar, double* u_prev_bar, double* v_prev_bar,
-                     double* p_bar, double* theta_bar, cfa_stream_t stream);
+                     double* p_bar, double* x_bar, double* y_bar, double* theta_bar, cfa_stream_t stream);
 /* Pressure increment system (OSS weights fem_kernels.cuh:697-719, PPE :201-282 with the outlet
  * rows of :460-531, OSS operator :763-885) at the solution dp with adjoint solution lam of the
- * same (self-adjoint on the free rows) operator: sensitivities to u*, v*, p^n and viscosity. */
+ * same (self-adjoint on the free rows) operator: sensitivities to u*, v*, p^n, the coordinates
+ * (including those of the lumped mass in M^-1) and viscosity. */
 int cfa_adj_pressure(const cfa_mesh_t* m, const double* elem_s, const double* u_star, const double* v_star,
                      const double* p, const double* dp, const double* lam, const double* mass, double nu, double rho,
                      double dt, double stab, double* ws, double* u_star_bar, double* v_star_bar, double* p_bar,
-                     double* theta_bar, cfa_stream_t stream);
+                     double* x_bar, double* y_bar, double* theta_bar, cfa_stream_t stream);
 /* Rotational pressure update and velocity correction (fem_kernels.cuh:536-649):
  * dp_bar = p1_bar + nu dt K M^-1 p1_bar - (dt/rho) G^T M^-1 u1_bar (corrected rows), zero on
- * outlet rows (overwritten, not added); theta_bar[0] += p1_bar . (-rho div_num / m). The
- * sensitivities u*_bar = u1_bar and p^n_bar = p1_bar are identities left to the caller. */
+ * outlet rows (overwritten, not added); theta_bar[0] += p1_bar . (-rho div_num / m); with
+ * x_bar, y_bar the coordinate sensitivities of G, K and M at the increment dp. The sensitivities
+ * u*_bar = u1_bar and p^n_bar = p1_bar are identities left to the caller. */
 int cfa_adj_correction(const cfa_mesh_t* m, double dt, double rho, double nu, const double* div_num,
-                       const double* mass, const double* p1_bar, const double* u1_bar, const double* v1_bar,
-                       double* ws, double* dp_bar, double* theta_bar, cfa_stream_t stream);
+                       const double* mass, const double* dp, const double* p1_bar, const double* u1_bar,
+                       const double* v1_bar, double* ws, double* dp_bar, double* x_bar, double* y_bar,
+                       double* theta_bar, cfa_stream_t stream);
 /* Volume-integral force (fem_kernels.cuh:1046-1123): gradient of wx F_x + wy F_y with respect
- * to u, v, u_old, v_old, u_prev, v_prev, p and viscosity. */
+ * to u, v, u_old, v_old, u_prev, v_prev, p, the coordinates and viscosity. */
 int cfa_adj_forces(const cfa_mesh_t* m, const double* u, const double* v, const double* u_old,
                    const double* v_old, const double* u_prev, const double* v_prev, const double* p, double b0,
                    double b1, double b2, double nu, double rho, double wx, double wy, double* ws, double* u_bar,
                    double* v_bar, double* u_old_bar, double* v_old_bar, double* u_prev_bar, double* v_prev_bar,
-                   double* p_bar, double* theta_bar, cfa_stream_t stream);
+                   double* p_bar, double* x_bar, double* y_bar, double* theta_bar, cfa_stream_t stream);
 /* out[k] = in[perm[k]], k < nnz (CSR values of A^T on a symmetric pattern); x *= a. */
 int cfa_gather_values(int nnz, const int* perm, const double* in, double* out, cfa_stream_t stream);
 int cfa_scale(int n, double a, double* x, cfa_stream_t stream);

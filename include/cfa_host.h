@@ -131,11 +131,15 @@ int cfa_amg_setup_record(int n, const int* rp, const int* ci, const double* val,
  * cfa_step_fwd: one step, forces[2] = (C_D, C_L) of the new state (may be NULL).
  * cfa_window_forward: W steps from state0; state_out (may be NULL), J, history [2 W] (may be
  * NULL) = (C_D, C_L) per step.
+ * cfa_adjoint_set_shape: deform the hole boundary by 64 radial modes, r(phi) = R (1 + sum_k
+ * eps_k g_k(phi)), g_k Gaussians centred at 2 pi k / 64 of width 2 pi / 64, carried into the
+ * mesh by a smooth radial cut-off that vanishes at 3 R (n_modes must be 64; all zero = the
+ * reference's geometry). The deformation holds for the following calls.
  * cfa_window_adjoint: J and dJ/dtheta [2] by the discrete adjoint with checkpoints every
- * checkpoint_every steps (0: one segment) and recomputation in between; dJ_dstate0 [6 n] (may
- * be NULL) = dJ/dstate0 (zero in the dp slot); stats [5] (may be NULL) = forward ms,
- * recompute + reverse ms, recomputed steps, adjoint PCG iterations, adjoint BiCGSTAB2
- * iterations. */
+ * checkpoint_every steps (0: one segment) and recomputation in between; dJ_dshape [64] (may be
+ * NULL) = dJ/deps at the current shape; dJ_dstate0 [6 n] (may be NULL) = dJ/dstate0 (zero in the
+ * dp slot); stats [5] (may be NULL) = forward ms, recompute + reverse ms, recomputed steps,
+ * adjoint PCG iterations, adjoint BiCGSTAB2 iterations. */
 typedef struct cfa_adjoint cfa_adjoint_t;
 cfa_adjoint_t* cfa_adjoint_create(int level, double dt, double tol);
 void cfa_adjoint_destroy(cfa_adjoint_t* ctx);
@@ -145,9 +149,10 @@ int cfa_step_fwd(cfa_adjoint_t* ctx, const double* theta, int step, const double
                  double* forces);
 int cfa_window_forward(cfa_adjoint_t* ctx, const double* theta, int first_step, int steps, const double* weights,
                        const double* state0, double* state_out, double* J, double* history);
+int cfa_adjoint_set_shape(cfa_adjoint_t* ctx, const double* eps, int n_modes);
 int cfa_window_adjoint(cfa_adjoint_t* ctx, const double* theta, int first_step, int steps, int checkpoint_every,
                        const double* weights, const double* state0, double* J, double* dJ_dtheta,
-                       double* dJ_dstate0, double* stats);
+                       double* dJ_dshape, double* dJ_dstate0, double* stats);
 
 #ifdef __cplusplus
 }

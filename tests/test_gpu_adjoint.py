@@ -8,6 +8,9 @@ cfa_window_adjoint), on the frozen uniform level-1 mesh (1912 nodes), dt 1e-3.
   spin-up (checkpoints every 5 steps), against central finite differences of the forward window
   with relative step 1e-6, both at solver tolerance 1e-12: agreement to 1e-5 relative.
 * dJ/dstate0 in a seeded random direction against a central difference (Taylor check).
+* dJ/deps for the 64 radial modes of the hole boundary (shape sensitivities through every
+  element kernel's coordinates) against central differences of the forward window on deformed
+  meshes, along a seeded random direction and for single modes, relative step 1e-5: 1e-5.
 * The gradient does not depend on the checkpoint spacing.
 * A torch.autograd.Function (this harness only) whose forward is cfa_window_forward and whose
   backward is cfa_window_adjoint gives the same gradient through theta.grad.
@@ -152,3 +155,27 @@ def test_level2_gradient_vs_oracle_central_differences():
     print(f"  J {J:.12f}  adjoint {g}  oracle central differences {fd}", flush=True)
     for k in range(2):
         assert abs(g[k] - fd[k]) <= 1e-5 * abs(fd[k]), (k, g[k], fd[k])
+
+
+def test_shape_gradient_vs_central_differences(adj, spun):
+    print("[case] dJ/deps (64 radial boundary modes) vs central differences", flush=True)
+    zero = np.zeros(64)
+    adj.set_shape(zero)
+    try:
+        _, _, _, st = adj.gradient(THETA0, FIRST, W, spun, EVERY, shape_gradient=True)
+        g = st["dJ_dshape"]
+        assert g.shape == (64,) and np.all(np.isfinite(g))
+        rng = np.random.default_rng(11)
+        directions = [rng.standard_normal(64), np.eye(64)[0], np.eye(64)[16]]
+        h = 1e-5
+        for d in directions:
+            adj.set_shape(h * d)
+            jp = adj.forward(THETA0, FIRST, W, spun)[0]
+            adj.set_shape(-h * d)
+            jm = adj.forward(THETA0, FIRST, W, spun)[0]
+            fd = (jp - jm) / (2 * h)
+            ad = float(g @ d)
+            print(f"  <dJ/deps, d> adjoint {ad:.10e}  central difference {fd:.10e}", flush=True)
+            assert abs(ad - fd) <= 1e-5 * max(abs(fd), 1e-3 * np.max(np.abs(g))), (ad, fd)
+    finally:
+        adj.set_shape(zero)

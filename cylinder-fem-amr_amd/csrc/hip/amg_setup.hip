@@ -8,8 +8,11 @@
 //   * a sparse product accumulates every output entry in the host's encounter order (row entries
 //     of the left factor in order, each right row in order), starting from 0.0;
 //   * contraction is off, so every product and sum is the host's own operation.
-// There are no atomics: products and P are built row by row, the transpose column by column from
-// the aggregates' member lists.
+// Nothing is summed by atomics: products and P are built row by row, the transpose is a stable
+// radix sort (rocPRIM) of P's entries by column that moves values without combining them.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
 #include "cfa_device.hpp"
 
 using namespace cfa_dev;
@@ -44,35 +47,28 @@ __global__ void k_power_rows(int n, const int* __restrict__ rp, const int* __res
     y[i] = dinv[i] * s;
 }
 
-// value of lane k (wave-uniform k) of a double held by every lane
-__device__ __forceinline__ double lane_value(double v, int k) {
-    const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
-    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
-    return __hiloint2double(hi, lo);
-}
-
-// per chunk of 4096 rows: sum y_i^2 and x_i^2 in row order. One wavefront per chunk: the lanes
-// form 64 squares at a time, then every lane adds them in lane order (the same sums in every
-// lane; lane 0 writes).
+// per chunk of 4096 rows: sum y_i^2 and x_i^2 in row order. One wavefront per chunk stages the
+// squares in LDS; then lane 0 adds the y^2 and lane 1 the x^2 in row order, two dependent chains
+// running side by side in one instruction stream with their LDS reads issued ahead.
 __global__ void __launch_bounds__(WAVE) k_power_chunks(int n, const double* __restrict__ x,
                                                        const double* __restrict__ y,
                                                        double* __restrict__ py, double* __restrict__ px) {
+    __shared__ double sq[2][CHUNK];
     const int c = blockIdx.x, lane = threadIdx.x;
-    const int lo = c * CHUNK, hi = min(n, lo + CHUNK);
-    double sy = 0.0, sx = 0.0;
-    for (int base = lo; base < hi; base += WAVE) {
-        const int i = base + lane;
-        const double yi = i < hi ? y[i] : 0.0, xi = i < hi ? x[i] : 0.0;
-        const double y2 = yi * yi, x2 = xi * xi;
-        const int cnt = min(WAVE, hi - base);
-        for (int k = 0; k < cnt; ++k) {
-            sy += lane_value(y2, k);
-            sx += lane_value(x2, k);
-        }
+    const int lo = c * CHUNK, cnt = min(n, lo + CHUNK) - lo;
+    for (int k = lane; k < cnt; k += WAVE) {
+        const double yi = y[lo + k], xi = x[lo + k];
+        sq[0][k] = yi * yi;
+        sq[1][k] = xi * xi;
     }
-    if (lane == 0) {
-        py[c] = sy;
-        px[c] = sx;
+    __syncthreads();
+    if (lane < 2) {
+        const double* q = sq[lane];
+        double acc = 0.0;
+#pragma unroll 16
+        for (int k = 0; k < cnt; ++k) acc += q[k];
+        if (lane == 0) py[c] = acc;
+        else px[c] = acc;
     }
 }
 
@@ -100,16 +96,19 @@ __global__ void k_power_scale(int n, const double* __restrict__ y, const double*
 // ---- sparse product C = A B, host csr_multiply semantics ------------------------------------------
 // Every row of B has sorted, unique columns, so the host's encounter order for an output column
 // j is the order of the entries of A's row: c_ij = sum over a of a_ia b_(k_a) j, in a's order,
-// from 0. One thread per row merges the B rows of its A entries (one cursor each) in ascending
-// column order. Rows of A longer than MERGE_MAX entries set *overflow = 1 (the caller then uses
-// k_spgemm_rows).
-constexpr int MERGE_MAX = 64;
+// from 0. One thread per row merges the B rows of its A entries in ascending column order, with
+// the cursors, the current head columns and the A values in registers (the loops over the
+// MERGE_MAX slots are unrolled): each output column costs one round of independent loads (the
+// values of the hits and the next heads). Rows of A longer than MERGE_MAX entries set
+// *overflow = 1 (the caller then uses k_spgemm_rows).
+constexpr int MERGE_MAX = 16;
+constexpr int NO_COL = 0x7fffffff;
 
-__global__ void k_spgemm_merge(int rows, const int* __restrict__ a_rp, const int* __restrict__ a_ci,
-                               const double* __restrict__ a_v, const int* __restrict__ b_rp,
-                               const int* __restrict__ b_ci, const double* __restrict__ b_v, int pass,
-                               int* __restrict__ cnt, const int* __restrict__ c_rp, int* __restrict__ c_ci,
-                               double* __restrict__ c_v, int* __restrict__ overflow) {
+__global__ void __launch_bounds__(BLOCK)
+k_spgemm_merge(int rows, const int* __restrict__ a_rp, const int* __restrict__ a_ci, const double* __restrict__ a_v,
+               const int* __restrict__ b_rp, const int* __restrict__ b_ci, const double* __restrict__ b_v, int pass,
+               int* __restrict__ cnt, const int* __restrict__ c_rp, int* __restrict__ c_ci, double* __restrict__ c_v,
+               int* __restrict__ overflow) {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= rows) return;
     const int a0 = a_rp[row], len = a_rp[row + 1] - a0;
@@ -118,30 +117,49 @@ __global__ void k_spgemm_merge(int rows, const int* __restrict__ a_rp, const int
         if (pass == 0) cnt[row] = 0;
         return;
     }
-    int cur[MERGE_MAX], end[MERGE_MAX];
-    for (int t = 0; t < len; ++t) {
-        const int k = a_ci[a0 + t];
-        cur[t] = b_rp[k];
-        end[t] = b_rp[k + 1];
+    int cur[MERGE_MAX], end[MERGE_MAX], head[MERGE_MAX];
+    double av[MERGE_MAX];
+#pragma unroll
+    for (int t = 0; t < MERGE_MAX; ++t) {
+        cur[t] = 0;
+        end[t] = 0;
+        av[t] = 0.0;
+        if (t < len) {
+            const int k = a_ci[a0 + t];
+            av[t] = a_v[a0 + t];
+            cur[t] = b_rp[k];
+            end[t] = b_rp[k + 1];
+        }
     }
+#pragma unroll
+    for (int t = 0; t < MERGE_MAX; ++t) head[t] = cur[t] < end[t] ? b_ci[cur[t]] : NO_COL;
     int out = 0;
     const int o0 = pass == 1 ? c_rp[row] : 0;
     while (true) {
-        int j = 0x7fffffff;
-        for (int t = 0; t < len; ++t)
-            if (cur[t] < end[t]) j = min(j, b_ci[cur[t]]);
-        if (j == 0x7fffffff) break;
-        double acc = 0.0;
-        for (int t = 0; t < len; ++t)
-            if (cur[t] < end[t] && b_ci[cur[t]] == j) {
-                const double prod = a_v[a0 + t] * b_v[cur[t]];
-                acc += prod;
-                ++cur[t];
-            }
+        int j = NO_COL;
+#pragma unroll
+        for (int t = 0; t < MERGE_MAX; ++t) j = min(j, head[t]);
+        if (j == NO_COL) break;
         if (pass == 1) {
+            double bv[MERGE_MAX];
+#pragma unroll
+            for (int t = 0; t < MERGE_MAX; ++t) bv[t] = head[t] == j ? b_v[cur[t]] : 0.0;
+            double acc = 0.0;
+#pragma unroll
+            for (int t = 0; t < MERGE_MAX; ++t)
+                if (head[t] == j) {
+                    const double prod = av[t] * bv[t];
+                    acc += prod;
+                }
             c_ci[o0 + out] = j;
             c_v[o0 + out] = acc;
         }
+#pragma unroll
+        for (int t = 0; t < MERGE_MAX; ++t)
+            if (head[t] == j) {
+                ++cur[t];
+                head[t] = cur[t] < end[t] ? b_ci[cur[t]] : NO_COL;
+            }
         ++out;
     }
     if (pass == 0) cnt[row] = out;
@@ -320,52 +338,52 @@ __global__ void k_prolongator_rows(int n, const int* __restrict__ ap_rp, const i
 }
 
 // ---- R = P^T (csr_transpose: entries of each column in ascending row order) ----------------------
-// Column c of P can only be non-zero in the rows of aggregate c and their A-neighbours. One
-// thread per column walks those candidate rows in ascending order (a merge of the members' sorted
-// neighbour lists) and looks the column up in each candidate's P row.
-__device__ __forceinline__ int find_col(const int* __restrict__ ci, int lo, int hi, int col) {
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (ci[mid] < col) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
+// ---- transpose (host csr_transpose: the entries of a column in ascending row order) ----------------
+// A stable radix sort of the entries by column, from their CSR order (rows ascending), lists
+// every column's entries in row order; the values are moved, never combined.
+__global__ void k_row_of_entries(int rows, const int* __restrict__ rp, int* __restrict__ row_of) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) row_of[k] = i;
 }
 
-__global__ void k_transpose_cols(int n_agg, const int* __restrict__ mem_ptr, const int* __restrict__ mem_rows,
-                                 const int* __restrict__ a_rp, const int* __restrict__ a_ci,
-                                 const int* __restrict__ p_rp, const int* __restrict__ p_ci,
-                                 const double* __restrict__ p_v, int pass, int* __restrict__ cnt,
-                                 const int* __restrict__ r_rp, int* __restrict__ r_ci, double* __restrict__ r_v) {
+__global__ void k_transpose_gather(int nnz, const int* __restrict__ order, const int* __restrict__ row_of,
+                                   const double* __restrict__ v, int* __restrict__ t_ci, double* __restrict__ t_v) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    const int k = order[p];
+    t_ci[p] = row_of[k];
+    t_v[p] = v[k];
+}
+
+// t_rp[c] = first sorted position whose column is >= c, for c = 0..cols
+__global__ void k_column_starts(int cols, int nnz, const unsigned* __restrict__ keys, int* __restrict__ t_rp) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_agg) return;
-    const int m0 = mem_ptr[c], m1 = mem_ptr[c + 1];
-    int out = 0;
-    const int o0 = pass == 1 ? r_rp[c] : 0;
-    int last = -1;
-    // repeatedly take the smallest candidate row above `last` over the members' (sorted) rows of
-    // A; each member's row of A contains the member itself (the diagonal)
-    while (true) {
-        int next = 0x7fffffff;
-        for (int m = m0; m < m1; ++m) {
-            const int r = mem_rows[m];
-            const int lo = a_rp[r], hi = a_rp[r + 1];
-            const int pos = find_col(a_ci, lo, hi, last + 1);
-            if (pos < hi && a_ci[pos] < next) next = a_ci[pos];
-        }
-        if (next == 0x7fffffff) break;
-        const int lo = p_rp[next], hi = p_rp[next + 1];
-        const int pos = find_col(p_ci, lo, hi, c);
-        if (pos < hi && p_ci[pos] == c) {
-            if (pass == 1) {
-                r_ci[o0 + out] = next;
-                r_v[o0 + out] = p_v[pos];
-            }
-            ++out;
-        }
-        last = next;
+    if (c > cols) return;
+    int lo = 0, hi = nnz;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys[mid] < (unsigned)c) lo = mid + 1;
+        else hi = mid;
     }
-    if (pass == 0) cnt[c] = out;
+    t_rp[c] = lo;
+}
+
+unsigned column_bits(int cols) {
+    unsigned b = 1;
+    while (b < 31 && (1u << b) < (unsigned)cols) ++b;
+    return b;
+}
+
+size_t align_up(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t transpose_sort_bytes(int nnz, int cols) {
+    size_t bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, bytes, static_cast<const unsigned*>(nullptr), static_cast<unsigned*>(nullptr),
+                                  rocprim::counting_iterator<int>(0), static_cast<int*>(nullptr), (size_t)nnz, 0,
+                                  column_bits(cols)) != hipSuccess)
+        return 0;
+    return bytes;
 }
 
 int blocks(int n) { return blocks_for(n > 0 ? n : 1); }
@@ -443,15 +461,33 @@ int cfa_amg_prolongator(int n, const int* ap_rp, const int* ap_ci, const double*
     return launch_status();
 }
 
-int cfa_amg_transpose(int n_agg, const int* mem_ptr, const int* mem_rows, const int* a_rp, const int* a_ci,
-                      const int* p_rp, const int* p_ci, const double* p_v, int pass, int* cnt, const int* r_rp,
-                      int* r_ci, double* r_v, cfa_stream_t stream) {
-    if (n_agg < 0 || (pass != 0 && pass != 1)) return CFA_E_ARG;
-    if (n_agg == 0) return CFA_OK;
-    if (!mem_ptr || !mem_rows || !a_rp || !a_ci || !p_rp || !p_ci || !p_v) return CFA_E_ARG;
-    if ((pass == 0 && !cnt) || (pass == 1 && (!r_rp || !r_ci || !r_v))) return CFA_E_ARG;
-    hipLaunchKernelGGL(k_transpose_cols, dim3(blocks(n_agg)), dim3(BLOCK), 0, as_stream(stream), n_agg, mem_ptr,
-                       mem_rows, a_rp, a_ci, p_rp, p_ci, p_v, pass, cnt, r_rp, r_ci, r_v);
+size_t cfa_csr_transpose_ws_bytes(int rows, int cols, int nnz) {
+    if (rows < 0 || cols < 0 || nnz < 0) return 0;
+    const size_t sort = transpose_sort_bytes(nnz, cols);
+    return sort ? align_up(3 * (size_t)nnz * sizeof(int)) + sort : 0;
+}
+
+int cfa_csr_transpose(int rows, int cols, int nnz, const int* rp, const int* ci, const double* v, void* ws,
+                      size_t ws_bytes, int* t_rp, int* t_ci, double* t_v, cfa_stream_t stream) {
+    if (rows < 0 || cols < 0 || nnz < 0 || !t_rp || (rows > 0 && !rp) || (nnz > 0 && (!ci || !v || !t_ci || !t_v)))
+        return CFA_E_ARG;
+    const size_t need = cfa_csr_transpose_ws_bytes(rows, cols, nnz);
+    if (need == 0 || !ws || ws_bytes < need) return CFA_E_ARG;
+    const hipStream_t s = as_stream(stream);
+    unsigned* keys = static_cast<unsigned*>(ws);
+    int* order = reinterpret_cast<int*>(keys + nnz);
+    int* row_of = order + nnz;
+    void* tmp = static_cast<char*>(ws) + align_up(3 * (size_t)nnz * sizeof(int));
+    size_t tmp_bytes = ws_bytes - align_up(3 * (size_t)nnz * sizeof(int));
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_row_of_entries, dim3(blocks(rows)), dim3(BLOCK), 0, s, rows, rp, row_of);
+        const hipError_t e = rocprim::radix_sort_pairs(tmp, tmp_bytes, reinterpret_cast<const unsigned*>(ci), keys,
+                                                       rocprim::counting_iterator<int>(0), order, (size_t)nnz, 0,
+                                                       column_bits(cols), s);
+        if (e != hipSuccess) return (int)e;
+        hipLaunchKernelGGL(k_transpose_gather, dim3(blocks(nnz)), dim3(BLOCK), 0, s, nnz, order, row_of, v, t_ci, t_v);
+    }
+    hipLaunchKernelGGL(k_column_starts, dim3(blocks(cols + 1)), dim3(BLOCK), 0, s, cols, nnz, keys, t_rp);
     return launch_status();
 }
 

@@ -404,8 +404,8 @@ HostCsr csr_transpose(const HostCsr& A) {
 
 bool amg_device_setup_default() {
     static const bool on = [] {
-        const char* e = std::getenv("CFA_AMG_HOST_SETUP");
-        return !(e && e[0] == '1');
+        const char* e = std::getenv("CFA_AMG_DEVICE_SETUP");
+        return e && e[0] == '1';
     }();
     return on;
 }
@@ -719,17 +719,11 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
         return 0;
     }
     if (first) last_aggregates = agg;
-    // P0 values, aggregate member lists (rows ascending), P0 = (iota, agg, p0v)
+    // P0 values; P0 = (iota, agg, p0v)
     std::vector<int> size(na, 0);
     for (int i = 0; i < n; ++i) size[agg[i]]++;
     std::vector<double> p0v(n);
     for (int i = 0; i < n; ++i) p0v[i] = 1.0 / std::sqrt((double)size[agg[i]]);
-    std::vector<int> mem_ptr(na + 1, 0), mem_rows(n);
-    for (int c = 0; c < na; ++c) mem_ptr[c + 1] = mem_ptr[c] + size[c];
-    {
-        std::vector<int> fill(mem_ptr.begin(), mem_ptr.end() - 1);
-        for (int i = 0; i < n; ++i) mem_rows[fill[agg[i]]++] = i;
-    }
     if (iota_dev_n_ < n + 1) {
         std::vector<int> iota(n + 1);
         for (int i = 0; i <= n; ++i) iota[i] = i;
@@ -738,8 +732,6 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
     }
     d_agg_.upload(agg);
     d_p0v_.upload(p0v);
-    d_mem_ptr_.upload(mem_ptr);
-    d_mem_rows_.upload(mem_rows);
     phase_end(PH_UPLOAD);
     // P = P0 - omega D^-1 A P0
     if (!device_product(n, arp, aci, av, d_iota_, d_agg_, d_p0v_, dev_tmp_)) {
@@ -762,23 +754,15 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
     L->P.tpr = csr_lanes_per_row(n, p_nnz);
     phase_end(PH_SMOOTH);
     // R = P^T
-    GPU_CHECK(cfa_amg_transpose(na, d_mem_ptr_, d_mem_rows_, arp, aci, L->P.rp, L->P.ci, L->P.v, 0, d_cnt_, nullptr,
-                                nullptr, nullptr, s));
     L->R.rp.reserve(na + 1);
-    GPU_CHECK(cfa_scan_counts(na, d_cnt_, L->R.rp, s));
-    int r_nnz = 0;
-    GPU_CHECK(cfa_memcpy(&r_nnz, L->R.rp.ptr + na, sizeof(int), 1, s));
-    if (r_nnz != p_nnz) {   // every entry of P lies in a member's neighbourhood; anything else is a bug
-        std::fprintf(stderr, "[AMG] device transpose found %d of %d entries\n", r_nnz, p_nnz);
-        std::exit(EXIT_FAILURE);
-    }
-    L->R.ci.reserve(std::max(r_nnz, 1));
-    L->R.v.reserve(std::max(r_nnz, 1));
-    GPU_CHECK(cfa_amg_transpose(na, d_mem_ptr_, d_mem_rows_, arp, aci, L->P.rp, L->P.ci, L->P.v, 1, nullptr, L->R.rp,
-                                L->R.ci, L->R.v, s));
+    L->R.ci.reserve(std::max(p_nnz, 1));
+    L->R.v.reserve(std::max(p_nnz, 1));
+    const size_t tws = cfa_csr_transpose_ws_bytes(n, na, p_nnz);
+    d_tws_.reserve(std::max<size_t>(tws, 1));
+    GPU_CHECK(cfa_csr_transpose(n, na, p_nnz, L->P.rp, L->P.ci, L->P.v, d_tws_, tws, L->R.rp, L->R.ci, L->R.v, s));
     L->R.rows = na;
-    L->R.nnz = r_nnz;
-    L->R.tpr = csr_lanes_per_row(na, r_nnz);
+    L->R.nnz = p_nnz;
+    L->R.tpr = csr_lanes_per_row(na, p_nnz);
     phase_end(PH_TRANSPOSE);
     // Galerkin product R (A P)
     if (!device_product(n, arp, aci, av, L->P.rp, L->P.ci, L->P.v, dev_tmp_) ||

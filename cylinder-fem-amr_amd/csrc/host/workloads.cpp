@@ -70,8 +70,14 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
     AmgPreconditioner amg;
     t0 = Clock::now();
     amg.setup(n, g.nnz, g.d_row_ptr, g.d_col_ind, g.d_Kp_val);
+    const double setup_cold_ms = host_ms(t0);
+    // the steady state of a time-stepping run: buffers already sized, kernels loaded
+    double cold_phases[AmgPreconditioner::PH_COUNT];
+    for (int k = 0; k < AmgPreconditioner::PH_COUNT; ++k) cold_phases[k] = amg.phase_ms[k];
+    t0 = Clock::now();
+    amg.setup(n, g.nnz, g.d_row_ptr, g.d_col_ind, g.d_Kp_val);
     const double setup_ms = host_ms(t0);
-    for (int k = 0; k < AmgPreconditioner::PH_COUNT; ++k) g_amg_phases[k] = amg.phase_ms[k];
+    for (int k = 0; k < AmgPreconditioner::PH_COUNT; ++k) g_amg_phases[k] = amg.phase_ms[k] - cold_phases[k];
     g_amg_phases[AmgPreconditioner::PH_COUNT] = amg.device_levels;
     GpuLinearSolver solver;
     double solve_ms = 0.0, worst = 0.0;
@@ -147,10 +153,10 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
             }), kernel_reps});
         }
     }
-    const double v[12] = {(double)n, (double)g.nnz, (double)g.n_elems, mesh_s, setup_ms, (double)amg.num_levels(),
+    const double v[13] = {(double)n, (double)g.nnz, (double)g.n_elems, mesh_s, setup_ms, (double)amg.num_levels(),
                           (double)nrhs, solve_ms, (double)iters, solve_ms / nrhs,
-                          iters ? 1000.0 * solve_ms / iters : 0.0, worst};
-    for (int k = 0; k < 12; ++k) out[k] = v[k];
+                          iters ? 1000.0 * solve_ms / iters : 0.0, worst, setup_cold_ms};
+    for (int k = 0; k < 13; ++k) out[k] = v[k];
     return CFA_OK;
 }
 

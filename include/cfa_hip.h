@@ -239,6 +239,14 @@ typedef struct {
 /* z = M^{-1} r, one V-cycle (src/cfa/linalg.cuh:1161-1169 AmgPreconditioner::apply). */
 int cfa_amg_apply(const cfa_amg_t* amg, const double* r, double* z, cfa_stream_t stream);
 
+/* The bottom levels of a V-cycle run in one single-workgroup kernel (vectors in LDS) from the
+ * first level l >= 1 whose operator has at most `limit` entries and from which every level fits
+ * the LDS. No counterpart in the reference (its cycle launches per level). cfa_amg_set_tail_limit
+ * sets the limit (0 = off, < 0 = leave; start value 0 or CFA_AMG_TAIL_NNZ) and returns the
+ * previous one; cfa_amg_tail_start returns that first level for a hierarchy, or -1. */
+long long cfa_amg_set_tail_limit(long long limit);
+int cfa_amg_tail_start(const cfa_amg_t* amg);
+
 /* out[i] = (float) in[i], i < n: the FP32 copies of a hierarchy. */
 int cfa_to_f32(size_t n, const double* in, float* out, cfa_stream_t stream);
 
@@ -348,20 +356,20 @@ int cfa_bicgstab_jacobi(int n, const int* rp, const int* ci, const double* val, 
  * D^-1 A from the device vector x0 (the reference starts from 1 + 0.1 sin(1 + i)), norms summed
  * per chunk of 4096 rows in row order and over the chunks in order; *lambda (host) = the last
  * ||D^-1 A x|| / ||x||. ws: cfa_amg_power_ws_doubles(n) doubles.
- * cfa_spgemm: C = A B in two passes (csr_multiply, linalg.cuh:1336-1380): pass 0 writes the
+ * cfa_spgemm: C = A B in two passes (csr_multiply, linalg.cuh:799-844): pass 0 writes the
  * number of distinct columns of each row to cnt and sets *overflow (device int, cleared by the
  * caller) when a row is beyond the method's limit; pass 1 writes the sorted columns and the
  * values, each summed from 0 in the order the products are met, at c_rp (from
  * cfa_scan_counts). method 0: one thread per row merging the right rows (rows of A with at
- * most 64 entries); method 1: one workgroup per row sorting up to 4096 products in LDS. Rows of
+ * most 16 entries); method 1: one workgroup per row sorting up to 4096 products in LDS. Rows of
  * B must have sorted, unique columns.
  * cfa_scan_counts: out[0..n] = exclusive prefix sums of cnt (out[n] = total), one workgroup.
  * cfa_amg_prolongator: P = P0 - omega D^-1 A P0 from AP0 = A P0 (smoothed_prolongator,
- * linalg.cuh:1400-1431: the row's own aggregate merged in column order, exact zeros dropped);
+ * linalg.cuh:1360-1392: the row's own aggregate merged in column order, exact zeros dropped);
  * pass 0 counts per row, pass 1 fills at p_rp.
- * cfa_amg_transpose: R = P^T (csr_transpose, entries of a column in ascending row order) from
- * the aggregates' member lists (mem_ptr [n_agg + 1], mem_rows ascending) and the pattern of A;
- * pass 0 counts per column, pass 1 fills at r_rp. No atomics anywhere. */
+ * cfa_csr_transpose: T = M^T for a rows x cols CSR matrix with nnz entries (csr_transpose,
+ * linalg.cuh:847-862: the entries of a column in ascending row order, values moved unchanged);
+ * t_rp [cols + 1], t_ci / t_v [nnz]; ws: device workspace of cfa_csr_transpose_ws_bytes bytes. */
 int cfa_amg_dinv(int n, const int* rp, const int* ci, const double* val, double* dinv, cfa_stream_t stream);
 size_t cfa_amg_power_ws_doubles(int n);
 int cfa_amg_power(int n, const int* rp, const int* ci, const double* val, const double* dinv, const double* x0,
@@ -373,9 +381,9 @@ int cfa_scan_counts(int n, const int* cnt, int* out, cfa_stream_t stream);
 int cfa_amg_prolongator(int n, const int* ap_rp, const int* ap_ci, const double* ap_v, const int* agg,
                         const double* p0v, const double* dinv, double omega, int pass, int* cnt, const int* p_rp,
                         int* p_ci, double* p_v, cfa_stream_t stream);
-int cfa_amg_transpose(int n_agg, const int* mem_ptr, const int* mem_rows, const int* a_rp, const int* a_ci,
-                      const int* p_rp, const int* p_ci, const double* p_v, int pass, int* cnt, const int* r_rp,
-                      int* r_ci, double* r_v, cfa_stream_t stream);
+size_t cfa_csr_transpose_ws_bytes(int rows, int cols, int nnz);
+int cfa_csr_transpose(int rows, int cols, int nnz, const int* rp, const int* ci, const double* v, void* ws,
+                      size_t ws_bytes, int* t_rp, int* t_ci, double* t_v, cfa_stream_t stream);
 
 /* ---- discrete adjoint of the step (no counterpart in the reference) --------------------------
  * Vector-Jacobian products of the step's assemblies and updates (the forward operations are

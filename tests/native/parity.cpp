@@ -429,71 +429,84 @@ oracle::Csr host_csr(const Fixture& f, const Vec& v) {
 void case_amg(Fixture& f) {
     std::printf("[SA-AMG set-up and V-cycle]\n");
     const PressureSystem o = case_pressure(f, false);
-    AmgPreconditioner amg;
-    amg.setup(f.n, f.nnz, f.g.d_row_ptr, f.g.d_col_ind, f.g.d_Kp_val);
     oracle::Amg oa;
     oa.setup(host_csr(f, o.Kp));
-    expect(amg.num_levels() == oa.num_levels(), "number of levels");
-    std::vector<int> rows_g, rows_o, nnz_g, nnz_o;
-    Vec om_g, om_o;
-    for (size_t l = 0; l < amg.levels.size(); ++l) {
-        rows_g.push_back(amg.levels[l]->A.rows);
-        nnz_g.push_back(amg.levels[l]->A.nnz);
-        nnz_g.push_back(amg.levels[l]->P.nnz);
-        om_g.push_back(amg.levels[l]->omega);
-    }
-    for (const auto& L : oa.levels) {
-        rows_o.push_back(L.A.rows);
-        nnz_o.push_back(L.A.rp.back());
-        nnz_o.push_back(L.P.rp.back());
-        om_o.push_back(L.omega);
-    }
-    same_ints("level rows", rows_g, rows_o);
-    same_ints("level nnz (A, P)", nnz_g, nnz_o);
-    same_bits("Jacobi weights omega", om_g, om_o);
-    expect(amg.coarse_n == oa.coarse_n, "coarse size");
-    std::printf("  levels built on the device: %d of %zu\n", amg.device_levels, amg.levels.size());
-    // every level's operator, prolongator, restriction and inverse diagonal, bit for bit
-    if (amg.levels.size() == oa.levels.size()) {
-        std::vector<int> pat_g, pat_o;
-        Vec val_g, val_o;
-        auto add = [&](const AmgPreconditioner::DevCsr& D, const oracle::Csr& C) {
-            const std::vector<int> rp = down_int(D.rp, D.rows + 1), ci = down_int(D.ci, D.nnz);
-            pat_g.insert(pat_g.end(), rp.begin(), rp.end());
-            pat_g.insert(pat_g.end(), ci.begin(), ci.end());
-            pat_o.insert(pat_o.end(), C.rp.begin(), C.rp.end());
-            pat_o.insert(pat_o.end(), C.ci.begin(), C.ci.end());
-            const Vec v = down(D.v, D.nnz);
-            val_g.insert(val_g.end(), v.begin(), v.end());
-            val_o.insert(val_o.end(), C.v.begin(), C.v.end());
-        };
+    for (const bool on_device : {true, false}) {
+        std::printf("  set-up on the %s\n", on_device ? "device" : "host");
+        AmgPreconditioner amg;
+        amg.device_setup = on_device;
+        amg.setup(f.n, f.nnz, f.g.d_row_ptr, f.g.d_col_ind, f.g.d_Kp_val);
+        if (on_device) expect(amg.device_levels == (int)amg.levels.size(), "every level built on the device");
+        expect(amg.num_levels() == oa.num_levels(), "number of levels");
+        std::vector<int> rows_g, rows_o, nnz_g, nnz_o;
+        Vec om_g, om_o;
         for (size_t l = 0; l < amg.levels.size(); ++l) {
-            const auto& L = *amg.levels[l];
-            const auto& O = oa.levels[l];
-            add(L.A, O.A);
-            add(L.P, O.P);
-            add(L.R, O.R);
-            const Vec d = down(L.dinv, L.A.rows);
-            val_g.insert(val_g.end(), d.begin(), d.end());
-            val_o.insert(val_o.end(), O.dinv.begin(), O.dinv.end());
+            rows_g.push_back(amg.levels[l]->A.rows);
+            nnz_g.push_back(amg.levels[l]->A.nnz);
+            nnz_g.push_back(amg.levels[l]->P.nnz);
+            om_g.push_back(amg.levels[l]->omega);
         }
-        same_ints("hierarchy patterns (A, P, R per level)", pat_g, pat_o);
-        same_bits("hierarchy values (A, P, R, D^-1 per level)", val_g, val_o);
+        for (const auto& L : oa.levels) {
+            rows_o.push_back(L.A.rows);
+            nnz_o.push_back(L.A.rp.back());
+            nnz_o.push_back(L.P.rp.back());
+            om_o.push_back(L.omega);
+        }
+        same_ints("level rows", rows_g, rows_o);
+        same_ints("level nnz (A, P)", nnz_g, nnz_o);
+        same_bits("Jacobi weights omega", om_g, om_o);
+        expect(amg.coarse_n == oa.coarse_n, "coarse size");
+        std::printf("  levels built on the device: %d of %zu\n", amg.device_levels, amg.levels.size());
+        // every level's operator, prolongator, restriction and inverse diagonal, bit for bit
+        if (amg.levels.size() == oa.levels.size()) {
+            std::vector<int> pat_g, pat_o;
+            Vec val_g, val_o;
+            auto add = [&](const AmgPreconditioner::DevCsr& D, const oracle::Csr& C) {
+                const std::vector<int> rp = down_int(D.rp, D.rows + 1), ci = down_int(D.ci, D.nnz);
+                pat_g.insert(pat_g.end(), rp.begin(), rp.end());
+                pat_g.insert(pat_g.end(), ci.begin(), ci.end());
+                pat_o.insert(pat_o.end(), C.rp.begin(), C.rp.end());
+                pat_o.insert(pat_o.end(), C.ci.begin(), C.ci.end());
+                const Vec v = down(D.v, D.nnz);
+                val_g.insert(val_g.end(), v.begin(), v.end());
+                val_o.insert(val_o.end(), C.v.begin(), C.v.end());
+            };
+            for (size_t l = 0; l < amg.levels.size(); ++l) {
+                const auto& L = *amg.levels[l];
+                const auto& O = oa.levels[l];
+                add(L.A, O.A);
+                add(L.P, O.P);
+                add(L.R, O.R);
+                const Vec d = down(L.dinv, L.A.rows);
+                val_g.insert(val_g.end(), d.begin(), d.end());
+                val_o.insert(val_o.end(), O.dinv.begin(), O.dinv.end());
+            }
+            same_ints("hierarchy patterns (A, P, R per level)", pat_g, pat_o);
+            same_bits("hierarchy values (A, P, R, D^-1 per level)", val_g, val_o);
+        }
+        if (!oa.levels.empty()) {
+            std::vector<int> agg;
+            oracle::smoothed_prolongator(oa.levels[0].A, oa.levels[0].dinv, oa.levels[0].omega, 0.08, &agg);
+            same_ints("level-0 aggregates", amg.last_aggregates, agg);
+        }
+        Vec r(f.n);
+        for (auto& v : r) v = std::uniform_real_distribution<double>(-1, 1)(f.rng);
+        DevArray<double> d_r, d_z;
+        d_r.upload(r);
+        d_z.reserve(f.n);
+        Vec z(f.n);
+        oa.apply(r.data(), z.data());
+        // with every level launched on its own, and with the bottom levels that fit the LDS in the
+        // single-workgroup tail kernel
+        const long long limit = cfa_amg_set_tail_limit(0);
+        amg.apply(d_r, d_z);
+        close_rel("one V-cycle", down(d_z, f.n), z, 1e-12);
+        cfa_amg_set_tail_limit(1LL << 40);
+        std::printf("  V-cycle tail from level %d of %d\n", cfa_amg_tail_start(amg.view()), amg.num_levels());
+        amg.apply(d_r, d_z);
+        close_rel("one V-cycle, tail kernel", down(d_z, f.n), z, 1e-12);
+        cfa_amg_set_tail_limit(limit);
     }
-    if (!oa.levels.empty()) {
-        std::vector<int> agg;
-        oracle::smoothed_prolongator(oa.levels[0].A, oa.levels[0].dinv, oa.levels[0].omega, 0.08, &agg);
-        same_ints("level-0 aggregates", amg.last_aggregates, agg);
-    }
-    Vec r(f.n);
-    for (auto& v : r) v = std::uniform_real_distribution<double>(-1, 1)(f.rng);
-    DevArray<double> d_r, d_z;
-    d_r.upload(r);
-    d_z.reserve(f.n);
-    amg.apply(d_r, d_z);
-    Vec z(f.n);
-    oa.apply(r.data(), z.data());
-    close_rel("one V-cycle", down(d_z, f.n), z, 1e-12);
 }
 
 void case_pcg(Fixture& f) {
@@ -546,6 +559,11 @@ void case_mixed(Fixture& f) {
     Vec z(f.n);
     oa.apply(r.data(), z.data());
     close_rel("one V-cycle, FP32 vs FP64", down(d_z, f.n), z, 1e-5);
+    const long long limit = cfa_amg_set_tail_limit(1LL << 40);
+    std::printf("  FP32 V-cycle tail from level %d of %d\n", cfa_amg_tail_start(amg.view()), amg.num_levels());
+    amg.apply(d_r, d_z);
+    close_rel("one V-cycle, FP32 vs FP64, tail kernel", down(d_z, f.n), z, 1e-5);
+    cfa_amg_set_tail_limit(limit);
     const double tol = 1e-10;
     f.g.d_dp.upload(Vec(f.n, 0.0));
     GpuLinearSolver solver;

@@ -11,7 +11,9 @@
 // Nothing is summed by atomics: products and P are built row by row, the transpose is a stable
 // radix sort (rocPRIM) of P's entries by column that moves values without combining them.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 #include "cfa_device.hpp"
 
@@ -36,14 +38,34 @@ __global__ void k_amg_dinv(int n, const int* __restrict__ rp, const int* __restr
     dinv[i] = fabs(d) > 1.0e-300 ? 1.0 / d : 1.0;
 }
 
-// y = D^-1 A x, one thread per row, entries in row order
-__global__ void k_power_rows(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                             const double* __restrict__ v, const double* __restrict__ dinv,
-                             const double* __restrict__ x, double* __restrict__ y) {
+// One power step y = D^-1 A x, one thread per row, entries in row order. The step's x is formed
+// where it is read: x0 on the first step, else y_prev / nrm with nrm = sqrt(sum of the previous
+// step's chunk sums, in chunk order), which every workgroup recomputes (the same division the
+// host applies to its stored x). Each row also stores its own x_i for the chunk sums.
+__global__ void __launch_bounds__(BLOCK)
+k_power_step(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
+             const double* __restrict__ dinv, const double* __restrict__ x0, const double* __restrict__ y_prev,
+             const double* __restrict__ py_prev, int nch, int first, double* __restrict__ x,
+             double* __restrict__ y) {
+    __shared__ double s_nrm;
+    if (!first) {
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int c = 0; c < nch; ++c) t += py_prev[c];
+            s_nrm = sqrt(t);
+        }
+        __syncthreads();
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double nrm = first ? 1.0 : s_nrm;
     double s = 0.0;
-    for (int a = rp[i]; a < rp[i + 1]; ++a) s += v[a] * x[ci[a]];
+    for (int a = rp[i]; a < rp[i + 1]; ++a) {
+        const int j = ci[a];
+        const double xj = first ? x0[j] : y_prev[j] / nrm;
+        s += v[a] * xj;
+    }
+    x[i] = first ? x0[i] : y_prev[i] / nrm;
     y[i] = dinv[i] * s;
 }
 
@@ -85,12 +107,6 @@ __global__ void k_power_final(int nch, const double* __restrict__ py, const doub
     nrm = sqrt(nrm);
     out[0] = nrm;
     out[1] = nrm / sqrt(xn);
-}
-
-__global__ void k_power_scale(int n, const double* __restrict__ y, const double* __restrict__ nrm,
-                              double* __restrict__ x) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = y[i] / nrm[0];
 }
 
 // ---- sparse product C = A B, host csr_multiply semantics ------------------------------------------
@@ -275,31 +291,25 @@ k_spgemm_rows(int rows, const int* __restrict__ a_rp, const int* __restrict__ a_
     if (pass == 0 && t == 0) cnt[row] = n_distinct;
 }
 
-// exclusive scan of cnt[0..n) into out[0..n] (out[n] = total), one workgroup, fixed order
-__global__ void __launch_bounds__(1024) k_scan_counts(int n, const int* __restrict__ cnt, int* __restrict__ out) {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int lo = min(n, t * per), hi = min(n, lo + per);
-    int s = 0;
-    for (int i = lo; i < hi; ++i) s += cnt[i];
-    part[t] = s;
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int k = 0; k < 1024; ++k) {
-            const int v = part[k];
-            part[k] = acc;
-            acc += v;
-        }
-        out[n] = acc;
-    }
-    __syncthreads();
-    int acc = part[t];
-    for (int i = lo; i < hi; ++i) {
-        out[i] = acc;
-        acc += cnt[i];
-    }
+// exclusive scan of cnt[0..n) into out[0..n] (out[n] = total): rocPRIM's device scan over n + 1
+// values (cnt, then 0); integer sums, so the result does not depend on its order
+struct CountAt {
+    const int* c;
+    int n;
+    __host__ __device__ int operator()(int i) const { return i < n ? c[i] : 0; }
+};
+
+__global__ void k_copy_int(const int* __restrict__ src, int* __restrict__ dst) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) dst[0] = src[0];
+}
+
+size_t scan_bytes(int n) {
+    size_t bytes = 0;
+    const auto in = rocprim::make_transform_iterator(rocprim::counting_iterator<int>(0), CountAt{nullptr, n});
+    if (rocprim::exclusive_scan(nullptr, bytes, in, static_cast<int*>(nullptr), 0, (size_t)n + 1,
+                                rocprim::plus<int>()) != hipSuccess)
+        return 0;
+    return bytes;
 }
 
 // ---- smoothed prolongator P = P0 - omega D^-1 A P0 (linalg.cuh smoothed_prolongator): per row,
@@ -399,32 +409,47 @@ int cfa_amg_dinv(int n, const int* rp, const int* ci, const double* val, double*
     return launch_status();
 }
 
+namespace {
+// power-iteration workspace: x [n], y [2][n], py [2][nch], px [nch], result {nrm, lambda}
+struct PowerWs {
+    double *x, *y[2], *py[2], *px, *out;
+};
+PowerWs power_ws(double* ws, int n) {
+    const int nch = (n + CHUNK - 1) / CHUNK;
+    PowerWs w;
+    w.x = ws;
+    w.y[0] = ws + n;
+    w.y[1] = ws + 2 * (size_t)n;
+    w.py[0] = ws + 3 * (size_t)n;
+    w.py[1] = w.py[0] + nch;
+    w.px = w.py[1] + nch;
+    w.out = w.px + nch;
+    return w;
+}
+}  // namespace
+
 size_t cfa_amg_power_ws_doubles(int n) {
-    return n < 0 ? 0 : 2 * (size_t)n + 2 * (size_t)((n + CHUNK - 1) / CHUNK) + 8;
+    return n < 0 ? 0 : 3 * (size_t)n + 3 * (size_t)((n + CHUNK - 1) / CHUNK) + 2;
 }
 
 int cfa_amg_power(int n, const int* rp, const int* ci, const double* val, const double* dinv, const double* x0,
                   int iterations, double* ws, double* lambda, cfa_stream_t stream) {
-    if (n <= 0 || !rp || !ci || !val || !dinv || !x0 || !ws || !lambda || iterations < 1) return CFA_E_ARG;
+    if (n <= 0 || !rp || !ci || !val || !dinv || !x0 || !ws || iterations < 1) return CFA_E_ARG;
     const hipStream_t s = as_stream(stream);
     const int nch = (n + CHUNK - 1) / CHUNK;
-    double* x = ws;
-    double* y = ws + n;
-    double* py = y + n;
-    double* px = py + nch;
-    double* out = px + nch;
-    hipError_t e = hipMemcpyAsync(x, x0, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) return (int)e;
+    const PowerWs w = power_ws(ws, n);
     for (int it = 0; it < iterations; ++it) {
-        hipLaunchKernelGGL(k_power_rows, dim3(blocks(n)), dim3(BLOCK), 0, s, n, rp, ci, val, dinv, x, y);
-        hipLaunchKernelGGL(k_power_chunks, dim3(nch), dim3(WAVE), 0, s, n, x, y, py, px);
-        hipLaunchKernelGGL(k_power_final, dim3(1), dim3(WAVE), 0, s, nch, py, px, out);
-        hipLaunchKernelGGL(k_power_scale, dim3(blocks(n)), dim3(BLOCK), 0, s, n, y, out, x);
+        const int cur = it & 1, prev = cur ^ 1;
+        hipLaunchKernelGGL(k_power_step, dim3(blocks(n)), dim3(BLOCK), 0, s, n, rp, ci, val, dinv, x0, w.y[prev],
+                           w.py[prev], nch, it == 0 ? 1 : 0, w.x, w.y[cur]);
+        hipLaunchKernelGGL(k_power_chunks, dim3(nch), dim3(WAVE), 0, s, n, w.x, w.y[cur], w.py[cur], w.px);
     }
-    e = hipMemcpyAsync(lambda, out + 1, sizeof(double), hipMemcpyDeviceToHost, s);
+    hipLaunchKernelGGL(k_power_final, dim3(1), dim3(WAVE), 0, s, nch, w.py[(iterations - 1) & 1], w.px, w.out);
+    int rc = launch_status();
+    if (rc || !lambda) return rc;
+    hipError_t e = hipMemcpyAsync(lambda, w.out + 1, sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return (int)e;
-    return launch_status();
+    return e == hipSuccess ? CFA_OK : (int)e;
 }
 
 int cfa_spgemm(int rows, const int* a_rp, const int* a_ci, const double* a_v, const int* b_rp, const int* b_ci,
@@ -443,9 +468,19 @@ int cfa_spgemm(int rows, const int* a_rp, const int* a_ci, const double* a_v, co
     return launch_status();
 }
 
-int cfa_scan_counts(int n, const int* cnt, int* out, cfa_stream_t stream) {
+size_t cfa_scan_counts_ws_bytes(int n) {
+    return n < 0 ? 0 : scan_bytes(n);
+}
+
+int cfa_scan_counts(int n, const int* cnt, int* out, int* total, void* ws, size_t ws_bytes, cfa_stream_t stream) {
     if (n < 0 || !out || (n > 0 && !cnt)) return CFA_E_ARG;
-    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, as_stream(stream), n, cnt, out);
+    size_t need = scan_bytes(n);
+    if (need == 0 || !ws || ws_bytes < need) return CFA_E_ARG;
+    const hipStream_t s = as_stream(stream);
+    const auto in = rocprim::make_transform_iterator(rocprim::counting_iterator<int>(0), CountAt{cnt, n});
+    const hipError_t e = rocprim::exclusive_scan(ws, need, in, out, 0, (size_t)n + 1, rocprim::plus<int>(), s);
+    if (e != hipSuccess) return (int)e;
+    if (total) hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(WAVE), 0, s, out + n, total);
     return launch_status();
 }
 

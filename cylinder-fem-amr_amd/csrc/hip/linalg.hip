@@ -1653,10 +1653,40 @@ namespace {
 // PCG with the reductions folded into the consuming kernels (k_fp_*), two iterations per
 // captured graph (the search direction and the r.z partials alternate between two buffers).
 // Same algorithm, guards and stopping rules as the reference (src/cfa/linalg.cuh:183-256).
+// The operator kernels run with 2 or 4 lanes per row (about four entries per lane).
+template <int TPR>
+void launch_fp_operator_t(bool oss, int grid, hipStream_t s, const PcgArgs& a, int h) {
+    if (oss) hipLaunchKernelGGL((k_fp_operator<TPR, true>), dim3(grid), dim3(BLOCK), 0, s, a, h);
+    else hipLaunchKernelGGL((k_fp_operator<TPR, false>), dim3(grid), dim3(BLOCK), 0, s, a, h);
+}
+void launch_fp_operator(int tpr, bool oss, int grid, hipStream_t s, const PcgArgs& a, int h) {
+    if (tpr == 2) launch_fp_operator_t<2>(oss, grid, s, a, h);
+    else launch_fp_operator_t<4>(oss, grid, s, a, h);
+}
+void launch_apply_K_B(int tpr, int grid, hipStream_t s, int n, int nnz, const int* rp, const int* ci,
+                      const double* val, const double* oss, const double* inv_m, const double* in, double* q,
+                      double* gx, double* gy, const int* st) {
+    if (tpr == 2)
+        hipLaunchKernelGGL(k_apply_K_B<2>, dim3(grid), dim3(BLOCK), 0, s, n, nnz, rp, ci, val, oss, inv_m, in, q, gx, gy,
+                           st);
+    else
+        hipLaunchKernelGGL(k_apply_K_B<4>, dim3(grid), dim3(BLOCK), 0, s, n, nnz, rp, ci, val, oss, inv_m, in, q, gx, gy,
+                           st);
+}
+void launch_apply_BT_dot(int tpr, int grid, hipStream_t s, int n, int nnz, const int* rp, const int* ci,
+                         const double* oss, const int* tag, const double* gx, const double* gy, const double* p,
+                         double* q, double* part, const int* st) {
+    if (tpr == 2)
+        hipLaunchKernelGGL(k_apply_BT_dot<2>, dim3(grid), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss, tag, gx, gy, p, q,
+                           part, st);
+    else
+        hipLaunchKernelGGL(k_apply_BT_dot<4>, dim3(grid), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss, tag, gx, gy, p, q,
+                           part, st);
+}
+
 int pcg_fused(int n, const int* rp, const int* ci, const double* val, const double* b, double* x, int max_iter,
               double tol, const cfa_oss_op_t* oss, const cfa_amg_t* amg, cfa_krylov_ws_t* ws,
               cfa_solve_result_t* result, hipStream_t s) {
-    constexpr int TPR = 4;
     PcgArgs a;
     std::memset(&a, 0, sizeof(a));   // the graph key compares the bytes, padding included
     a.n = n;
@@ -1666,15 +1696,18 @@ int pcg_fused(int n, const int* rp, const int* ci, const double* val, const doub
     double* inv_diag = ws->vec[4];
     a.inv_diag = inv_diag;
     a.x = x;
-    a.prz[0] = ws->part; a.prz[1] = ws->part + RED_BLOCKS; a.ppq = ws->part + 2 * RED_BLOCKS;
-    a.prr = ws->part + 3 * RED_BLOCKS;
-    double* scratch = ws->part + 4 * RED_BLOCKS;   // prologue partials (up to OP_BLOCKS)
+    // block partials: four segments of OP_BLOCKS doubles, then the prologue's (5 * 4096 of the
+    // workspace's 8 * 4096)
+    a.prz[0] = ws->part; a.prz[1] = ws->part + OP_BLOCKS; a.ppq = ws->part + 2 * OP_BLOCKS;
+    a.prr = ws->part + 3 * OP_BLOCKS;
+    double* scratch = ws->part + 4 * OP_BLOCKS;
     a.sc = ws->scal; a.st = state_words(ws); a.status = ws->status_dev;
+    CFA_TRY(hipMemcpyAsync(&a.nnz, rp + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    CFA_TRY(hipStreamSynchronize(s));
     if (oss) {
-        CFA_TRY(hipMemcpyAsync(&a.nnz, rp + n, sizeof(int), hipMemcpyDeviceToHost, s));
-        CFA_TRY(hipStreamSynchronize(s));
         a.oss = oss->oss_val; a.inv_m = oss->inv_m; a.tag = oss->bc_tag; a.gx = oss->gx; a.gy = oss->gy;
     }
+    const int TPR = a.nnz <= 12LL * n ? 2 : 4;
     const double init_sc[2] = {tol, -1.0};   // S_TOL, and S_REL "not computed"
     CFA_TRY(hipMemcpyAsync(a.sc + S_TOL, &init_sc[0], sizeof(double), hipMemcpyHostToDevice, s));
     CFA_TRY(hipMemcpyAsync(a.sc + S_REL, &init_sc[1], sizeof(double), hipMemcpyHostToDevice, s));
@@ -1682,8 +1715,7 @@ int pcg_fused(int n, const int* rp, const int* ci, const double* val, const doub
     ws->status_host[1] = ST_RUNNING;
     const int nb = red_blocks(n);
     const int blocks = blocks_for(n);
-    const int blocks_tpr = std::min(blocks_for((long long)n * TPR), OP_BLOCKS);
-    const int nb_op = std::min(blocks_for((long long)n * TPR), RED_BLOCKS);
+    const int nb_op = std::min(blocks_for((long long)n * TPR), OP_BLOCKS);
     const int* st_r = a.st;
 
     // prologue (linalg.cuh:205-227): ||b||, x = 0 when trivial, r = b - A x, initial test, z = M^-1 r
@@ -1692,10 +1724,8 @@ int pcg_fused(int n, const int* rp, const int* ci, const double* val, const doub
     hipLaunchKernelGGL(k_zero_if_trivial, dim3(blocks), dim3(BLOCK), 0, s, n, x, st_r);
     if (!amg) hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(BLOCK), 0, s, n, rp, ci, val, 1.0e-14, inv_diag);
     if (oss) {
-        hipLaunchKernelGGL(k_apply_K_B<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, a.nnz, rp, ci, val, a.oss, a.inv_m,
-                           x, a.q, a.gx, a.gy, st_r);
-        hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, a.nnz, rp, ci, a.oss, a.tag,
-                           a.gx, a.gy, a.q, a.q, scratch, st_r);
+        launch_apply_K_B(TPR, nb_op, s, n, a.nnz, rp, ci, val, a.oss, a.inv_m, x, a.q, a.gx, a.gy, st_r);
+        launch_apply_BT_dot(TPR, nb_op, s, n, a.nnz, rp, ci, a.oss, a.tag, a.gx, a.gy, a.q, a.q, scratch, st_r);
     } else {
         csr_op_launch<OP_SPMV, double>(TPR, n, rp, ci, val, x, nullptr, nullptr, 0.0, a.q, st_r, s);
     }
@@ -1721,13 +1751,9 @@ int pcg_fused(int n, const int* rp, const int* ci, const double* val, const doub
 
     // iteration k with parity h (linalg.cuh:231-253)
     auto iteration = [&](int h, hipStream_t q_s) {
-        if (oss) {
-            hipLaunchKernelGGL((k_fp_operator<TPR, true>), dim3(nb_op), dim3(BLOCK), 0, q_s, a, h);
-            hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(nb_op), dim3(BLOCK), 0, q_s, n, a.nnz, rp, ci, a.oss, a.tag,
-                               a.gx, a.gy, a.p[h], a.q, a.ppq, st_r);
-        } else {
-            hipLaunchKernelGGL((k_fp_operator<TPR, false>), dim3(nb_op), dim3(BLOCK), 0, q_s, a, h);
-        }
+        launch_fp_operator(TPR, oss != nullptr, nb_op, q_s, a, h);
+        if (oss)
+            launch_apply_BT_dot(TPR, nb_op, q_s, n, a.nnz, rp, ci, a.oss, a.tag, a.gx, a.gy, a.p[h], a.q, a.ppq, st_r);
         hipLaunchKernelGGL(k_fp_update, dim3(nb), dim3(BLOCK), 0, q_s, a, h, nb_op);
         (void)apply_prec(1 - h, true, q_s);
     };

@@ -404,8 +404,8 @@ HostCsr csr_transpose(const HostCsr& A) {
 
 bool amg_device_setup_default() {
     static const bool on = [] {
-        const char* e = std::getenv("CFA_AMG_DEVICE_SETUP");
-        return e && e[0] == '1';
+        const char* e = std::getenv("CFA_AMG_HOST_SETUP");
+        return !(e && e[0] == '1');
     }();
     return on;
 }
@@ -647,23 +647,25 @@ void AmgPreconditioner::release() {
 // C = A B on the device (host csr_multiply results, bit for bit); false when a row has more
 // products than the device kernel holds
 bool AmgPreconditioner::device_product(int rows, const int* arp, const int* aci, const double* av, const int* brp,
-                                       const int* bci, const double* bv, DevCsr& C) {
+                                       const int* bci, const double* bv, DevCsr& C, int left_max_row) {
     const cfa_stream_t s = solver_stream();
     d_cnt_.reserve(std::max(rows, 1));
-    d_flag_.reserve(1);
-    // method 0: one thread per row merging the right rows (rows of A up to 64 entries);
-    // method 1: one workgroup per row sorting the products in LDS (up to 4096 per row)
-    for (int method = 0; method < 2; ++method) {
-        dev_zero(d_flag_, sizeof(int));
+    d_flag_.reserve(2);
+    const size_t sws = cfa_scan_counts_ws_bytes(rows);
+    d_sws_.reserve(std::max<size_t>(sws, 1));
+    C.rp.reserve(rows + 1);
+    // method 0: one thread per row merging the right rows (rows of A up to 16 entries);
+    // method 1: one workgroup per row sorting the products in LDS (up to 4096 per row).
+    // The count pass, the scan and one download of {overflow flag, total} per attempt.
+    for (int method = left_max_row >= 0 && left_max_row <= 16 ? 0 : 1; method < 2; ++method) {
+        dev_zero(d_flag_, 2 * sizeof(int));
         GPU_CHECK(cfa_spgemm(rows, arp, aci, av, brp, bci, bv, 0, method, d_cnt_, nullptr, nullptr, nullptr, d_flag_,
                              s));
-        int flag = 0;
-        d_flag_.download(&flag, 1);
-        if (flag) continue;
-        C.rp.reserve(rows + 1);
-        GPU_CHECK(cfa_scan_counts(rows, d_cnt_, C.rp, s));
-        int total = 0;
-        GPU_CHECK(cfa_memcpy(&total, C.rp.ptr + rows, sizeof(int), 1, s));
+        GPU_CHECK(cfa_scan_counts(rows, d_cnt_, C.rp, d_flag_.ptr + 1, d_sws_, sws, s));
+        int ft[2] = {0, 0};
+        d_flag_.download(ft, 2);
+        if (ft[0]) continue;
+        const int total = ft[1];
         C.ci.reserve(std::max(total, 1));
         C.v.reserve(std::max(total, 1));
         GPU_CHECK(cfa_spgemm(rows, arp, aci, av, brp, bci, bv, 1, method, nullptr, C.rp, C.ci, C.v, d_flag_, s));
@@ -706,14 +708,20 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
         d_x0_.upload(x0_host_);
         x0_dev_n_ = n;
     }
-    d_pw_.reserve(cfa_amg_power_ws_doubles(n));
-    double lambda = 0.0;
-    GPU_CHECK(cfa_amg_power(n, arp, aci, av, L->dinv, d_x0_, 20, d_pw_, &lambda, s));
-    const double omega = 4.0 / (3.0 * (std::max(lambda, 1.0e-12) * 1.05));
+    const size_t pw = cfa_amg_power_ws_doubles(n);
+    d_pw_.reserve(pw);
+    // the power iteration runs on the device while the host aggregates (which needs no omega)
+    GPU_CHECK(cfa_amg_power(n, arp, aci, av, L->dinv, d_x0_, 20, d_pw_, nullptr, s));
     phase_end(PH_RHO);
     int na = 0;
     const std::vector<int> agg = aggregate(A, dinv, na);
     phase_end(PH_AGG);
+    double lambda = 0.0;
+    GPU_CHECK(cfa_memcpy(&lambda, d_pw_.ptr + pw - 1, sizeof(double), 1, s));
+    const double omega = 4.0 / (3.0 * (std::max(lambda, 1.0e-12) * 1.05));
+    phase_end(PH_RHO);
+    int left_max = 0;
+    for (int i = 0; i < n; ++i) left_max = std::max(left_max, A.rp[i + 1] - A.rp[i]);
     if (na >= (int)(0.85 * n) || na == 0) {
         give_back();
         return 0;
@@ -734,7 +742,7 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
     d_p0v_.upload(p0v);
     phase_end(PH_UPLOAD);
     // P = P0 - omega D^-1 A P0
-    if (!device_product(n, arp, aci, av, d_iota_, d_agg_, d_p0v_, dev_tmp_)) {
+    if (!device_product(n, arp, aci, av, d_iota_, d_agg_, d_p0v_, dev_tmp_, left_max)) {
         give_back();
         return -1;
     }
@@ -742,9 +750,12 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
     GPU_CHECK(cfa_amg_prolongator(n, dev_tmp_.rp, dev_tmp_.ci, dev_tmp_.v, d_agg_, d_p0v_, L->dinv, omega, 0, d_cnt_,
                                   nullptr, nullptr, nullptr, s));
     L->P.rp.reserve(n + 1);
-    GPU_CHECK(cfa_scan_counts(n, d_cnt_, L->P.rp, s));
+    const size_t sws = cfa_scan_counts_ws_bytes(n);
+    d_sws_.reserve(std::max<size_t>(sws, 1));
+    d_flag_.reserve(2);
+    GPU_CHECK(cfa_scan_counts(n, d_cnt_, L->P.rp, d_flag_.ptr + 1, d_sws_, sws, s));
     int p_nnz = 0;
-    GPU_CHECK(cfa_memcpy(&p_nnz, L->P.rp.ptr + n, sizeof(int), 1, s));
+    GPU_CHECK(cfa_memcpy(&p_nnz, d_flag_.ptr + 1, sizeof(int), 1, s));
     L->P.ci.reserve(std::max(p_nnz, 1));
     L->P.v.reserve(std::max(p_nnz, 1));
     GPU_CHECK(cfa_amg_prolongator(n, dev_tmp_.rp, dev_tmp_.ci, dev_tmp_.v, d_agg_, d_p0v_, L->dinv, omega, 1, nullptr,
@@ -765,8 +776,9 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
     L->R.tpr = csr_lanes_per_row(na, p_nnz);
     phase_end(PH_TRANSPOSE);
     // Galerkin product R (A P)
-    if (!device_product(n, arp, aci, av, L->P.rp, L->P.ci, L->P.v, dev_tmp_) ||
-        !device_product(na, L->R.rp, L->R.ci, L->R.v, dev_tmp_.rp, dev_tmp_.ci, dev_tmp_.v, dev_next2_)) {
+    // (rows of R cover an aggregate's neighbourhood: longer than the merge kernel takes)
+    if (!device_product(n, arp, aci, av, L->P.rp, L->P.ci, L->P.v, dev_tmp_, left_max) ||
+        !device_product(na, L->R.rp, L->R.ci, L->R.v, dev_tmp_.rp, dev_tmp_.ci, dev_tmp_.v, dev_next2_, -1)) {
         give_back();
         return -1;
     }

@@ -152,8 +152,9 @@ public:
     // numeric set-up phases (D^-1, spectral radius, P smoothing, transpose, Galerkin product) on
     // the device, bit-identical to the host path; the aggregation stays on the host. A level
     // whose products exceed the device kernel's row capacity falls back to the host path.
-    // Off by default: measured on MI355X the host path is faster (16.7 ms against 41 ms for a
-    // warm set-up of the 115k-row pressure matrix); CFA_AMG_DEVICE_SETUP=1 turns it on.
+    // Measured on MI355X (warm set-up of the pressure matrix): 11.5 ms against the host path's
+    // 14.1 ms at 115k rows, 32.5 ms against 54.2 ms at 460k rows. CFA_AMG_HOST_SETUP=1 in the
+    // environment selects the host path, for comparisons.
     bool device_setup = amg_device_setup_default();
     bool keep_p0_host = true;   // download the level-0 prolongator for remap()
     int device_levels = 0;      // levels the last set-up built on the device
@@ -190,7 +191,7 @@ public:
 private:
     DevCsr dev_next_, dev_tmp_, dev_next2_;    // device set-up: next operator, products
     DevArray<int> d_cnt_, d_flag_, d_agg_, d_iota_;
-    DevArray<unsigned char> d_tws_;   // transpose workspace
+    DevArray<unsigned char> d_tws_, d_sws_;   // transpose and scan workspaces
     DevArray<double> d_p0v_, d_x0_, d_pw_;
     std::vector<double> x0_host_;
     int x0_dev_n_ = 0, iota_dev_n_ = 0;
@@ -198,8 +199,11 @@ private:
     // stalled), -1 fall back to the host for this and the following levels
     int device_level(HostCsr& A, const int* arp, const int* aci, const double* av,
                      std::vector<std::unique_ptr<Level>>& pool);
+    // C = A B on the device; left_max_row: the longest row of A when known (rows of at most 16
+    // entries take the merge kernel), -1 for the LDS-sort kernel straight away. false: a row is
+    // beyond both kernels (the caller falls back to the host).
     bool device_product(int rows, const int* arp, const int* aci, const double* av, const int* brp, const int* bci,
-                        const double* bv, DevCsr& C);
+                        const double* bv, DevCsr& C, int left_max_row);
     std::vector<std::unique_ptr<Level>> spare_;
     std::vector<cfa_amg_level_t> desc_;
     cfa_amg_t view_{};

@@ -363,7 +363,10 @@ int cfa_bicgstab_jacobi(int n, const int* rp, const int* ci, const double* val, 
  * cfa_scan_counts). method 0: one thread per row merging the right rows (rows of A with at
  * most 16 entries); method 1: one workgroup per row sorting up to 4096 products in LDS. Rows of
  * B must have sorted, unique columns.
- * cfa_scan_counts: out[0..n] = exclusive prefix sums of cnt (out[n] = total), one workgroup.
+ * cfa_scan_counts: out[0..n] = exclusive prefix sums of cnt (out[n] = total; also copied to the
+ * device int *total when given); ws: device workspace of cfa_scan_counts_ws_bytes(n) bytes.
+ * cfa_amg_power with lambda == NULL does not synchronize: {||D^-1 A x||, lambda} of the last step
+ * are then left in ws[cfa_amg_power_ws_doubles(n) - 2 .. - 1].
  * cfa_amg_prolongator: P = P0 - omega D^-1 A P0 from AP0 = A P0 (smoothed_prolongator,
  * linalg.cuh:1360-1392: the row's own aggregate merged in column order, exact zeros dropped);
  * pass 0 counts per row, pass 1 fills at p_rp.
@@ -377,7 +380,8 @@ int cfa_amg_power(int n, const int* rp, const int* ci, const double* val, const 
 int cfa_spgemm(int rows, const int* a_rp, const int* a_ci, const double* a_v, const int* b_rp, const int* b_ci,
                const double* b_v, int pass, int method, int* cnt, const int* c_rp, int* c_ci, double* c_v,
                int* overflow, cfa_stream_t stream);
-int cfa_scan_counts(int n, const int* cnt, int* out, cfa_stream_t stream);
+size_t cfa_scan_counts_ws_bytes(int n);
+int cfa_scan_counts(int n, const int* cnt, int* out, int* total, void* ws, size_t ws_bytes, cfa_stream_t stream);
 int cfa_amg_prolongator(int n, const int* ap_rp, const int* ap_ci, const double* ap_v, const int* agg,
                         const double* p0v, const double* dinv, double omega, int pass, int* cnt, const int* p_rp,
                         int* p_ci, double* p_v, cfa_stream_t stream);

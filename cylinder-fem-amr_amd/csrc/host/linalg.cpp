@@ -703,10 +703,13 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
     ensure_level_vectors(*L, n);
     GPU_CHECK(cfa_amg_dinv(n, arp, aci, av, L->dinv, s));
     if (x0_dev_n_ < n) {   // power-iteration start vector of spectral_radius_DinvA
-        x0_host_.resize(n);
-        for (int i = 0; i < n; ++i) x0_host_[i] = 1.0 + 0.1 * std::sin(1.0 + i);
+        // its entries do not depend on n: extend it, with headroom for the meshes that follow
+        const int old_n = (int)x0_host_.size();
+        const int new_n = std::max(n, old_n + old_n / 2);
+        x0_host_.resize(new_n);
+        for (int i = old_n; i < new_n; ++i) x0_host_[i] = 1.0 + 0.1 * std::sin(1.0 + i);
         d_x0_.upload(x0_host_);
-        x0_dev_n_ = n;
+        x0_dev_n_ = new_n;
     }
     const size_t pw = cfa_amg_power_ws_doubles(n);
     d_pw_.reserve(pw);
@@ -732,11 +735,12 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
     for (int i = 0; i < n; ++i) size[agg[i]]++;
     std::vector<double> p0v(n);
     for (int i = 0; i < n; ++i) p0v[i] = 1.0 / std::sqrt((double)size[agg[i]]);
-    if (iota_dev_n_ < n + 1) {
-        std::vector<int> iota(n + 1);
-        for (int i = 0; i <= n; ++i) iota[i] = i;
+    if (iota_dev_n_ < n + 1) {   // row pointer of P0 (one entry per row), with headroom
+        const int m = std::max(n + 1, iota_dev_n_ + iota_dev_n_ / 2);
+        std::vector<int> iota(m);
+        for (int i = 0; i < m; ++i) iota[i] = i;
         d_iota_.upload(iota);
-        iota_dev_n_ = n + 1;
+        iota_dev_n_ = m;
     }
     d_agg_.upload(agg);
     d_p0v_.upload(p0v);

@@ -39,14 +39,13 @@ __device__ __forceinline__ bool halted(const int* state) { return state && state
 // both loads are in flight at once, so the state word does not add a round trip in front of the
 // row's loads (measured: 0.25-0.3 us of a 3-4.4 us sweep on 18k-115k rows). The empty asm
 // statement makes the three values complete before the early exit.
+// Rows past the end get the empty range [ptr[rows], ptr[rows]); the loads are unconditional (and
+// in bounds: ptr has rows + 1 entries) so that no branch separates them.
 __device__ __forceinline__ bool running_with_range(const int* __restrict__ ptr, int row, int rows,
                                                    const int* __restrict__ state, int& k0, int& k1) {
-    k0 = 0;
-    k1 = 0;
-    if (row < rows) {
-        k0 = ptr[row];
-        k1 = ptr[row + 1];
-    }
+    const int r = row < rows ? row : rows;
+    k0 = ptr[r];
+    k1 = ptr[r + (row < rows ? 1 : 0)];
     const int sv = state ? state[0] : ST_RUNNING;
     asm volatile("" ::"v"(k0), "v"(k1), "s"(sv));
     return sv == ST_RUNNING;
@@ -715,9 +714,9 @@ k_ilu_lower(int m, int n, const int* __restrict__ rows, const int* __restrict__ 
             const T* __restrict__ lv, const T* __restrict__ dinv, int last, const double* __restrict__ r,
             double* __restrict__ z, const int* __restrict__ st) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = rows[t < m ? t : m - 1];   // in flight with the range and the state (m >= 1)
     int e0, e1;
     if (!running_with_range(lp, t, m, st, e0, e1) || t >= m) return;
-    const int i = rows[t];
     double acc[NRHS];
 #pragma unroll
     for (int c = 0; c < NRHS; ++c) acc[c] = r[(size_t)c * n + i];
@@ -738,9 +737,9 @@ k_ilu_upper(int m, int n, const int* __restrict__ rows, const int* __restrict__ 
             const T* __restrict__ uv, const T* __restrict__ dinv, double* __restrict__ z,
             const int* __restrict__ st) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = rows[t < m ? t : m - 1];   // in flight with the range and the state (m >= 1)
     int e0, e1;
     if (!running_with_range(up, t, m, st, e0, e1) || t >= m) return;
-    const int i = rows[t];
     double acc[NRHS];
 #pragma unroll
     for (int c = 0; c < NRHS; ++c) acc[c] = z[(size_t)c * n + i];

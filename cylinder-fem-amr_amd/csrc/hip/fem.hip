@@ -90,7 +90,13 @@ __device__ Mms mms_exact(double x, double y, double t) {
 // ---- momentum system -----------------------------------------------------------------------
 // One thread per row i. Element terms as in src/cfa/fem_kernels.cuh:89-180; Dirichlet rows as
 // in :431-453.
-__global__ void __launch_bounds__(BLOCK)
+// Row assembly kernels accumulate a row's matrix values in per-thread LDS slots (one wavefront
+// per workgroup) instead of read-modify-writes in global memory; the sums are the same, from 0
+// in ascending element order. Rows longer than ROW_SLOTS accumulate in place.
+constexpr int ROW_SLOTS = 16;
+constexpr int ROWS_PER_BLOCK = 64;
+
+__global__ void __launch_bounds__(ROWS_PER_BLOCK)
 k_momentum_rows(int n, const int* __restrict__ ev, const double* __restrict__ x, const double* __restrict__ y,
                 const int* __restrict__ tag, const int* __restrict__ rp, const int* __restrict__ ci,
                 const int* __restrict__ inc_ptr, const int* __restrict__ inc_elem, const int* __restrict__ inc_pos,
@@ -100,13 +106,19 @@ k_momentum_rows(int n, const int* __restrict__ ev, const double* __restrict__ x,
                 const double* __restrict__ fx, const double* __restrict__ fy, double u_max, double mms_time,
                 double* __restrict__ A, double* __restrict__ rhs_u, double* __restrict__ rhs_v,
                 double* __restrict__ mass, double* __restrict__ u, double* __restrict__ v) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double acc[ROW_SLOTS][ROWS_PER_BLOCK];
+    const int lane = threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + lane;
     if (i >= n) return;
     const int t = tag[i];
     const bool dirichlet = (t == BC_INLET || t == BC_WALL || t == BC_CYLINDER);
     const int rs = rp[i], re = rp[i + 1];
+    const bool in_lds = re - rs <= ROW_SLOTS;
     if (!dirichlet)
-        for (int k = rs; k < re; ++k) A[k] = 0.0;
+        for (int k = rs; k < re; ++k) {
+            if (in_lds) acc[k - rs][lane] = 0.0;
+            else A[k] = 0.0;
+        }
     double fu_acc = 0.0, fv_acc = 0.0, m_acc = 0.0;
     for (int s = inc_ptr[i]; s < inc_ptr[i + 1]; ++s) {
         const int ea = inc_elem[s];
@@ -138,17 +150,24 @@ k_momentum_rows(int n, const int* __restrict__ ev, const double* __restrict__ x,
         double ug[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) ug[k] = wu_bar * g.dNdx[k] + wv_bar * g.dNdy[k];
+        // the owner's values, selected without a dynamic index
+        const double dxa = a == 0 ? g.dNdx[0] : a == 1 ? g.dNdx[1] : g.dNdx[2];
+        const double dya = a == 0 ? g.dNdy[0] : a == 1 ? g.dNdy[1] : g.dNdy[2];
+        const double uga = a == 0 ? ug[0] : a == 1 ? ug[1] : ug[2];
         double f_u = 0.0, f_v = 0.0;
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const double M = (a == b) ? (area / 6.0) : (area / 12.0);
-            const double K = area * (g.dNdx[a] * g.dNdx[b] + g.dNdy[a] * g.dNdy[b]);
+            const double K = area * (dxa * g.dNdx[b] + dya * g.dNdy[b]);
             const double C = (area / 3.0) * ug[b] + skew * div_w * M;
-            const double S = area * tau * ug[a] * ug[b];
-            const double Ms = (area / 3.0) * tau * ug[a];
+            const double S = area * tau * uga * ug[b];
+            const double Ms = (area / 3.0) * tau * uga;
             const double Aab = a0 * (M + Ms) + nu * K + C + S;
             const int pos = inc_pos[3 * s + b];
-            if (pos >= 0) A[pos] += Aab;
+            if (pos >= 0) {
+                if (in_lds) acc[pos - rs][lane] += Aab;
+                else A[pos] += Aab;
+            }
             f_u += (M + Ms) * uh[b];
             f_v += (M + Ms) * vh[b];
             if (fx) {
@@ -156,7 +175,7 @@ k_momentum_rows(int n, const int* __restrict__ ev, const double* __restrict__ x,
                 f_v += (M + Ms) * fy[nd[b]];
             }
         }
-        const double tw = (area / 3.0) + area * tau * ug[a];
+        const double tw = (area / 3.0) + area * tau * uga;
         f_u -= tw * dpdx;
         f_v -= tw * dpdy;
         fu_acc += f_u;
@@ -177,6 +196,8 @@ k_momentum_rows(int n, const int* __restrict__ ev, const double* __restrict__ x,
         rhs_u[i] = ub;  rhs_v[i] = vb;
         u[i] = ub;      v[i] = vb;
     } else {
+        if (in_lds)
+            for (int k = rs; k < re; ++k) A[k] = acc[k - rs][lane];
         rhs_u[i] = fu_acc;
         rhs_v[i] = fv_acc;
     }
@@ -252,7 +273,7 @@ __global__ void k_lumped_mass_rows(int n, const int* __restrict__ ev, const doub
 
 // PPE rows with OSS operator and outlet Dirichlet rows (src/cfa/fem_kernels.cuh:201-282, 460-490,
 // 807-852).
-__global__ void __launch_bounds__(BLOCK)
+__global__ void __launch_bounds__(ROWS_PER_BLOCK)
 k_pressure_rows(int n, int nnz, const int* __restrict__ ev, const double* __restrict__ x,
                 const double* __restrict__ y, const int* __restrict__ tag, const int* __restrict__ rp,
                 const int* __restrict__ ci, const int* __restrict__ inc_ptr, const int* __restrict__ inc_elem,
@@ -261,18 +282,36 @@ k_pressure_rows(int n, int nnz, const int* __restrict__ ev, const double* __rest
                 const double* __restrict__ mass, const double* __restrict__ s_el, double dt, double rho,
                 double stab, double* __restrict__ K, double* __restrict__ rhs, double* __restrict__ div_num,
                 double* __restrict__ dp, double* __restrict__ oss_val, double* __restrict__ inv_m) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // the row's values (K, Bx, By, BTx, BTy) accumulate in this thread's LDS slots, from 0 in
+    // ascending element order as before; rows longer than ROW_SLOTS accumulate in place
+    __shared__ double acc[5][ROW_SLOTS][ROWS_PER_BLOCK];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + t;
     if (i >= n) return;
     const bool dirichlet = tag[i] == BC_OUTLET;
     const int rs = rp[i], re = rp[i + 1];
+    const bool in_lds = re - rs <= ROW_SLOTS;
     double* Bx = oss_val;
     double* By = oss_val ? oss_val + nnz : nullptr;
     double* BTx = oss_val ? oss_val + 2 * (size_t)nnz : nullptr;
     double* BTy = oss_val ? oss_val + 3 * (size_t)nnz : nullptr;
+    const int n_arr = oss_val ? 5 : 1;
     for (int k = rs; k < re; ++k) {
-        K[k] = 0.0;
-        if (oss_val) { Bx[k] = 0.0; By[k] = 0.0; BTx[k] = 0.0; BTy[k] = 0.0; }
+        if (in_lds) {
+            for (int f = 0; f < n_arr; ++f) acc[f][k - rs][t] = 0.0;
+        } else {
+            K[k] = 0.0;
+            if (oss_val) { Bx[k] = 0.0; By[k] = 0.0; BTx[k] = 0.0; BTy[k] = 0.0; }
+        }
     }
+    auto add = [&](int f, int pos, double v) {
+        if (in_lds) {
+            acc[f][pos - rs][t] += v;
+            return;
+        }
+        double* dst = f == 0 ? K : f == 1 ? Bx : f == 2 ? By : f == 3 ? BTx : BTy;
+        dst[pos] += v;
+    };
     double r_acc = 0.0, d_acc = 0.0;
     for (int s = inc_ptr[i]; s < inc_ptr[i + 1]; ++s) {
         const int ea = inc_elem[s];
@@ -281,6 +320,9 @@ k_pressure_rows(int n, int nnz, const int* __restrict__ ev, const double* __rest
         Geo g;
         if (!geometry(x, y, nd[0], nd[1], nd[2], g)) continue;
         const double area = g.area;
+        // the owner's shape-function gradient, selected without a dynamic index into g
+        const double dxa = a == 0 ? g.dNdx[0] : a == 1 ? g.dNdx[1] : g.dNdx[2];
+        const double dya = a == 0 ? g.dNdy[0] : a == 1 ? g.dNdy[1] : g.dNdy[2];
         const double div_u = us[nd[0]] * g.dNdx[0] + us[nd[1]] * g.dNdx[1] + us[nd[2]] * g.dNdx[2] +
                              vs[nd[0]] * g.dNdy[0] + vs[nd[1]] * g.dNdy[1] + vs[nd[2]] * g.dNdy[2];
         const double rc = -(rho / dt) * (area / 3.0) * div_u;
@@ -301,11 +343,11 @@ k_pressure_rows(int n, int nnz, const int* __restrict__ ev, const double* __rest
         }
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            const double Kab = area * (g.dNdx[a] * g.dNdx[b] + g.dNdy[a] * g.dNdy[b]);
+            const double Kab = area * (dxa * g.dNdx[b] + dya * g.dNdy[b]);
             const int pos = inc_pos[3 * s + b];
-            if (pos >= 0) K[pos] += (1.0 + sk) * Kab;
+            if (pos >= 0) add(0, pos, (1.0 + sk) * Kab);
         }
-        const double stab_a = sk * area * (g.dNdx[a] * gxp + g.dNdy[a] * gyp) - area * (g.dNdx[a] * sx + g.dNdy[a] * sy);
+        const double stab_a = sk * area * (dxa * gxp + dya * gyp) - area * (dxa * sx + dya * sy);
         r_acc += rc - stab_a;
         d_acc += (area / 3.0) * div_u;
         if (oss_val) {
@@ -314,13 +356,23 @@ k_pressure_rows(int n, int nnz, const int* __restrict__ ev, const double* __rest
             for (int b = 0; b < 3; ++b) {
                 const int pos = inc_pos[3 * s + b];
                 if (pos < 0) continue;
-                Bx[pos] += w * g.dNdx[b];
-                By[pos] += w * g.dNdy[b];
-                BTx[pos] += w * g.dNdx[a];
-                BTy[pos] += w * g.dNdy[a];
+                add(1, pos, w * g.dNdx[b]);
+                add(2, pos, w * g.dNdy[b]);
+                add(3, pos, w * dxa);
+                add(4, pos, w * dya);
             }
         }
     }
+    if (in_lds)
+        for (int k = rs; k < re; ++k) {
+            K[k] = acc[0][k - rs][t];
+            if (oss_val) {
+                Bx[k] = acc[1][k - rs][t];
+                By[k] = acc[2][k - rs][t];
+                BTx[k] = acc[3][k - rs][t];
+                BTy[k] = acc[4][k - rs][t];
+            }
+        }
     div_num[i] = d_acc;
     if (oss_val) inv_m[i] = 1.0 / fmax(mass[i], 1.0e-14);
     if (dirichlet) {
@@ -776,7 +828,8 @@ int cfa_assemble_momentum(const cfa_mesh_t* m, const double* u_n, const double* 
         return CFA_E_ARG;
     if (u_n == u || v_n == v || ((fx == nullptr) != (fy == nullptr))) return CFA_E_ARG;
     if (m->n_nodes == 0) return CFA_OK;
-    hipLaunchKernelGGL(k_momentum_rows, dim3(blocks_for(m->n_nodes)), dim3(BLOCK), 0, as_stream(stream),
+    hipLaunchKernelGGL(k_momentum_rows, dim3(blocks_for(m->n_nodes, ROWS_PER_BLOCK)), dim3(ROWS_PER_BLOCK), 0,
+                       as_stream(stream),
                        m->n_nodes, m->elem_v, m->x, m->y, m->bc_tag, m->row_ptr, m->col_ind, m->inc_ptr,
                        m->inc_elem, m->inc_pos, u_n, v_n, u_prev, v_prev, p, nu, rho, a0, a1, a2, c1, c2,
                        tau_time, skew, fx, fy, u_max, mms_time, A_val, rhs_u, rhs_v, mass, u, v);
@@ -826,7 +879,8 @@ int cfa_assemble_pressure(const cfa_mesh_t* m, const double* u_star, const doubl
     if (stab > 0.0 && (!gpx || !gpy || !elem_s)) return CFA_E_ARG;
     if (oss_val && (!inv_m || !elem_s)) return CFA_E_ARG;
     if (m->n_nodes == 0) return CFA_OK;
-    hipLaunchKernelGGL(k_pressure_rows, dim3(blocks_for(m->n_nodes)), dim3(BLOCK), 0, as_stream(stream),
+    hipLaunchKernelGGL(k_pressure_rows, dim3(blocks_for(m->n_nodes, ROWS_PER_BLOCK)), dim3(ROWS_PER_BLOCK), 0,
+                       as_stream(stream),
                        m->n_nodes, m->nnz, m->elem_v, m->x, m->y, m->bc_tag, m->row_ptr, m->col_ind,
                        m->inc_ptr, m->inc_elem, m->inc_pos, u_star, v_star, p, gpx, gpy, mass, elem_s, dt,
                        rho, stab, Kp, rhs, div_num, dp, oss_val, inv_m);

@@ -52,6 +52,14 @@ PressureProbe find_cylinder_probe(const Mesh2D& mesh, double theta) {
     return best;
 }
 
+bool device_topology_default() {
+    static const bool on = [] {
+        const char* e = std::getenv("CFA_HOST_TOPOLOGY");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
+
 void build_incidence(const Mesh2D& mesh, std::vector<int>& inc_ptr, std::vector<int>& inc_elem,
                      std::vector<int>& inc_pos) {
     const int n = mesh.num_nodes, ne = mesh.num_elems;
@@ -94,9 +102,89 @@ cfa_oss_op_t DeviceMeshBuffers::oss_op() const {
 void DeviceMeshBuffers::sync_from_host(const Mesh2D& mesh, const std::vector<double>& h_u,
                                        const std::vector<double>& h_v, const std::vector<double>& h_p,
                                        const std::vector<double>& h_uo, const std::vector<double>& h_vo) {
+    upload_elements(mesh);
+    nnz = mesh.nnz;
+    d_row_ptr.upload(mesh.csr_row_ptr);
+    d_col_ind.upload(mesh.csr_col_ind);
+    std::vector<int> inc_ptr, inc_elem, inc_pos;
+    build_incidence(mesh, inc_ptr, inc_elem, inc_pos);
+    d_inc_ptr.upload(inc_ptr);
+    d_inc_elem.upload(inc_elem);
+    d_inc_pos.upload(inc_pos);
+    upload_fields(mesh, h_u, h_v, h_p, h_uo, h_vo);
+}
+
+void DeviceMeshBuffers::upload_elements(const Mesh2D& mesh) {
     n_nodes = mesh.num_nodes;
     n_elems = mesh.num_elems;
-    nnz = mesh.nnz;
+    const size_t ne = n_elems;
+    std::vector<int> flat(3 * ne);
+    for (size_t e = 0; e < ne; ++e)
+        for (int k = 0; k < 3; ++k) flat[3 * e + k] = mesh.elems[e].v[k];
+    d_elem_v.upload(flat);
+    d_bc_tag.upload(mesh.bc_tag);
+    d_x.upload(mesh.x);
+    d_y.upload(mesh.y);
+}
+
+bool DeviceMeshBuffers::sync_from_host_device_topology(Mesh2D& mesh, const std::vector<double>& h_u,
+                                                       const std::vector<double>& h_v,
+                                                       const std::vector<double>& h_p,
+                                                       const std::vector<double>& h_uo,
+                                                       const std::vector<double>& h_vo, MeshQuality& q,
+                                                       bool& valid) {
+    mesh.num_nodes = (int)mesh.x.size();
+    mesh.num_elems = (int)mesh.elems.size();
+    upload_elements(mesh);
+    const int n = n_nodes, ne = n_elems;
+    const cfa_stream_t s = solver_stream();
+    const size_t cap = (size_t)n + 6 * (size_t)ne;   // upper bound of the graph's entries
+    d_row_ptr.reserve(n + 1);
+    d_col_ind.reserve(cap);
+    d_inc_ptr.reserve(n + 1);
+    d_inc_elem.reserve(3 * (size_t)ne);
+    d_inc_pos.reserve(9 * (size_t)ne);
+    const size_t wsb = cfa_topology_ws_bytes(n, ne);
+    if (wsb == 0) return false;
+    d_topo_ws_.reserve(wsb);
+    d_topo_out_.reserve(8);
+    d_topo_nnz_.reserve(1);
+    GPU_CHECK(cfa_mesh_topology(n, ne, d_elem_v, d_topo_ws_, wsb, d_inc_ptr, d_inc_elem, d_row_ptr, d_col_ind,
+                                d_inc_pos, d_topo_nnz_, s));
+    int nz = 0;
+    d_topo_nnz_.download(&nz, 1);
+    if (nz < 0) return false;   // a row longer than the device kernel takes
+    GPU_CHECK(cfa_mesh_check(n, ne, d_elem_v, d_x, d_y, d_bc_tag, d_row_ptr, d_col_ind, nz, Sim::H_CHANNEL,
+                             Sim::L_CHANNEL, d_topo_ws_, wsb, d_topo_out_, s));
+    double st[6];
+    d_topo_out_.download(st, 6);
+    mesh.nnz = nz;
+    mesh.csr_row_ptr.resize(n + 1);
+    mesh.csr_col_ind.resize(nz);
+    d_row_ptr.download(mesh.csr_row_ptr.data(), n + 1);
+    d_col_ind.download(mesh.csr_col_ind.data(), nz);
+    build_boundary_topology(mesh);
+    q = MeshQuality();
+    q.min_area = st[3];
+    q.total_area = st[4];
+    q.min_angle_deg = std::acos(std::max(-1.0, std::min(1.0, st[5]))) * 180.0 / Sim::PI;
+    for (const Triangle& t : mesh.elems) {
+        q.max_level = std::max(q.max_level, t.level);
+        q.max_gen = std::max(q.max_gen, t.gen);
+    }
+    valid = st[0] == 0.0 && st[1] == 0.0 && st[2] == 0.0;
+    if (!valid) {   // the host check names the offending elements and edges
+        MeshQuality qh;
+        (void)validate_mesh(mesh, qh);
+    }
+    nnz = nz;
+    upload_fields(mesh, h_u, h_v, h_p, h_uo, h_vo);
+    return true;
+}
+
+void DeviceMeshBuffers::upload_fields(const Mesh2D& mesh, const std::vector<double>& h_u,
+                                      const std::vector<double>& h_v, const std::vector<double>& h_p,
+                                      const std::vector<double>& h_uo, const std::vector<double>& h_vo) {
     const size_t n = n_nodes, ne = n_elems, nz = nnz;
     for (DevArray<double>* a : {&d_x, &d_y, &d_u, &d_v, &d_u_old, &d_v_old, &d_u_prev, &d_v_prev, &d_p, &d_dp, &d_vort,
                                 &d_rhs_u, &d_rhs_v, &d_rhs_p, &d_mass_lumped, &d_grad_dp_x, &d_grad_dp_y, &d_vort_num,
@@ -111,18 +199,6 @@ void DeviceMeshBuffers::sync_from_host(const Mesh2D& mesh, const std::vector<dou
     d_diag.reserve(6, 0);
     d_ws.reserve(8 * 1024, 0);
 
-    std::vector<int> flat(3 * ne);
-    for (size_t e = 0; e < ne; ++e)
-        for (int k = 0; k < 3; ++k) flat[3 * e + k] = mesh.elems[e].v[k];
-    d_elem_v.upload(flat);
-    d_bc_tag.upload(mesh.bc_tag);
-    d_row_ptr.upload(mesh.csr_row_ptr);
-    d_col_ind.upload(mesh.csr_col_ind);
-    std::vector<int> inc_ptr, inc_elem, inc_pos;
-    build_incidence(mesh, inc_ptr, inc_elem, inc_pos);
-    d_inc_ptr.upload(inc_ptr);
-    d_inc_elem.upload(inc_elem);
-    d_inc_pos.upload(inc_pos);
     std::vector<int> layer;
     for (size_t e = 0; e < ne; ++e) {
         const Triangle& t = mesh.elems[e];
@@ -132,8 +208,6 @@ void DeviceMeshBuffers::sync_from_host(const Mesh2D& mesh, const std::vector<dou
     n_layer = (int)layer.size();
     d_layer.upload(layer);
 
-    d_x.upload(mesh.x);
-    d_y.upload(mesh.y);
     d_u.upload(h_u);
     d_v.upload(h_v);
     d_p.upload(h_p);
@@ -328,6 +402,7 @@ void RemeshProfile::report(double total_ms) const {
     item("CSR build", csr);
     item("validate_mesh", validate);
     item("upload", upload);
+    if (device_topology > 0.0) item("upload + CSR/checks on GPU", device_topology);
     item("ILU colouring", colouring);
     item("re-projection AMG setup", amg_setup);
     item("re-projection solve", projection);

@@ -36,15 +36,37 @@ struct DeviceMeshBuffers {
     const cfa_mesh_t* view() const { return &view_; }
     cfa_oss_op_t oss_op() const;
 
+    // Upload of a mesh whose CSR graph the host built (build_csr_and_boundary_topology).
     void sync_from_host(const Mesh2D& mesh, const std::vector<double>& h_u, const std::vector<double>& h_v,
                         const std::vector<double>& h_p, const std::vector<double>& h_uo,
                         const std::vector<double>& h_vo);
+    // Upload of a freshly extracted mesh (elements, coordinates, tags): the incidence, the CSR
+    // graph, the position table and the mesh checks are built on the device (cfa_mesh_topology,
+    // cfa_mesh_check) with the host's results; the graph is copied into mesh for the host users
+    // and the cylinder edges are found on the host. valid = the checks passed (the host check
+    // then prints what failed). Returns false, with nothing changed on the device side that
+    // sync_from_host would not redo, when a row is too long for the device kernel.
+    bool sync_from_host_device_topology(Mesh2D& mesh, const std::vector<double>& h_u,
+                                        const std::vector<double>& h_v, const std::vector<double>& h_p,
+                                        const std::vector<double>& h_uo, const std::vector<double>& h_vo,
+                                        MeshQuality& q, bool& valid);
     void copy_solution_to_host(std::vector<double>& h_u, std::vector<double>& h_v, std::vector<double>& h_p,
                                std::vector<double>& h_uo, std::vector<double>& h_vo, std::vector<double>& h_vort,
                                std::vector<double>& h_elem_err);
 private:
     cfa_mesh_t view_{};
+    DevArray<unsigned char> d_topo_ws_;
+    DevArray<double> d_topo_out_;
+    DevArray<int> d_topo_nnz_;
+    void upload_elements(const Mesh2D& mesh);
+    void upload_fields(const Mesh2D& mesh, const std::vector<double>& h_u, const std::vector<double>& h_v,
+                       const std::vector<double>& h_p, const std::vector<double>& h_uo,
+                       const std::vector<double>& h_vo);
 };
+
+// Synchronous remeshes build the CSR graph and check the mesh on the device unless
+// CFA_HOST_TOPOLOGY=1 is set (the host path, for comparisons).
+bool device_topology_default();
 
 // node -> incident element table of the owner-computes kernels (ascending element order) and
 // the CSR position of each (node, element vertex) pair
@@ -100,6 +122,7 @@ struct RemeshProfile {
     double indicator = 0.0;
     double lookahead = 0.0, mark = 0.0, neighbours = 0.0, coarsen = 0.0, refine = 0.0;
     double extract = 0.0, csr = 0.0, validate = 0.0, upload = 0.0, colouring = 0.0;
+    double device_topology = 0.0;   // upload, CSR graph and checks built on the device
     double amg_setup = 0.0, projection = 0.0;
     double overlapped = 0.0, join_wait = 0.0, transfer = 0.0;
     void add(const AmrStats& st) {

@@ -424,10 +424,13 @@ int run_driver(int argc, char** argv) {
             return proj_iters;
         };
         auto finish_remesh = [&](const AmrStats& st, const MeshQuality& q, const std::vector<int>& old_vids,
-                                 const std::vector<double>& old_x, const std::vector<double>& old_y) {
-            g.sync_from_host(mesh, h_u, h_v, h_p, h_uo, h_vo);
-            dev_sync();
-            remesh_prof.upload += lap();
+                                 const std::vector<double>& old_x, const std::vector<double>& old_y,
+                                 bool uploaded) {
+            if (!uploaded) {
+                g.sync_from_host(mesh, h_u, h_v, h_p, h_uo, h_vo);
+                dev_sync();
+                remesh_prof.upload += lap();
+            }
             if (opt.momentum_ilu) {
                 if (pend.has_ilu) {   // colouring done by the remesh worker, only the upload here
                     mom_ilu.set_pattern(pend.ilu, g.d_row_ptr, g.d_col_ind);
@@ -494,15 +497,26 @@ int run_driver(int argc, char** argv) {
                 lap();
                 amr.extract(mesh, h_u, h_v, h_p, h_uo, h_vo);
                 remesh_prof.extract += lap();
-                build_csr_and_boundary_topology(mesh);
-                remesh_prof.csr += lap();
                 MeshQuality q;
-                if (!validate_mesh(mesh, q)) {
+                bool valid = true;
+                // the graph, the position table and the checks on the device, the host path when
+                // a row is too long for it (or CFA_HOST_TOPOLOGY=1)
+                const bool on_device = device_topology_default() &&
+                                       g.sync_from_host_device_topology(mesh, h_u, h_v, h_p, h_uo, h_vo, q, valid);
+                if (on_device) {
+                    dev_sync();
+                    remesh_prof.device_topology += lap();
+                } else {
+                    build_csr_and_boundary_topology(mesh);
+                    remesh_prof.csr += lap();
+                    valid = validate_mesh(mesh, q);
+                    remesh_prof.validate += lap();
+                }
+                if (!valid) {
                     std::fprintf(stderr, "[AMR] mesh validation failed, aborting\n");
                     return EXIT_FAILURE;
                 }
-                remesh_prof.validate += lap();
-                finish_remesh(st, q, old_vids, old_x, old_y);
+                finish_remesh(st, q, old_vids, old_x, old_y, on_device);
             } else {
                 prm_now.lookahead_time += opt.amr_overlap * dt;
                 pend.old_vids = amr.node_vertices();
@@ -560,7 +574,7 @@ int run_driver(int argc, char** argv) {
             h_vo.swap(pend.vo);
             remesh_prof.transfer += lap();
             pend.join_step = -1;
-            finish_remesh(pend.st, pend.q, pend.old_vids, old_x, old_y);
+            finish_remesh(pend.st, pend.q, pend.old_vids, old_x, old_y, false);
         }
         g_stats.hist[hist_row + RunStats::HIST_COLS - 1] = ms_since(wall_start) / 1000.0;
     }

@@ -163,7 +163,11 @@ void build_csr_and_boundary_topology(Mesh2D& mesh) {
         std::copy(scratch.begin() + slot[i], scratch.begin() + slot[i] + len[i],
                   mesh.csr_col_ind.begin() + mesh.csr_row_ptr[i]);
     mesh.nnz = (int)mesh.csr_col_ind.size();
+    build_boundary_topology(mesh);
+}
 
+void build_boundary_topology(Mesh2D& mesh) {
+    const int ne = (int)mesh.elems.size();
     // Cylinder edges: element edges whose two ends are cylinder nodes and that belong to exactly
     // one element, in order of first appearance (src/cfa/mesh.cuh:251-269).
     std::unordered_map<uint64_t, std::pair<int, int>> seen;   // key -> (count, last element)

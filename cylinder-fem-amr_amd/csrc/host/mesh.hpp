@@ -47,6 +47,8 @@ void generate_cylinder_mesh(Mesh2D& mesh, int n_quarter = 8, int n_radial = 7, i
 
 // Sorted-row node adjacency and the cylinder boundary edges (edges of exactly one element).
 void build_csr_and_boundary_topology(Mesh2D& mesh);
+// the cylinder edge list alone (the part of build_csr_and_boundary_topology after the graph)
+void build_boundary_topology(Mesh2D& mesh);
 
 // True for a conforming, positively oriented triangulation; fills q with quality figures.
 bool validate_mesh(const Mesh2D& mesh, MeshQuality& q);

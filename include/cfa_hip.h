@@ -389,6 +389,28 @@ size_t cfa_csr_transpose_ws_bytes(int rows, int cols, int nnz);
 int cfa_csr_transpose(int rows, int cols, int nnz, const int* rp, const int* ci, const double* v, void* ws,
                       size_t ws_bytes, int* t_rp, int* t_ci, double* t_v, cfa_stream_t stream);
 
+/* ---- mesh topology on the device (src/cfa/mesh.cuh:198-376, host code in the reference) -------
+ * cfa_mesh_topology: from the element list elem_v [3 ne] of a mesh with n nodes, the node-element
+ * incidence (inc_ptr [n + 1]; inc_elem [3 ne] = 3 e + corner, each node's elements ascending),
+ * the CSR graph (row i = {i} and the other vertices of its elements, sorted: row_ptr [n + 1],
+ * col_ind with room for n + 6 ne entries) and the owner-computes position table inc_pos [9 ne]
+ * (CSR position of each element vertex in the row of each incidence), as
+ * build_csr_and_boundary_topology (mesh.cuh:198-270) and the incidence build of the kernels;
+ * *nnz_out (device int) = the graph's entries, or -1 when a node has more than 47 elements (the
+ * caller then builds on the host).
+ * cfa_mesh_check: validate_mesh (mesh.cuh:283-376): stats (device) [6] = non-positive elements,
+ * element edges missing from the graph, non-conforming edges (shared by more than two
+ * elements, or by one off a boundary piece of the channel h_channel x l_channel or the
+ * cylinder), minimum signed area, total area (summed in a fixed tree order), maximum corner
+ * cosine (the minimum angle is its arccosine).
+ * ws: device workspace of cfa_topology_ws_bytes(n, ne) bytes, shared by both calls. */
+size_t cfa_topology_ws_bytes(int n_nodes, int n_elems);
+int cfa_mesh_topology(int n, int ne, const int* elem_v, void* ws, size_t ws_bytes, int* inc_ptr, int* inc_elem,
+                      int* row_ptr, int* col_ind, int* inc_pos, int* nnz_out, cfa_stream_t stream);
+int cfa_mesh_check(int n, int ne, const int* elem_v, const double* x, const double* y, const int* bc_tag,
+                   const int* row_ptr, const int* col_ind, int nnz, double h_channel, double l_channel, void* ws,
+                   size_t ws_bytes, double* stats, cfa_stream_t stream);
+
 /* ---- discrete adjoint of the step (no counterpart in the reference) --------------------------
  * Vector-Jacobian products of the step's assemblies and updates (the forward operations are
  * those of src/cylinder_fem_amr.cu:277-538; the reference has no derivative code). A linear

@@ -5,7 +5,7 @@
 // block reductions or other summation orders must match to the stated relative tolerance.
 //
 //   parity <case> [level]    case: momentum pressure correction forces indicator mms oss ilu
-//                                  spmv amg pcg bicgstab mixed all
+//                                  spmv amg pcg bicgstab mixed topology all
 //
 // Prints one line per compared quantity and exits 0 when every comparison passes.
 #include <cmath>
@@ -509,6 +509,80 @@ void case_amg(Fixture& f) {
     }
 }
 
+// Mesh topology on the device against the host build: the CSR graph, the incidence and position
+// tables, the cylinder edges and the checks, on the fixture's uniform mesh and on a locally
+// refined NVB mesh; then on two planted defects (a flipped element, a missing element), which
+// both paths must reject.
+void topology_pair(const char* name, const Mesh2D& m, bool expect_valid) {
+    std::printf("  %s: %d nodes, %d elements\n", name, (int)m.x.size(), (int)m.elems.size());
+    Mesh2D h = m;
+    build_csr_and_boundary_topology(h);
+    MeshQuality qh;
+    const bool vh = validate_mesh(h, qh);
+    Mesh2D d = m;
+    d.csr_row_ptr.clear();
+    d.csr_col_ind.clear();
+    d.nnz = 0;
+    d.cylinder_edges.clear();
+    d.cylinder_edge_elem.clear();
+    DeviceMeshBuffers g;
+    const Vec z(m.x.size(), 0.0);
+    MeshQuality qd;
+    bool vd = false;
+    const bool took = g.sync_from_host_device_topology(d, z, z, z, z, z, qd, vd);
+    expect(took, "the device path takes the mesh");
+    expect(vh == expect_valid, "host check as planted");
+    expect(vd == vh, "device check = host check");
+    same_ints("CSR row pointers", d.csr_row_ptr, h.csr_row_ptr);
+    same_ints("CSR columns", d.csr_col_ind, h.csr_col_ind);
+    std::vector<int> ce_d, ce_h;
+    for (size_t k = 0; k < d.cylinder_edges.size(); ++k)
+        ce_d.insert(ce_d.end(), {d.cylinder_edges[k].first, d.cylinder_edges[k].second, d.cylinder_edge_elem[k]});
+    for (size_t k = 0; k < h.cylinder_edges.size(); ++k)
+        ce_h.insert(ce_h.end(), {h.cylinder_edges[k].first, h.cylinder_edges[k].second, h.cylinder_edge_elem[k]});
+    same_ints("cylinder edges", ce_d, ce_h);
+    same_bits("minimum area, minimum angle", Vec{qd.min_area, qd.min_angle_deg}, Vec{qh.min_area, qh.min_angle_deg});
+    // summed in another order: bounded by (elements) x eps = 4.5e-11 at 2e5 elements
+    close_rel("total area", Vec{qd.total_area}, Vec{qh.total_area}, 1e-10);
+    expect(qd.max_level == qh.max_level && qd.max_gen == qh.max_gen, "maximum level and generation");
+    std::vector<int> ip, ie, ipos;
+    build_incidence(h, ip, ie, ipos);
+    same_ints("incidence pointers", down_int(g.d_inc_ptr, (int)ip.size()), ip);
+    same_ints("incidence elements", down_int(g.d_inc_elem, (int)ie.size()), ie);
+    same_ints("owner-computes positions", down_int(g.d_inc_pos, (int)ipos.size()), ipos);
+}
+
+void case_topology(Fixture& f) {
+    std::printf("[mesh topology on the device]\n");
+    topology_pair("uniform mesh", f.mesh, true);
+    // locally refined: elements with their centroid within 0.08 of (0.35, 0.2), three levels up
+    Mesh2D m = f.mesh;
+    Vec z(m.num_nodes, 0.0), u = z, v = z, p = z, uo = z, vo = z;
+    AdaptiveMesh amr(m, z, z, z);
+    AmrParams prm;
+    prm.max_level = 5;
+    prm.min_level = 0;
+    for (int pass = 0; pass < 3; ++pass) {
+        amr.extract(m, u, v, p, uo, vo);
+        Vec eta(2 * m.elems.size(), 0.0);
+        for (size_t e = 0; e < m.elems.size(); ++e) {
+            const Triangle& t = m.elems[e];
+            const double cx = (m.x[t.v[0]] + m.x[t.v[1]] + m.x[t.v[2]]) / 3.0;
+            const double cy = (m.y[t.v[0]] + m.y[t.v[1]] + m.y[t.v[2]]) / 3.0;
+            if (std::hypot(cx - 0.35, cy - 0.2) < 0.08) eta[e] = 1.0e3;
+        }
+        amr.adapt(eta, prm);
+    }
+    amr.extract(m, u, v, p, uo, vo);
+    topology_pair("locally refined mesh", m, true);
+    Mesh2D flipped = m;
+    std::swap(flipped.elems[flipped.elems.size() / 2].v[1], flipped.elems[flipped.elems.size() / 2].v[2]);
+    topology_pair("planted: one element flipped", flipped, false);
+    Mesh2D holed = m;
+    holed.elems.erase(holed.elems.begin() + (long)(holed.elems.size() / 3));
+    topology_pair("planted: one element removed", holed, false);
+}
+
 void case_pcg(Fixture& f) {
     std::printf("[PCG with OSS and SA-AMG]\n");
     const PressureSystem o = case_pressure(f, false);
@@ -639,6 +713,7 @@ int main(int argc, char** argv) {
         if (all || what == "pcg") case_pcg(f);
         if (all || what == "bicgstab") case_bicgstab(f);
         if (all || what == "mixed") case_mixed(f);
+        if (all || what == "topology") case_topology(f);
         std::printf(failures ? "%d FAILURE(S)\n" : "ALL PASSED\n", failures);
         status = failures ? 1 : 0;
     }   // the fixture's device memory is freed here

@@ -1,7 +1,9 @@
 """Kernel-group parity through the C ABI against the oracle (tests/native/parity.cpp): assembly,
 BCs, OSS operator, correction, indicators bit for bit; forces, MMS integrals, SpMV, ILU sweeps,
-V-cycle, PCG and BiCGSTAB2 to the relative tolerances printed by the program. Meshes: uniform
-levels 1 and 3 of the reference's generator; fields seeded."""
+V-cycle, PCG and BiCGSTAB2 to the relative tolerances printed by the program; the mesh topology
+built on the device (CSR graph, incidence, checks) against the host build, on uniform and locally
+refined meshes and two planted defects. Meshes: uniform levels 1 and 3 of the reference's
+generator; fields seeded."""
 import subprocess
 from pathlib import Path
 
@@ -9,7 +11,7 @@ import pytest
 
 BIN = Path(__file__).resolve().parent / "native" / "build" / "parity"
 CASES = ["momentum", "pressure", "correction", "forces", "indicator", "mms", "oss", "ilu", "spmv", "amg", "pcg",
-         "bicgstab", "mixed"]
+         "bicgstab", "mixed", "topology"]
 
 pytestmark = pytest.mark.gpu
 # (case, level) pairs kept off the GPU: each ended once with SIGSEGV in the parity process, the

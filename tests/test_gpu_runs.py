@@ -11,7 +11,13 @@ reference's stored runs.
   trajectory (nodes, elements per step) is integer work and must match exactly; forces follow the
   reference to 1e-8 relative over the first 200 steps (remesh decisions are sensitive to
   round-off, so the reference itself drifts later, paper/manuscript.tex:919-922).
+* The same flow with synchronous remeshes (--amr-overlap 0), 300 steps: the CSR graph and mesh
+  checks built on the device give the same run as the host build, forces CSV byte for byte.
 """
+import os
+import subprocess
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -92,3 +98,22 @@ def test_adaptive_trajectory_vs_reference(golden):
     for c in ("C_D", "C_L", "delta_p"):
         d = rel_dev(hist[:, H[c]], ref[c])
         assert d <= 1e-8, f"{c}: relative deviation {d:.3e} from the reference"
+
+
+def test_synchronous_remesh_topology_on_device_matches_host(tmp_path):
+    print("[case] adaptive A2, synchronous remesh, 300 steps: device vs host topology", flush=True)
+    binary = Path(__file__).resolve().parents[1] / "cylinder-fem-amr_amd" / "build" / "cylinder_fem_amr"
+    flags = ["--steps", "300", "--dt", "0.0005", "--amr-interval", "20", "--amr-overlap", "0", "--amr-tol-vort",
+             "0.05", "--amr-tol-interp", "0.005", "--max-level", "5", "--max-elems", "400000", "--min-level", "2"]
+    forces = {}
+    for mode in ("device", "host"):
+        env = dict(os.environ)
+        env.pop("CFA_HOST_TOPOLOGY", None)
+        if mode == "host":
+            env["CFA_HOST_TOPOLOGY"] = "1"
+        r = subprocess.run([str(binary)] + flags + ["--prefix", str(tmp_path / mode)], env=env, capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        assert ("CSR/checks on GPU" in r.stdout) == (mode == "device"), r.stdout[-3000:]
+        forces[mode] = (tmp_path / f"{mode}_forces.csv").read_bytes()
+    assert forces["device"] == forces["host"]

@@ -1669,6 +1669,8 @@ namespace {
 // captured graph (the search direction and the r.z partials alternate between two buffers).
 // Same algorithm, guards and stopping rules as the reference (src/cfa/linalg.cuh:183-256).
 // The operator kernels run with 2 or 4 lanes per row (about four entries per lane).
+int operator_lanes(int n, long long nnz) { return nnz <= 12LL * n ? 2 : 4; }
+
 template <int TPR>
 void launch_fp_operator_t(bool oss, int grid, hipStream_t s, const PcgArgs& a, int h) {
     if (oss) hipLaunchKernelGGL((k_fp_operator<TPR, true>), dim3(grid), dim3(BLOCK), 0, s, a, h);
@@ -1722,7 +1724,7 @@ int pcg_fused(int n, const int* rp, const int* ci, const double* val, const doub
     if (oss) {
         a.oss = oss->oss_val; a.inv_m = oss->inv_m; a.tag = oss->bc_tag; a.gx = oss->gx; a.gy = oss->gy;
     }
-    const int TPR = a.nnz <= 12LL * n ? 2 : 4;
+    const int TPR = operator_lanes(n, a.nnz);
     const double init_sc[2] = {tol, -1.0};   // S_TOL, and S_REL "not computed"
     CFA_TRY(hipMemcpyAsync(a.sc + S_TOL, &init_sc[0], sizeof(double), hipMemcpyHostToDevice, s));
     CFA_TRY(hipMemcpyAsync(a.sc + S_REL, &init_sc[1], sizeof(double), hipMemcpyHostToDevice, s));
@@ -1846,14 +1848,14 @@ int cfa_pcg_operator(int n, const int* rp, const int* ci, const double* val, con
     int nnz = 0;
     CFA_TRY(hipMemcpyAsync(&nnz, rp + n, sizeof(int), hipMemcpyDeviceToHost, s));
     CFA_TRY(hipStreamSynchronize(s));
-    constexpr int TPR = 4;
-    const int blocks_tpr = std::min(blocks_for((long long)n * TPR), OP_BLOCKS);
+    const int tpr = operator_lanes(n, nnz);
+    const int blocks_tpr = std::min(blocks_for((long long)n * tpr), OP_BLOCKS);
     if (stage != 2)
-        hipLaunchKernelGGL(k_apply_K_B<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, nnz, rp, ci, val, oss->oss_val,
-                           oss->inv_m, in, out, oss->gx, oss->gy, nullptr);
+        launch_apply_K_B(tpr, blocks_tpr, s, n, nnz, rp, ci, val, oss->oss_val, oss->inv_m, in, out, oss->gx, oss->gy,
+                         nullptr);
     if (stage != 1)
-        hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss->oss_val,
-                           oss->bc_tag, oss->gx, oss->gy, in, out, part, nullptr);
+        launch_apply_BT_dot(tpr, blocks_tpr, s, n, nnz, rp, ci, oss->oss_val, oss->bc_tag, oss->gx, oss->gy, in, out,
+                            part, nullptr);
     return launch_status();
 }
 
@@ -2028,13 +2030,30 @@ int cfa_bicgstab2(int n, const int* rp, const int* ci, const double* val, const 
     ws->status_host[0] = 0;
     ws->status_host[1] = ST_RUNNING;
 
-    constexpr int TPR = 4;
+    // 2 lanes per row for rows of about 7 entries (4 for longer ones): with the partial sums
+    // capped at RED_BLOCKS workgroups, 2 lanes also keep one row per group up to 131k rows
+    int nnz = 0;
+    CFA_TRY(hipMemcpyAsync(&nnz, rp + n, sizeof(int), hipMemcpyDeviceToHost, s));
+    CFA_TRY(hipStreamSynchronize(s));
+    const int TPR = operator_lanes(n, nnz);
     const int groups_per_block = BLOCK / TPR;
     const int nb_rows = std::min((n + groups_per_block - 1) / groups_per_block, RED_BLOCKS);
     const int nb2 = red_blocks(2LL * n);
     const int blocks2 = blocks_for(2LL * n);
+    auto spmv2_dots = [&](const double* xin, double* yout, const double* w0, int mode, double* partials,
+                          hipStream_t q_s) {
+        if (TPR == 2)
+            hipLaunchKernelGGL(k_spmv2_dots<2>, dim3(nb_rows), dim3(BLOCK), 0, q_s, n, rp, ci, val, xin, yout, w0, mode,
+                               partials, st);
+        else
+            hipLaunchKernelGGL(k_spmv2_dots<4>, dim3(nb_rows), dim3(BLOCK), 0, q_s, n, rp, ci, val, xin, yout, w0, mode,
+                               partials, st);
+    };
 
-    hipLaunchKernelGGL(k_bicg_init<TPR>, dim3(nb_rows), dim3(BLOCK), 0, s, n, rp, ci, val, bb, x, r, r0, p, v, part_a);
+    if (TPR == 2)
+        hipLaunchKernelGGL(k_bicg_init<2>, dim3(nb_rows), dim3(BLOCK), 0, s, n, rp, ci, val, bb, x, r, r0, p, v, part_a);
+    else
+        hipLaunchKernelGGL(k_bicg_init<4>, dim3(nb_rows), dim3(BLOCK), 0, s, n, rp, ci, val, bb, x, r, r0, p, v, part_a);
     hipLaunchKernelGGL(k_bicg_final_init, dim3(1), dim3(BLOCK), 0, s, nb_rows, part_a, sc, st, status, max_iter);
     hipLaunchKernelGGL(k_bicg_pre, dim3(1), dim3(1), 0, s, sc, st, status);
     int rc = launch_status();
@@ -2044,11 +2063,11 @@ int cfa_bicgstab2(int n, const int* rp, const int* ci, const double* val, const 
     auto iteration = [&](hipStream_t q_s) {
         hipLaunchKernelGGL(k_bicg_p, dim3(blocks2), dim3(BLOCK), 0, q_s, n, r, v, sc, p, st);
         ilu_apply_any(ilu, use_f32, p, ph, 2, st, q_s);
-        hipLaunchKernelGGL(k_spmv2_dots<TPR>, dim3(nb_rows), dim3(BLOCK), 0, q_s, n, rp, ci, val, ph, v, r0, 1, part_a, st);
+        spmv2_dots(ph, v, r0, 1, part_a, q_s);
         hipLaunchKernelGGL(k_bicg_s, dim3(nb2), dim3(BLOCK), 0, q_s, n, r, v, sc, sv, part_b, nb_rows, part_a, st,
                            status);
         ilu_apply_any(ilu, use_f32, sv, sh, 2, st, q_s);
-        hipLaunchKernelGGL(k_spmv2_dots<TPR>, dim3(nb_rows), dim3(BLOCK), 0, q_s, n, rp, ci, val, sh, t, sv, 2, part_c, st);
+        spmv2_dots(sh, t, sv, 2, part_c, q_s);
         hipLaunchKernelGGL(k_bicg_xr, dim3(nb2), dim3(BLOCK), 0, q_s, n, ph, sh, sv, t, r0, sc, x, r, part_d, nb2,
                            part_b, nb_rows, part_c, st);
         hipLaunchKernelGGL(k_bicg_final_post, dim3(1), dim3(BLOCK), 0, q_s, nb2, part_d, sc, st, status);
@@ -2056,7 +2075,7 @@ int cfa_bicgstab2(int n, const int* rp, const int* ci, const double* val, const 
     hipGraphExec_t graph = nullptr;
     if (max_iter > 0) {
         KeyBuilder kb;
-        kb.add(n); kb.add(rp); kb.add(ci); kb.add(val); kb.add(use_f32); kb.add(ilu->n_colours);
+        kb.add(n); kb.add(rp); kb.add(ci); kb.add(val); kb.add(use_f32); kb.add(TPR); kb.add(ilu->n_colours);
         kb.add_bytes(ilu->colour_start, sizeof(int) * (size_t)(ilu->n_colours + 1));
         kb.add(ilu->rows); kb.add(ilu->lp); kb.add(ilu->up); kb.add(ilu->lc); kb.add(ilu->uc); kb.add(ilu->lv);
         kb.add(ilu->uv); kb.add(ilu->dinv); kb.add(ilu->lv32); kb.add(ilu->uv32); kb.add(ilu->dinv32);

@@ -60,6 +60,63 @@ __device__ __forceinline__ double sum_partials(const double* part, int count, in
     return s;
 }
 
+// K sums of partials at once, each in the order of sum_partials (lane-strided, the wave tree, the
+// waves in order), so the values are bit-identical to K calls of it, with one pair of barriers
+// instead of K: out[q] = sum of parts[q][k * stride], k < count[q]; valid in all threads.
+// lds: at least K * blockDim.x / 64 doubles.
+template <int K>
+__device__ __forceinline__ void sum_partials_k(const double* const (&parts)[K], const int (&count)[K], int stride,
+                                               double (&out)[K], double* lds) {
+    double s[K];
+    int cmax = 0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        s[q] = 0.0;
+        cmax = count[q] > cmax ? count[q] : cmax;
+    }
+    for (int k = threadIdx.x; k < cmax; k += blockDim.x)
+#pragma unroll
+        for (int q = 0; q < K; ++q)
+            if (k < count[q]) s[q] += parts[q][(size_t)k * stride];
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int nw = (blockDim.x + WAVE - 1) / WAVE;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        s[q] = wave_sum(s[q]);
+        if (lane == 0) lds[q * nw + wid] = s[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        double t = 0.0;
+        for (int w = 0; w < nw; ++w) t += lds[q * nw + w];
+        out[q] = t;
+    }
+    __syncthreads();
+}
+
+// K block sums at once (block_sum's order: the wave tree, then the waves in order); valid in
+// thread 0. lds: at least K * blockDim.x / 64 doubles.
+template <int K>
+__device__ __forceinline__ void block_sum_k(double (&v)[K], double* lds) {
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int nw = (blockDim.x + WAVE - 1) / WAVE;
+#pragma unroll
+    for (int q = 0; q < K; ++q) {
+        v[q] = wave_sum(v[q]);
+        if (lane == 0) lds[q * nw + wid] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            double t = 0.0;
+            for (int w = 0; w < nw; ++w) t += lds[q * nw + w];
+            v[q] = t;
+        }
+    __syncthreads();
+}
+
 // Sum over TPR consecutive lanes (TPR a power of two <= 64); lane 0 of the group holds it.
 template <int TPR>
 __device__ __forceinline__ double group_sum(double v) {

@@ -472,7 +472,7 @@ __device__ __forceinline__ int block_state(const int* st, int* it_out) {
 template <int TPR, bool OSS>
 __global__ void __launch_bounds__(BLOCK)
 k_fp_operator(PcgArgs a, int h) {
-    __shared__ double lds[BLOCK / WAVE];
+    __shared__ double lds[2 * BLOCK / WAVE];
     int it;
     if (block_state(a.st, &it) != ST_RUNNING) return;
     if (it >= a.max_iter) {
@@ -482,9 +482,9 @@ k_fp_operator(PcgArgs a, int h) {
     const bool first = it == 0;
     double beta = 0.0;
     if (!first) {
-        const double rz_new = sum_partials(a.prz[h], a.nb_rz, 1, lds);
-        const double rz_old = sum_partials(a.prz[1 - h], a.nb_rz, 1, lds);
-        beta = rz_new / (rz_old + 1.0e-30);
+        double rz[2];
+        sum_partials_k<2>({a.prz[h], a.prz[1 - h]}, {a.nb_rz, a.nb_rz}, 1, rz, lds);
+        beta = rz[0] / (rz[1] + 1.0e-30);
     }
     const double* __restrict__ z = a.z;
     const double* __restrict__ po = a.p[1 - h];
@@ -533,10 +533,12 @@ k_fp_operator(PcgArgs a, int h) {
 // the completed-iteration counter advances here.
 __global__ void __launch_bounds__(BLOCK)
 k_fp_update(PcgArgs a, int h, int nb_pq) {
-    __shared__ double lds[BLOCK / WAVE];
+    __shared__ double lds[2 * BLOCK / WAVE];
     int it;
     if (block_state(a.st, &it) != ST_RUNNING) return;
-    const double pq = sum_partials(a.ppq, nb_pq, 1, lds);
+    double sums[2];   // p.q, r.z
+    sum_partials_k<2>({a.ppq, a.prz[h]}, {nb_pq, a.nb_rz}, 1, sums, lds);
+    const double pq = sums[0];
     if (fabs(pq) < 1.0e-30) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             a.st[0] = ST_STOPPED;
@@ -545,7 +547,7 @@ k_fp_update(PcgArgs a, int h, int nb_pq) {
         }
         return;
     }
-    const double alpha = sum_partials(a.prz[h], a.nb_rz, 1, lds) / pq;
+    const double alpha = sums[1] / pq;
     const double* __restrict__ p = a.p[h];
     const double* __restrict__ q = a.q;
     double* __restrict__ x = a.x;
@@ -766,12 +768,18 @@ enum {
 // [6 + c] converged, [8 + c] half-step done, [10] current iteration, [11] iteration cap
 
 // Two-component partial sums: part[b * 6 + 2 q + c]
-__device__ __forceinline__ void write_partials2(double* part, int nq, double acc[][2], double* lds) {
-    for (int q = 0; q < nq; ++q)
-        for (int c = 0; c < 2; ++c) {
-            const double s = block_sum(acc[q][c], lds);
-            if (threadIdx.x == 0) part[(size_t)blockIdx.x * 6 + 2 * q + c] = s;
-        }
+// (lds: at least 2 nq * blockDim.x / 64 doubles)
+template <int NQ>
+__device__ __forceinline__ void write_partials2(double* part, double (&acc)[NQ][2], double* lds) {
+    double v[2 * NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) v[2 * q + c] = acc[q][c];
+    block_sum_k<2 * NQ>(v, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < 2 * NQ; ++k) part[(size_t)blockIdx.x * 6 + k] = v[k];
 }
 
 // r = b - A x (both components), r0 = p = r, v = 0; partial r.r
@@ -780,7 +788,7 @@ __global__ void __launch_bounds__(BLOCK)
 k_bicg_init(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ val,
             const double* __restrict__ b, const double* __restrict__ x, double* __restrict__ r,
             double* __restrict__ r0, double* __restrict__ p, double* __restrict__ v, double* __restrict__ part) {
-    __shared__ double lds[BLOCK / WAVE];
+    __shared__ double lds[2 * BLOCK / WAVE];
     const int groups = gridDim.x * (blockDim.x / TPR);
     const int g0 = (blockIdx.x * blockDim.x + threadIdx.x) / TPR, lane = threadIdx.x % TPR;
     double acc[1][2] = {{0.0, 0.0}};
@@ -804,7 +812,7 @@ k_bicg_init(int n, const int* __restrict__ rp, const int* __restrict__ ci, const
             acc[0][1] += q1 * q1;
         }
     }
-    write_partials2(part, 1, acc, lds);
+    write_partials2<1>(part, acc, lds);
 }
 
 __device__ __forceinline__ void bicg_stop(int* st, int c, int it, int ok) {
@@ -815,9 +823,9 @@ __device__ __forceinline__ void bicg_stop(int* st, int c, int it, int ok) {
 
 __global__ void k_bicg_final_init(int nb, const double* __restrict__ part, double* __restrict__ sc,
                                   int* __restrict__ st, int* __restrict__ status, int max_iter) {
-    __shared__ double lds[BLOCK / WAVE];
+    __shared__ double lds[2 * BLOCK / WAVE];
     double rr[2];
-    for (int c = 0; c < 2; ++c) rr[c] = sum_partials(part + c, nb, 6, lds);
+    sum_partials_k<2>({part, part + 1}, {nb, nb}, 6, rr, lds);
     if (threadIdx.x != 0) return;
     for (int c = 0; c < 2; ++c) {
         sc[B_RN0 + c] = sqrt(rr[c]);
@@ -874,7 +882,7 @@ __global__ void __launch_bounds__(BLOCK)
 k_spmv2_dots(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ val,
              const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ w0, int mode,
              double* __restrict__ part, const int* __restrict__ st) {
-    __shared__ double lds[BLOCK / WAVE];
+    __shared__ double lds[4 * BLOCK / WAVE];
     if (halted(st)) return;
     const int groups = gridDim.x * (blockDim.x / TPR);
     const int g0 = (blockIdx.x * blockDim.x + threadIdx.x) / TPR, lane = threadIdx.x % TPR;
@@ -903,7 +911,8 @@ k_spmv2_dots(int n, const int* __restrict__ rp, const int* __restrict__ ci, cons
             }
         }
     }
-    write_partials2(part, mode == 1 ? 1 : 2, acc, lds);
+    if (mode == 1) write_partials2<1>(part, reinterpret_cast<double(&)[1][2]>(acc), lds);
+    else write_partials2<2>(part, acc, lds);
 }
 
 // The scalar steps between the vector kernels are folded into the kernels that consume them:
@@ -918,15 +927,17 @@ __global__ void __launch_bounds__(BLOCK)
 k_bicg_s(int n, const double* __restrict__ r, const double* __restrict__ v, double* __restrict__ sc,
          double* __restrict__ s, double* __restrict__ part, int nb_a, const double* __restrict__ part_a,
          int* __restrict__ st, int* __restrict__ status) {
-    __shared__ double lds[BLOCK / WAVE];
+    __shared__ double lds[2 * BLOCK / WAVE];
     __shared__ int s_state, s_act[2];
     if (threadIdx.x == 0) { s_state = st[0]; s_act[0] = st[2]; s_act[1] = st[3]; }
     __syncthreads();
     if (s_state != ST_RUNNING) return;
     double ca[2];
     bool act[2];
+    double o2[2];
+    sum_partials_k<2>({part_a, part_a + 1}, {nb_a, nb_a}, 6, o2, lds);
     for (int c = 0; c < 2; ++c) {
-        const double o = sum_partials(part_a + c, nb_a, 6, lds);
+        const double o = o2[c];
         act[c] = s_act[c] && !(fabs(o) < sc[B_BD + c]);
         ca[c] = act[c] ? sc[B_RHO + c] / o : 0.0;
     }
@@ -947,7 +958,7 @@ k_bicg_s(int n, const double* __restrict__ r, const double* __restrict__ v, doub
         s[i] = si;
         acc[0][c] += si * si;
     }
-    write_partials2(part, 1, acc, lds);
+    write_partials2<1>(part, acc, lds);
 }
 
 // omega = (t . s) / (t . t) unless the half step already converged (from the s.s and t.t, t.s
@@ -957,17 +968,20 @@ k_bicg_xr(int n, const double* __restrict__ ph, const double* __restrict__ sh, c
           const double* __restrict__ t, const double* __restrict__ r0, double* __restrict__ sc, double* __restrict__ x,
           double* __restrict__ r, double* __restrict__ part, int nb_s, const double* __restrict__ part_s, int nb_t,
           const double* __restrict__ part_t, int* __restrict__ st) {
-    __shared__ double lds[BLOCK / WAVE];
+    __shared__ double lds[6 * BLOCK / WAVE];
     __shared__ int s_state, s_act[2];
     if (threadIdx.x == 0) { s_state = st[0]; s_act[0] = st[2]; s_act[1] = st[3]; }
     __syncthreads();
     if (s_state != ST_RUNNING) return;
     double a1[2], a2[2], om[2];
     int half[2];
+    double sums[6];   // s.s, t.t, t.s of both components
+    sum_partials_k<6>({part_s, part_s + 1, part_t, part_t + 1, part_t + 2, part_t + 3},
+                      {nb_s, nb_s, nb_t, nb_t, nb_t, nb_t}, 6, sums, lds);
     for (int c = 0; c < 2; ++c) {
-        const double ss = sum_partials(part_s + c, nb_s, 6, lds);
-        const double tt = sum_partials(part_t + c, nb_t, 6, lds);
-        const double ts = sum_partials(part_t + 2 + c, nb_t, 6, lds);
+        const double ss = sums[c];
+        const double tt = sums[2 + c];
+        const double ts = sums[4 + c];
         a1[c] = a2[c] = om[c] = 0.0;
         half[c] = 0;
         if (!s_act[c]) continue;
@@ -997,20 +1011,18 @@ k_bicg_xr(int n, const double* __restrict__ ph, const double* __restrict__ sh, c
         acc[0][c] += ri * ri;
         acc[1][c] += r0[i] * ri;
     }
-    write_partials2(part, 2, acc, lds);
+    write_partials2<2>(part, acc, lds);
 }
 
 // end of an iteration: half-step stops, residual test, rho update, iteration cap; then the start
 // of the next one (breakdown check on rho, p-update coefficients)
 __global__ void k_bicg_final_post(int nb, const double* __restrict__ part, double* __restrict__ sc,
                                   int* __restrict__ st, int* __restrict__ status) {
-    __shared__ double lds[BLOCK / WAVE];
+    __shared__ double lds[4 * BLOCK / WAVE];
     if (st[0] != ST_RUNNING) return;
-    double rr[2], r0r[2];
-    for (int c = 0; c < 2; ++c) {
-        rr[c] = sum_partials(part + c, nb, 6, lds);
-        r0r[c] = sum_partials(part + 2 + c, nb, 6, lds);
-    }
+    double sums[4];
+    sum_partials_k<4>({part, part + 1, part + 2, part + 3}, {nb, nb, nb, nb}, 6, sums, lds);
+    const double rr[2] = {sums[0], sums[1]}, r0r[2] = {sums[2], sums[3]};
     if (threadIdx.x != 0) return;
     const int it = st[10], max_iter = st[11];
     for (int c = 0; c < 2; ++c)

@@ -349,4 +349,24 @@ int cfa_mesh_check(int n, int ne, const int* elem_v, const double* x, const doub
     return launch_status();
 }
 
+size_t cfa_sort_pairs_ws_bytes(int n) {
+    if (n <= 0) return 0;
+    size_t bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, bytes, static_cast<const unsigned*>(nullptr), static_cast<unsigned*>(nullptr),
+                                  static_cast<const int*>(nullptr), static_cast<int*>(nullptr), (size_t)n) != hipSuccess)
+        return 0;
+    return bytes;
+}
+
+int cfa_sort_pairs_u32(int n, const unsigned* keys_in, const int* vals_in, unsigned* keys_out, int* vals_out,
+                       void* ws, size_t ws_bytes, cfa_stream_t stream) {
+    if (n < 0 || (n > 0 && (!keys_in || !vals_in || !keys_out || !vals_out || !ws))) return CFA_E_ARG;
+    if (n == 0) return CFA_OK;
+    size_t need = cfa_sort_pairs_ws_bytes(n);
+    if (need == 0 || ws_bytes < need) return CFA_E_ARG;
+    const hipError_t e =
+        rocprim::radix_sort_pairs(ws, need, keys_in, keys_out, vals_in, vals_out, (size_t)n, 0, 32, as_stream(stream));
+    return e == hipSuccess ? launch_status() : (int)e;
+}
+
 }  // extern "C"

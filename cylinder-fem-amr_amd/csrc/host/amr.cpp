@@ -108,7 +108,7 @@ void AdaptiveMesh::extract(Mesh2D& mesh, std::vector<double>& h_u, std::vector<d
     }
     dbg_lap(D_KEYS);
     // (key, id) is a total order, so any correct sort gives the same permutation
-    std::sort(keyed.begin(), keyed.end());
+    if (!leaf_sorter || !leaf_sorter(keyed)) std::sort(keyed.begin(), keyed.end());
     for (long long i = 0; i < nl; ++i) leaf_ids_[i] = keyed[i].second;
     dbg_lap(D_SORT);
 

@@ -10,6 +10,7 @@
 
 #include <array>
 #include <atomic>
+#include <functional>
 #include <mutex>
 #include <unordered_map>
 
@@ -77,6 +78,11 @@ public:
                     const std::vector<double>& h_uo, const std::vector<double>& h_vo,
                     std::vector<double>& n_u, std::vector<double>& n_v, std::vector<double>& n_p,
                     std::vector<double>& n_uo, std::vector<double>& n_vo);
+    // Sorts extract's (Morton key, forest id) pairs ascending in place (ids arrive ascending, so a
+    // stable sort by key is the same order); when unset or returning false, extract sorts on the
+    // host. The driver sets a device sorter for synchronous remeshes only: with overlap the
+    // extraction runs on the worker thread, which does not use the GPU.
+    std::function<bool(std::vector<std::pair<uint32_t, int>>&)> leaf_sorter;
     void extract(Mesh2D& mesh, std::vector<double>& h_u, std::vector<double>& h_v,
                  std::vector<double>& h_p, std::vector<double>& h_uo, std::vector<double>& h_vo);
     AmrStats adapt(const std::vector<double>& eta, const AmrParams& prm);

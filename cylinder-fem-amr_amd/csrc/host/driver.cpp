@@ -168,6 +168,11 @@ int run_driver(int argc, char** argv) {
         }
     DeviceMeshBuffers g;
     g.sync_from_host(mesh, h_u, h_v, h_p, h_uo, h_vo);
+    // synchronous remeshes sort the leaves' Morton keys on the device (the overlapped ones run the
+    // extraction on the worker thread, which keeps to the host)
+    DeviceLeafSorter leaf_sorter;
+    if (opt.amr_overlap <= 0 && device_topology_default())
+        amr.leaf_sorter = [&leaf_sorter](std::vector<std::pair<uint32_t, int>>& keyed) { return leaf_sorter(keyed); };
     GpuLinearSolver solver;
     if (opt.init_amr > 0)
         for (int cycle = 0; cycle < opt.init_amr; ++cycle) {

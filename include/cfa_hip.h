@@ -405,6 +405,13 @@ int cfa_csr_transpose(int rows, int cols, int nnz, const int* rp, const int* ci,
  * cosine (the minimum angle is its arccosine).
  * ws: device workspace of cfa_topology_ws_bytes(n, ne) bytes, shared by both calls. */
 size_t cfa_topology_ws_bytes(int n_nodes, int n_elems);
+/* cfa_sort_pairs_u32: stable ascending sort of n (key, value) pairs by the 32-bit key (ties keep
+ * their input order); the Morton-curve ordering of the leaves in AdaptiveMesh::extract
+ * (src/cfa/amr.cuh:152-228, a std::sort of (key, id) on the host there) when the ids come in
+ * ascending order. ws: cfa_sort_pairs_ws_bytes(n) bytes. */
+size_t cfa_sort_pairs_ws_bytes(int n);
+int cfa_sort_pairs_u32(int n, const unsigned* keys_in, const int* vals_in, unsigned* keys_out, int* vals_out,
+                       void* ws, size_t ws_bytes, cfa_stream_t stream);
 int cfa_mesh_topology(int n, int ne, const int* elem_v, void* ws, size_t ws_bytes, int* inc_ptr, int* inc_elem,
                       int* row_ptr, int* col_ind, int* inc_pos, int* nnz_out, cfa_stream_t stream);
 int cfa_mesh_check(int n, int ne, const int* elem_v, const double* x, const double* y, const int* bc_tag,

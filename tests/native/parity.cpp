@@ -574,6 +574,22 @@ void case_topology(Fixture& f) {
         amr.adapt(eta, prm);
     }
     amr.extract(m, u, v, p, uo, vo);
+    {   // the Morton ordering of the leaves sorted on the device: the same mesh, numbering included
+        Mesh2D md;
+        Vec ud, vd, pd, uod, vod;
+        DeviceLeafSorter sorter;
+        amr.leaf_sorter = [&sorter](std::vector<std::pair<uint32_t, int>>& keyed) { return sorter(keyed); };
+        amr.extract(md, ud, vd, pd, uod, vod);
+        amr.leaf_sorter = nullptr;
+        std::vector<int> ev_h, ev_d;
+        for (const Triangle& t : m.elems) ev_h.insert(ev_h.end(), {t.v[0], t.v[1], t.v[2], t.gen});
+        for (const Triangle& t : md.elems) ev_d.insert(ev_d.end(), {t.v[0], t.v[1], t.v[2], t.gen});
+        std::printf("  extraction with the device leaf sort\n");
+        same_ints("elements (vertices, generation)", ev_d, ev_h);
+        same_bits("node coordinates", [&] { Vec c = md.x; c.insert(c.end(), md.y.begin(), md.y.end()); return c; }(),
+                  [&] { Vec c = m.x; c.insert(c.end(), m.y.begin(), m.y.end()); return c; }());
+        same_ints("boundary tags", md.bc_tag, m.bc_tag);
+    }
     topology_pair("locally refined mesh", m, true);
     Mesh2D flipped = m;
     std::swap(flipped.elems[flipped.elems.size() / 2].v[1], flipped.elems[flipped.elems.size() / 2].v[2]);

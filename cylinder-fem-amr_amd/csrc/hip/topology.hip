@@ -329,6 +329,27 @@ int cfa_mesh_topology(int n, int ne, const int* elem_v, void* ws, size_t ws_byte
     return launch_status();
 }
 
+int cfa_mesh_tables(int n, int ne, const int* elem_v, const int* row_ptr, const int* col_ind, void* ws,
+                    size_t ws_bytes, int* inc_ptr, int* inc_elem, int* inc_pos, cfa_stream_t stream) {
+    if (n <= 0 || ne <= 0 || !elem_v || !row_ptr || !col_ind || !ws || !inc_ptr || !inc_elem || !inc_pos)
+        return CFA_E_ARG;
+    const size_t need = cfa_topology_ws_bytes(n, ne);
+    if (need == 0 || ws_bytes < need) return CFA_E_ARG;
+    TopoWs w;
+    ws_layout(n, ne, static_cast<char*>(ws), &w);
+    const hipStream_t s = as_stream(stream);
+    const int total = 3 * ne;
+    size_t tb = w.tmp_bytes;
+    const hipError_t e = rocprim::radix_sort_pairs(w.tmp, tb, reinterpret_cast<const unsigned*>(elem_v), w.keys,
+                                                   rocprim::counting_iterator<int>(0), inc_elem, (size_t)total, 0,
+                                                   key_bits(n), s);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_node_starts, dim3(blocks(n + 1)), dim3(BLOCK), 0, s, n, total, w.keys, inc_ptr);
+    hipLaunchKernelGGL(k_inc_pos, dim3(blocks(total)), dim3(BLOCK), 0, s, total, elem_v, w.keys, inc_elem, row_ptr,
+                       col_ind, inc_pos);
+    return launch_status();
+}
+
 int cfa_mesh_check(int n, int ne, const int* elem_v, const double* x, const double* y, const int* bc_tag,
                    const int* row_ptr, const int* col_ind, int nnz, double h_channel, double l_channel, void* ws,
                    size_t ws_bytes, double* stats, cfa_stream_t stream) {

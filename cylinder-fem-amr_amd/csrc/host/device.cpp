@@ -139,6 +139,26 @@ void DeviceMeshBuffers::sync_from_host(const Mesh2D& mesh, const std::vector<dou
     upload_fields(mesh, h_u, h_v, h_p, h_uo, h_vo);
 }
 
+void DeviceMeshBuffers::sync_from_host_device_tables(const Mesh2D& mesh, const std::vector<double>& h_u,
+                                                     const std::vector<double>& h_v,
+                                                     const std::vector<double>& h_p,
+                                                     const std::vector<double>& h_uo,
+                                                     const std::vector<double>& h_vo) {
+    upload_elements(mesh);
+    nnz = mesh.nnz;
+    d_row_ptr.upload(mesh.csr_row_ptr);
+    d_col_ind.upload(mesh.csr_col_ind);
+    const int n = n_nodes, ne = n_elems;
+    d_inc_ptr.reserve(n + 1);
+    d_inc_elem.reserve(3 * (size_t)ne);
+    d_inc_pos.reserve(9 * (size_t)ne);
+    const size_t wsb = cfa_topology_ws_bytes(n, ne);
+    d_topo_ws_.reserve(std::max<size_t>(wsb, 1));
+    GPU_CHECK(cfa_mesh_tables(n, ne, d_elem_v, d_row_ptr, d_col_ind, d_topo_ws_, wsb, d_inc_ptr, d_inc_elem,
+                              d_inc_pos, solver_stream()));
+    upload_fields(mesh, h_u, h_v, h_p, h_uo, h_vo);
+}
+
 void DeviceMeshBuffers::upload_elements(const Mesh2D& mesh) {
     n_nodes = mesh.num_nodes;
     n_elems = mesh.num_elems;

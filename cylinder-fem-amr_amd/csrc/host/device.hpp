@@ -46,6 +46,11 @@ struct DeviceMeshBuffers {
     // and the cylinder edges are found on the host. valid = the checks passed (the host check
     // then prints what failed). Returns false, with nothing changed on the device side that
     // sync_from_host would not redo, when a row is too long for the device kernel.
+    // Upload of a mesh whose CSR graph the host built, with the incidence and position tables
+    // formed on the device (cfa_mesh_tables) instead of on the host and uploaded.
+    void sync_from_host_device_tables(const Mesh2D& mesh, const std::vector<double>& h_u,
+                                      const std::vector<double>& h_v, const std::vector<double>& h_p,
+                                      const std::vector<double>& h_uo, const std::vector<double>& h_vo);
     bool sync_from_host_device_topology(Mesh2D& mesh, const std::vector<double>& h_u,
                                         const std::vector<double>& h_v, const std::vector<double>& h_p,
                                         const std::vector<double>& h_uo, const std::vector<double>& h_vo,

@@ -431,8 +431,9 @@ int run_driver(int argc, char** argv) {
         auto finish_remesh = [&](const AmrStats& st, const MeshQuality& q, const std::vector<int>& old_vids,
                                  const std::vector<double>& old_x, const std::vector<double>& old_y,
                                  bool uploaded) {
-            if (!uploaded) {
-                g.sync_from_host(mesh, h_u, h_v, h_p, h_uo, h_vo);
+            if (!uploaded) {   // the graph from the host; the tables built on the device by default
+                if (device_topology_default()) g.sync_from_host_device_tables(mesh, h_u, h_v, h_p, h_uo, h_vo);
+                else g.sync_from_host(mesh, h_u, h_v, h_p, h_uo, h_vo);
                 dev_sync();
                 remesh_prof.upload += lap();
             }

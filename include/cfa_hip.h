@@ -414,6 +414,10 @@ int cfa_sort_pairs_u32(int n, const unsigned* keys_in, const int* vals_in, unsig
                        void* ws, size_t ws_bytes, cfa_stream_t stream);
 int cfa_mesh_topology(int n, int ne, const int* elem_v, void* ws, size_t ws_bytes, int* inc_ptr, int* inc_elem,
                       int* row_ptr, int* col_ind, int* inc_pos, int* nnz_out, cfa_stream_t stream);
+/* cfa_mesh_incidence: the incidence and position tables of cfa_mesh_topology for a graph the
+ * caller already has (row_ptr, col_ind on the device). */
+int cfa_mesh_tables(int n, int ne, const int* elem_v, const int* row_ptr, const int* col_ind, void* ws,
+                    size_t ws_bytes, int* inc_ptr, int* inc_elem, int* inc_pos, cfa_stream_t stream);
 int cfa_mesh_check(int n, int ne, const int* elem_v, const double* x, const double* y, const int* bc_tag,
                    const int* row_ptr, const int* col_ind, int nnz, double h_channel, double l_channel, void* ws,
                    size_t ws_bytes, double* stats, cfa_stream_t stream);

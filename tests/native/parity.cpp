@@ -550,6 +550,11 @@ void topology_pair(const char* name, const Mesh2D& m, bool expect_valid) {
     same_ints("incidence pointers", down_int(g.d_inc_ptr, (int)ip.size()), ip);
     same_ints("incidence elements", down_int(g.d_inc_elem, (int)ie.size()), ie);
     same_ints("owner-computes positions", down_int(g.d_inc_pos, (int)ipos.size()), ipos);
+    // the tables alone, from the host's graph (the upload of an overlapped remesh)
+    DeviceMeshBuffers g2;
+    g2.sync_from_host_device_tables(h, z, z, z, z, z);
+    same_ints("tables from the host graph: incidence", down_int(g2.d_inc_elem, (int)ie.size()), ie);
+    same_ints("tables from the host graph: positions", down_int(g2.d_inc_pos, (int)ipos.size()), ipos);
 }
 
 void case_topology(Fixture& f) {

@@ -144,6 +144,10 @@ void DeviceMeshBuffers::sync_from_host_device_tables(const Mesh2D& mesh, const s
                                                      const std::vector<double>& h_p,
                                                      const std::vector<double>& h_uo,
                                                      const std::vector<double>& h_vo) {
+    if (mesh.num_nodes <= 0 || mesh.num_elems <= 0) {   // nothing for the device kernels to sort
+        sync_from_host(mesh, h_u, h_v, h_p, h_uo, h_vo);
+        return;
+    }
     upload_elements(mesh);
     nnz = mesh.nnz;
     d_row_ptr.upload(mesh.csr_row_ptr);
@@ -180,6 +184,7 @@ bool DeviceMeshBuffers::sync_from_host_device_topology(Mesh2D& mesh, const std::
                                                        bool& valid) {
     mesh.num_nodes = (int)mesh.x.size();
     mesh.num_elems = (int)mesh.elems.size();
+    if (mesh.num_nodes <= 0 || mesh.num_elems <= 0) return false;
     upload_elements(mesh);
     const int n = n_nodes, ne = n_elems;
     const cfa_stream_t s = solver_stream();

@@ -14,6 +14,18 @@ constexpr int BLOCK = 256;   // default workgroup: 4 waves
 // Boundary tags (src/cfa/common.cuh:68-74)
 constexpr int BC_INTERIOR = 0, BC_INLET = 1, BC_OUTLET = 2, BC_WALL = 3, BC_CYLINDER = 4;
 
+// Outlet condition of a mesh (cfa_mesh_t fields; src/cfa/common.cuh:76-89)
+struct OutletMode {
+    int dirichlet;   // 0: p = 0 on outlet nodes; else inflow parabola on the outlet, dp pinned
+    int pin;         // the node with dp = 0 when dirichlet
+    double scale;    // outlet parabola factor
+};
+
+// Pressure-increment Dirichlet node (src/cfa/common.cuh:87-89 pressure_dirichlet)
+__device__ __forceinline__ bool pressure_dirichlet(int i, int tag, const OutletMode& om) {
+    return om.dirichlet ? i == om.pin : tag == BC_OUTLET;
+}
+
 inline hipStream_t as_stream(cfa_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline int launch_status() {

@@ -104,14 +104,14 @@ k_momentum_rows(int n, const int* __restrict__ ev, const double* __restrict__ x,
                 const double* __restrict__ v_prev, const double* __restrict__ p, double nu, double rho, double a0,
                 double a1, double a2, double c1, double c2, double tau_time, double skew,
                 const double* __restrict__ fx, const double* __restrict__ fy, double u_max, double mms_time,
-                double* __restrict__ A, double* __restrict__ rhs_u, double* __restrict__ rhs_v,
+                OutletMode om, double* __restrict__ A, double* __restrict__ rhs_u, double* __restrict__ rhs_v,
                 double* __restrict__ mass, double* __restrict__ u, double* __restrict__ v) {
     __shared__ double acc[ROW_SLOTS][ROWS_PER_BLOCK];
     const int lane = threadIdx.x;
     const int i = blockIdx.x * blockDim.x + lane;
     if (i >= n) return;
     const int t = tag[i];
-    const bool dirichlet = (t == BC_INLET || t == BC_WALL || t == BC_CYLINDER);
+    const bool dirichlet = (t == BC_INLET || t == BC_WALL || t == BC_CYLINDER || (t == BC_OUTLET && om.dirichlet));
     const int rs = rp[i], re = rp[i + 1];
     const bool in_lds = re - rs <= ROW_SLOTS;
     if (!dirichlet)
@@ -188,9 +188,9 @@ k_momentum_rows(int n, const int* __restrict__ ev, const double* __restrict__ x,
             const Mms r = mms_exact(x[i], y[i], mms_time);
             ub = r.u;
             vb = r.v;
-        } else if (t == BC_INLET) {
+        } else if (t == BC_INLET || t == BC_OUTLET) {   // the outlet only with om.dirichlet
             const double H = 0.41, yi = y[i];
-            ub = 4.0 * u_max * yi * (H - yi) / (H * H);
+            ub = 4.0 * u_max * yi * (H - yi) / (H * H) * (t == BC_OUTLET ? om.scale : 1.0);
         }
         for (int k = rs; k < re; ++k) A[k] = (ci[k] == i) ? 1.0 : 0.0;
         rhs_u[i] = ub;  rhs_v[i] = vb;
@@ -281,14 +281,14 @@ k_pressure_rows(int n, int nnz, const int* __restrict__ ev, const double* __rest
                 const double* __restrict__ p, const double* __restrict__ gpx, const double* __restrict__ gpy,
                 const double* __restrict__ mass, const double* __restrict__ s_el, double dt, double rho,
                 double stab, double* __restrict__ K, double* __restrict__ rhs, double* __restrict__ div_num,
-                double* __restrict__ dp, double* __restrict__ oss_val, double* __restrict__ inv_m) {
+                double* __restrict__ dp, double* __restrict__ oss_val, double* __restrict__ inv_m, OutletMode om) {
     // the row's values (K, Bx, By, BTx, BTy) accumulate in this thread's LDS slots, from 0 in
     // ascending element order as before; rows longer than ROW_SLOTS accumulate in place
     __shared__ double acc[5][ROW_SLOTS][ROWS_PER_BLOCK];
     const int t = threadIdx.x;
     const int i = blockIdx.x * blockDim.x + t;
     if (i >= n) return;
-    const bool dirichlet = tag[i] == BC_OUTLET;
+    const bool dirichlet = pressure_dirichlet(i, tag[i], om);
     const int rs = rp[i], re = rp[i + 1];
     const bool in_lds = re - rs <= ROW_SLOTS;
     double* Bx = oss_val;
@@ -377,11 +377,12 @@ k_pressure_rows(int n, int nnz, const int* __restrict__ ev, const double* __rest
     if (oss_val) inv_m[i] = 1.0 / fmax(mass[i], 1.0e-14);
     if (dirichlet) {
         for (int k = rs; k < re; ++k) K[k] = (ci[k] == i) ? 1.0 : 0.0;
-        rhs[i] = 0.0;
+        // a pinned row keeps its assembled value for the mean removal, which then zeroes it
+        rhs[i] = om.dirichlet ? r_acc : 0.0;
         dp[i] = 0.0;
     } else {
         for (int k = rs; k < re; ++k)
-            if (ci[k] != i && tag[ci[k]] == BC_OUTLET) K[k] = 0.0;
+            if (ci[k] != i && pressure_dirichlet(ci[k], tag[ci[k]], om)) K[k] = 0.0;
         rhs[i] = r_acc;
     }
 }
@@ -444,14 +445,14 @@ __global__ void k_oss_B(int n, int nnz, const int* __restrict__ rp, const int* _
 
 template <int TPR>
 __global__ void k_oss_BT(int n, int nnz, const int* __restrict__ rp, const int* __restrict__ ci,
-                         const double* __restrict__ oss, const int* __restrict__ tag,
+                         const double* __restrict__ oss, const int* __restrict__ tag, OutletMode om,
                          const double* __restrict__ gx, const double* __restrict__ gy, double* __restrict__ out) {
     const int gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = gtid / TPR, lane = gtid % TPR;
     const double* BTx = oss + 2 * (size_t)nnz;
     const double* BTy = oss + 3 * (size_t)nnz;
     double s = 0.0;
-    const bool active = i < n && tag[i] != BC_OUTLET;
+    const bool active = i < n && !pressure_dirichlet(i, tag[i], om);
     if (active)
         for (int a = rp[i] + lane; a < rp[i + 1]; a += TPR) {
             const int k = ci[a];
@@ -463,13 +464,13 @@ __global__ void k_oss_BT(int n, int nnz, const int* __restrict__ rp, const int* 
 
 // Matrix-free OSS subtraction (src/cfa/fem_kernels.cuh:763-797), row gather
 __global__ void k_oss_matrix_free_rows(int n, const int* __restrict__ ev, const double* __restrict__ x,
-                                       const double* __restrict__ y, const int* __restrict__ tag,
+                                       const double* __restrict__ y, const int* __restrict__ tag, OutletMode om,
                                        const int* __restrict__ inc_ptr, const int* __restrict__ inc_elem,
                                        const double* __restrict__ s_el, const double* __restrict__ gx,
                                        const double* __restrict__ gy, const double* __restrict__ mass,
                                        double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || tag[i] == BC_OUTLET) return;
+    if (i >= n || pressure_dirichlet(i, tag[i], om)) return;
     double acc = 0.0;
     for (int s = inc_ptr[i]; s < inc_ptr[i + 1]; ++s) {
         const int ea = inc_elem[s];
@@ -542,7 +543,8 @@ __global__ void k_grad_dp_vort_rows(int n, const int* __restrict__ ev, const dou
 }
 
 // src/cfa/fem_kernels.cuh:592-622 (default outlet: outlet velocities are corrected)
-__global__ void k_correction(int n, const int* __restrict__ tag, double dt, double rho, double p_update,
+__global__ void k_correction(int n, const int* __restrict__ tag, int outlet_dirichlet, double dt, double rho,
+                             double p_update,
                              double rot, const double* __restrict__ div_num, const double* __restrict__ mass,
                              const double* __restrict__ gx, const double* __restrict__ gy,
                              const double* __restrict__ vn, const double* __restrict__ dp, double* __restrict__ u,
@@ -553,10 +555,55 @@ __global__ void k_correction(int n, const int* __restrict__ tag, double dt, doub
     vort[i] = vn[i] / m;
     p[i] += p_update * (dp[i] - rot * div_num[i] / m);
     const int t = tag[i];
-    if (t == BC_INTERIOR || t == BC_OUTLET) {
+    if (t == BC_INTERIOR || (t == BC_OUTLET && !outlet_dirichlet)) {
         u[i] -= (dt / rho) * (gx[i] / m);
         v[i] -= (dt / rho) * (gy[i] / m);
     }
+}
+
+// ---- pure-Neumann PPE (outlet Dirichlet) --------------------------------------------------------
+constexpr int MEAN_BLOCK = 256;
+
+// Block sums of f and m over MEAN_BLOCK entries with the in-block tree of
+// src/cfa/fem_kernels.cuh:493-503 k_sum (s[t] += s[t + w], w = 128, 64, ..., 1).
+// part[b] = sum of f over block b, part[nb + b] = sum of m.
+__global__ void __launch_bounds__(MEAN_BLOCK)
+k_mean_block_sums(int n, const double* __restrict__ f, const double* __restrict__ m, double* __restrict__ part) {
+    __shared__ double s[2][MEAN_BLOCK];
+    const int t = threadIdx.x;
+    const int i = blockIdx.x * MEAN_BLOCK + t;
+    s[0][t] = i < n ? f[i] : 0.0;
+    s[1][t] = i < n ? m[i] : 0.0;
+    __syncthreads();
+    for (int w = MEAN_BLOCK / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            s[0][t] += s[0][t + w];
+            s[1][t] += s[1][t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        part[blockIdx.x] = s[0][0];
+        part[gridDim.x + blockIdx.x] = s[1][0];
+    }
+}
+
+// sums[q] = 0 + part[q nb] + part[q nb + 1] + ... in ascending block order (thread q)
+__global__ void k_mean_final(int nb, const double* __restrict__ part, double* __restrict__ sums) {
+    const int q = threadIdx.x;
+    if (q >= 2) return;
+    double acc = 0.0;
+    for (int b = 0; b < nb; ++b) acc += part[(size_t)q * nb + b];
+    sums[q] = acc;
+}
+
+// f_i -= (sum f / sum m) m_i (src/cfa/fem_kernels.cuh:506-510 k_subtract_mean), then the pinned
+// row's rhs = 0 (:475-479 k_apply_bc_pressure)
+__global__ void k_subtract_mean(int n, const double* __restrict__ sums, const double* __restrict__ m, int pin,
+                                double* __restrict__ f) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    f[i] = i == pin ? 0.0 : f[i] - sums[0] / sums[1] * m[i];
 }
 
 // ---- forces ------------------------------------------------------------------------------------
@@ -811,7 +858,12 @@ __global__ void k_sum_partials6(int nb, const double* __restrict__ ws, double* _
 
 inline bool mesh_ok(const cfa_mesh_t* m) {
     return m && m->n_nodes >= 0 && m->n_elems >= 0 && m->elem_v && m->x && m->y && m->bc_tag &&
-           m->row_ptr && m->col_ind && m->inc_ptr && m->inc_elem && m->inc_pos;
+           m->row_ptr && m->col_ind && m->inc_ptr && m->inc_elem && m->inc_pos &&
+           (!m->outlet_dirichlet || m->n_nodes == 0 || (m->p_pin >= 0 && m->p_pin < m->n_nodes));
+}
+
+inline OutletMode outlet_of(const cfa_mesh_t* m) {
+    return OutletMode{m->outlet_dirichlet, m->p_pin, m->outlet_scale};
 }
 
 }  // namespace
@@ -832,7 +884,7 @@ int cfa_assemble_momentum(const cfa_mesh_t* m, const double* u_n, const double* 
                        as_stream(stream),
                        m->n_nodes, m->elem_v, m->x, m->y, m->bc_tag, m->row_ptr, m->col_ind, m->inc_ptr,
                        m->inc_elem, m->inc_pos, u_n, v_n, u_prev, v_prev, p, nu, rho, a0, a1, a2, c1, c2,
-                       tau_time, skew, fx, fy, u_max, mms_time, A_val, rhs_u, rhs_v, mass, u, v);
+                       tau_time, skew, fx, fy, u_max, mms_time, outlet_of(m), A_val, rhs_u, rhs_v, mass, u, v);
     return launch_status();
 }
 
@@ -883,7 +935,24 @@ int cfa_assemble_pressure(const cfa_mesh_t* m, const double* u_star, const doubl
                        as_stream(stream),
                        m->n_nodes, m->nnz, m->elem_v, m->x, m->y, m->bc_tag, m->row_ptr, m->col_ind,
                        m->inc_ptr, m->inc_elem, m->inc_pos, u_star, v_star, p, gpx, gpy, mass, elem_s, dt,
-                       rho, stab, Kp, rhs, div_num, dp, oss_val, inv_m);
+                       rho, stab, Kp, rhs, div_num, dp, oss_val, inv_m, outlet_of(m));
+    return launch_status();
+}
+
+size_t cfa_pressure_mean_ws_doubles(int n) {
+    return 2 * (size_t)blocks_for(n > 0 ? n : 1, MEAN_BLOCK) + 2;
+}
+
+int cfa_pressure_remove_mean(const cfa_mesh_t* m, const double* mass, double* rhs, double* ws,
+                             cfa_stream_t stream) {
+    if (!mesh_ok(m) || !m->outlet_dirichlet || !mass || !rhs || !ws) return CFA_E_ARG;
+    const int n = m->n_nodes;
+    if (n == 0) return CFA_OK;
+    const int nb = blocks_for(n, MEAN_BLOCK);
+    hipLaunchKernelGGL(k_mean_block_sums, dim3(nb), dim3(MEAN_BLOCK), 0, as_stream(stream), n, rhs, mass, ws);
+    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(WAVE), 0, as_stream(stream), nb, ws, ws + 2 * (size_t)nb);
+    hipLaunchKernelGGL(k_subtract_mean, dim3(blocks_for(n)), dim3(BLOCK), 0, as_stream(stream), n,
+                       ws + 2 * (size_t)nb, mass, m->p_pin, rhs);
     return launch_status();
 }
 
@@ -907,7 +976,7 @@ int cfa_oss_apply(const cfa_mesh_t* m, const double* oss_val, const double* inv_
     hipLaunchKernelGGL(k_oss_B<TPR>, dim3(blocks), dim3(BLOCK), 0, as_stream(stream), n, m->nnz, m->row_ptr,
                        m->col_ind, oss_val, inv_m, in, gx, gy);
     hipLaunchKernelGGL(k_oss_BT<TPR>, dim3(blocks), dim3(BLOCK), 0, as_stream(stream), n, m->nnz, m->row_ptr,
-                       m->col_ind, oss_val, m->bc_tag, gx, gy, out);
+                       m->col_ind, oss_val, m->bc_tag, outlet_of(m), gx, gy, out);
     return launch_status();
 }
 
@@ -920,7 +989,8 @@ int cfa_oss_apply_matrix_free(const cfa_mesh_t* m, const double* elem_s, const d
     hipLaunchKernelGGL(k_scalar_gradient_rows, dim3(blocks_for(n)), dim3(BLOCK), 0, as_stream(stream), n,
                        m->elem_v, m->x, m->y, m->inc_ptr, m->inc_elem, in, elem_s, gx, gy);
     hipLaunchKernelGGL(k_oss_matrix_free_rows, dim3(blocks_for(n)), dim3(BLOCK), 0, as_stream(stream), n,
-                       m->elem_v, m->x, m->y, m->bc_tag, m->inc_ptr, m->inc_elem, elem_s, gx, gy, mass, out);
+                       m->elem_v, m->x, m->y, m->bc_tag, outlet_of(m), m->inc_ptr, m->inc_elem, elem_s, gx, gy, mass,
+                       out);
     return launch_status();
 }
 
@@ -943,8 +1013,9 @@ int cfa_correction(const cfa_mesh_t* m, double dt, double rho, double p_update, 
     if (n == 0) return CFA_OK;
     hipLaunchKernelGGL(k_grad_dp_vort_rows, dim3(blocks_for(n)), dim3(BLOCK), 0, as_stream(stream), n,
                        m->elem_v, m->x, m->y, m->inc_ptr, m->inc_elem, dp, u, v, gx, gy, vort_num);
-    hipLaunchKernelGGL(k_correction, dim3(blocks_for(n)), dim3(BLOCK), 0, as_stream(stream), n, m->bc_tag, dt,
-                       rho, p_update, rot, div_num, mass, gx, gy, vort_num, dp, u, v, p, vort);
+    hipLaunchKernelGGL(k_correction, dim3(blocks_for(n)), dim3(BLOCK), 0, as_stream(stream), n, m->bc_tag,
+                       m->outlet_dirichlet, dt, rho, p_update, rot, div_num, mass, gx, gy, vort_num, dp, u, v, p,
+                       vort);
     return launch_status();
 }
 

@@ -402,9 +402,9 @@ k_apply_K_B(int n, int nnz, const int* __restrict__ rp, const int* __restrict__ 
 template <int TPR>
 __global__ void __launch_bounds__(BLOCK)
 k_apply_BT_dot(int n, int nnz, const int* __restrict__ rp, const int* __restrict__ ci,
-               const double* __restrict__ oss, const int* __restrict__ tag, const double* __restrict__ gx,
-               const double* __restrict__ gy, const double* __restrict__ p, double* __restrict__ q,
-               double* __restrict__ part, const int* __restrict__ st) {
+               const double* __restrict__ oss, const int* __restrict__ tag, OutletMode om,
+               const double* __restrict__ gx, const double* __restrict__ gy, const double* __restrict__ p,
+               double* __restrict__ q, double* __restrict__ part, const int* __restrict__ st) {
     __shared__ double lds[BLOCK / WAVE];
     if (halted(st)) return;
     const double* BTx = oss + 2 * (size_t)nnz;
@@ -413,7 +413,7 @@ k_apply_BT_dot(int n, int nnz, const int* __restrict__ rp, const int* __restrict
     const int lane = threadIdx.x % TPR;
     double contrib = 0.0;
     for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < n; row += groups) {
-        const bool active = tag[row] != BC_OUTLET;
+        const bool active = !pressure_dirichlet(row, tag[row], om);
         double s = 0.0;
         if (active)
             for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
@@ -451,6 +451,7 @@ struct PcgArgs {
     int n, nnz, max_iter;
     const int* rp; const int* ci; const double* val;
     const double* oss; const double* inv_m; const int* tag; double* gx; double* gy;
+    OutletMode om;
     double* x; double* r; double* z; double* q; double* p[2];
     double* prz[2]; double* ppq; double* prr;
     int nb_rz, nb_rr;
@@ -1703,14 +1704,19 @@ void launch_apply_K_B(int tpr, int grid, hipStream_t s, int n, int nnz, const in
                            st);
 }
 void launch_apply_BT_dot(int tpr, int grid, hipStream_t s, int n, int nnz, const int* rp, const int* ci,
-                         const double* oss, const int* tag, const double* gx, const double* gy, const double* p,
-                         double* q, double* part, const int* st) {
+                         const double* oss, const int* tag, OutletMode om, const double* gx, const double* gy,
+                         const double* p, double* q, double* part, const int* st) {
     if (tpr == 2)
-        hipLaunchKernelGGL(k_apply_BT_dot<2>, dim3(grid), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss, tag, gx, gy, p, q,
-                           part, st);
+        hipLaunchKernelGGL(k_apply_BT_dot<2>, dim3(grid), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss, tag, om, gx, gy, p,
+                           q, part, st);
     else
-        hipLaunchKernelGGL(k_apply_BT_dot<4>, dim3(grid), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss, tag, gx, gy, p, q,
-                           part, st);
+        hipLaunchKernelGGL(k_apply_BT_dot<4>, dim3(grid), dim3(BLOCK), 0, s, n, nnz, rp, ci, oss, tag, om, gx, gy, p,
+                           q, part, st);
+}
+
+// pressure-Dirichlet rows of the operator: node p_pin with the outlet Dirichlet condition, else outlet nodes
+OutletMode outlet_of(const cfa_oss_op_t* oss) {
+    return OutletMode{oss->outlet_dirichlet, oss->p_pin, 0.0};
 }
 
 int pcg_fused(int n, const int* rp, const int* ci, const double* val, const double* b, double* x, int max_iter,
@@ -1735,6 +1741,7 @@ int pcg_fused(int n, const int* rp, const int* ci, const double* val, const doub
     CFA_TRY(hipStreamSynchronize(s));
     if (oss) {
         a.oss = oss->oss_val; a.inv_m = oss->inv_m; a.tag = oss->bc_tag; a.gx = oss->gx; a.gy = oss->gy;
+        a.om = outlet_of(oss);
     }
     const int TPR = operator_lanes(n, a.nnz);
     const double init_sc[2] = {tol, -1.0};   // S_TOL, and S_REL "not computed"
@@ -1754,7 +1761,7 @@ int pcg_fused(int n, const int* rp, const int* ci, const double* val, const doub
     if (!amg) hipLaunchKernelGGL(k_inv_diag, dim3(blocks), dim3(BLOCK), 0, s, n, rp, ci, val, 1.0e-14, inv_diag);
     if (oss) {
         launch_apply_K_B(TPR, nb_op, s, n, a.nnz, rp, ci, val, a.oss, a.inv_m, x, a.q, a.gx, a.gy, st_r);
-        launch_apply_BT_dot(TPR, nb_op, s, n, a.nnz, rp, ci, a.oss, a.tag, a.gx, a.gy, a.q, a.q, scratch, st_r);
+        launch_apply_BT_dot(TPR, nb_op, s, n, a.nnz, rp, ci, a.oss, a.tag, a.om, a.gx, a.gy, a.q, a.q, scratch, st_r);
     } else {
         csr_op_launch<OP_SPMV, double>(TPR, n, rp, ci, val, x, nullptr, nullptr, 0.0, a.q, st_r, s);
     }
@@ -1782,7 +1789,8 @@ int pcg_fused(int n, const int* rp, const int* ci, const double* val, const doub
     auto iteration = [&](int h, hipStream_t q_s) {
         launch_fp_operator(TPR, oss != nullptr, nb_op, q_s, a, h);
         if (oss)
-            launch_apply_BT_dot(TPR, nb_op, q_s, n, a.nnz, rp, ci, a.oss, a.tag, a.gx, a.gy, a.p[h], a.q, a.ppq, st_r);
+            launch_apply_BT_dot(TPR, nb_op, q_s, n, a.nnz, rp, ci, a.oss, a.tag, a.om, a.gx, a.gy, a.p[h], a.q, a.ppq,
+                                st_r);
         hipLaunchKernelGGL(k_fp_update, dim3(nb), dim3(BLOCK), 0, q_s, a, h, nb_op);
         (void)apply_prec(1 - h, true, q_s);
     };
@@ -1855,6 +1863,7 @@ int cfa_pcg_operator(int n, const int* rp, const int* ci, const double* val, con
                      const double* in, double* out, double* part, int stage, cfa_stream_t stream) {
     if (n < 0 || !rp || !ci || !val || !oss || !in || !out || !part || stage < 0 || stage > 2) return CFA_E_ARG;
     if (!oss->oss_val || !oss->inv_m || !oss->bc_tag || !oss->gx || !oss->gy) return CFA_E_ARG;
+    if (oss->outlet_dirichlet && n > 0 && (oss->p_pin < 0 || oss->p_pin >= n)) return CFA_E_ARG;
     if (n == 0) return CFA_OK;
     hipStream_t s = S(stream);
     int nnz = 0;
@@ -1866,8 +1875,8 @@ int cfa_pcg_operator(int n, const int* rp, const int* ci, const double* val, con
         launch_apply_K_B(tpr, blocks_tpr, s, n, nnz, rp, ci, val, oss->oss_val, oss->inv_m, in, out, oss->gx, oss->gy,
                          nullptr);
     if (stage != 1)
-        launch_apply_BT_dot(tpr, blocks_tpr, s, n, nnz, rp, ci, oss->oss_val, oss->bc_tag, oss->gx, oss->gy, in, out,
-                            part, nullptr);
+        launch_apply_BT_dot(tpr, blocks_tpr, s, n, nnz, rp, ci, oss->oss_val, oss->bc_tag, outlet_of(oss), oss->gx,
+                            oss->gy, in, out, part, nullptr);
     return launch_status();
 }
 
@@ -1878,6 +1887,7 @@ int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double
     if (n < 0 || !rp || !ci || !val || !b || !x || !ws || !result || ws->cap < n || !ws->status_host || max_iter < 0)
         return CFA_E_ARG;
     if (oss && (!oss->oss_val || !oss->inv_m || !oss->bc_tag || !oss->gx || !oss->gy)) return CFA_E_ARG;
+    if (oss && oss->outlet_dirichlet && n > 0 && (oss->p_pin < 0 || oss->p_pin >= n)) return CFA_E_ARG;
     if (amg && !amg_valid(amg)) return CFA_E_ARG;
     hipStream_t s = S(stream);
     result->iters[0] = 0;
@@ -1923,7 +1933,7 @@ int cfa_pcg(int n, const int* rp, const int* ci, const double* val, const double
     auto finish_op_dot = [&](const double* pv, double* qv, hipStream_t q_s) {
         if (oss) {
             hipLaunchKernelGGL(k_apply_BT_dot<TPR>, dim3(blocks_tpr), dim3(BLOCK), 0, q_s, n, nnz, rp, ci,
-                               oss->oss_val, oss->bc_tag, oss->gx, oss->gy, pv, qv, part, st);
+                               oss->oss_val, oss->bc_tag, outlet_of(oss), oss->gx, oss->gy, pv, qv, part, st);
             return blocks_tpr;
         }
         hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(BLOCK), 0, q_s, n, pv, qv, part, st);

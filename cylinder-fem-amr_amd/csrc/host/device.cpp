@@ -2,6 +2,9 @@
 #include "device.hpp"
 
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <iomanip>
 #include <iostream>
 
@@ -120,8 +123,70 @@ cfa_oss_op_t DeviceMeshBuffers::oss_op() const {
     op.bc_tag = d_bc_tag;
     op.gx = d_oss_gx;
     op.gy = d_oss_gy;
+    op.outlet_dirichlet = outlet_dirichlet ? 1 : 0;
+    op.p_pin = p_pin;
     return op;
 }
+
+void DeviceMeshBuffers::pressure_outlet_bc(const double* mass, double* rhs, cfa_stream_t s) {
+    if (!outlet_dirichlet || n_nodes == 0) return;
+    d_mean_ws_.reserve(cfa_pressure_mean_ws_doubles(n_nodes), 0);
+    GPU_CHECK(cfa_pressure_remove_mean(&view_, mass, rhs, d_mean_ws_, s));
+}
+
+namespace {
+
+// src/cfa/device.cuh:153-172: the pinned node (the outlet node nearest mid-height, the first in
+// index order on ties) and the outlet parabola factor, inlet over outlet of the trapezoidal flux
+// of the unit parabola y (H - y) through the boundary's nodes and the two corners.
+void outlet_pin_and_scale(const Mesh2D& mesh, int& pin, double& scale) {
+    const double H = Sim::H_CHANNEL;
+    const int n = (int)mesh.x.size();
+    pin = -1;
+    for (int i = 0; i < n; ++i)
+        if (mesh.bc_tag[i] == BC_OUTLET &&
+            (pin < 0 || std::fabs(mesh.y[i] - 0.5 * H) < std::fabs(mesh.y[pin] - 0.5 * H)))
+            pin = i;
+    if (pin < 0) {
+        std::fprintf(stderr, "--outlet dirichlet: no outlet node\n");
+        std::exit(EXIT_FAILURE);
+    }
+    auto flux = [&](int tag) {
+        std::vector<double> ys = {0.0, H};
+        for (int i = 0; i < n; ++i)
+            if (mesh.bc_tag[i] == tag) ys.push_back(mesh.y[i]);
+        std::sort(ys.begin(), ys.end());
+        auto prof = [H](double y) { return y * (H - y); };
+        double q = 0.0;
+        for (size_t k = 1; k < ys.size(); ++k) q += 0.5 * (ys[k] - ys[k - 1]) * (prof(ys[k]) + prof(ys[k - 1]));
+        return q;
+    };
+    scale = flux(BC_INLET) / flux(BC_OUTLET);
+}
+
+// src/cfa/device.cuh:179-199: outlet values of a transferred field scaled by its own inlet over
+// outlet trapezoidal flux (corners at 0), when the outlet flux is not negligible
+std::vector<double> flux_balanced(const Mesh2D& mesh, const std::vector<double>& f) {
+    const int n = (int)mesh.x.size();
+    std::vector<double> g = f;
+    auto flux = [&](int tag) {
+        std::vector<std::pair<double, double>> pts = {{0.0, 0.0}, {Sim::H_CHANNEL, 0.0}};
+        for (int i = 0; i < n; ++i)
+            if (mesh.bc_tag[i] == tag) pts.push_back({mesh.y[i], f[i]});
+        std::sort(pts.begin(), pts.end());
+        double q = 0.0;
+        for (size_t k = 1; k < pts.size(); ++k)
+            q += 0.5 * (pts[k].first - pts[k - 1].first) * (pts[k].second + pts[k - 1].second);
+        return q;
+    };
+    const double qi = flux(BC_INLET), qo = flux(BC_OUTLET);
+    if (std::fabs(qo) > 1.0e-14)
+        for (int i = 0; i < n; ++i)
+            if (mesh.bc_tag[i] == BC_OUTLET) g[i] *= qi / qo;
+    return g;
+}
+
+}  // namespace
 
 // src/cfa/device.cuh:131-228 (default outlet)
 void DeviceMeshBuffers::sync_from_host(const Mesh2D& mesh, const std::vector<double>& h_u,
@@ -258,10 +323,18 @@ void DeviceMeshBuffers::upload_fields(const Mesh2D& mesh, const std::vector<doub
     n_layer = (int)layer.size();
     d_layer.upload(layer);
 
-    d_u.upload(h_u);
+    if (outlet_dirichlet && n > 0) {
+        outlet_pin_and_scale(mesh, p_pin, outlet_scale);
+        d_u.upload(flux_balanced(mesh, h_u));
+        d_u_old.upload(flux_balanced(mesh, h_uo));
+    } else {
+        p_pin = -1;
+        outlet_scale = 1.0;
+        d_u.upload(h_u);
+        d_u_old.upload(h_uo);
+    }
     d_v.upload(h_v);
     d_p.upload(h_p);
-    d_u_old.upload(h_uo);
     d_v_old.upload(h_vo);
     dev_copy(d_u_prev, d_u_old, n);
     dev_copy(d_v_prev, d_v_old, n);
@@ -292,6 +365,9 @@ void DeviceMeshBuffers::upload_fields(const Mesh2D& mesh, const std::vector<doub
     view_.inc_ptr = d_inc_ptr;
     view_.inc_elem = d_inc_elem;
     view_.inc_pos = d_inc_pos;
+    view_.outlet_dirichlet = outlet_dirichlet && n > 0 ? 1 : 0;
+    view_.p_pin = p_pin;
+    view_.outlet_scale = outlet_scale;
 }
 
 void DeviceMeshBuffers::copy_solution_to_host(std::vector<double>& h_u, std::vector<double>& h_v,
@@ -351,6 +427,7 @@ int project_scheme_consistent(DeviceMeshBuffers& g, GpuLinearSolver& solver, dou
         GPU_CHECK(cfa_assemble_pressure(m, u, v, g.d_p, g.d_grad_dp_x, g.d_grad_dp_y, g.d_mass_lumped, g.d_elem_s,
                                         dt_proj, Sim::RHO, stab, g.d_Kp_val, g.d_rhs_p, g.d_div_num, g.d_dp,
                                         oss ? g.d_oss_val.ptr : nullptr, g.d_inv_m, s));
+        g.pressure_outlet_bc(g.d_mass_lumped, g.d_rhs_p, s);
         if (amg && f == first) {
             if (build_amg) build_amg();
             else amg->setup(g.n_nodes, g.nnz, g.d_row_ptr, g.d_col_ind, g.d_Kp_val);
@@ -383,6 +460,7 @@ int project_divergence_free(DeviceMeshBuffers& g, GpuLinearSolver& solver, doubl
     GPU_CHECK(cfa_lumped_mass(m, g.d_mass_lumped, s));
     GPU_CHECK(cfa_assemble_pressure(m, u, v, g.d_p, nullptr, nullptr, g.d_mass_lumped, nullptr, dt, Sim::RHO, 0.0,
                                     g.d_Kp_val, g.d_rhs_p, g.d_div_num, g.d_dp, nullptr, nullptr, s));
+    g.pressure_outlet_bc(g.d_mass_lumped, g.d_rhs_p, s);
     int iters;
     if (amg) {
         if (setup_amg) amg->setup(g.n_nodes, g.nnz, g.d_row_ptr, g.d_col_ind, g.d_Kp_val);

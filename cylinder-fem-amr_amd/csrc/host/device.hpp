@@ -33,8 +33,19 @@ struct DeviceMeshBuffers {
 
     PressureProbe probe_front, probe_back;
 
+    // Outlet condition (src/cfa/common.cuh:76-89; the reference's global g_outlet_dirichlet, set
+    // by the program from --outlet). With outlet_dirichlet every upload picks the pinned node and
+    // the outlet parabola factor (device.cuh:153-172) and flux-balances the transferred outlet
+    // velocities (:179-199).
+    bool outlet_dirichlet = false;
+    int p_pin = -1;
+    double outlet_scale = 1.0;
+
     const cfa_mesh_t* view() const { return &view_; }
     cfa_oss_op_t oss_op() const;
+    // Mean removal and pinned row of the pure-Neumann PPE after cfa_assemble_pressure
+    // (src/cfa/fem_kernels.cuh:519-531 apply_bc_pressure); nothing with the default outlet.
+    void pressure_outlet_bc(const double* mass, double* rhs, cfa_stream_t s);
 
     // Upload of a mesh whose CSR graph the host built (build_csr_and_boundary_topology).
     void sync_from_host(const Mesh2D& mesh, const std::vector<double>& h_u, const std::vector<double>& h_v,
@@ -63,6 +74,7 @@ private:
     DevArray<unsigned char> d_topo_ws_;
     DevArray<double> d_topo_out_;
     DevArray<int> d_topo_nnz_;
+    DevArray<double> d_mean_ws_;
     void upload_elements(const Mesh2D& mesh);
     void upload_fields(const Mesh2D& mesh, const std::vector<double>& h_u, const std::vector<double>& h_v,
                        const std::vector<double>& h_p, const std::vector<double>& h_uo,

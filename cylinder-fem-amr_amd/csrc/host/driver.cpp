@@ -95,13 +95,11 @@ int run_driver(int argc, char** argv) {
     if (!parse_options(argc, argv, opt)) return 0;
     g_stats = RunStats();
     kernel_timings().clear();
-    if (opt.outlet_dirichlet) {
-        std::fprintf(stderr, "--outlet dirichlet is not supported by this program\n");
-        return EXIT_FAILURE;
-    }
     GPU_CHECK(cfa_init(0));
     std::cout.unsetf(std::ios::floatfield);   // a previous in-process run may have left fixed format
     std::cout << std::setprecision(6);
+    if (opt.outlet_dirichlet)   // src/cylinder_fem_amr.cu:122-127
+        std::cout << "[Setup] outlet: inflow parabola (Dirichlet), pressure increment pinned at one outlet node\n";
     const double dt = opt.dt;
     opt.amr.lookahead_time = opt.lookahead * opt.amr_interval * dt;
     const double rot_coeff = opt.rotational ? Sim::RHO * Sim::NU : 0.0;
@@ -167,6 +165,7 @@ int run_driver(int argc, char** argv) {
             h_p[i] = r.p;
         }
     DeviceMeshBuffers g;
+    g.outlet_dirichlet = opt.outlet_dirichlet;
     g.sync_from_host(mesh, h_u, h_v, h_p, h_uo, h_vo);
     // synchronous remeshes sort the leaves' Morton keys on the device (the overlapped ones run the
     // extraction on the worker thread, which keeps to the host)
@@ -307,6 +306,7 @@ int run_driver(int argc, char** argv) {
         GPU_CHECK(cfa_assemble_pressure(m, g.d_u, g.d_v, g.d_p, g.d_grad_dp_x, g.d_grad_dp_y, g.d_mass_lumped,
                                         g.d_elem_s, dt_proj, Sim::RHO, opt.pressure_stab, g.d_Kp_val, g.d_rhs_p,
                                         g.d_div_num, g.d_dp, oss ? g.d_oss_val.ptr : nullptr, g.d_inv_m, s));
+        g.pressure_outlet_bc(g.d_mass_lumped, g.d_rhs_p, s);
         if (shift_now) GPU_CHECK(cfa_axpy(n_nodes, -1.0, g.d_phi, g.d_p, s));
         if (dump_now) {   // int32 n, nnz; rp, ci; K_p values, rhs, initial guess
             dump.open(opt.prefix + "_pressure.bin", std::ios::binary);

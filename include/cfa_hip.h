@@ -76,13 +76,21 @@ typedef struct {
     const int* inc_ptr;  /* [n_nodes + 1] */
     const int* inc_elem; /* [3 n_elems] */
     const int* inc_pos;  /* [9 n_elems] */
+    /* Outlet condition (src/cfa/common.cuh:76-89, device.cuh:153-172). outlet_dirichlet = 0 (the
+     * default, also the zero-initialised struct): p = 0 on outlet nodes, nu du/dn = 0. Otherwise
+     * the outlet carries the inflow parabola times outlet_scale, and the pressure increment is
+     * Dirichlet (dp = 0) at node p_pin only. */
+    int outlet_dirichlet;
+    int p_pin;
+    double outlet_scale;
 } cfa_mesh_t;
 
 /* Momentum system (src/cfa/fem_kernels.cuh:40-181 k_assemble_momentum_fem, fused with
  * :411-453 k_apply_bc_momentum). Writes A_val, rhs_u, rhs_v and mass (no prior zeroing), then
- * Dirichlet rows (inlet/wall/cylinder) become identity rows whose rhs and u, v entries hold the
- * boundary value: the inflow parabola with peak u_max, zero on walls and cylinder, or the
- * manufactured solution at mms_time when mms_time >= 0. u_n, v_n must not alias u, v.
+ * Dirichlet rows (inlet/wall/cylinder, and the outlet with m->outlet_dirichlet) become identity
+ * rows whose rhs and u, v entries hold the boundary value: the inflow parabola with peak u_max
+ * (times m->outlet_scale on the outlet), zero on walls and cylinder, or the manufactured solution
+ * at mms_time when mms_time >= 0. u_n, v_n must not alias u, v.
  * fx, fy (nodal body force) may be NULL. */
 int cfa_assemble_momentum(const cfa_mesh_t* m, const double* u_n, const double* v_n,
                           const double* u_prev, const double* v_prev, const double* p,
@@ -121,6 +129,16 @@ int cfa_assemble_pressure(const cfa_mesh_t* m, const double* u_star, const doubl
                           double stab, double* Kp, double* rhs, double* div_num, double* dp,
                           double* oss_val, double* inv_m, cfa_stream_t stream);
 
+/* With m->outlet_dirichlet, cfa_assemble_pressure leaves rhs[p_pin] at its assembled value and
+ * this call completes the boundary condition (src/cfa/fem_kernels.cuh:493-531 k_sum,
+ * k_subtract_mean and the pinned row of k_apply_bc_pressure): rhs -= (sum rhs / sum mass) mass,
+ * then rhs[p_pin] = 0. Each sum adds blocks of 256 entries by the reference's in-block tree and
+ * the block results in ascending block order (one of the orders its atomics allow). ws: at least
+ * cfa_pressure_mean_ws_doubles(n) doubles. */
+size_t cfa_pressure_mean_ws_doubles(int n);
+int cfa_pressure_remove_mean(const cfa_mesh_t* m, const double* mass, double* rhs, double* ws,
+                             cfa_stream_t stream);
+
 /* OSS operator alone (src/cfa/device.cuh:304-311 assemble_oss_projection: fem_kernels.cuh:807-852
  * k_assemble_oss_operator and k_inverse_lumped_mass): writes oss_val [Bx|By|BTx|BTy] and inv_m
  * from the weights elem_s and the lumped mass. */
@@ -145,7 +163,8 @@ int cfa_stiffness_apply(const cfa_mesh_t* m, const double* f, double c, double* 
 
 /* Velocity correction and pressure update of src/cfa/fem_kernels.cuh:536-582
  * (k_accumulate_grad_dp_and_vorticity) and :592-622 (k_velocity_pressure_correction):
- *   u -= (dt/rho) M^-1 G dp on interior and outlet rows, p += p_update (dp - rot div_num / m),
+ *   u -= (dt/rho) M^-1 G dp on interior rows and, unless m->outlet_dirichlet, outlet rows,
+ *   p += p_update (dp - rot div_num / m),
  *   vort = M^-1 (N, omega_h(u*)). gx, gy, vort_num receive the nodal sums. */
 int cfa_correction(const cfa_mesh_t* m, double dt, double rho, double p_update, double rot,
                    const double* div_num, const double* mass, const double* dp, double* u,
@@ -199,6 +218,8 @@ typedef struct {
     const int* bc_tag;     /* [n] */
     double* gx;            /* [n] scratch */
     double* gy;
+    int outlet_dirichlet;  /* as in cfa_mesh_t: skipped rows are node p_pin, else outlet nodes */
+    int p_pin;
 } cfa_oss_op_t;
 
 /* One SA-AMG level (src/cfa/linalg.cuh:939-957 Level): operator A (rows x rows), prolongator

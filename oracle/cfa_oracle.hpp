@@ -33,6 +33,22 @@ struct Csr {
 };
 
 // ---- element kernels (src/cfa/fem_kernels.cuh) --------------------------------------------
+// Outlet condition (src/cfa/common.cuh:76-89): default p = 0 on outlet nodes; with dirichlet the
+// outlet carries the inflow parabola times scale and dp = 0 at node pin only.
+struct Outlet {
+    bool dirichlet = false;
+    int pin = -1;
+    double scale = 1.0;
+    bool p_dirichlet(int i, int tag) const { return dirichlet ? i == pin : tag == cfa::BC_OUTLET; }
+};
+// src/cfa/device.cuh:153-172: the pinned node and the outlet parabola factor of a mesh
+Outlet outlet_dirichlet_setup(const Mesh2D& mesh);
+// src/cfa/device.cuh:179-199: a transferred field with its outlet values flux-balanced
+Vec flux_balanced(const Mesh2D& mesh, const Vec& f);
+// src/cfa/fem_kernels.cuh:493-510 k_sum + k_subtract_mean: f -= (sum f / sum m) m, each sum by
+// blocks of 256 (in-block tree), the block results added in ascending order
+void subtract_mean(int n, double* f, const double* m);
+
 struct MmsValues { double u, v, p, fx, fy, ux, uy, vx, vy; };
 MmsValues mms_exact(double x, double y, double t);
 
@@ -46,7 +62,7 @@ void assemble_momentum(int ne, const int* ev, const double* x, const double* y,
                        double* rhs_v, double* mass, const double* fx, const double* fy);
 void apply_bc_momentum(int n, const int* tag, const double* x, const double* y, double u_max,
                        double mms_time, const int* rp, const int* ci, double* A, double* rhs_u,
-                       double* rhs_v, double* u, double* v);
+                       double* rhs_v, double* u, double* v, const Outlet& outlet = Outlet());
 void mms_forcing(int n, const double* x, const double* y, double t, double* fx, double* fy);
 void compute_oss_weights(int ne, const int* ev, const double* x, const double* y, const double* us,
                          const double* vs, double dt, double rho, double nu, double stab, double* s);
@@ -58,22 +74,24 @@ void assemble_pressure_poisson(int ne, const int* ev, const double* x, const dou
                                const double* s, double dt, double rho, double stab, const int* rp,
                                const int* ci, double* K, double* rhs, double* div_num);
 void apply_bc_pressure(int n, const int* tag, const int* rp, const int* ci, double* K, double* rhs,
-                       double* dp);
+                       double* dp, const Outlet& outlet = Outlet());
 void assemble_lumped_mass(int ne, const int* ev, const double* x, const double* y, double* m);
 void assemble_oss_operator(int ne, const int* ev, const double* x, const double* y, const double* s,
                            const int* rp, const int* ci, int nnz, double* oss_val);
 void oss_apply(int n, const int* rp, const int* ci, int nnz, const double* oss_val,
-               const double* inv_m, const int* tag, const double* in, double* out);
+               const double* inv_m, const int* tag, const double* in, double* out,
+               const Outlet& outlet = Outlet());
 void oss_apply_matrix_free(int ne, int n, const int* ev, const double* x, const double* y,
                            const int* tag, const double* s, const double* mass, const double* in,
-                           double* out);
+                           double* out, const Outlet& outlet = Outlet());
 void accumulate_grad_dp_and_vorticity(int ne, const int* ev, const double* x, const double* y,
                                       const double* dp, const double* u, const double* v,
                                       double* gx, double* gy, double* vort_num);
 void velocity_pressure_correction(int n, const int* tag, double dt, double rho, double p_update,
                                   double rot, const double* div_num, const double* mass,
                                   const double* gx, const double* gy, const double* vort_num,
-                                  const double* dp, double* u, double* v, double* p, double* vort);
+                                  const double* dp, double* u, double* v, double* p, double* vort,
+                                  const Outlet& outlet = Outlet());
 void stiffness_apply(int ne, const int* ev, const double* x, const double* y, const double* f,
                      double c, double* out);
 void volume_forces(int ne, const int* ev, const int* tag, const double* x, const double* y,
@@ -158,6 +176,7 @@ struct RunConfig {
     int dump_systems = 0;  // > 0 (with a prefix): write the momentum and pressure systems of this step
     double nu = cfa::Sim::NU;        // viscosity of the step (the reference's constant by default)
     double u_max = -1.0;             // > 0: inflow peak of the Dirichlet data instead of inflow_peak(t)
+    bool outlet_dirichlet = false;   // --outlet dirichlet
 };
 struct StepRow {
     int step;

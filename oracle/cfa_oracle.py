@@ -31,6 +31,9 @@ def load():
         _lib.oracle_run_params.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, d]
         _lib.oracle_run_params.restype = ctypes.c_int
+        _lib.oracle_run_outlet_dirichlet.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.c_int, ctypes.c_double, d]
+        _lib.oracle_run_outlet_dirichlet.restype = ctypes.c_int
     return _lib
 
 
@@ -56,4 +59,15 @@ def run_params(steps: int, dt: float, uniform_level: int, nu: float, u_max: floa
     rows = np.zeros((steps, len(ROW_COLUMNS)))
     k = lib.oracle_run_params(steps, dt, uniform_level, nu, u_max, tol,
                               rows.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return rows[:k]
+
+
+def run_outlet_dirichlet(steps: int, dt: float, uniform_level: int, min_level: int = 2, case_2d3: bool = False,
+                         tol: float = 1e-7):
+    """Fixed-mesh DFG run with the inflow parabola on the outlet and the pressure increment pinned
+    at one outlet node (--outlet dirichlet), solvers at relative tolerance tol. Returns rows."""
+    lib = load()
+    rows = np.zeros((steps, len(ROW_COLUMNS)))
+    k = lib.oracle_run_outlet_dirichlet(steps, dt, uniform_level, min_level, 1 if case_2d3 else 0, tol,
+                                        rows.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return rows[:k]

@@ -66,4 +66,29 @@ int oracle_run_params(int steps, double dt, int uniform_level, double nu, double
     return (int)r.rows.size();
 }
 
+// Fixed-mesh DFG run with the outlet parabola (--outlet dirichlet), both solvers at relative
+// tolerance tol, case 2D-2 or (case_2d3) 2D-3: rows as oracle_run_fixed.
+int oracle_run_outlet_dirichlet(int steps, double dt, int uniform_level, int min_level, int case_2d3, double tol,
+                                double* rows) {
+    RunConfig c;
+    c.steps = steps;
+    c.dt = dt;
+    c.uniform_level = uniform_level;
+    c.min_level = min_level;
+    c.case_2d3 = case_2d3 != 0;
+    c.outlet_dirichlet = true;
+    c.mom_tol = tol;
+    c.ppe_tol = tol;
+    c.quiet = true;
+    RunResult r = run_fixed(c);
+    for (size_t k = 0; k < r.rows.size(); ++k) {
+        const StepRow& s = r.rows[k];
+        double* o = rows + 13 * k;
+        o[0] = s.step;  o[1] = s.t;  o[2] = s.cd;  o[3] = s.cl;  o[4] = s.dp;  o[5] = s.nodes;
+        o[6] = s.elems;  o[7] = s.p_front;  o[8] = s.p_back;  o[9] = s.cd_s;  o[10] = s.cl_s;
+        o[11] = s.it_u + 1000.0 * s.it_v;  o[12] = s.it_p;
+    }
+    return (int)r.rows.size();
+}
+
 }  // extern "C"

@@ -118,6 +118,14 @@ RunResult run_fixed(const RunConfig& cfg) {
     }
     const Probe pf = find_probe(mesh, Sim::PI), pb = find_probe(mesh, 0.0);
 
+    // device upload (src/cfa/device.cuh:131-213): with the outlet parabola, the pinned node, the
+    // flux factor and flux-balanced u, u_old; u_prev, v_prev start as copies of u_old, v_old
+    Outlet outlet;
+    if (cfg.outlet_dirichlet) {
+        outlet = outlet_dirichlet_setup(mesh);
+        hu = flux_balanced(mesh, hu);
+        huo = flux_balanced(mesh, huo);
+    }
     Vec u = hu, v = hv, p = hp, u_old = huo, v_old = hvo, u_prev = huo, v_prev = hvo;
     Vec dp(n, 0.0), vort(n, 0.0), A(nnz), Kp(nnz), rhs_u(n), rhs_v(n), rhs_p(n), mass(n);
     Vec gx(n), gy(n), vort_num(n), div_num(n), elem_s(ne), oss_val(4 * (size_t)nnz), inv_m(n);
@@ -151,7 +159,7 @@ RunResult run_fixed(const RunConfig& cfg) {
                           cfg.mms ? gy.data() : nullptr);
         apply_bc_momentum(n, tag, x, y, cfg.u_max > 0.0 ? cfg.u_max : cfa::inflow_peak(step * dt, cfg.case_2d3),
                           cfg.mms ? step * dt : -1.0,
-                          rp, ci, A.data(), rhs_u.data(), rhs_v.data(), u.data(), v.data());
+                          rp, ci, A.data(), rhs_u.data(), rhs_v.data(), u.data(), v.data(), outlet);
         const bool dump_now = step == cfg.dump_systems && !cfg.prefix.empty();
         std::FILE* dump = dump_now ? std::fopen((cfg.prefix + "_momentum.bin").c_str(), "wb") : nullptr;
         if (dump) {
@@ -190,7 +198,8 @@ RunResult run_fixed(const RunConfig& cfg) {
         assemble_pressure_poisson(ne, ev.data(), x, y, u.data(), v.data(), p.data(), gx.data(), gy.data(),
                                   mass.data(), elem_s.data(), dt_proj, Sim::RHO, cfg.pressure_stab, rp, ci,
                                   Kp.data(), rhs_p.data(), div_num.data());
-        apply_bc_pressure(n, tag, rp, ci, Kp.data(), rhs_p.data(), dp.data());
+        if (outlet.dirichlet) subtract_mean(n, rhs_p.data(), mass.data());   // fem_kernels.cuh:522-529
+        apply_bc_pressure(n, tag, rp, ci, Kp.data(), rhs_p.data(), dp.data(), outlet);
         if (dump_now) {
             if (std::FILE* f = std::fopen((cfg.prefix + "_pressure.bin").c_str(), "wb")) {
                 const int hdr[2] = {n, nnz};
@@ -209,7 +218,7 @@ RunResult run_fixed(const RunConfig& cfg) {
             for (int i = 0; i < n; ++i) inv_m[i] = 1.0 / std::fmax(mass[i], 1.0e-14);
         }
         Op oss_op = [&](const double* in, double* out) {
-            oss_apply(n, rp, ci, nnz, oss_val.data(), inv_m.data(), tag, in, out);
+            oss_apply(n, rp, ci, nnz, oss_val.data(), inv_m.data(), tag, in, out, outlet);
         };
         if (amg_dirty || step - amg_built >= cfg.amg_rebuild) {
             Csr K;
@@ -238,7 +247,7 @@ RunResult run_fixed(const RunConfig& cfg) {
                                          gy.data(), vort_num.data());
         velocity_pressure_correction(n, tag, dt_proj, Sim::RHO, 1.0, rot_coeff, div_num.data(), mass.data(),
                                      gx.data(), gy.data(), vort_num.data(), dp.data(), u.data(), v.data(),
-                                     p.data(), vort.data());
+                                     p.data(), vort.data(), outlet);
 
         // Step D: forces and probes
         double diag[6] = {0, 0, 0, 0, 0, 0};

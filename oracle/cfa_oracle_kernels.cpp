@@ -161,11 +161,12 @@ void assemble_momentum(int ne, const int* ev, const double* x, const double* y,
 // src/cfa/fem_kernels.cuh:411-453 (default outlet: p = 0, no velocity condition there)
 void apply_bc_momentum(int n, const int* tag, const double* x, const double* y, double u_max,
                        double mms_time, const int* rp, const int* ci, double* A, double* rhs_u,
-                       double* rhs_v, double* u, double* v) {
+                       double* rhs_v, double* u, double* v, const Outlet& outlet) {
     const double H = cfa::Sim::H_CHANNEL;
     for (int i = 0; i < n; ++i) {
         const int t = tag[i];
-        if (!(t == BC_INLET || t == BC_WALL || t == BC_CYLINDER)) continue;
+        const bool outlet_row = t == BC_OUTLET && outlet.dirichlet;
+        if (!(t == BC_INLET || t == BC_WALL || t == BC_CYLINDER || outlet_row)) continue;
         double ub = 0.0, vb = 0.0;
         if (mms_time >= 0.0) {
             const MmsValues r = mms_exact(x[i], y[i], mms_time);
@@ -174,6 +175,9 @@ void apply_bc_momentum(int n, const int* tag, const double* x, const double* y, 
         } else if (t == BC_INLET) {
             const double yi = y[i];
             ub = 4.0 * u_max * yi * (H - yi) / (H * H);
+        } else if (outlet_row) {   // :440, the parabola times the flux factor
+            const double yi = y[i];
+            ub = 4.0 * u_max * yi * (H - yi) / (H * H) * outlet.scale;
         }
         for (int k = rp[i]; k < rp[i + 1]; ++k) A[k] = (ci[k] == i) ? 1.0 : 0.0;
         rhs_u[i] = ub;  rhs_v[i] = vb;
@@ -266,19 +270,83 @@ void assemble_pressure_poisson(int ne, const int* ev, const double* x, const dou
     }
 }
 
-// src/cfa/fem_kernels.cuh:460-490 (default outlet: Dirichlet rows = outlet nodes)
+// src/cfa/fem_kernels.cuh:460-490 (Dirichlet rows: outlet nodes, or the pinned node)
 void apply_bc_pressure(int n, const int* tag, const int* rp, const int* ci, double* K, double* rhs,
-                       double* dp) {
+                       double* dp, const Outlet& outlet) {
     for (int i = 0; i < n; ++i) {
-        if (tag[i] == BC_OUTLET) {
+        if (outlet.p_dirichlet(i, tag[i])) {
             for (int k = rp[i]; k < rp[i + 1]; ++k) K[k] = (ci[k] == i) ? 1.0 : 0.0;
             rhs[i] = 0.0;
             dp[i] = 0.0;
         } else {
             for (int k = rp[i]; k < rp[i + 1]; ++k)
-                if (ci[k] != i && tag[ci[k]] == BC_OUTLET) K[k] = 0.0;
+                if (ci[k] != i && outlet.p_dirichlet(ci[k], tag[ci[k]])) K[k] = 0.0;
         }
     }
+}
+
+// src/cfa/fem_kernels.cuh:493-510: k_sum reduces blocks of 256 threads by the tree
+// s[t] += s[t + w], w = 128 ... 1, and adds each block's s[0] to a zeroed total (ascending
+// block order here); k_subtract_mean then forms f_i -= (sum f / sum m) m_i
+void subtract_mean(int n, double* f, const double* m) {
+    auto total = [n](const double* a) {
+        double acc = 0.0;
+        double s[256];
+        for (int b0 = 0; b0 < n; b0 += 256) {
+            for (int t = 0; t < 256; ++t) s[t] = b0 + t < n ? a[b0 + t] : 0.0;
+            for (int w = 128; w > 0; w /= 2)
+                for (int t = 0; t < w; ++t) s[t] += s[t + w];
+            acc += s[0];
+        }
+        return acc;
+    };
+    const double sf = total(f), sm = total(m);
+    for (int i = 0; i < n; ++i) f[i] -= sf / sm * m[i];
+}
+
+// src/cfa/device.cuh:153-172
+Outlet outlet_dirichlet_setup(const Mesh2D& mesh) {
+    const double H = cfa::Sim::H_CHANNEL;
+    const int n = mesh.num_nodes;
+    Outlet o;
+    o.dirichlet = true;
+    for (int i = 0; i < n; ++i) {
+        if (mesh.bc_tag[i] != BC_OUTLET) continue;
+        if (o.pin < 0 || std::fabs(mesh.y[i] - 0.5 * H) < std::fabs(mesh.y[o.pin] - 0.5 * H)) o.pin = i;
+    }
+    auto unit_flux = [&](int t) {
+        std::vector<double> ys{0.0, H};
+        for (int i = 0; i < n; ++i)
+            if (mesh.bc_tag[i] == t) ys.push_back(mesh.y[i]);
+        std::sort(ys.begin(), ys.end());
+        double q = 0.0;
+        for (size_t k = 1; k < ys.size(); ++k)
+            q += 0.5 * (ys[k] - ys[k - 1]) * (ys[k] * (H - ys[k]) + ys[k - 1] * (H - ys[k - 1]));
+        return q;
+    };
+    o.scale = unit_flux(BC_INLET) / unit_flux(BC_OUTLET);
+    return o;
+}
+
+// src/cfa/device.cuh:179-199
+Vec flux_balanced(const Mesh2D& mesh, const Vec& f) {
+    const int n = mesh.num_nodes;
+    auto flux = [&](int t) {
+        std::vector<std::pair<double, double>> pts{{0.0, 0.0}, {cfa::Sim::H_CHANNEL, 0.0}};
+        for (int i = 0; i < n; ++i)
+            if (mesh.bc_tag[i] == t) pts.emplace_back(mesh.y[i], f[i]);
+        std::sort(pts.begin(), pts.end());
+        double q = 0.0;
+        for (size_t k = 1; k < pts.size(); ++k)
+            q += 0.5 * (pts[k].first - pts[k - 1].first) * (pts[k].second + pts[k - 1].second);
+        return q;
+    };
+    const double qi = flux(BC_INLET), qo = flux(BC_OUTLET);
+    Vec g = f;
+    if (std::fabs(qo) > 1.0e-14)
+        for (int i = 0; i < n; ++i)
+            if (mesh.bc_tag[i] == BC_OUTLET) g[i] *= qi / qo;
+    return g;
 }
 
 // src/cfa/fem_kernels.cuh:890-909
@@ -319,7 +387,7 @@ void assemble_oss_operator(int ne, const int* ev, const double* x, const double*
 
 // src/cfa/fem_kernels.cuh:855-885: out -= B^T M^-1 B in, Dirichlet (outlet) rows untouched
 void oss_apply(int n, const int* rp, const int* ci, int nnz, const double* oss_val,
-               const double* inv_m, const int* tag, const double* in, double* out) {
+               const double* inv_m, const int* tag, const double* in, double* out, const Outlet& outlet) {
     std::vector<double> gx(n), gy(n);
     for (int k = 0; k < n; ++k) {
         double sx = 0.0, sy = 0.0;
@@ -334,7 +402,7 @@ void oss_apply(int n, const int* rp, const int* ci, int nnz, const double* oss_v
     const double* BTx = oss_val + 2 * (size_t)nnz;
     const double* BTy = oss_val + 3 * (size_t)nnz;
     for (int i = 0; i < n; ++i) {
-        if (tag[i] == BC_OUTLET) continue;
+        if (outlet.p_dirichlet(i, tag[i])) continue;
         double s = 0.0;
         for (int a = rp[i]; a < rp[i + 1]; ++a) s += BTx[a] * gx[ci[a]] + BTy[a] * gy[ci[a]];
         out[i] -= s;
@@ -344,7 +412,7 @@ void oss_apply(int n, const int* rp, const int* ci, int nnz, const double* oss_v
 // src/cfa/fem_kernels.cuh:763-797 with device.cuh:287-297: the matrix-free element form
 void oss_apply_matrix_free(int ne, int n, const int* ev, const double* x, const double* y,
                            const int* tag, const double* s, const double* mass, const double* in,
-                           double* out) {
+                           double* out, const Outlet& outlet) {
     std::vector<double> gx(n, 0.0), gy(n, 0.0);
     accumulate_scalar_gradient(ne, ev, x, y, in, s, gx.data(), gy.data());
     for (int e = 0; e < ne; ++e) {
@@ -359,7 +427,7 @@ void oss_apply_matrix_free(int ne, int n, const int* ev, const double* x, const 
         }
         const double w = std::sqrt(s[e]) * g.area / 3.0;
         for (int i = 0; i < 3; ++i) {
-            if (tag[nd[i]] == BC_OUTLET) continue;
+            if (outlet.p_dirichlet(nd[i], tag[nd[i]])) continue;
             out[nd[i]] += -w * (g.dNdx[i] * sx + g.dNdy[i] * sy);
         }
     }
@@ -387,16 +455,17 @@ void accumulate_grad_dp_and_vorticity(int ne, const int* ev, const double* x, co
     }
 }
 
-// src/cfa/fem_kernels.cuh:592-622 (default outlet: outlet velocities are corrected)
+// src/cfa/fem_kernels.cuh:592-622 (outlet velocities are corrected with the default outlet only)
 void velocity_pressure_correction(int n, const int* tag, double dt, double rho, double p_update,
                                   double rot, const double* div_num, const double* mass,
                                   const double* gx, const double* gy, const double* vort_num,
-                                  const double* dp, double* u, double* v, double* p, double* vort) {
+                                  const double* dp, double* u, double* v, double* p, double* vort,
+                                  const Outlet& outlet) {
     for (int i = 0; i < n; ++i) {
         const double m = std::fmax(mass[i], 1.0e-14);
         vort[i] = vort_num[i] / m;
         p[i] += p_update * (dp[i] - rot * div_num[i] / m);
-        if (tag[i] == BC_INTERIOR || tag[i] == BC_OUTLET) {
+        if (tag[i] == BC_INTERIOR || (tag[i] == BC_OUTLET && !outlet.dirichlet)) {
             u[i] -= (dt / rho) * (gx[i] / m);
             v[i] -= (dt / rho) * (gy[i] / m);
         }

@@ -20,6 +20,7 @@ int main(int argc, char** argv) {
         else if (a == "--uniform-level") c.uniform_level = std::atoi(val());
         else if (a == "--min-level") c.min_level = std::atoi(val());
         else if (a == "--mms") c.mms = true;
+        else if (a == "--outlet") c.outlet_dirichlet = std::string(val()) == "dirichlet";
         else if (a == "--case") c.case_2d3 = std::string(val()) == "2d3";
         else if (a == "--time-scheme") c.bdf2 = std::string(val()) == "bdf2";
         else if (a == "--pressure-update") c.rotational = std::string(val()) == "rotational";

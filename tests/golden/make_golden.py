@@ -21,6 +21,9 @@ ADAPTIVE = ("d2_A2_dt05", 2000)
 # uniform level-3 DFG 2D-2 run to t = 8 s (runs/v14/timing/U3_r1): the force history from
 # t >= 5.9 s (its statistics window was the last 1.5 s) and the statistics the run wrote
 STATS_RUN = (os.path.join("v14", "timing", "U3_r1"), 5.9)
+# --outlet dirichlet runs (runs/review, source = the reference's src/): the first 200 steps of a
+# uniform level-4 DFG 2D-2 and 2D-3 run, and of the adaptive A1 2D-2 run (levels 2-4)
+OUTLET_RUNS = (("d2_U4_outdir", 200), ("d3_U4_outdir", 200), ("d2_A1_outdir", 200))
 
 
 def main(ref):
@@ -52,6 +55,14 @@ def main(ref):
             if float(v[1]) >= t_from:
                 fo.write(",".join(v[c] for c in cols) + "\n")
     shutil.copyfile(os.path.join(ref, "runs", run, "cylinder_fem_amr_stats.csv"), os.path.join(here, "U3_r1_stats.csv"))
+    for run, rows in OUTLET_RUNS:
+        src = os.path.join(ref, "runs", "review", run, "cylinder_fem_amr_forces.csv")
+        with open(src) as fi, open(os.path.join(here, f"{run}_forces_first{rows}.csv"), "w") as fo:
+            for k, line in enumerate(fi):
+                if k > rows:
+                    break
+                fo.write(line)
+        shutil.copyfile(os.path.join(ref, "runs", "review", run, "cmd.txt"), os.path.join(here, f"{run}_cmd.txt"))
 
 
 if __name__ == "__main__":

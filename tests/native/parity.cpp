@@ -5,7 +5,7 @@
 // block reductions or other summation orders must match to the stated relative tolerance.
 //
 //   parity <case> [level]    case: momentum pressure correction forces indicator mms oss ilu
-//                                  spmv amg pcg bicgstab mixed topology all
+//                                  spmv amg pcg bicgstab mixed topology outlet all
 //
 // Prints one line per compared quantity and exits 0 when every comparison passes.
 #include <cmath>
@@ -712,6 +712,151 @@ void case_bicgstab(Fixture& f) {
 
 }  // namespace
 
+// --outlet dirichlet: the upload's pinned node, flux factor and balanced velocities, then the
+// momentum rows, the mean-free PPE, the OSS operator, the fused PCG and the correction with the
+// outlet parabola, each against the oracle's restatement of the reference's outlet branches
+void case_outlet(Fixture& f) {
+    std::printf("[outlet dirichlet]\n");
+    const Vec u0 = smooth_field(f.mesh, f.rng, 1.5), v0 = smooth_field(f.mesh, f.rng, 0.5);
+    const Vec uo0 = smooth_field(f.mesh, f.rng, 1.5), z(f.n, 0.0);
+    f.g.outlet_dirichlet = true;
+    f.g.sync_from_host(f.mesh, u0, v0, z, uo0, v0);
+    const oracle::Outlet out = oracle::outlet_dirichlet_setup(f.mesh);
+    expect(f.g.p_pin == out.pin && f.g.p_pin >= 0 && f.tag[f.g.p_pin] == 2, "pinned node (outlet, nearest mid-height)");
+    same_bits("outlet parabola factor", Vec{f.g.outlet_scale}, Vec{out.scale});
+    same_bits("u flux-balanced on upload", down(f.g.d_u, f.n), oracle::flux_balanced(f.mesh, u0));
+    same_bits("u_old flux-balanced on upload", down(f.g.d_u_old, f.n), oracle::flux_balanced(f.mesh, uo0));
+    same_bits("u_prev = balanced u_old", down(f.g.d_u_prev, f.n), oracle::flux_balanced(f.mesh, uo0));
+    const cfa_mesh_t* m = f.g.view();
+
+    // momentum: outlet rows carry the scaled parabola
+    const Vec un = smooth_field(f.mesh, f.rng, 1.5), vn = smooth_field(f.mesh, f.rng, 0.5);
+    const Vec p = smooth_field(f.mesh, f.rng, 2.0);
+    const double dt = 1.0e-3, a0 = 1.5 / dt, a1 = 2.0 / dt, a2 = -0.5 / dt, umax = 1.2345;
+    f.g.d_u_old.upload(un);
+    f.g.d_v_old.upload(vn);
+    f.g.d_u_prev.upload(u0);
+    f.g.d_v_prev.upload(v0);
+    f.g.d_p.upload(p);
+    f.g.d_u.upload(un);
+    f.g.d_v.upload(vn);
+    GPU_CHECK(cfa_assemble_momentum(m, f.g.d_u_old, f.g.d_v_old, f.g.d_u_prev, f.g.d_v_prev, f.g.d_p, Sim::NU,
+                                    Sim::RHO, a0, a1, a2, 2.0, -1.0, 1.0e-3, 0.5, nullptr, nullptr, umax, -1.0,
+                                    f.g.d_A_val, f.g.d_rhs_u, f.g.d_rhs_v, f.g.d_mass_lumped, f.g.d_u, f.g.d_v, f.s()));
+    Vec A(f.nnz, 0.0), ru(f.n, 0.0), rv(f.n, 0.0), mass(f.n, 0.0), u = un, v = vn;
+    oracle::assemble_momentum(f.ne, f.ev.data(), f.x, f.y, un.data(), vn.data(), u0.data(), v0.data(), p.data(),
+                              Sim::NU, Sim::RHO, a0, a1, a2, 2.0, -1.0, 1.0e-3, 0.5, f.rp, f.ci, A.data(), ru.data(),
+                              rv.data(), mass.data(), nullptr, nullptr);
+    oracle::apply_bc_momentum(f.n, f.tag, f.x, f.y, umax, -1.0, f.rp, f.ci, A.data(), ru.data(), rv.data(), u.data(),
+                              v.data(), out);
+    same_bits("A (outlet rows identity)", down(f.g.d_A_val, f.nnz), A);
+    same_bits("rhs_u", down(f.g.d_rhs_u, f.n), ru);
+    same_bits("rhs_v", down(f.g.d_rhs_v, f.n), rv);
+    same_bits("u (outlet parabola)", down(f.g.d_u, f.n), u);
+    same_bits("v", down(f.g.d_v, f.n), v);
+
+    // pure-Neumann PPE: assembled, made mean-free, pinned
+    const Vec us = smooth_field(f.mesh, f.rng, 1.5), vs = smooth_field(f.mesh, f.rng, 0.5);
+    const double dtp = (2.0 / 3.0) * 1.0e-3, stab = 0.3;
+    f.g.d_u.upload(us);
+    f.g.d_v.upload(vs);
+    const Vec dp_in = smooth_field(f.mesh, f.rng, 0.1);
+    f.g.d_dp.upload(dp_in);
+    GPU_CHECK(cfa_lumped_mass(m, f.g.d_mass_lumped, f.s()));
+    GPU_CHECK(cfa_oss_weights(m, f.g.d_u, f.g.d_v, dtp, Sim::RHO, Sim::NU, stab, f.g.d_elem_s, f.s()));
+    GPU_CHECK(cfa_scalar_gradient(m, f.g.d_p, f.g.d_elem_s, f.g.d_grad_dp_x, f.g.d_grad_dp_y, f.s()));
+    GPU_CHECK(cfa_assemble_pressure(m, f.g.d_u, f.g.d_v, f.g.d_p, f.g.d_grad_dp_x, f.g.d_grad_dp_y, f.g.d_mass_lumped,
+                                    f.g.d_elem_s, dtp, Sim::RHO, stab, f.g.d_Kp_val, f.g.d_rhs_p, f.g.d_div_num,
+                                    f.g.d_dp, f.g.d_oss_val, f.g.d_inv_m, f.s()));
+    f.g.pressure_outlet_bc(f.g.d_mass_lumped, f.g.d_rhs_p, f.s());
+    Vec pm(f.n, 0.0), es(f.ne, 0.0), gx(f.n, 0.0), gy(f.n, 0.0), div(f.n, 0.0), Kp(f.nnz, 0.0), rhs(f.n, 0.0);
+    Vec dp = dp_in, oss(4 * (size_t)f.nnz, 0.0), inv_m(f.n);
+    oracle::assemble_lumped_mass(f.ne, f.ev.data(), f.x, f.y, pm.data());
+    oracle::compute_oss_weights(f.ne, f.ev.data(), f.x, f.y, us.data(), vs.data(), dtp, Sim::RHO, Sim::NU, stab,
+                                es.data());
+    oracle::accumulate_scalar_gradient(f.ne, f.ev.data(), f.x, f.y, p.data(), es.data(), gx.data(), gy.data());
+    oracle::assemble_pressure_poisson(f.ne, f.ev.data(), f.x, f.y, us.data(), vs.data(), p.data(), gx.data(),
+                                      gy.data(), pm.data(), es.data(), dtp, Sim::RHO, stab, f.rp, f.ci, Kp.data(),
+                                      rhs.data(), div.data());
+    double rhs_abs = 0.0;
+    for (double r : rhs) rhs_abs += std::fabs(r);
+    oracle::subtract_mean(f.n, rhs.data(), pm.data());
+    double rhs_sum = 0.0;
+    for (double r : rhs) rhs_sum += r;
+    std::printf("  sum rhs after the mean removal %.3e (sum |rhs| %.3e)\n", rhs_sum, rhs_abs);
+    expect(std::fabs(rhs_sum) <= 1e-12 * rhs_abs, "oracle rhs mean-free before the pin");
+    oracle::apply_bc_pressure(f.n, f.tag, f.rp, f.ci, Kp.data(), rhs.data(), dp.data(), out);
+    same_bits("Kp (pinned row and column)", down(f.g.d_Kp_val, f.nnz), Kp);
+    same_bits("rhs_p (mean-free, pinned)", down(f.g.d_rhs_p, f.n), rhs);
+    same_bits("dp (0 at the pin only)", down(f.g.d_dp, f.n), dp);
+    oracle::assemble_oss_operator(f.ne, f.ev.data(), f.x, f.y, es.data(), f.rp, f.ci, f.nnz, oss.data());
+    for (int i = 0; i < f.n; ++i) inv_m[i] = 1.0 / std::fmax(pm[i], 1.0e-14);
+
+    // OSS operator: the pinned row is skipped, outlet rows are not
+    const Vec in = smooth_field(f.mesh, f.rng, 1.0);
+    DevArray<double> d_in, d_out;
+    d_in.upload(in);
+    d_out.upload(Vec(f.n, 0.0));
+    GPU_CHECK(cfa_oss_apply(m, f.g.d_oss_val, f.g.d_inv_m, d_in, d_out, f.g.d_oss_gx, f.g.d_oss_gy, f.s()));
+    Vec oo(f.n, 0.0);
+    oracle::oss_apply(f.n, f.rp, f.ci, f.nnz, oss.data(), inv_m.data(), f.tag, in.data(), oo.data(), out);
+    close_rel("-B^T M^-1 B in (assembled)", down(d_out, f.n), oo, 1e-13);
+    d_out.upload(Vec(f.n, 0.0));
+    GPU_CHECK(cfa_oss_apply_matrix_free(m, f.g.d_elem_s, f.g.d_mass_lumped, d_in, d_out, f.g.d_oss_gx, f.g.d_oss_gy,
+                                        f.s()));
+    Vec mf(f.n, 0.0);
+    oracle::oss_apply_matrix_free(f.ne, f.n, f.ev.data(), f.x, f.y, f.tag, es.data(), pm.data(), in.data(), mf.data(),
+                                  out);
+    close_rel("-B^T M^-1 B in (matrix-free)", down(d_out, f.n), mf, 1e-13);
+
+    // fused PCG with SA-AMG on the pinned system
+    AmgPreconditioner amg;
+    amg.setup(f.n, f.nnz, f.g.d_row_ptr, f.g.d_col_ind, f.g.d_Kp_val);
+    oracle::Amg oa;
+    oa.setup(host_csr(f, Kp));
+    const double tol = 1e-10;
+    f.g.d_dp.upload(z);
+    GpuLinearSolver solver;
+    const cfa_oss_op_t op = f.g.oss_op();
+    const int it = solver.solve_pcg_fused(f.n, f.g.d_row_ptr, f.g.d_col_ind, f.g.d_Kp_val, f.g.d_rhs_p, f.g.d_dp, 4000,
+                                          tol, &op, &amg);
+    Vec x(f.n, 0.0);
+    oracle::Op oss_op = [&](const double* a, double* b) {
+        oracle::oss_apply(f.n, f.rp, f.ci, f.nnz, oss.data(), inv_m.data(), f.tag, a, b, out);
+    };
+    oracle::Op prec = [&](const double* r, double* zz) { oa.apply(r, zz); };
+    const oracle::SolveInfo si = oracle::pcg(f.n, f.rp, f.ci, Kp.data(), rhs.data(), x.data(), 4000, tol, &oss_op,
+                                             &prec);
+    std::printf("  iterations gpu %d oracle %d\n", it, si.iters);
+    expect(std::abs(it - si.iters) <= 1 && solver.converged && si.converged, "PCG iterations and convergence");
+    close_rel("PCG solution", down(f.g.d_dp, f.n), x, 1e-8);
+
+    // correction: outlet velocities are Dirichlet data and stay
+    const Vec dpc = smooth_field(f.mesh, f.rng, 0.1), uc = smooth_field(f.mesh, f.rng, 1.5),
+              vc = smooth_field(f.mesh, f.rng, 0.5), pc = smooth_field(f.mesh, f.rng, 2.0);
+    f.g.d_dp.upload(dpc);
+    f.g.d_u.upload(uc);
+    f.g.d_v.upload(vc);
+    f.g.d_p.upload(pc);
+    f.g.d_mass_lumped.upload(pm);
+    f.g.d_div_num.upload(div);
+    GPU_CHECK(cfa_correction(m, dtp, Sim::RHO, 1.0, Sim::RHO * Sim::NU, f.g.d_div_num, f.g.d_mass_lumped, f.g.d_dp,
+                             f.g.d_u, f.g.d_v, f.g.d_p, f.g.d_vort, f.g.d_grad_dp_x, f.g.d_grad_dp_y, f.g.d_vort_num,
+                             f.s()));
+    Vec cgx(f.n, 0.0), cgy(f.n, 0.0), cvn(f.n, 0.0), uu = uc, vv = vc, pp = pc, vort(f.n, 0.0);
+    oracle::accumulate_grad_dp_and_vorticity(f.ne, f.ev.data(), f.x, f.y, dpc.data(), uc.data(), vc.data(), cgx.data(),
+                                             cgy.data(), cvn.data());
+    oracle::velocity_pressure_correction(f.n, f.tag, dtp, Sim::RHO, 1.0, Sim::RHO * Sim::NU, div.data(), pm.data(),
+                                         cgx.data(), cgy.data(), cvn.data(), dpc.data(), uu.data(), vv.data(),
+                                         pp.data(), vort.data(), out);
+    same_bits("u corrected (outlet untouched)", down(f.g.d_u, f.n), uu);
+    same_bits("v corrected", down(f.g.d_v, f.n), vv);
+    same_bits("p updated", down(f.g.d_p, f.n), pp);
+
+    f.g.outlet_dirichlet = false;   // back to the default outlet for later cases
+    f.g.sync_from_host(f.mesh, z, z, z, z, z);
+}
+
 int main(int argc, char** argv) {
     std::setvbuf(stdout, nullptr, _IOLBF, 0);   // each line reaches a pipe at once
     const std::string what = argc > 1 ? argv[1] : "all";
@@ -735,6 +880,7 @@ int main(int argc, char** argv) {
         if (all || what == "bicgstab") case_bicgstab(f);
         if (all || what == "mixed") case_mixed(f);
         if (all || what == "topology") case_topology(f);
+        if (all || what == "outlet") case_outlet(f);
         std::printf(failures ? "%d FAILURE(S)\n" : "ALL PASSED\n", failures);
         status = failures ? 1 : 0;
     }   // the fixture's device memory is freed here

@@ -13,6 +13,11 @@ reference's stored runs.
   round-off, so the reference itself drifts later, paper/manuscript.tex:919-922).
 * The same flow with synchronous remeshes (--amr-overlap 0), 300 steps: the CSR graph and mesh
   checks built on the device give the same run as the host build, forces CSV byte for byte.
+* --outlet dirichlet on the uniform level-4 mesh (runs/review/d2_U4_outdir), 15 steps: against
+  the oracle at relative 1e-10 per column with identical iteration counts, and against the
+  reference's CSV to its print resolution; on the adaptive A1 flow (runs/review/d2_A1_outdir,
+  remesh every 10 steps overlapped by 5, transferred outlet velocities flux-balanced), 200 steps:
+  mesh trajectory identical, forces within 1e-8 relative.
 """
 import os
 import subprocess
@@ -117,3 +122,38 @@ def test_synchronous_remesh_topology_on_device_matches_host(tmp_path):
         assert ("CSR/checks on GPU" in r.stdout) == (mode == "device"), r.stdout[-3000:]
         forces[mode] = (tmp_path / f"{mode}_forces.csv").read_bytes()
     assert forces["device"] == forces["host"]
+
+
+def test_outlet_dirichlet_uniform_vs_oracle_and_reference(golden):
+    print("[case] outlet dirichlet U4 15 steps", flush=True)
+    steps = 15
+    assert cfa.run(["--uniform-level", "4", "--steps", str(steps), "--outlet", "dirichlet", "--quiet",
+                    "--no-csv"]) == 0
+    hist = cfa.last_history()
+    rows = cfa_oracle.run_outlet_dirichlet(steps, 1e-3, 4, 2)
+    O = {c: i for i, c in enumerate(cfa_oracle.ROW_COLUMNS)}
+    assert hist.shape[0] == rows.shape[0] == steps
+    for c in COLS:
+        d = rel_dev(hist[:, H[c]], rows[:, O[c]])
+        assert d <= 1e-10, f"{c}: relative deviation {d:.3e} from the oracle"
+    np.testing.assert_array_equal(hist[:, H["it_u"]] + 1000 * hist[:, H["it_v"]], rows[:, O["it_uv"]])
+    np.testing.assert_array_equal(hist[:, H["it_p"]], rows[:, O["it_p"]])
+    ref = read_csv(golden / "d2_U4_outdir_forces_first200.csv")[:steps]
+    np.testing.assert_array_equal(hist[:, H["nodes"]], ref["nodes"])
+    for c in COLS:
+        assert_matches_printed(hist[:, H[c]], ref[c], f"GPU outlet dirichlet U4 {c}")
+
+
+def test_outlet_dirichlet_adaptive_vs_reference(golden):
+    print("[case] outlet dirichlet adaptive A1, 200 steps", flush=True)
+    steps = 200
+    assert cfa.run(["--steps", str(steps), "--amr-tol-vort", "0.1", "--amr-tol-interp", "0.01", "--max-level",
+                    "4", "--max-elems", "200000", "--min-level", "2", "--outlet", "dirichlet", "--quiet",
+                    "--no-csv"]) == 0
+    hist = cfa.last_history()
+    ref = read_csv(golden / "d2_A1_outdir_forces_first200.csv")[:steps]
+    np.testing.assert_array_equal(hist[:, H["nodes"]], ref["nodes"])
+    np.testing.assert_array_equal(hist[:, H["elements"]], ref["elements"])
+    for c in ("C_D", "C_L", "delta_p"):
+        d = rel_dev(hist[:, H[c]], ref[c])
+        assert d <= 1e-8, f"{c}: relative deviation {d:.3e} from the reference"

@@ -23,11 +23,11 @@ def print_quantum(v):
     return np.where(a > 0, 10.0 ** (e - 9), 1e-300)
 
 
-def assert_matches_printed(ours, printed, what, rel=1e-10):
+def assert_matches_printed(ours, printed, what, rel=1e-10, floor=0.0):
     ours = np.asarray(ours, float)
     printed = np.asarray(printed, float)
     d = np.abs(ours - printed)
-    bound = np.maximum(print_quantum(printed), rel * np.max(np.abs(printed)))
+    bound = np.maximum(np.maximum(print_quantum(printed), rel * np.max(np.abs(printed))), floor)
     bad = np.nonzero(d > bound)[0]
     assert bad.size == 0, f"{what}: {bad.size} values off, first at {bad[0]}: {ours[bad[0]]!r} vs {printed[bad[0]]!r}"
 
@@ -57,3 +57,24 @@ def test_oracle_plants_are_caught(golden):
     bad[37] += 1e-8 * np.max(np.abs(bad))
     with pytest.raises(AssertionError):
         assert_matches_printed(bad, ref["C_D"], "planted")
+
+
+@pytest.mark.parametrize("case", ["d2", "d3"])
+def test_oracle_outlet_dirichlet_history(golden, case):
+    """--outlet dirichlet (inflow parabola on the outlet scaled to the inlet's discrete flux, pressure
+    increment pinned at the outlet node nearest mid-height, pure-Neumann PPE made mean-free) on the
+    uniform level-4 mesh, DFG 2D-2 and 2D-3: the first 15 steps of runs/review/{d2,d3}_U4_outdir.
+    2D-3 starts from rest under the inflow ramp, so its first forces are small (C_D 0.14-0.16,
+    C_L 2e-4) and fixed only to the solvers' stopping rule: the oracle at relative tolerance 1e-7
+    and at 1e-9 differ by 3.5e-9 in C_D. Round-off-level differences (the reference's atomic
+    assembly against the oracle's element order) move them by up to 5e-11 (measured), so that
+    case carries an absolute floor of 1e-10 on top of the printed-digit bound."""
+    steps = 15
+    rows = cfa_oracle.run_outlet_dirichlet(steps, 1e-3, 4, 2, case_2d3=case == "d3")
+    ref = read_csv(golden / f"{case}_U4_outdir_forces_first200.csv")
+    idx = {c: i for i, c in enumerate(cfa_oracle.ROW_COLUMNS)}
+    assert len(rows) == steps
+    np.testing.assert_array_equal(rows[:, idx["nodes"]], ref["nodes"][:steps])
+    for c in COLS:
+        assert_matches_printed(rows[:, idx[c]], ref[c][:steps], f"{case} outlet dirichlet {c}",
+                               floor=1e-10 if case == "d3" else 0.0)

@@ -201,6 +201,10 @@ def mixed_comparison(flags: list[str], hist64, warmup: int, total: int) -> dict:
         "steps_on_identical_meshes": upto,
         "max_abs_dC_D": float(d_cd.max()) if upto else None,
         "max_abs_dC_L": float(d_cl.max()) if upto else None,
+        # the same after the impulsive start (C_D near 276 at step 1, where a 1e-10 relative
+        # difference is already 3e-8 absolute)
+        "max_abs_dC_D_from_step_100": float(d_cd[99:].max()) if upto > 99 else None,
+        "max_abs_dC_L_from_step_100": float(d_cl[99:].max()) if upto > 99 else None,
         "mean_iters_u_v_p": [round(float(h[warmup:total, col[c]].mean()), 2) for c in ("it_u", "it_v", "it_p")],
     }
 

@@ -11,8 +11,9 @@ reference's stored runs.
   trajectory (nodes, elements per step) is integer work and must match exactly; forces follow the
   reference to 1e-8 relative over the first 200 steps (remesh decisions are sensitive to
   round-off, so the reference itself drifts later, paper/manuscript.tex:919-922).
-* The same flow with synchronous remeshes (--amr-overlap 0), 300 steps: the CSR graph and mesh
-  checks built on the device give the same run as the host build, forces CSV byte for byte.
+* The same flow with synchronous remeshes (--amr-overlap 0), 300 steps, both outlet modes: the
+  CSR graph and mesh checks built on the device give the same run as the host build, forces CSV
+  byte for byte.
 * --outlet dirichlet on the uniform level-4 mesh (runs/review/d2_U4_outdir), 15 steps: against
   the oracle at relative 1e-10 per column with identical iteration counts, and against the
   reference's CSV to its print resolution; on the adaptive A1 flow (runs/review/d2_A1_outdir,
@@ -105,11 +106,13 @@ def test_adaptive_trajectory_vs_reference(golden):
         assert d <= 1e-8, f"{c}: relative deviation {d:.3e} from the reference"
 
 
-def test_synchronous_remesh_topology_on_device_matches_host(tmp_path):
-    print("[case] adaptive A2, synchronous remesh, 300 steps: device vs host topology", flush=True)
+@pytest.mark.parametrize("outlet", ["pressure", "dirichlet"])
+def test_synchronous_remesh_topology_on_device_matches_host(tmp_path, outlet):
+    print(f"[case] adaptive A2, synchronous remesh, 300 steps, outlet {outlet}: device vs host topology", flush=True)
     binary = Path(__file__).resolve().parents[1] / "cylinder-fem-amr_amd" / "build" / "cylinder_fem_amr"
     flags = ["--steps", "300", "--dt", "0.0005", "--amr-interval", "20", "--amr-overlap", "0", "--amr-tol-vort",
-             "0.05", "--amr-tol-interp", "0.005", "--max-level", "5", "--max-elems", "400000", "--min-level", "2"]
+             "0.05", "--amr-tol-interp", "0.005", "--max-level", "5", "--max-elems", "400000", "--min-level", "2",
+             "--outlet", outlet]
     forces = {}
     for mode in ("device", "host"):
         env = dict(os.environ)

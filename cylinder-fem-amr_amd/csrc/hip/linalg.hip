@@ -1126,226 +1126,6 @@ void jacobi_from_zero_check_launch(int tpr, int rows, const int* rp, const int* 
     });
 }
 
-// ---- coarse tail of the V-cycle -------------------------------------------------------------------
-// The bottom levels of the hierarchy hold a few hundred to a few thousand rows; there a kernel per
-// smoothing step, residual and transfer costs more in launch and ramp than in work. The tail
-// kernel runs the V-cycle from level t down through the dense coarse solve and back up in one
-// workgroup of 1024 lanes: every tail level's vectors sit in LDS, the operators are read through
-// L2, and the steps are the row products of k_csr_op (G lanes per row, group sum) separated by
-// workgroup barriers. Pre-smoothing starts from x = omega D^-1 b followed by ss - 1 sweeps, which
-// is the arithmetic of k_jacobi_from_zero (ss >= 2) or k_jacobi_init (ss < 2).
-constexpr int TAIL_THREADS = 1024;
-constexpr int TAIL_MAX_LEVELS = 8;
-constexpr int TAIL_LDS_BYTES = 144 * 1024;
-
-struct TailLevel {
-    int rows, cols, ga, gp, gr;
-    const int* a_rp; const int* a_ci; const void* a_v;
-    const int* p_rp; const int* p_ci; const void* p_v;
-    const int* r_rp; const int* r_ci; const void* r_v;
-    const void* dinv;
-    double omega;
-};
-struct TailArgs {
-    int nt, ss, coarse_n;
-    const double* coarse_inv;
-    const void* b_in;   // right-hand side of the first tail level (global)
-    void* x_out;        // its V-cycle result (global)
-    TailLevel lv[TAIL_MAX_LEVELS];
-};
-
-// One row product over `rows` rows with G lanes per row; the entries of a lane are loaded four at a
-// time before they are summed (in entry order), so a lane has four loads in flight.
-template <int G, int OP, typename T>
-__device__ __forceinline__ void tail_rows_g(int rows, const int* __restrict__ rp, const int* __restrict__ ci,
-                                            const T* __restrict__ val, const T* x, const T* b,
-                                            const T* __restrict__ dinv, T omega, T* y) {
-    constexpr int PER = TAIL_THREADS / G;
-    const int lane = threadIdx.x % G;
-    for (int base = 0; base < rows; base += PER) {
-        const int row = base + (int)threadIdx.x / G;
-        T s = 0;
-        if (row < rows) {
-            const int e = rp[row + 1];
-            for (int k = rp[row] + lane; k < e; k += 4 * G) {
-                int c[4];
-                T v[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const bool in = k + u * G < e;
-                    c[u] = in ? ci[k + u * G] : 0;
-                    v[u] = in ? val[k + u * G] : (T)0;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (k + u * G < e) s += v[u] * x[c[u]];
-            }
-        }
-        s = group_sum<G>(s);
-        if (row < rows && lane == 0) {
-            if (OP == OP_SPMV) y[row] = s;
-            else if (OP == OP_ADD) y[row] += s;
-            else if (OP == OP_RESID) y[row] = b[row] - s;
-            else y[row] = x[row] + omega * dinv[row] * (b[row] - s);
-        }
-    }
-    __syncthreads();
-}
-
-template <int OP, typename T>
-__device__ __forceinline__ void tail_rows(int g, int rows, const int* rp, const int* ci, const T* val, const T* x,
-                                          const T* b, const T* dinv, T omega, T* y) {
-    switch (g) {
-        case 1: tail_rows_g<1, OP, T>(rows, rp, ci, val, x, b, dinv, omega, y); break;
-        case 2: tail_rows_g<2, OP, T>(rows, rp, ci, val, x, b, dinv, omega, y); break;
-        case 4: tail_rows_g<4, OP, T>(rows, rp, ci, val, x, b, dinv, omega, y); break;
-        case 8: tail_rows_g<8, OP, T>(rows, rp, ci, val, x, b, dinv, omega, y); break;
-        case 16: tail_rows_g<16, OP, T>(rows, rp, ci, val, x, b, dinv, omega, y); break;
-        case 32: tail_rows_g<32, OP, T>(rows, rp, ci, val, x, b, dinv, omega, y); break;
-        default: tail_rows_g<64, OP, T>(rows, rp, ci, val, x, b, dinv, omega, y); break;
-    }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(TAIL_THREADS) k_vcycle_tail(TailArgs a, const int* __restrict__ state) {
-    if (halted(state)) return;
-    __shared__ __align__(16) unsigned char lds_raw[TAIL_LDS_BYTES];
-    T* const lds = reinterpret_cast<T*>(lds_raw);
-    // LDS offsets (elements): per level x, y (ping-pong pair) and b; then the coarse cb, cx
-    int ox[TAIL_MAX_LEVELS], oy[TAIL_MAX_LEVELS], ob[TAIL_MAX_LEVELS];
-    int off = 0;
-    for (int j = 0; j < a.nt; ++j) {
-        ox[j] = off; off += a.lv[j].rows;
-        oy[j] = off; off += a.lv[j].rows;
-        ob[j] = off; off += a.lv[j].rows;
-    }
-    const int ocb = off, ocx = off + a.coarse_n;
-    const int t = threadIdx.x;
-    const T* b_in = static_cast<const T*>(a.b_in);
-    for (int i = t; i < a.lv[0].rows; i += TAIL_THREADS) lds[ob[0] + i] = b_in[i];
-    __syncthreads();
-    // down: pre-smoothing, residual, restriction
-    for (int j = 0; j < a.nt; ++j) {
-        const TailLevel& L = a.lv[j];
-        const T* av = static_cast<const T*>(L.a_v);
-        const T* dv = static_cast<const T*>(L.dinv);
-        const T om = (T)L.omega;
-        const int n = L.rows;
-        int xo = ox[j], yo = oy[j];
-        for (int i = t; i < n; i += TAIL_THREADS) lds[xo + i] = om * dv[i] * lds[ob[j] + i];
-        __syncthreads();
-        for (int k = 1; k < a.ss; ++k) {
-            tail_rows<OP_JACOBI, T>(L.ga, n, L.a_rp, L.a_ci, av, lds + xo, lds + ob[j], dv, om, lds + yo);
-            const int tmp = xo; xo = yo; yo = tmp;
-        }
-        ox[j] = xo;
-        oy[j] = yo;
-        tail_rows<OP_RESID, T>(L.ga, n, L.a_rp, L.a_ci, av, lds + xo, lds + ob[j], nullptr, (T)0, lds + yo);
-        const int next_b = j + 1 < a.nt ? ob[j + 1] : ocb;
-        tail_rows<OP_SPMV, T>(L.gr, L.cols, L.r_rp, L.r_ci, static_cast<const T*>(L.r_v), lds + yo, nullptr, nullptr,
-                              (T)0, lds + next_b);
-    }
-    // dense coarse solve in FP64, one wavefront per row (the arithmetic of k_dense_matvec)
-    {
-        const int cn = a.coarse_n, w = t / WAVE, ln = t % WAVE;
-        for (int i = w; i < cn; i += TAIL_THREADS / WAVE) {
-            double s = 0.0;
-            for (int c = ln; c < cn; c += WAVE) s += a.coarse_inv[(size_t)i * cn + c] * (double)lds[ocb + c];
-            s = wave_sum(s);
-            if (ln == 0) lds[ocx + i] = (T)s;
-        }
-        __syncthreads();
-    }
-    // up: prolongation and post-smoothing
-    for (int j = a.nt - 1; j >= 0; --j) {
-        const TailLevel& L = a.lv[j];
-        const T* av = static_cast<const T*>(L.a_v);
-        const T* dv = static_cast<const T*>(L.dinv);
-        const T om = (T)L.omega;
-        const int n = L.rows;
-        int xo = ox[j], yo = oy[j];
-        const int cx = j + 1 < a.nt ? ox[j + 1] : ocx;
-        tail_rows<OP_ADD, T>(L.gp, n, L.p_rp, L.p_ci, static_cast<const T*>(L.p_v), lds + cx, nullptr, nullptr, (T)0,
-                             lds + xo);
-        for (int k = 0; k < a.ss; ++k) {
-            tail_rows<OP_JACOBI, T>(L.ga, n, L.a_rp, L.a_ci, av, lds + xo, lds + ob[j], dv, om, lds + yo);
-            const int tmp = xo; xo = yo; yo = tmp;
-        }
-        ox[j] = xo;
-    }
-    T* x_out = static_cast<T*>(a.x_out);
-    for (int i = t; i < a.lv[0].rows; i += TAIL_THREADS) x_out[i] = lds[ox[0] + i];
-}
-
-// Lanes per row inside the tail kernel: a power of two near half the mean row length (at most 64),
-// lowered while the level would take more than four rounds of the workgroup (at least 1).
-int tail_lanes(int rows, long long nnz) {
-    if (rows <= 0) return 1;
-    const long long per_row = (nnz + rows - 1) / rows;
-    int g = 1;
-    while (g < 64 && 2 * g <= (per_row + 1) / 2) g *= 2;
-    while (g > 1 && (long long)rows * g > 4LL * TAIL_THREADS) g /= 2;
-    return g;
-}
-
-// Entries of A allowed at the first tail level (0 turns the tail off); CFA_AMG_TAIL_NNZ in the
-// environment sets the start value, cfa_amg_set_tail_limit changes it. Off by default: measured on
-// MI355X (115k-row pressure hierarchy) the per-level kernels are faster, 222.8 us per PCG
-// iteration against 240.3 us with the 399-row level in the tail, because one workgroup walks the
-// rows in rounds and every round waits for its own global loads.
-long long g_tail_limit = -1;
-long long tail_max_nnz() {
-    if (g_tail_limit < 0) {
-        const char* e = std::getenv("CFA_AMG_TAIL_NNZ");
-        g_tail_limit = e ? std::max(0LL, std::atoll(e)) : 0LL;
-    }
-    return g_tail_limit;
-}
-
-// First level of the tail for element type T, or -1: every level from it down must fit the LDS
-// (three vectors per level, two coarse vectors) and the first one stay under the entry limit. The
-// tail never starts at level 0 (the PCG hooks live there).
-template <typename T>
-int tail_start(const cfa_amg_t* amg) {
-    const long long max_nnz = tail_max_nnz();
-    if (max_nnz <= 0 || amg->n_levels < 2) return -1;
-    long long bytes = 2LL * amg->coarse_n * (long long)sizeof(T);
-    int start = -1;
-    for (int l = amg->n_levels - 1; l >= 1; --l) {
-        const cfa_amg_level_t& L = amg->levels[l];
-        bytes += 3LL * L.rows * (long long)sizeof(T);
-        if (bytes > TAIL_LDS_BYTES || L.a_nnz > max_nnz || amg->n_levels - l > TAIL_MAX_LEVELS) break;
-        start = l;
-    }
-    return start;
-}
-
-template <typename T>
-void tail_launch(const cfa_amg_t* amg, int l0, const int* st, hipStream_t s) {
-    TailArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.nt = amg->n_levels - l0;
-    a.ss = amg->smooth_steps;
-    a.coarse_n = amg->coarse_n;
-    a.coarse_inv = amg->coarse_inv;
-    a.b_in = Lv<T>::b(amg->levels[l0]);
-    a.x_out = Lv<T>::x(amg->levels[l0]);
-    for (int j = 0; j < a.nt; ++j) {
-        const cfa_amg_level_t& L = amg->levels[l0 + j];
-        TailLevel& d = a.lv[j];
-        d.rows = L.rows; d.cols = L.cols;
-        d.ga = tail_lanes(L.rows, L.a_nnz);
-        d.gp = tail_lanes(L.rows, L.p_nnz);
-        d.gr = tail_lanes(L.cols, L.r_nnz);
-        d.a_rp = L.a_rp; d.a_ci = L.a_ci; d.a_v = Lv<T>::a(L);
-        d.p_rp = L.p_rp; d.p_ci = L.p_ci; d.p_v = Lv<T>::p(L);
-        d.r_rp = L.r_rp; d.r_ci = L.r_ci; d.r_v = Lv<T>::r(L);
-        d.dinv = Lv<T>::dinv(L);
-        d.omega = L.omega;
-    }
-    hipLaunchKernelGGL(k_vcycle_tail<T>, dim3(1), dim3(TAIL_THREADS), 0, s, a, st);
-}
-
 // One V-cycle (src/cfa/linalg.cuh:1184-1206) for level l with right-hand side b, in FP64 or FP32
 // (T). Each level's x / ax buffers form a ping-pong pair; the function returns the buffer holding
 // the level's iterate. On level 0 the last smoothing step writes straight into the FP64 `out`
@@ -1360,10 +1140,6 @@ T* vcycle(const cfa_amg_t* amg, int l, const T* b, double* out, const double* rd
     T* xb = Lv<T>::ax(L);
     const T* av = Lv<T>::a(L);
     const T* dv = Lv<T>::dinv(L);
-    if (l >= 1 && l == tail_start<T>(amg)) {
-        tail_launch<T>(amg, l, st, s);
-        return xa;
-    }
     // the residual test of a fused PCG iteration (the FP32 cycle takes it in its entry kernel)
     const bool check_here = std::is_same<T, double>::value && hk && hk->prr && l == 0;
     if (ss >= 2) {
@@ -1655,17 +1431,6 @@ int cfa_inv_diag(int n, const int* rp, const int* ci, const double* val, double 
     if (n == 0) return CFA_OK;
     hipLaunchKernelGGL(k_inv_diag, dim3(blocks_for(n)), dim3(BLOCK), 0, S(stream), n, rp, ci, val, eps, inv_diag);
     return launch_status();
-}
-
-long long cfa_amg_set_tail_limit(long long max_nnz) {
-    const long long prev = tail_max_nnz();
-    if (max_nnz >= 0) g_tail_limit = max_nnz;
-    return prev;
-}
-
-int cfa_amg_tail_start(const cfa_amg_t* amg) {
-    if (!amg || !amg_valid(amg)) return CFA_E_ARG;
-    return amg->use_f32 ? tail_start<float>(amg) : tail_start<double>(amg);
 }
 
 int cfa_amg_apply(const cfa_amg_t* amg, const double* r, double* z, cfa_stream_t stream) {

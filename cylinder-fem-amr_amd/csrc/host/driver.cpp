@@ -69,8 +69,10 @@ MmsHost mms_exact_host(double x, double y, double t) {
     return r;
 }
 
+// What an overlapped remesh hands back at its join. The worker thread writes the members, so
+// `job` is declared last: it is destroyed first, and ~future waits for the worker before any
+// member it writes goes away (an early return from the time loop with a remesh in flight).
 struct PendingRemesh {
-    std::future<bool> job;
     int join_step = -1;
     AmrStats st;
     MeshQuality q;
@@ -80,7 +82,40 @@ struct PendingRemesh {
     double extract_ms = 0.0, csr_ms = 0.0, validate_ms = 0.0;
     MulticolourIlu0::Pattern ilu;   // prepared by the worker when the momentum ILU is on
     bool has_ilu = false;
+    std::future<bool> job;
+    ~PendingRemesh() {
+        if (job.valid()) job.wait();
+    }
 };
+
+// State file at the end of a step: int32 magic 0x43464153, n, ne, step; float64 dt; then x, y
+// (float64 [n]), bc_tag (int32 [n]), elements (int32 [4 ne]: v0, v1, v2, gen), and the fields
+// u, v, p, u_old, v_old, dp (float64 [n] each): what a fixed-mesh continuation from that step needs.
+void write_state(const std::string& path, const Mesh2D& mesh, const DeviceMeshBuffers& g, int step, double dt) {
+    const int n = mesh.num_nodes, ne = mesh.num_elems;
+    std::ofstream f(path, std::ios::binary);
+    if (!f.is_open()) {
+        std::fprintf(stderr, "[Output] cannot open %s\n", path.c_str());
+        std::exit(EXIT_FAILURE);
+    }
+    const int hdr[4] = {0x43464153, n, ne, step};
+    f.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
+    f.write(reinterpret_cast<const char*>(&dt), sizeof(dt));
+    f.write(reinterpret_cast<const char*>(mesh.x.data()), n * sizeof(double));
+    f.write(reinterpret_cast<const char*>(mesh.y.data()), n * sizeof(double));
+    f.write(reinterpret_cast<const char*>(mesh.bc_tag.data()), n * sizeof(int));
+    std::vector<int> ev(4 * (size_t)ne);
+    for (int e = 0; e < ne; ++e) {
+        for (int k = 0; k < 3; ++k) ev[4 * (size_t)e + k] = mesh.elems[e].v[k];
+        ev[4 * (size_t)e + 3] = mesh.elems[e].gen;
+    }
+    f.write(reinterpret_cast<const char*>(ev.data()), ev.size() * sizeof(int));
+    for (const DevArray<double>* a : {&g.d_u, &g.d_v, &g.d_p, &g.d_u_old, &g.d_v_old, &g.d_dp}) {
+        const std::vector<double> h = a->to_host(n);
+        f.write(reinterpret_cast<const char*>(h.data()), n * sizeof(double));
+    }
+    std::printf("[Dump] state at the end of step %d (%d nodes, %d elements) -> %s\n", step, n, ne, path.c_str());
+}
 
 void dump_device(std::ofstream& f, const void* d, size_t bytes) {
     std::vector<char> h(bytes);
@@ -382,6 +417,7 @@ int run_driver(int argc, char** argv) {
                       << " | C_L = " << C_L << " | dP = " << delta_p << std::endl;
 
         history.add(t_now, C_D, C_L, delta_p);
+        if (step == opt.dump_state) write_state(opt.prefix + "_state.bin", mesh, g, step, dt);
         // Step E: VTK snapshots (src/cylinder_fem_amr.cu:540-547)
         if (opt.write_csv && opt.vtk_interval > 0 && step % opt.vtk_interval == 0 && step < opt.num_steps) {
             compute_error_indicator(g);

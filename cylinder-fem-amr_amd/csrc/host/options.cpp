@@ -67,7 +67,9 @@ void print_usage(const char* prog) {
         << "  --precision fp64|mixed                  ILU factors and AMG cycle in FP64 or FP32 (fp64)\n"
         << "  --quiet             no per-step console lines\n"
         << "  --no-csv            write no CSV files (the history stays available in-process)\n"
-        << "  --kernel-timing R   after the run, time the main kernels R launches each (HIP events)\n";
+        << "  --kernel-timing R   after the run, time the main kernels R launches each (HIP events)\n"
+        << "  --dump-state N      write the mesh and the solution state at the end of step N to\n"
+        << "                      <prefix>_state.bin (restarts and CPU comparisons)\n";
 }
 
 static double parse_double(const char* s, const char* name) {
@@ -169,6 +171,7 @@ bool parse_options(int argc, char** argv, RunOptions& opt) {
         else if (a == "--quiet") opt.quiet = true;
         else if (a == "--no-csv") opt.write_csv = false;
         else if (a == "--kernel-timing") opt.kernel_timing = parse_int(value(), "--kernel-timing");
+        else if (a == "--dump-state") opt.dump_state = parse_int(value(), "--dump-state");
         else if (a.rfind("--", 0) == 0) {
             std::fprintf(stderr, "Unknown option %s (see --help)\n", a.c_str());
             std::exit(EXIT_FAILURE);
@@ -193,6 +196,7 @@ bool parse_options(int argc, char** argv, RunOptions& opt) {
     require(opt.num_steps > 0, "--steps must be > 0");
     require(opt.dt > 0.0, "--dt must be > 0");
     require(opt.kernel_timing >= 0, "--kernel-timing must be >= 0");
+    require(opt.dump_state >= 0, "--dump-state must be >= 0");
     require(opt.amr_interval >= 0 && opt.init_amr >= 0, "AMR intervals must be >= 0");
     require(opt.amr.max_level >= 0 && opt.amr.max_level <= 12, "--max-level must be in [0, 12]");
     require(opt.amr.tol_vort > 0.0 || opt.amr.tol_interp > 0.0,

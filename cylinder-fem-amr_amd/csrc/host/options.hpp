@@ -48,6 +48,7 @@ struct RunOptions {
     bool quiet = false;         // no per-step console lines
     bool write_csv = true;
     int kernel_timing = 0;      // > 0: time the main kernels this many launches each on the final state
+    int dump_state = 0;         // > 0: write the mesh and solution state at the end of this step
 };
 
 void print_usage(const char* prog);

@@ -260,14 +260,6 @@ typedef struct {
 /* z = M^{-1} r, one V-cycle (src/cfa/linalg.cuh:1161-1169 AmgPreconditioner::apply). */
 int cfa_amg_apply(const cfa_amg_t* amg, const double* r, double* z, cfa_stream_t stream);
 
-/* The bottom levels of a V-cycle run in one single-workgroup kernel (vectors in LDS) from the
- * first level l >= 1 whose operator has at most `limit` entries and from which every level fits
- * the LDS. No counterpart in the reference (its cycle launches per level). cfa_amg_set_tail_limit
- * sets the limit (0 = off, < 0 = leave; start value 0 or CFA_AMG_TAIL_NNZ) and returns the
- * previous one; cfa_amg_tail_start returns that first level for a hierarchy, or -1. */
-long long cfa_amg_set_tail_limit(long long limit);
-int cfa_amg_tail_start(const cfa_amg_t* amg);
-
 /* out[i] = (float) in[i], i < n: the FP32 copies of a hierarchy. */
 int cfa_to_f32(size_t n, const double* in, float* out, cfa_stream_t stream);
 

@@ -192,5 +192,11 @@ struct RunResult {
 };
 // Fixed (uniform) mesh run, optionally starting from `initial_mesh_levels` (the product mesh)
 RunResult run_fixed(const RunConfig& cfg);
+// cfg.steps steps on a fixed mesh from a given state (uⁿ, vⁿ, pⁿ, uⁿ⁻¹, vⁿ⁻¹ and the pressure
+// increment as warm start) after step first_step - 1; wall time counted from t_wall0
+RunResult run_loop(const RunConfig& cfg, Mesh2D& mesh, Vec hu, Vec hv, Vec hp, Vec huo, Vec hvo, Vec dp0,
+                   int first_step, cfa::Clock::time_point t_wall0);
+// run_loop from a state file written by the program's --dump-state (mesh and fields of a step)
+RunResult run_state_file(const RunConfig& cfg, const std::string& path);
 
 }  // namespace oracle

@@ -34,6 +34,9 @@ def load():
         _lib.oracle_run_outlet_dirichlet.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                                      ctypes.c_int, ctypes.c_double, d]
         _lib.oracle_run_outlet_dirichlet.restype = ctypes.c_int
+        _lib.oracle_run_state.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_int, d, d]
+        _lib.oracle_run_state.restype = ctypes.c_int
     return _lib
 
 
@@ -71,3 +74,14 @@ def run_outlet_dirichlet(steps: int, dt: float, uniform_level: int, min_level: i
     k = lib.oracle_run_outlet_dirichlet(steps, dt, uniform_level, min_level, 1 if case_2d3 else 0, tol,
                                         rows.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return rows[:k]
+
+
+def run_state(path: str, steps: int, dt: float, tol: float = 1e-8, threads: int = 0):
+    """Fixed-mesh continuation of a --dump-state file for `steps` steps on `threads` OpenMP
+    threads (0: the runtime default). Returns (rows (steps x 13), wall seconds of the loop)."""
+    lib = load()
+    rows = np.zeros((steps, len(ROW_COLUMNS)))
+    wall = np.zeros(1)
+    d = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    k = lib.oracle_run_state(str(path).encode(), steps, dt, tol, threads, d(rows), d(wall))
+    return rows[:k], float(wall[0])

@@ -2,6 +2,10 @@
 // caller-owned host buffers; each function restates one reference routine (see the C++ side).
 #include <cstring>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include "cfa_oracle.hpp"
 
 using namespace oracle;
@@ -88,6 +92,35 @@ int oracle_run_outlet_dirichlet(int steps, double dt, int uniform_level, int min
         o[6] = s.elems;  o[7] = s.p_front;  o[8] = s.p_back;  o[9] = s.cd_s;  o[10] = s.cl_s;
         o[11] = s.it_u + 1000.0 * s.it_v;  o[12] = s.it_p;
     }
+    return (int)r.rows.size();
+}
+
+// Continues a run on a fixed mesh from a --dump-state file for `steps` steps (DFG inflow, 2D-2,
+// default flags), both solvers at relative tolerance tol, on `threads` OpenMP threads (<= 0: the
+// runtime's default): rows as oracle_run_fixed, wall seconds of the time loop in *wall_s.
+int oracle_run_state(const char* path, int steps, double dt, double tol, int threads, double* rows, double* wall_s) {
+#ifdef _OPENMP
+    const int prev = omp_get_max_threads();
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+    RunConfig c;
+    c.steps = steps;
+    c.dt = dt;
+    c.mom_tol = tol;
+    c.ppe_tol = tol;
+    c.quiet = true;
+    RunResult r = run_state_file(c, path);
+#ifdef _OPENMP
+    omp_set_num_threads(prev);
+#endif
+    for (size_t k = 0; k < r.rows.size(); ++k) {
+        const StepRow& s = r.rows[k];
+        double* o = rows + 13 * k;
+        o[0] = s.step;  o[1] = s.t;  o[2] = s.cd;  o[3] = s.cl;  o[4] = s.dp;  o[5] = s.nodes;
+        o[6] = s.elems;  o[7] = s.p_front;  o[8] = s.p_back;  o[9] = s.cd_s;  o[10] = s.cl_s;
+        o[11] = s.it_u + 1000.0 * s.it_v;  o[12] = s.it_p;
+    }
+    if (wall_s) *wall_s = r.wall_s;
     return (int)r.rows.size();
 }
 

@@ -66,12 +66,7 @@ void dump_raw(std::FILE* f, const void* data, size_t bytes) {
 }  // namespace
 
 RunResult run_fixed(const RunConfig& cfg) {
-    RunResult res;
     const auto t_wall0 = cfa::Clock::now();
-    const double dt = cfg.dt;
-    const double rot_coeff = cfg.rotational ? Sim::RHO * cfg.nu : 0.0;
-    const double force_scale = -2.0 / (Sim::U_MEAN * Sim::U_MEAN * 2.0 * Sim::R_CYL);
-    const double surface_scale = -force_scale;
 
     // mesh set-up as in src/cylinder_fem_amr.cu:169-207
     Mesh2D mesh;
@@ -101,6 +96,59 @@ RunResult run_fixed(const RunConfig& cfg) {
             hp[i] = r.p;
         }
 
+    return run_loop(cfg, mesh, hu, hv, hp, huo, hvo, Vec(mesh.num_nodes, 0.0), 1, t_wall0);
+}
+
+RunResult run_state_file(const RunConfig& cfg, const std::string& path) {
+    const auto t0 = cfa::Clock::now();
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) {
+        std::fprintf(stderr, "[oracle] cannot open %s\n", path.c_str());
+        return RunResult();
+    }
+    int hdr[4] = {0, 0, 0, 0};
+    double dt_file = 0.0;
+    bool ok = std::fread(hdr, sizeof(int), 4, f) == 4 && hdr[0] == 0x43464153 && hdr[1] > 0 && hdr[2] > 0 &&
+              std::fread(&dt_file, sizeof(double), 1, f) == 1;
+    const int n = ok ? hdr[1] : 0, ne = ok ? hdr[2] : 0;
+    Mesh2D mesh;
+    mesh.num_nodes = n;
+    mesh.num_elems = ne;
+    mesh.x.resize(n);
+    mesh.y.resize(n);
+    mesh.bc_tag.resize(n);
+    std::vector<int> ev(4 * (size_t)ne);
+    Vec fields[6];
+    ok = ok && std::fread(mesh.x.data(), sizeof(double), n, f) == (size_t)n &&
+         std::fread(mesh.y.data(), sizeof(double), n, f) == (size_t)n &&
+         std::fread(mesh.bc_tag.data(), sizeof(int), n, f) == (size_t)n &&
+         std::fread(ev.data(), sizeof(int), ev.size(), f) == ev.size();
+    for (Vec& a : fields) {
+        a.resize(n);
+        ok = ok && std::fread(a.data(), sizeof(double), n, f) == (size_t)n;
+    }
+    std::fclose(f);
+    if (!ok) {
+        std::fprintf(stderr, "[oracle] %s is not a state file\n", path.c_str());
+        return RunResult();
+    }
+    mesh.elems.resize(ne);
+    for (int e = 0; e < ne; ++e) {
+        for (int k = 0; k < 3; ++k) mesh.elems[e].v[k] = ev[4 * (size_t)e + k];
+        mesh.elems[e].gen = ev[4 * (size_t)e + 3];
+        mesh.elems[e].level = mesh.elems[e].gen / 2;
+    }
+    cfa::finalize_mesh(mesh);
+    return run_loop(cfg, mesh, fields[0], fields[1], fields[2], fields[3], fields[4], fields[5], hdr[3] + 1, t0);
+}
+
+RunResult run_loop(const RunConfig& cfg, Mesh2D& mesh, Vec hu, Vec hv, Vec hp, Vec huo, Vec hvo, Vec dp0,
+                   int first_step, cfa::Clock::time_point t_wall0) {
+    RunResult res;
+    const double dt = cfg.dt;
+    const double rot_coeff = cfg.rotational ? Sim::RHO * cfg.nu : 0.0;
+    const double force_scale = -2.0 / (Sim::U_MEAN * Sim::U_MEAN * 2.0 * Sim::R_CYL);
+    const double surface_scale = -force_scale;
     const int n = mesh.num_nodes, ne = mesh.num_elems, nnz = mesh.nnz;
     std::vector<int> ev(3 * (size_t)ne);
     for (int e = 0; e < ne; ++e)
@@ -127,7 +175,7 @@ RunResult run_fixed(const RunConfig& cfg) {
         huo = flux_balanced(mesh, huo);
     }
     Vec u = hu, v = hv, p = hp, u_old = huo, v_old = hvo, u_prev = huo, v_prev = hvo;
-    Vec dp(n, 0.0), vort(n, 0.0), A(nnz), Kp(nnz), rhs_u(n), rhs_v(n), rhs_p(n), mass(n);
+    Vec dp = dp0, vort(n, 0.0), A(nnz), Kp(nnz), rhs_u(n), rhs_v(n), rhs_p(n), mass(n);
     Vec gx(n), gy(n), vort_num(n), div_num(n), elem_s(ne), oss_val(4 * (size_t)nnz), inv_m(n);
     Ilu0 ilu;
     ilu.set_pattern(n, rp, ci);
@@ -135,7 +183,8 @@ RunResult run_fixed(const RunConfig& cfg) {
     bool amg_dirty = true;
     int amg_built = 0;
 
-    for (int step = 1; step <= cfg.steps; ++step) {
+    const int last_step = first_step + cfg.steps - 1;
+    for (int step = first_step; step <= last_step; ++step) {
         const bool bdf2 = cfg.bdf2 && step > 1;
         const double dt_proj = bdf2 ? (2.0 / 3.0) * dt : dt;
         const double a0 = bdf2 ? 1.5 / dt : 1.0 / dt;
@@ -277,14 +326,14 @@ RunResult run_fixed(const RunConfig& cfg) {
             std::fprintf(stderr, "[Step %d] non-finite solution detected, aborting\n", step);
             break;
         }
-        if (!cfg.quiet && (step % cfg.print_interval == 0 || step == 1 || step == cfg.steps))
+        if (!cfg.quiet && (step % cfg.print_interval == 0 || step == first_step || step == last_step))
             std::printf("[oracle step %5d] Iters(u,v,p): (%d,%d,%d) | C_D = %.10g | C_L = %.10g | dP = %.10g\n",
                         step, row.it_u, row.it_v, row.it_p, row.cd, row.cl, row.dp);
     }
     res.wall_s = cfa::ms_since(t_wall0) / 1000.0;
 
     if (cfg.mms) {
-        const double t_end = cfg.steps * dt;
+        const double t_end = last_step * dt;
         double e2[6] = {0, 0, 0, 0, 0, 0};
         mms_errors(ne, ev.data(), x, y, u.data(), v.data(), p.data(), t_end, e2);
         for (int k = 0; k < 6; ++k) res.mms[k] = std::sqrt(e2[k]);
@@ -316,7 +365,7 @@ RunResult run_fixed(const RunConfig& cfg) {
             if (g) {
                 std::fprintf(g, "nodes,elements,h_max,dt,t,err_u_L2,err_u_H1,err_p_L2,norm_u_L2,norm_u_H1,norm_p_L2,"
                                 "err_p_cyl_max,err_p_probe_max\n");
-                std::fprintf(g, "%d,%d,%.10g,%.10g,%.10g", n, ne, res.h_max, dt, cfg.steps * dt);
+                std::fprintf(g, "%d,%d,%.10g,%.10g,%.10g", n, ne, res.h_max, dt, last_step * dt);
                 for (int k = 0; k < 6; ++k) std::fprintf(g, ",%.10g", res.mms[k]);
                 std::fprintf(g, ",%.10g,%.10g\n", res.cyl_max, res.probe_max);
                 std::fclose(g);

@@ -1,6 +1,9 @@
 // ORACLE (test infrastructure only): Krylov solvers, multicolour ILU(0) and SA-AMG of
-// src/cfa/linalg.cuh restated serially. Stopping rules, breakdown guards and AMG constants are
-// the reference's; dot products are left-to-right sums.
+// src/cfa/linalg.cuh restated on the host. Stopping rules, breakdown guards and AMG constants are
+// the reference's. Row loops run under OpenMP (each row is computed by one thread, so results do
+// not depend on the thread count); dot products sum fixed 4096-entry chunks left to right and
+// then the chunk sums left to right, the same order for any number of threads. The AMG set-up
+// stays serial.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -11,13 +14,24 @@
 namespace oracle {
 
 double dot(int n, const double* a, const double* b) {
+    constexpr int CHUNK = 4096;
+    const int nch = (n + CHUNK - 1) / CHUNK;
+    Vec part(nch > 0 ? nch : 1, 0.0);
+#pragma omp parallel for schedule(static) if (nch > 4)
+    for (int c = 0; c < nch; ++c) {
+        const int hi = std::min(n, (c + 1) * CHUNK);
+        double s = 0.0;
+        for (int i = c * CHUNK; i < hi; ++i) s += a[i] * b[i];
+        part[c] = s;
+    }
     double s = 0.0;
-    for (int i = 0; i < n; ++i) s += a[i] * b[i];
+    for (int c = 0; c < nch; ++c) s += part[c];
     return s;
 }
 
 // src/cfa/linalg.cuh:8-26
 void spmv(int n, const int* rp, const int* ci, const double* val, const double* x, double* y) {
+#pragma omp parallel for schedule(static) if (n > 16384)
     for (int i = 0; i < n; ++i) {
         double s = 0.0;
         for (int k = rp[i]; k < rp[i + 1]; ++k) s += val[k] * x[ci[k]];
@@ -52,7 +66,9 @@ SolveInfo pcg(int n, const int* rp, const int* ci, const double* val, const doub
     if (r_norm < tol * b_norm) return info;
     auto apply_prec = [&]() {
         if (prec) (*prec)(r.data(), z.data());
-        else for (int i = 0; i < n; ++i) z[i] = inv_diag[i] * r[i];
+        else
+#pragma omp parallel for schedule(static) if (n > 16384)
+            for (int i = 0; i < n; ++i) z[i] = inv_diag[i] * r[i];
     };
     apply_prec();
     p = z;
@@ -64,8 +80,11 @@ SolveInfo pcg(int n, const int* rp, const int* ci, const double* val, const doub
         const double pq = dot(n, p.data(), q.data());
         if (std::fabs(pq) < 1.0e-30) break;
         const double alpha = rz_old / pq;
-        for (int i = 0; i < n; ++i) x[i] += alpha * p[i];
-        for (int i = 0; i < n; ++i) r[i] += -alpha * q[i];
+#pragma omp parallel for schedule(static) if (n > 16384)
+        for (int i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] += -alpha * q[i];
+        }
         r_norm = std::sqrt(dot(n, r.data(), r.data()));
         info.rel = r_norm / b_norm;
         if (r_norm < tol * b_norm) {
@@ -76,6 +95,7 @@ SolveInfo pcg(int n, const int* rp, const int* ci, const double* val, const doub
         apply_prec();
         const double rz_new = dot(n, r.data(), z.data());
         const double beta = rz_new / (rz_old + 1.0e-30);
+#pragma omp parallel for schedule(static) if (n > 16384)
         for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
         rz_old = rz_new;
     }
@@ -232,6 +252,7 @@ void Ilu0::apply(const double* r, double* z, int nrhs) const {
         double* zc = z + (size_t)comp * n;
         for (int c = 0; c < n_colours; ++c) {
             const bool last = (c == n_colours - 1);
+#pragma omp parallel for schedule(static) if (n > 16384)
             for (int t = colour_start[c]; t < colour_start[c + 1]; ++t) {
                 const int i = rows[t];
                 double acc = rc[i];
@@ -240,6 +261,7 @@ void Ilu0::apply(const double* r, double* z, int nrhs) const {
             }
         }
         for (int c = n_colours - 2; c >= 0; --c)
+#pragma omp parallel for schedule(static) if (n > 16384)
             for (int t = colour_start[c]; t < colour_start[c + 1]; ++t) {
                 const int i = rows[t];
                 double acc = zc[i];
@@ -292,6 +314,7 @@ Bicg2Info bicgstab2(int n, const int* rp, const int* ci, const double* val, cons
         }
         if (!(active[0] || active[1])) break;
         if (iter > 0)
+#pragma omp parallel for schedule(static) if (n > 16384)
             for (size_t i = 0; i < N; ++i) {
                 const int c = i >= (size_t)n;
                 p[i] = r[i] + cb[c] * (p[i] - co[c] * v[i]);
@@ -305,6 +328,7 @@ Bicg2Info bicgstab2(int n, const int* rp, const int* ci, const double* val, cons
             if (active[c]) { alpha[c] = rho[c] / o0[c]; ca[c] = alpha[c]; }
         }
         if (!(active[0] || active[1])) break;
+#pragma omp parallel for schedule(static) if (n > 16384)
         for (size_t i = 0; i < N; ++i) s[i] = r[i] - ca[i >= (size_t)n] * v[i];
         ilu.apply(s.data(), sh.data(), 2);
         spmv2(sh.data(), t.data());
@@ -322,6 +346,7 @@ Bicg2Info bicgstab2(int n, const int* rp, const int* ci, const double* val, cons
             a2[c] = omega[c];
             cw[c] = omega[c];
         }
+#pragma omp parallel for schedule(static) if (n > 16384)
         for (size_t i = 0; i < N; ++i) {
             const int c = i >= (size_t)n;
             x[i] += a1[c] * ph[i] + a2[c] * sh[i];
@@ -571,6 +596,7 @@ void Amg::setup(const Csr& A0) {
 
 namespace {
 void dense_matvec(int n, const Vec& Ainv, const double* x, double* y) {
+#pragma omp parallel for schedule(static) if (n > 128)
     for (int i = 0; i < n; ++i) {
         double s = 0.0;
         for (int j = 0; j < n; ++j) s += Ainv[(size_t)i * n + j] * x[j];
@@ -579,6 +605,7 @@ void dense_matvec(int n, const Vec& Ainv, const double* x, double* y) {
 }
 // y = x + omega D^-1 (b - A x)
 void jacobi(const Csr& A, const Vec& dinv, double omega, const double* x, const double* b, double* y) {
+#pragma omp parallel for schedule(static) if (A.rows > 16384)
     for (int i = 0; i < A.rows; ++i) {
         double s = 0.0;
         for (int k = A.rp[i]; k < A.rp[i + 1]; ++k) s += A.v[k] * x[A.ci[k]];
@@ -590,11 +617,13 @@ void jacobi(const Csr& A, const Vec& dinv, double omega, const double* x, const 
 void Amg::vcycle(int l, const double* b) {
     Level& L = levels[l];
     const int n = L.A.rows;
+#pragma omp parallel for schedule(static) if (n > 16384)
     for (int i = 0; i < n; ++i) L.x[i] = L.omega * L.dinv[i] * b[i];
     for (int s = 1; s < smooth_steps; ++s) {
         jacobi(L.A, L.dinv, L.omega, L.x.data(), b, L.Ax.data());
         std::swap(L.x, L.Ax);
     }
+#pragma omp parallel for schedule(static) if (n > 16384)
     for (int i = 0; i < n; ++i) {
         double s = 0.0;
         for (int k = L.A.rp[i]; k < L.A.rp[i + 1]; ++k) s += L.A.v[k] * L.x[L.A.ci[k]];
@@ -611,6 +640,7 @@ void Amg::vcycle(int l, const double* b) {
         dense_matvec(coarse_n, coarse_inv, cb.data(), cx.data());
         cxv = cx.data();
     }
+#pragma omp parallel for schedule(static) if (n > 16384)
     for (int i = 0; i < n; ++i) {
         double s = 0.0;
         for (int k = L.P.rp[i]; k < L.P.rp[i + 1]; ++k) s += L.P.v[k] * cxv[L.P.ci[k]];

@@ -496,16 +496,8 @@ void case_amg(Fixture& f) {
         d_z.reserve(f.n);
         Vec z(f.n);
         oa.apply(r.data(), z.data());
-        // with every level launched on its own, and with the bottom levels that fit the LDS in the
-        // single-workgroup tail kernel
-        const long long limit = cfa_amg_set_tail_limit(0);
         amg.apply(d_r, d_z);
         close_rel("one V-cycle", down(d_z, f.n), z, 1e-12);
-        cfa_amg_set_tail_limit(1LL << 40);
-        std::printf("  V-cycle tail from level %d of %d\n", cfa_amg_tail_start(amg.view()), amg.num_levels());
-        amg.apply(d_r, d_z);
-        close_rel("one V-cycle, tail kernel", down(d_z, f.n), z, 1e-12);
-        cfa_amg_set_tail_limit(limit);
     }
 }
 
@@ -654,11 +646,6 @@ void case_mixed(Fixture& f) {
     Vec z(f.n);
     oa.apply(r.data(), z.data());
     close_rel("one V-cycle, FP32 vs FP64", down(d_z, f.n), z, 1e-5);
-    const long long limit = cfa_amg_set_tail_limit(1LL << 40);
-    std::printf("  FP32 V-cycle tail from level %d of %d\n", cfa_amg_tail_start(amg.view()), amg.num_levels());
-    amg.apply(d_r, d_z);
-    close_rel("one V-cycle, FP32 vs FP64, tail kernel", down(d_z, f.n), z, 1e-5);
-    cfa_amg_set_tail_limit(limit);
     const double tol = 1e-10;
     f.g.d_dp.upload(Vec(f.n, 0.0));
     GpuLinearSolver solver;

@@ -1,22 +1,30 @@
 #!/usr/bin/env python3
 """Benchmark of the per-step projection solve on one MI355X.
 
-Workload (default): the adaptive obstacle flow of the reference's runs/paper/d2_A2_dt05 —
-parabolic inflow (mean 1, Re 100), NVB levels 2-5 with coarsening and look-ahead, remesh every
-20 steps (overlapped by 10), dt 5e-4, FP64 throughout — with the momentum BiCGSTAB and pressure
-PCG run to a relative residual of 1e-8. A step is one time step (momentum predictor, pressure
-increment, correction, forces, and the remesh work that falls on it). `--warmup W` steps run
-first, untimed; then exactly `--steps K` steps are timed. The program synchronises the device
-at the end of every step (the forces are read back), and the timed region runs from the end of
-step W to the end of step W + K on the host's steady clock.
+Workload (default): the adaptive obstacle flow with NVB refinement capped at level 6 (levels 2-6,
+element budget 600k, remesh every 20 steps overlapped by 10, look-ahead, coarsening, dt 5e-4,
+parabolic inflow of mean 1, Re 100), the momentum BiCGSTAB and pressure PCG run to a relative
+residual of 1e-8. A step is one time step: momentum predictor, pressure increment, correction,
+forces, and the remesh work that falls on it (the launch of a remesh, and at its join the upload of
+the new mesh, the solution transfer, the scheme-consistent re-projection and the AMG set-up).
 
-After the timed run the main kernels are launched alone on the final mesh and state
+The run goes through an untimed spin-up (--spinup, 1000 steps by default, so the window sees a
+developed wake and its refined mesh), then --warmup W untimed steps, then exactly --steps K timed
+steps, then an untimed tail of overlap + 1 steps (a remesh launched in the window is only allowed
+when its join falls before the last step). Any 20 consecutive steps hold one remesh launch and one
+join. The program synchronises the device at the end of every step (the forces are read back), and
+the timed region runs from the end of step spin-up + W to the end of step spin-up + W + K on the
+host's steady clock. The whole run is done twice: with the FP32 ILU(0) factors and FP32 SA-AMG
+cycle inside the FP64 Krylov solvers (`value`), and in FP64 (`fp64`); the force coefficients of
+the two runs are compared step by step while their meshes are identical.
+
+After the FP64 run the main kernels are launched alone on the final mesh and state
 (--kernel-timing) and timed with HIP events on the solver stream ("kernels"). The roofline object
 reports the dominant kernel family's level-0 member, the damped-Jacobi sweep of the SA-AMG
-V-cycle, measured the same way on a fixed shape (the pressure matrix of the uniform level-4
-mesh, 115,648 rows, the size of the adaptive run's late meshes), so that the hardware-counter
-passes can profile exactly that launch (--roofline-only). The CPU baseline is the oracle (the
-serial FP64 CPU restatement of the same step) on a bounded sample, run on this host.
+V-cycle, measured the same way on a fixed shape so that the hardware-counter passes can profile
+exactly that launch (--roofline-only). The CPU baseline is the oracle (the FP64 CPU restatement of
+the step, OpenMP) continuing the FP64 run from the state at the start of the timed window, on all
+cores this process may use and with the reference's thread rule min(8, nproc / 2).
 
 Prints one JSON line.
 """
@@ -54,19 +62,48 @@ DESCRIPTION = {
 }
 
 
-def cpu_baseline(seconds_target: float = 15.0) -> dict:
-    """The oracle on a bounded sample: fixed uniform level-3 mesh (29,152 nodes, the size of the
-    adaptive run's early meshes), DFG inflow, dt 5e-4, tolerances 1e-8, serial FP64."""
+def host_cores() -> dict:
+    """Threads this process may use, the machine's physical cores (lscpu) and its CPU model."""
+    import subprocess
+    avail = len(os.sched_getaffinity(0))
+    phys, model = None, ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        kv = {k.strip(): v.strip() for k, v in (ln.split(":", 1) for ln in out.splitlines() if ":" in ln)}
+        phys = int(kv.get("Core(s) per socket", "0")) * int(kv.get("Socket(s)", "0")) or None
+        model = kv.get("Model name", "")
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    share = int(omp) if omp and omp.isdigit() and int(omp) > 0 else avail
+    return {"affinity": avail, "physical": phys, "omp_num_threads": omp, "model": model,
+            "threads": max(1, min(share, avail, phys or avail))}
+
+
+def cpu_baseline(state_file: str, dt: float, tol: float, seconds_target: float = 12.0) -> dict:
+    """The oracle (the FP64 CPU restatement of the step, OpenMP row loops) continuing the benchmark
+    run from the state at the start of the timed window (mesh and fields written by --dump-state),
+    on a fixed mesh for a bounded number of steps: once on every core this process may use and once
+    with the reference's host thread rule min(8, nproc / 2) (src/cylinder_fem_amr.cu:138)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import cfa_oracle
-    steps = 5
-    rows, _, wall = cfa_oracle.run_fixed(steps, 5e-4, 3, 2, mms=False, mom_tol=1e-8, ppe_tol=1e-8)
-    per_step = wall / max(len(rows), 1)
-    steps = max(5, min(400, int(seconds_target / max(per_step, 1e-6))))
-    rows, _, wall = cfa_oracle.run_fixed(steps, 5e-4, 3, 2, mms=False, mom_tol=1e-8, ppe_tol=1e-8)
-    return {"value": round(len(rows) / wall, 3), "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (serial FP64 CPU restatement) on a fixed uniform level-3 mesh (29152 nodes), "
-                      f"DFG inflow, dt 5e-4, tol 1e-8, {len(rows)} steps in {wall:.1f} s"}
+    cores = host_cores()
+    runs = {}
+    for label, threads in (("all_cores", cores["threads"]), ("reference_rule", max(1, min(8, cores["affinity"] // 2)))):
+        rows, wall = cfa_oracle.run_state(state_file, 2, dt, tol, threads)
+        per_step = wall / max(len(rows), 1)
+        steps = max(2, min(200, int(seconds_target / max(per_step, 1e-6))))
+        rows, wall = cfa_oracle.run_state(state_file, steps, dt, tol, threads)
+        runs[label] = {"value": round(len(rows) / wall, 4), "threads": threads, "steps": len(rows),
+                       "seconds": round(wall, 2),
+                       "nodes": int(rows[0, cfa_oracle.ROW_COLUMNS.index("nodes")]) if len(rows) else 0}
+    main = runs["all_cores"]
+    return {"value": main["value"], "unit": "steps/s", "cores": main["threads"], "kind": "port",
+            "sample": f"oracle (FP64 CPU restatement, OpenMP) continuing the FP64 run from the state at the start "
+                      f"of the timed window, fixed mesh of {main['nodes']} nodes, dt {dt:g}, tol {tol:g}: "
+                      f"{main['steps']} steps in {main['seconds']} s on {main['threads']} threads (the window's "
+                      f"remesh work is not in the sample)",
+            "reference_thread_rule": runs["reference_rule"], "host": cores}
 
 
 def trace_summary(path: str, reps: int, top: int = 25) -> int:
@@ -176,44 +213,58 @@ def roofline_object(dom: dict) -> dict:
     }
 
 
-def mixed_comparison(flags: list[str], hist64, warmup: int, total: int) -> dict:
-    """The same workload with FP32 ILU factors and FP32 SA-AMG cycle inside the FP64 Krylov solvers:
-    steps/s over the same timed steps and the force deviation from the FP64 run, step by step
-    while the two runs still have the same mesh (remesh decisions are made from the solution, so
-    the adaptive trajectories may part)."""
-    import cfa
+def remesh_events(flags: list[str], total: int, first: int, last: int) -> dict:
+    """Remesh launches and joins in steps first..last of a run of `total` steps (the program's rule:
+    a remesh is launched at the end of every interval-th step unless its join, `overlap` steps
+    later, would fall at or after the last step; the join is the end of step launch + overlap)."""
+    f = {flags[i]: flags[i + 1] for i in range(len(flags) - 1) if flags[i].startswith("--")}
+    interval, overlap = int(f.get("--amr-interval", 0)), int(f.get("--amr-overlap", 0))
+    launches = [s for s in range(interval, total, interval) if not (overlap and s + overlap >= total)] if interval else []
+    joins = [s + overlap for s in launches] if overlap else list(launches)
+    return {"launch_steps": [s for s in launches if first <= s <= last],
+            "join_steps": [s for s in joins if first <= s <= last]}
+
+
+def window_summary(hist, first: int, last: int) -> dict:
+    """Steps/s over the window of steps first..last (1-based, from the end of step first - 1 to the
+    end of step last on the host's steady clock) and what the window ran on."""
     import numpy as np
-    mixed_flags = [f if f != "fp64" else "mixed" for f in flags]
-    if cfa.run(mixed_flags) != 0:
-        return {"error": "mixed-precision run failed"}
-    h = cfa.last_history()
-    col = {c: i for i, c in enumerate(cfa.HISTORY_COLUMNS)}
-    t0 = h[warmup - 1, col["elapsed_s"]] if warmup > 0 else 0.0
-    timed = h[total - 1, col["elapsed_s"]] - t0
-    same = (h[:, col["nodes"]] == hist64[:, col["nodes"]]) & (h[:, col["elements"]] == hist64[:, col["elements"]])
+    col = {c: i for i, c in enumerate(__import__("cfa").HISTORY_COLUMNS)}
+    t0 = hist[first - 2, col["elapsed_s"]] if first >= 2 else 0.0
+    timed = hist[last - 1, col["elapsed_s"]] - t0
+    w = hist[first - 1:last]
+    k = last - first + 1
+    return {"value": round(k / timed, 3), "ms_per_step": round(1000.0 * timed / k, 4), "seconds": timed,
+            "nodes_range": [int(w[:, col["nodes"]].min()), int(w[:, col["nodes"]].max())],
+            "elements_range": [int(w[:, col["elements"]].min()), int(w[:, col["elements"]].max())],
+            "mean_iters_u_v_p": [round(float(w[:, col[c]].mean()), 2) for c in ("it_u", "it_v", "it_p")],
+            "mesh_changes": int(np.count_nonzero(np.diff(hist[first - 2:last, col["elements"]]))) if first >= 2 else 0}
+
+
+def force_deviation(h64, hmx, first: int, last: int) -> dict:
+    """Per-step |dC_D|, |dC_L| between the FP64 and the mixed-precision runs: over the window, over
+    the whole run up to the first step whose mesh differs, and from step 100 on (after the impulsive
+    start, C_D near 276 at step 1)."""
+    import numpy as np
+    col = {c: i for i, c in enumerate(__import__("cfa").HISTORY_COLUMNS)}
+    same = (h64[:, col["nodes"]] == hmx[:, col["nodes"]]) & (h64[:, col["elements"]] == hmx[:, col["elements"]])
     upto = int(np.argmin(same)) if not same.all() else len(same)
-    d_cd = np.abs(h[:upto, col["C_D"]] - hist64[:upto, col["C_D"]])
-    d_cl = np.abs(h[:upto, col["C_L"]] - hist64[:upto, col["C_L"]])
-    return {
-        "value": round((total - warmup) / timed, 3),
-        "unit": "steps/s",
-        "ms_per_step": round(1000.0 * timed / (total - warmup), 4),
-        "steps_on_identical_meshes": upto,
-        "max_abs_dC_D": float(d_cd.max()) if upto else None,
-        "max_abs_dC_L": float(d_cl.max()) if upto else None,
-        # the same after the impulsive start (C_D near 276 at step 1, where a 1e-10 relative
-        # difference is already 3e-8 absolute)
-        "max_abs_dC_D_from_step_100": float(d_cd[99:].max()) if upto > 99 else None,
-        "max_abs_dC_L_from_step_100": float(d_cl[99:].max()) if upto > 99 else None,
-        "mean_iters_u_v_p": [round(float(h[warmup:total, col[c]].mean()), 2) for c in ("it_u", "it_v", "it_p")],
-    }
+    out = {"steps_on_identical_meshes": upto, "window_on_identical_meshes": bool(upto >= last)}
+    for c in ("C_D", "C_L"):
+        d = np.abs(h64[:upto, col[c]] - hmx[:upto, col[c]])
+        out[f"max_abs_d{c}"] = float(d.max()) if upto else None
+        out[f"max_abs_d{c}_from_step_100"] = float(d[99:].max()) if upto > 99 else None
+        out[f"max_abs_d{c}_window"] = float(d[first - 1:last].max()) if upto >= last else None
+    out["target"] = "|dC_D|, |dC_L| <= 1e-8"
+    return out
 
 
 def adjoint_cost(level: int, spinup: int, window: int, every: int, tol: float) -> dict:
     """Discrete adjoint of the window functional J = mean C_D over `window` steps on the frozen
-    uniform level-`level` mesh after a `spinup`-step FP64 run from the DFG initial condition,
-    theta = (nu, U_max): time of the forward window, time of the gradient (forward pass with
-    checkpoints every `every` steps, segment recomputation and reverse sweep), and their ratio."""
+    uniform level-`level` mesh after a `spinup`-step FP64 run from the DFG initial condition, with
+    respect to all 66 parameters (nu, U_max and the 64 radial modes of the hole boundary): time of
+    the forward window, time of the gradient (forward pass with checkpoints every `every` steps,
+    segment recomputation and reverse sweep), and their ratio."""
     import time
     import cfa
     theta = [1.0e-3, 1.5]
@@ -225,13 +276,15 @@ def adjoint_cost(level: int, spinup: int, window: int, every: int, tol: float) -
     J, _, _ = a.forward(theta, spinup + 1, window, state)
     fwd_s = time.perf_counter() - t
     t = time.perf_counter()
-    J2, g, _, st = a.gradient(theta, spinup + 1, window, state, every)
+    J2, g, _, st = a.gradient(theta, spinup + 1, window, state, every, shape_gradient=True)
     grad_s = time.perf_counter() - t
     a.close()
     return {
         "workload": f"uniform level-{level} mesh ({a.n} nodes, {a.ne} elements), {spinup}-step spin-up, "
                     f"{window}-step window, checkpoints every {every}, tol {tol:g}",
+        "parameters": 2 + len(st["dJ_dshape"]),
         "J": J, "dJ_dnu": float(g[0]), "dJ_dUmax": float(g[1]),
+        "dJ_dshape_norm": float(sum(v * v for v in st["dJ_dshape"]) ** 0.5),
         "forward_s": round(fwd_s, 4), "gradient_s": round(grad_s, 4),
         "adjoint_over_forward": round(grad_s / fwd_s, 3),
         "reverse_over_forward": round(st["reverse_ms"] / 1000.0 / fwd_s, 3),
@@ -244,10 +297,11 @@ def adjoint_cost(level: int, spinup: int, window: int, every: int, tol: float) -
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="adaptive")
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--spinup", type=int, default=None,
+                    help="untimed steps before the warm-up (default: 1000 for the adaptive workloads, 0 otherwise)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="adaptive_l6")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--kernel-reps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -257,7 +311,6 @@ def main() -> int:
     ap.add_argument("--solver-workloads", type=int, metavar="LEVEL",
                     help="run the isolated solver workloads on a uniform mesh of this level and exit")
     ap.add_argument("--nrhs", type=int, default=32)
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run of the same workload")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward timing")
     ap.add_argument("--adjoint-level", type=int, default=4)
     ap.add_argument("--adjoint-spinup", type=int, default=1000)
@@ -270,6 +323,10 @@ def main() -> int:
         return trace_summary(args.trace_summary, args.kernel_reps)
     if args.counter_summary:
         return counter_summary(args.counter_summary, args.kernel_reps)
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU (the path's sharding is a design note only); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        return 2
     if args.solver_workloads is not None:
         return solver_workloads(args.solver_workloads, args.nrhs, args.tol, args.kernel_reps)
     if args.roofline_only:
@@ -278,64 +335,81 @@ def main() -> int:
                "kernels": {k: {"us": round(v["us"], 3), "bytes": v["bytes"]} for k, v in ks.items()}}
         print(json.dumps(out), flush=True)
         return 0
-    if args.gpus != 1:
-        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
-        return 2
     if args.steps < 1 or args.warmup < 0:
         print("bench.py: --steps must be >= 1 and --warmup >= 0", file=sys.stderr)
         return 2
     if args.no_graphs:
         os.environ["CFA_KRYLOV_GRAPHS"] = "0"
+    import tempfile
     import cfa
 
-    total = args.warmup + args.steps
-    flags = WORKLOADS[args.workload] + [
-        "--steps", str(total), "--mom-tol", str(args.tol), "--ppe-tol", str(args.tol), "--precision", args.precision,
-        "--quiet", "--no-csv", "--kernel-timing", str(args.kernel_reps)]
-    rc = cfa.run(flags)
-    if rc != 0:
-        print(f"bench.py: run failed with status {rc}", file=sys.stderr)
-        return 1
-    hist = cfa.last_history()
-    col = {c: i for i, c in enumerate(cfa.HISTORY_COLUMNS)}
-    t_end = hist[total - 1, col["elapsed_s"]]
-    t_start = hist[args.warmup - 1, col["elapsed_s"]] if args.warmup > 0 else 0.0
-    timed = t_end - t_start
-    window = hist[args.warmup:total]
-    kernels = cfa.last_kernels()
+    base = WORKLOADS[args.workload]
+    adaptive = "--amr-interval" in base
+    spinup = args.spinup if args.spinup is not None else (1000 if adaptive else 0)
+    first = spinup + args.warmup + 1                 # first timed step
+    last = spinup + args.warmup + args.steps         # last timed step
+    # untimed tail: a remesh launched inside the window is only allowed when its join falls
+    # before the last step of the run
+    tail = int(base[base.index("--amr-overlap") + 1]) + 1 if adaptive else 0
+    total = last + tail
+    dt = float(base[base.index("--dt") + 1])
+    tmp = tempfile.mkdtemp(prefix="cfa_bench_")
+    common = base + ["--steps", str(total), "--mom-tol", str(args.tol), "--ppe-tol", str(args.tol), "--quiet",
+                     "--no-csv", "--prefix", os.path.join(tmp, "run")]
+    hist, summary = {}, {}
+    kernels = {}
+    for prec in ("fp64", "mixed"):
+        flags = common + ["--precision", prec]
+        if prec == "fp64":
+            flags += ["--dump-state", str(first - 1), "--kernel-timing", str(args.kernel_reps)]
+        rc = cfa.run(flags)
+        if rc != 0:
+            print(f"bench.py: {prec} run failed with status {rc}", file=sys.stderr)
+            return 1
+        hist[prec] = cfa.last_history()
+        summary[prec] = window_summary(hist[prec], first, last)
+        if prec == "fp64":
+            kernels = cfa.last_kernels()
+    mx, f64 = summary["mixed"], summary["fp64"]
+    events = remesh_events(common, total, first, last)
     line = {
         "metric": "steps/s at 1e-8 rel. residual, FP32-precond vs FP64 GPU; adjoint/forward time",
-        "value": round(args.steps / timed, 3),
+        "value": mx["value"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * timed / args.steps, 4),
+        "ms_per_step": mx["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64 (FP32 preconditioner factors)",
-        "data": "synthetic (deterministic: the reference's inflow / manufactured-solution set-up, no seeds)",
+        "dtype": "f64 Krylov, residuals, coarse solve and forces; f32 ILU(0) factors and SA-AMG cycle",
+        "data": "synthetic (deterministic: the reference's inflow set-up, no seeds)",
         "config": {
-            "workload": f"{args.workload}: {DESCRIPTION[args.workload]}, tol {args.tol:g}, steps "
-                        f"{args.warmup + 1}-{total}",
-            "precision": args.precision,
-            "nodes_range": [int(window[:, col["nodes"]].min()), int(window[:, col["nodes"]].max())],
-            "elements_range": [int(window[:, col["elements"]].min()), int(window[:, col["elements"]].max())],
-            "mean_iters_u_v_p": [round(float(window[:, col[c]].mean()), 2) for c in ("it_u", "it_v", "it_p")],
+            "workload": f"{args.workload}: {DESCRIPTION[args.workload]}, tol {args.tol:g}; untimed spin-up of "
+                        f"{spinup} steps and warm-up of {args.warmup}, timed steps {first}-{last}, untimed tail "
+                        f"of {tail}; value = FP32-preconditioned run, fp64 = the same window in FP64",
+            "precision": "mixed",
+            "spinup_steps": spinup,
+            "timed_steps": [first, last],
+            "remesh_in_window": events,
+            "nodes_range": mx["nodes_range"],
+            "elements_range": mx["elements_range"],
+            "mean_iters_u_v_p": mx["mean_iters_u_v_p"],
             "parallelism": "1 GPU",
         },
+        "fp64": {k: f64[k] for k in ("value", "ms_per_step", "nodes_range", "elements_range", "mean_iters_u_v_p")},
+        "mixed_precision": dict(force_deviation(hist["fp64"], hist["mixed"], first, last),
+                                speedup_vs_fp64=round(mx["value"] / f64["value"], 3)),
         "roofline": None,
         "kernels": {k: {"us": round(v["us"], 3), "GB/s": round(v["bytes"] / (v["us"] * 1e3), 1)}
                     for k, v in kernels.items()},
-        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(),
+        "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(os.path.join(tmp, "run_state.bin"), dt, args.tol)
     dom, _ = roofline_measurement(args.kernel_reps, args.tol)
     line["roofline"] = roofline_object(dom)
-    if not args.no_mixed and args.precision == "fp64":
-        line["mixed_precision"] = mixed_comparison(flags, hist, args.warmup, total)
-        if "value" in line["mixed_precision"]:
-            line["mixed_precision"]["speedup_vs_fp64"] = round(line["mixed_precision"]["value"] / line["value"], 3)
     if not args.no_adjoint:
         line["adjoint"] = adjoint_cost(args.adjoint_level, args.adjoint_spinup, args.adjoint_window,
                                        args.adjoint_every, args.tol)

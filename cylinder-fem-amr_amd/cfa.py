@@ -181,6 +181,43 @@ def workload_momentum(level: int, tol: float = 1e-8, kernel_reps: int = 0) -> di
     return dict(zip(MOMENTUM_FIELDS, out.tolist()))
 
 
+def workload_keep_systems(on: bool = True) -> None:
+    """Keep host copies of the following workload systems (cfa_workload_keep_systems)."""
+    _, host = load()
+    host.cfa_workload_keep_systems.argtypes = [ctypes.c_int]
+    host.cfa_workload_keep_systems.restype = ctypes.c_int
+    host.cfa_workload_keep_systems(1 if on else 0)
+
+
+def workload_system(which: str) -> dict:
+    """The kept system of the last pressure or momentum workload: rp, ci, tag, val, b, x0 (one row
+    per right-hand side) and x (pressure: one solution per right-hand side; momentum: u, v with FP64
+    factors, then u, v with FP32 factors)."""
+    _, host = load()
+    k = {"pressure": 0, "momentum": 1}[which]
+    ip = ctypes.POINTER(ctypes.c_int)
+    host.cfa_workload_system_sizes.argtypes = [ctypes.c_int, ip, ip, ip]
+    host.cfa_workload_system_sizes.restype = ctypes.c_int
+    host.cfa_workload_system.argtypes = [ctypes.c_int, ip, ip, ip] + [ctypes.POINTER(ctypes.c_double)] * 4 + [ip]
+    host.cfa_workload_system.restype = ctypes.c_int
+    n, nnz, cols = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(host.cfa_workload_system_sizes(k, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(cols)),
+           "cfa_workload_system_sizes")
+    n, nnz, cols = n.value, nnz.value, cols.value
+    rp = np.zeros(n + 1, np.int32)
+    ci = np.zeros(nnz, np.int32)
+    tag = np.zeros(n, np.int32)
+    val = np.zeros(nnz)
+    b = np.zeros((cols, n))
+    x0 = np.zeros((cols, n))
+    x = np.zeros((cols if k == 0 else 4, n))
+    iters = np.zeros(x.shape[0], np.int32)
+    _check(host.cfa_workload_system(k, _iptr(rp), _iptr(ci), _iptr(tag), _dptr(val), _dptr(b), _dptr(x0), _dptr(x),
+                                    _iptr(iters)), "cfa_workload_system")
+    return {"n": n, "nnz": nnz, "rp": rp, "ci": ci, "tag": tag, "val": val, "b": b, "x0": x0, "x": x,
+            "iters": iters}
+
+
 DUMP_MOMENTUM_FIELDS = ("n", "nnz", "colours", "colouring_ms", "factor_ms", "solve_ms", "it_u", "it_v",
                         "converged_u", "converged_v", "rel_dev_from_dumped")
 DUMP_PRESSURE_FIELDS = ("n", "nnz", "amg_levels", "amg_setup_ms", "solve_ms", "iterations", "converged",

@@ -98,6 +98,20 @@ int cfa_stream_sync(cfa_stream_t stream) {
     return e == hipSuccess ? CFA_OK : (int)e;
 }
 
+int cfa_stream_create(cfa_stream_t* stream) {
+    if (!stream) return CFA_E_ARG;
+    hipStream_t s = nullptr;
+    const hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    *stream = s;
+    return e == hipSuccess ? CFA_OK : (int)e;
+}
+
+int cfa_stream_destroy(cfa_stream_t stream) {
+    if (!stream) return CFA_OK;
+    const hipError_t e = hipStreamDestroy(as_stream(stream));
+    return e == hipSuccess ? CFA_OK : (int)e;
+}
+
 int cfa_device_sync(void) {
     const hipError_t e = hipDeviceSynchronize();
     return e == hipSuccess ? CFA_OK : (int)e;

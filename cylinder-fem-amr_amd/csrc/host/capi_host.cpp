@@ -91,6 +91,7 @@ int cfa_amg_setup_record(int n, const int* rp, const int* ci, const double* val,
     d_ci.upload(ci, rp[n]);
     d_v.upload(val, rp[n]);
     AmgPreconditioner amg;
+    amg.keep_aggregates = agg0 != nullptr;
     amg.setup(n, rp[n], d_rp, d_ci, d_v);
     const int nl = (int)amg.levels.size();
     for (int l = 0; l < nl && l < 16; ++l) {

@@ -55,31 +55,6 @@ PressureProbe find_cylinder_probe(const Mesh2D& mesh, double theta) {
     return best;
 }
 
-bool DeviceLeafSorter::operator()(std::vector<std::pair<uint32_t, int>>& keyed) {
-    const int n = (int)keyed.size();
-    if (n == 0) return true;
-    for (int i = 1; i < n; ++i)   // the stable sort reproduces the host order only for ascending ids
-        if (keyed[i].second <= keyed[i - 1].second) return false;
-    const size_t wsb = cfa_sort_pairs_ws_bytes(n);
-    if (wsb == 0) return false;
-    keys_.resize(n);
-    ids_.resize(n);
-    for (int i = 0; i < n; ++i) {
-        keys_[i] = keyed[i].first;
-        ids_[i] = keyed[i].second;
-    }
-    keys_in_.upload(keys_);
-    ids_in_.upload(ids_);
-    keys_out_.reserve(n);
-    ids_out_.reserve(n);
-    ws_.reserve(wsb);
-    GPU_CHECK(cfa_sort_pairs_u32(n, keys_in_, ids_in_, keys_out_, ids_out_, ws_, wsb, solver_stream()));
-    keys_out_.download(keys_.data(), n);
-    ids_out_.download(ids_.data(), n);
-    for (int i = 0; i < n; ++i) keyed[i] = {keys_[i], ids_[i]};
-    return true;
-}
-
 bool device_topology_default() {
     static const bool on = [] {
         const char* e = std::getenv("CFA_HOST_TOPOLOGY");

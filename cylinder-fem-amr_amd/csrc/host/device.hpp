@@ -81,20 +81,6 @@ private:
                        const std::vector<double>& h_vo);
 };
 
-// Device sorter for AdaptiveMesh::leaf_sorter: the (Morton key, id) pairs sorted by a stable radix
-// sort on the solver stream (cfa_sort_pairs_u32), the same order as the host's std::sort when the
-// ids arrive ascending (they do: extract collects the leaves in forest order).
-class DeviceLeafSorter {
-public:
-    bool operator()(std::vector<std::pair<uint32_t, int>>& keyed);
-private:
-    DevArray<unsigned> keys_in_, keys_out_;
-    DevArray<int> ids_in_, ids_out_;
-    DevArray<unsigned char> ws_;
-    std::vector<unsigned> keys_;
-    std::vector<int> ids_;
-};
-
 // Synchronous remeshes build the CSR graph and check the mesh on the device unless
 // CFA_HOST_TOPOLOGY=1 is set (the host path, for comparisons).
 bool device_topology_default();

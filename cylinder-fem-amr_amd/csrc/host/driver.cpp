@@ -202,11 +202,6 @@ int run_driver(int argc, char** argv) {
     DeviceMeshBuffers g;
     g.outlet_dirichlet = opt.outlet_dirichlet;
     g.sync_from_host(mesh, h_u, h_v, h_p, h_uo, h_vo);
-    // synchronous remeshes sort the leaves' Morton keys on the device (the overlapped ones run the
-    // extraction on the worker thread, which keeps to the host)
-    DeviceLeafSorter leaf_sorter;
-    if (opt.amr_overlap <= 0 && device_topology_default())
-        amr.leaf_sorter = [&leaf_sorter](std::vector<std::pair<uint32_t, int>>& keyed) { return leaf_sorter(keyed); };
     GpuLinearSolver solver;
     if (opt.init_amr > 0)
         for (int cycle = 0; cycle < opt.init_amr; ++cycle) {
@@ -741,8 +736,8 @@ int run_driver(int argc, char** argv) {
     std::cout << " | pressure " << pressure_ms_total / 1000.0 << " s (assembly, AMG setup, CG)\n";
     remesh_prof.report(amr_ms_total);
     {
-        static const char* dn[] = {"adjacency", "traces", "trace merge", "leaf collect", "morton keys",
-                                   "morton sort", "vertex compaction", "mesh fill"};
+        static const char* dn[] = {"marking", "coarsening", "refinement", "leaf lists", "extraction",
+                                   "mesh download"};
         std::cout << "[Timing] AMR sub-phases:";
         for (int i = 0; i < AdaptiveMesh::D_COUNT; ++i)
             std::cout << " " << dn[i] << " " << std::setprecision(2) << amr.dbg_ms[i] / 1000.0 << " s;";

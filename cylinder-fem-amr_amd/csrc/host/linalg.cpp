@@ -334,82 +334,10 @@ void GpuBicgstab2::solve(int n, const int* d_row_ptr, const int* d_col_ind, cons
     }
 }
 
+
 // ---------------------------------------------------------------------------------------------
-// Host CSR products (src/cfa/linalg.cuh:799-862)
+// SA-AMG (src/cfa/linalg.cuh:939-1432): the set-up runs on the device level by level
 // ---------------------------------------------------------------------------------------------
-HostCsr csr_multiply(const HostCsr& A, const HostCsr& B) {
-    HostCsr C;
-    C.rows = A.rows;
-    C.cols = B.cols;
-    C.rp.assign(A.rows + 1, 0);
-#pragma omp parallel
-    {
-        std::vector<int> marker(B.cols, -1);
-#pragma omp for schedule(static)
-        for (int i = 0; i < A.rows; ++i) {
-            int count = 0;
-            for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
-                for (int b = B.rp[A.ci[a]]; b < B.rp[A.ci[a] + 1]; ++b) {
-                    const int j = B.ci[b];
-                    if (marker[j] != i) { marker[j] = i; ++count; }
-                }
-            C.rp[i + 1] = count;
-        }
-    }
-    for (int i = 0; i < A.rows; ++i) C.rp[i + 1] += C.rp[i];
-    C.ci.resize(C.rp[A.rows]);
-    C.v.resize(C.rp[A.rows]);
-#pragma omp parallel
-    {
-        std::vector<int> marker(B.cols, -1), cols;
-        std::vector<double> acc(B.cols, 0.0);
-#pragma omp for schedule(static)
-        for (int i = 0; i < A.rows; ++i) {
-            cols.clear();
-            for (int a = A.rp[i]; a < A.rp[i + 1]; ++a) {
-                const int k = A.ci[a];
-                const double av = A.v[a];
-                for (int b = B.rp[k]; b < B.rp[k + 1]; ++b) {
-                    const int j = B.ci[b];
-                    if (marker[j] != i) { marker[j] = i; acc[j] = 0.0; cols.push_back(j); }
-                    acc[j] += av * B.v[b];
-                }
-            }
-            std::sort(cols.begin(), cols.end());
-            int o = C.rp[i];
-            for (int j : cols) { C.ci[o] = j; C.v[o] = acc[j]; ++o; }
-        }
-    }
-    return C;
-}
-
-HostCsr csr_transpose(const HostCsr& A) {
-    HostCsr T;
-    T.rows = A.cols;
-    T.cols = A.rows;
-    T.rp.assign(A.cols + 1, 0);
-    for (int c : A.ci) T.rp[c + 1]++;
-    for (int i = 0; i < A.cols; ++i) T.rp[i + 1] += T.rp[i];
-    T.ci.resize(A.ci.size());
-    T.v.resize(A.v.size());
-    std::vector<int> fill(T.rp.begin(), T.rp.end() - 1);
-    for (int i = 0; i < A.rows; ++i)
-        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a) {
-            const int k = fill[A.ci[a]]++;
-            T.ci[k] = i;
-            T.v[k] = A.v[a];
-        }
-    return T;
-}
-
-bool amg_device_setup_default() {
-    static const bool on = [] {
-        const char* e = std::getenv("CFA_AMG_HOST_SETUP");
-        return !(e && e[0] == '1');
-    }();
-    return on;
-}
-
 int csr_lanes_per_row(int rows, long long nnz) {
     // Measured on MI355X (one Jacobi sweep per level, lanes 1..32, uniform level-4 and level-5
     // pressure hierarchies): rows of ~7 entries run fastest with 2-4 lanes (one lane is 2.2x
@@ -421,198 +349,6 @@ int csr_lanes_per_row(int rows, long long nnz) {
     int tpr = 2;
     while (tpr < 32 && tpr < target) tpr *= 2;
     return tpr;
-}
-
-// ---------------------------------------------------------------------------------------------
-// AmgPreconditioner
-// ---------------------------------------------------------------------------------------------
-double AmgPreconditioner::spectral_radius_DinvA(const HostCsr& A, const std::vector<double>& dinv) {
-    const int n = A.rows, chunk = 4096, nch = (n + chunk - 1) / chunk;
-    std::vector<double> x(n), y(n), py(nch), px(nch);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) x[i] = 1.0 + 0.1 * std::sin(1.0 + i);
-    double lambda = 1.0;
-    for (int it = 0; it < 20; ++it) {
-#pragma omp parallel for schedule(static)
-        for (int c = 0; c < nch; ++c) {
-            double sy = 0.0, sx = 0.0;
-            for (int i = c * chunk; i < std::min(n, (c + 1) * chunk); ++i) {
-                double s = 0.0;
-                for (int a = A.rp[i]; a < A.rp[i + 1]; ++a) s += A.v[a] * x[A.ci[a]];
-                y[i] = dinv[i] * s;
-                sy += y[i] * y[i];
-                sx += x[i] * x[i];
-            }
-            py[c] = sy;
-            px[c] = sx;
-        }
-        double nrm = 0.0, xn = 0.0;
-        for (int c = 0; c < nch; ++c) { nrm += py[c]; xn += px[c]; }
-        nrm = std::sqrt(nrm);
-        lambda = nrm / std::sqrt(xn);
-#pragma omp parallel for schedule(static)
-        for (int i = 0; i < n; ++i) x[i] = y[i] / nrm;
-    }
-    return std::max(lambda, 1.0e-12) * 1.05;
-}
-
-std::vector<int> AmgPreconditioner::aggregate(const HostCsr& A, const std::vector<double>& dinv, int& n_agg) const {
-    const int n = A.rows;
-    std::vector<double> diag(n);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) diag[i] = 1.0 / dinv[i];
-    std::vector<char> strong(A.ci.size());
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i)
-        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a) {
-            const int j = A.ci[a];
-            strong[a] = j != i && std::fabs(A.v[a]) >= theta * std::sqrt(std::fabs(diag[i] * diag[j]));
-        }
-    // phase 1: aggregates seeded at rows whose strong neighbourhood is still free (row order)
-    std::vector<int> agg(n, -1);
-    int na = 0;
-    for (int i = 0; i < n; ++i) {
-        if (agg[i] >= 0) continue;
-        bool free_nb = true;
-        for (int a = A.rp[i]; a < A.rp[i + 1] && free_nb; ++a)
-            if (strong[a] && agg[A.ci[a]] >= 0) free_nb = false;
-        if (!free_nb) continue;
-        agg[i] = na;
-        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
-            if (strong[a]) agg[A.ci[a]] = na;
-        ++na;
-    }
-    // phase 2: unassigned rows join the aggregate of their first strong neighbour from phase 1
-    std::vector<int> agg2(agg);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) {
-        if (agg[i] >= 0) continue;
-        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
-            if (strong[a] && agg[A.ci[a]] >= 0) { agg2[i] = agg[A.ci[a]]; break; }
-    }
-    agg.swap(agg2);
-    // phase 3: the rest seed new aggregates with their unassigned strong neighbours
-    for (int i = 0; i < n; ++i) {
-        if (agg[i] >= 0) continue;
-        agg[i] = na;
-        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
-            if (strong[a] && agg[A.ci[a]] < 0) agg[A.ci[a]] = na;
-        ++na;
-    }
-    n_agg = na;
-    return agg;
-}
-
-HostCsr AmgPreconditioner::smoothed_prolongator(const HostCsr& A, const std::vector<double>& dinv, double omega,
-                                                std::vector<int>* agg_out) const {
-    const int n = A.rows;
-    int na = 0;
-    const std::vector<int> agg = aggregate(A, dinv, na);
-    if (agg_out) *agg_out = agg;
-    std::vector<int> size(na, 0);
-    for (int i = 0; i < n; ++i) size[agg[i]]++;
-    HostCsr P0;
-    P0.rows = n;
-    P0.cols = na;
-    P0.rp.resize(n + 1);
-    P0.ci.resize(n);
-    P0.v.resize(n);
-    for (int i = 0; i < n; ++i) {
-        P0.rp[i] = i;
-        P0.ci[i] = agg[i];
-        P0.v[i] = 1.0 / std::sqrt((double)size[agg[i]]);
-    }
-    P0.rp[n] = n;
-    // P = P0 - omega D^-1 A P0, rows merged in column order, exact zeros dropped
-    const HostCsr AP0 = csr_multiply(A, P0);
-    HostCsr P;
-    P.rows = n;
-    P.cols = na;
-    P.rp.assign(n + 1, 0);
-    auto build_row = [&](int i, int* cols, double* vals) {
-        int out = 0;
-        bool own = false;
-        const int oc = agg[i];
-        for (int a = AP0.rp[i]; a < AP0.rp[i + 1]; ++a) {
-            const int j = AP0.ci[a];
-            if (!own && j > oc) {
-                if (cols) { cols[out] = oc; vals[out] = P0.v[i]; }
-                ++out;
-                own = true;
-            }
-            double val = -omega * dinv[i] * AP0.v[a];
-            if (j == oc) { val += P0.v[i]; own = true; }
-            if (val != 0.0) {
-                if (cols) { cols[out] = j; vals[out] = val; }
-                ++out;
-            }
-        }
-        if (!own) {
-            if (cols) { cols[out] = oc; vals[out] = P0.v[i]; }
-            ++out;
-        }
-        return out;
-    };
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) P.rp[i + 1] = build_row(i, nullptr, nullptr);
-    for (int i = 0; i < n; ++i) P.rp[i + 1] += P.rp[i];
-    P.ci.resize(P.rp[n]);
-    P.v.resize(P.rp[n]);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) build_row(i, P.ci.data() + P.rp[i], P.v.data() + P.rp[i]);
-    return P;
-}
-
-std::vector<double> AmgPreconditioner::dense_spd_inverse(std::vector<double> M, int n) {
-    for (int j = 0; j < n; ++j) {
-        double d = M[(size_t)j * n + j];
-        for (int k = 0; k < j; ++k) d -= M[(size_t)j * n + k] * M[(size_t)j * n + k];
-        if (!(d > 0.0)) {
-            std::fprintf(stderr, "[AMG] coarse operator not SPD (pivot %.3e), aborting\n", d);
-            std::exit(EXIT_FAILURE);
-        }
-        const double ljj = std::sqrt(d);
-        M[(size_t)j * n + j] = ljj;
-        for (int i = j + 1; i < n; ++i) {
-            double s = M[(size_t)i * n + j];
-            for (int k = 0; k < j; ++k) s -= M[(size_t)i * n + k] * M[(size_t)j * n + k];
-            M[(size_t)i * n + j] = s / ljj;
-        }
-    }
-    std::vector<double> inv((size_t)n * n, 0.0);
-#pragma omp parallel
-    {
-        std::vector<double> col(n);
-#pragma omp for schedule(dynamic, 8)
-        for (int c = 0; c < n; ++c) {
-            for (int i = 0; i < n; ++i) {
-                double s = (i == c) ? 1.0 : 0.0;
-                for (int k = 0; k < i; ++k) s -= M[(size_t)i * n + k] * col[k];
-                col[i] = s / M[(size_t)i * n + i];
-            }
-            for (int i = n - 1; i >= 0; --i) {
-                double s = col[i];
-                for (int k = i + 1; k < n; ++k) s -= M[(size_t)k * n + i] * col[k];
-                col[i] = s / M[(size_t)i * n + i];
-            }
-            for (int i = 0; i < n; ++i) inv[(size_t)i * n + c] = col[i];
-        }
-    }
-    return inv;
-}
-
-void AmgPreconditioner::upload_csr(const HostCsr& M, DevCsr& D) {
-    D.rows = M.rows;
-    D.nnz = (int)M.ci.size();
-    D.tpr = csr_lanes_per_row(D.rows, D.nnz);
-    D.rp.upload(M.rp);
-    if (D.nnz) {
-        D.ci.upload(M.ci);
-        D.v.upload(M.v);
-    } else {
-        D.ci.reserve(1);
-        D.v.reserve(1);
-    }
 }
 
 void AmgPreconditioner::upload_csr_from_device(int rows, int nnz, const int* rp, const int* ci, const double* v,
@@ -677,21 +413,65 @@ bool AmgPreconditioner::device_product(int rows, const int* arp, const int* aci,
     return false;
 }
 
-int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, const double* av,
-                                    std::vector<std::unique_ptr<Level>>& pool) {
+
+namespace {
+std::vector<int> down_ints(const int* d, size_t n) {
+    std::vector<int> h(n);
+    if (n) GPU_CHECK(cfa_memcpy(h.data(), d, n * sizeof(int), 1, solver_stream()));
+    return h;
+}
+
+// Inverse of a small SPD matrix (the coarse level, at most a few hundred rows): Cholesky factor
+// L (row by row, in place in the lower triangle), W = L^-1 by forward substitution per column,
+// then M^-1 = W^T W. The reference inverts the same matrix through its Cholesky factor
+// (src/cfa/linalg.cuh:1395-1430); an indefinite pivot ends the run as there.
+std::vector<double> spd_inverse(std::vector<double> M, int n) {
+    auto at = [&](int i, int j) -> double& { return M[(size_t)i * n + j]; };
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = at(i, j);
+            for (int k = 0; k < j; ++k) s -= at(i, k) * at(j, k);
+            if (i == j) {
+                if (!(s > 0.0)) {
+                    std::fprintf(stderr, "[AMG] coarse operator not SPD (pivot %.3e), aborting\n", s);
+                    std::exit(EXIT_FAILURE);
+                }
+                at(i, i) = std::sqrt(s);
+            } else {
+                at(i, j) = s / at(j, j);
+            }
+        }
+    std::vector<double> W((size_t)n * n, 0.0);
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int c = 0; c < n; ++c) {
+        W[(size_t)c * n + c] = 1.0 / at(c, c);
+        for (int i = c + 1; i < n; ++i) {
+            double s = 0.0;
+            for (int k = c; k < i; ++k) s += at(i, k) * W[(size_t)k * n + c];
+            W[(size_t)i * n + c] = -s / at(i, i);
+        }
+    }
+    std::vector<double> inv((size_t)n * n);
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double s = 0.0;
+            for (int k = i; k < n; ++k) s += W[(size_t)k * n + i] * W[(size_t)k * n + j];
+            inv[(size_t)i * n + j] = inv[(size_t)j * n + i] = s;
+        }
+    return inv;
+}
+}  // namespace
+
+// One level on the device: D^-1, the spectral radius of D^-1 A (power iteration), the
+// three-phase aggregation, P = P0 - omega D^-1 A P0, R = P^T and the Galerkin product R (A P),
+// which becomes the next level's operator (dev_next_). Returns 1 with the next level's size, 0
+// when coarsening stalls (aggregates >= 0.85 rows or none).
+int AmgPreconditioner::device_level(int n, int nnz, const int* arp, const int* aci, const double* av,
+                                    std::vector<std::unique_ptr<Level>>& pool, int& n_next, int& nnz_next) {
     const cfa_stream_t s = solver_stream();
-    const int n = A.rows, nnz = (int)A.ci.size();
     const bool first = levels.empty();
     phase_start();
-    // D^-1 (host copy for the strength test, device copy for the level), spectral radius
-    std::vector<double> dinv(n);
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) {
-        double d = 0.0;
-        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
-            if (A.ci[a] == i) d = A.v[a];
-        dinv[i] = (std::fabs(d) > 1.0e-300) ? 1.0 / d : 1.0;
-    }
     std::unique_ptr<Level> L;
     if (!pool.empty()) {
         L = std::move(pool.front());
@@ -702,7 +482,7 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
     auto give_back = [&]() { pool.insert(pool.begin(), std::move(L)); };
     ensure_level_vectors(*L, n);
     GPU_CHECK(cfa_amg_dinv(n, arp, aci, av, L->dinv, s));
-    if (x0_dev_n_ < n) {   // power-iteration start vector of spectral_radius_DinvA
+    if (x0_dev_n_ < n) {   // power-iteration start vector 1 + 0.1 sin(1 + i) (linalg.cuh:1279)
         // its entries do not depend on n: extend it, with headroom for the meshes that follow
         const int old_n = (int)x0_host_.size();
         const int new_n = std::max(n, old_n + old_n / 2);
@@ -713,28 +493,28 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
     }
     const size_t pw = cfa_amg_power_ws_doubles(n);
     d_pw_.reserve(pw);
-    // the power iteration runs on the device while the host aggregates (which needs no omega)
     GPU_CHECK(cfa_amg_power(n, arp, aci, av, L->dinv, d_x0_, 20, d_pw_, nullptr, s));
     phase_end(PH_RHO);
-    int na = 0;
-    const std::vector<int> agg = aggregate(A, dinv, na);
-    phase_end(PH_AGG);
+    // aggregation; then one download: {aggregates, phase-1 aggregates, longest row, rounds} and lambda
+    const size_t aws = cfa_amg_aggregate_ws_bytes(n, nnz);
+    d_aws_.reserve(std::max<size_t>(aws, 1));
+    d_agg_.reserve(n);
+    d_p0v_.reserve(n);
+    d_info_.reserve(4);
+    GPU_CHECK(cfa_amg_aggregate(n, nnz, arp, aci, av, L->dinv, theta, d_aws_, aws, d_agg_, d_p0v_, d_info_, s));
+    int info[4] = {0, 0, 0, 0};
+    d_info_.download(info, 4);
     double lambda = 0.0;
     GPU_CHECK(cfa_memcpy(&lambda, d_pw_.ptr + pw - 1, sizeof(double), 1, s));
+    phase_end(PH_AGG);
+    const int na = info[0], left_max = info[2];
+    front_rounds += info[3];
     const double omega = 4.0 / (3.0 * (std::max(lambda, 1.0e-12) * 1.05));
-    phase_end(PH_RHO);
-    int left_max = 0;
-    for (int i = 0; i < n; ++i) left_max = std::max(left_max, A.rp[i + 1] - A.rp[i]);
     if (na >= (int)(0.85 * n) || na == 0) {
         give_back();
         return 0;
     }
-    if (first) last_aggregates = agg;
-    // P0 values; P0 = (iota, agg, p0v)
-    std::vector<int> size(na, 0);
-    for (int i = 0; i < n; ++i) size[agg[i]]++;
-    std::vector<double> p0v(n);
-    for (int i = 0; i < n; ++i) p0v[i] = 1.0 / std::sqrt((double)size[agg[i]]);
+    if (first && keep_aggregates) last_aggregates = d_agg_.to_host(n);
     if (iota_dev_n_ < n + 1) {   // row pointer of P0 (one entry per row), with headroom
         const int m = std::max(n + 1, iota_dev_n_ + iota_dev_n_ / 2);
         std::vector<int> iota(m);
@@ -742,9 +522,6 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
         d_iota_.upload(iota);
         iota_dev_n_ = m;
     }
-    d_agg_.upload(agg);
-    d_p0v_.upload(p0v);
-    phase_end(PH_UPLOAD);
     // P = P0 - omega D^-1 A P0
     if (!device_product(n, arp, aci, av, d_iota_, d_agg_, d_p0v_, dev_tmp_, left_max)) {
         give_back();
@@ -779,8 +556,7 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
     L->R.nnz = p_nnz;
     L->R.tpr = csr_lanes_per_row(na, p_nnz);
     phase_end(PH_TRANSPOSE);
-    // Galerkin product R (A P)
-    // (rows of R cover an aggregate's neighbourhood: longer than the merge kernel takes)
+    // Galerkin product R (A P) (rows of R cover an aggregate's neighbourhood: the LDS-sort kernel)
     if (!device_product(n, arp, aci, av, L->P.rp, L->P.ci, L->P.v, dev_tmp_, left_max) ||
         !device_product(na, L->R.rp, L->R.ci, L->R.v, dev_tmp_.rp, dev_tmp_.ci, dev_tmp_.v, dev_next2_, -1)) {
         give_back();
@@ -792,28 +568,17 @@ int AmgPreconditioner::device_level(HostCsr& A, const int* arp, const int* aci, 
     else std::swap(L->A, dev_next_);
     L->A.tpr = csr_lanes_per_row(n, L->A.nnz);
     L->omega = omega;
-    // host copy of the coarse operator for the next aggregation (and the coarse inverse)
-    HostCsr Ac;
-    Ac.rows = Ac.cols = na;
-    Ac.rp.resize(na + 1);
-    dev_next2_.rp.download(Ac.rp.data(), na + 1);
-    Ac.ci.resize(dev_next2_.nnz);
-    Ac.v.resize(dev_next2_.nnz);
-    dev_next2_.ci.download(Ac.ci.data(), dev_next2_.nnz);
-    dev_next2_.v.download(Ac.v.data(), dev_next2_.nnz);
-    if (first && keep_p0_host) {
+    if (first && keep_p0_host) {   // remap() reads the level-0 prolongator on the host
         P0_host.rows = n;
         P0_host.cols = na;
-        P0_host.rp.resize(n + 1);
-        P0_host.ci.resize(p_nnz);
-        P0_host.v.resize(p_nnz);
-        L->P.rp.download(P0_host.rp.data(), n + 1);
-        L->P.ci.download(P0_host.ci.data(), p_nnz);
-        L->P.v.download(P0_host.v.data(), p_nnz);
+        P0_host.rp = down_ints(L->P.rp, n + 1);
+        P0_host.ci = down_ints(L->P.ci, p_nnz);
+        P0_host.v = L->P.v.to_host(p_nnz);
     }
     std::swap(dev_next_, dev_next2_);
     levels.push_back(std::move(L));
-    A = std::move(Ac);
+    n_next = na;
+    nnz_next = dev_next_.nnz;
     phase_end(PH_DOWNLOAD);
     return 1;
 }
@@ -825,88 +590,38 @@ void AmgPreconditioner::setup(int n, int nnz, const int* d_row_ptr, const int* d
     old.swap(levels);
     for (auto it = spare_.begin(); it != spare_.end(); ++it) old.push_back(std::move(*it));
     spare_.clear();
-    auto take_level = [&]() {
-        if (!old.empty()) {
-            auto L = std::move(old.front());
-            old.erase(old.begin());
-            return L;
-        }
-        return std::make_unique<Level>();
-    };
-    phase_start();
-    HostCsr A;
-    A.rows = A.cols = n;
-    A.rp.resize(n + 1);
-    A.ci.resize(nnz);
-    A.v.resize(nnz);
-    GPU_CHECK(cfa_memcpy(A.rp.data(), d_row_ptr, (n + 1) * sizeof(int), 1, solver_stream()));
-    GPU_CHECK(cfa_memcpy(A.ci.data(), d_col_ind, nnz * sizeof(int), 1, solver_stream()));
-    GPU_CHECK(cfa_memcpy(A.v.data(), d_val, nnz * sizeof(double), 1, solver_stream()));
-    phase_end(PH_DOWNLOAD);
     last_aggregates.clear();
     device_levels = 0;
-    bool on_device = device_setup;
-    // device copy of the current level's operator: the caller's matrix, then the products
+    front_rounds = 0;
+    // the current level's operator: the caller's matrix, then the Galerkin products
     const int* arp = d_row_ptr;
     const int* aci = d_col_ind;
     const double* av = d_val;
-
-    while (A.rows > coarse_target && (int)levels.size() < max_levels - 1) {
-        if (on_device) {
-            const int r = device_level(A, arp, aci, av, old);
-            if (r == 1) {
-                ++device_levels;
-                arp = dev_next_.rp;
-                aci = dev_next_.ci;
-                av = dev_next_.v;
-                continue;
-            }
-            if (r == 0) break;
-            on_device = false;
-            phase_start();
-        }
-        std::vector<double> dinv(A.rows);
-#pragma omp parallel for schedule(static)
-        for (int i = 0; i < A.rows; ++i) {
-            double d = 0.0;
-            for (int a = A.rp[i]; a < A.rp[i + 1]; ++a)
-                if (A.ci[a] == i) d = A.v[a];
-            dinv[i] = (std::fabs(d) > 1.0e-300) ? 1.0 / d : 1.0;
-        }
-        const double omega = 4.0 / (3.0 * spectral_radius_DinvA(A, dinv));
-        phase_end(PH_RHO);
-        phase_start();
-        std::vector<int> agg;
-        HostCsr P = smoothed_prolongator(A, dinv, omega, levels.empty() ? &agg : nullptr);
-        phase_end(PH_SMOOTH);
-        if (P.cols >= (int)(0.85 * A.rows) || P.cols == 0) break;
-        if (levels.empty()) last_aggregates = agg;
-        HostCsr R = csr_transpose(P);
-        phase_end(PH_TRANSPOSE);
-        HostCsr Ac = csr_multiply(R, csr_multiply(A, P));
-        phase_end(PH_GALERKIN);
-
-        auto L = take_level();
-        if (levels.empty()) upload_csr_from_device(A.rows, nnz, d_row_ptr, d_col_ind, d_val, L->A);
-        else upload_csr(A, L->A);
-        upload_csr(P, L->P);
-        upload_csr(R, L->R);
-        L->omega = omega;
-        ensure_level_vectors(*L, A.rows);
-        L->dinv.upload(dinv.data(), dinv.size());
-        if (levels.empty()) P0_host = std::move(P);
-        levels.push_back(std::move(L));
-        A = std::move(Ac);
-        phase_end(PH_UPLOAD);
+    int rows = n, entries = nnz;
+    while (rows > coarse_target && (int)levels.size() < max_levels - 1) {
+        int n_next = 0, nnz_next = 0;
+        const int r = device_level(rows, entries, arp, aci, av, old, n_next, nnz_next);
+        if (r == 0) break;
+        if (r < 0) fatal("[AMG] a Galerkin product row holds more than 4096 products; the hierarchy cannot be built");
+        ++device_levels;
+        arp = dev_next_.rp;
+        aci = dev_next_.ci;
+        av = dev_next_.v;
+        rows = n_next;
+        entries = nnz_next;
     }
     for (auto& L : old) spare_.push_back(std::move(L));
 
+    // coarse operator to the host, dense inverse back to the device
     phase_start();
-    coarse_n = A.rows;
+    coarse_n = rows;
+    const std::vector<int> crp = down_ints(arp, rows + 1), cci = down_ints(aci, entries);
+    std::vector<double> cv(entries);
+    if (entries) GPU_CHECK(cfa_memcpy(cv.data(), av, entries * sizeof(double), 1, solver_stream()));
     std::vector<double> D((size_t)coarse_n * coarse_n, 0.0);
     for (int i = 0; i < coarse_n; ++i)
-        for (int a = A.rp[i]; a < A.rp[i + 1]; ++a) D[(size_t)i * coarse_n + A.ci[a]] = A.v[a];
-    const std::vector<double> inv = dense_spd_inverse(D, coarse_n);
+        for (int a = crp[i]; a < crp[i + 1]; ++a) D[(size_t)i * coarse_n + cci[a]] = cv[a];
+    const std::vector<double> inv = spd_inverse(D, coarse_n);
     d_coarse_inv.upload(inv);
     d_cx.reserve(std::max(coarse_n, 1), 16);
     d_cb.reserve(std::max(coarse_n, 1), 16);
@@ -915,6 +630,7 @@ void AmgPreconditioner::setup(int n, int nnz, const int* d_row_ptr, const int* d
     phase_end(PH_COARSE);
     setup_ms = ms_since(t0);
 }
+
 
 void AmgPreconditioner::refresh_view() {
     desc_.resize(levels.size());
@@ -977,6 +693,14 @@ void AmgPreconditioner::apply(const double* d_r, double* d_z) {
 // Carries the level-0 prolongator to a new mesh (src/cfa/linalg.cuh:1074-1157): kept nodes keep
 // their row, a new node gets the mean of the rows of its already-set graph neighbours, pass by
 // pass; coarser levels, omega and the coarse inverse stay.
+
+// Carries the level-0 prolongator to a new mesh instead of a new set-up (remap, the reference's
+// src/cfa/linalg.cuh:1074-1157, off by default): a node kept by the remesh (new_to_old[i] >= 0)
+// keeps its old row; the new nodes are filled layer by layer outwards from the kept ones, each
+// taking the mean of the rows of its graph neighbours filled in earlier layers (columns merged,
+// equal columns summed in neighbour order). Coarser levels, omega and the coarse inverse stay;
+// level 0 gets the new operator, its inverse diagonal, P and R = P^T (transposed on the device).
+// A new node that no layer reaches (no kept node in its part of the graph) means a new set-up.
 void AmgPreconditioner::remap(int n, int nnz, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
                               const int* h_rp, const int* h_ci, const std::vector<int>& new_to_old) {
     if (levels.empty() || P0_host.rows == 0) {
@@ -986,78 +710,87 @@ void AmgPreconditioner::remap(int n, int nnz, const int* d_row_ptr, const int* d
     const auto t0 = Clock::now();
     phase_start();
     const HostCsr& Po = P0_host;
+    // layer of every node: 0 kept, k filled from layers < k, -1 not reached yet
+    std::vector<int> layer(n, -1), frontier, next;
+    for (int i = 0; i < n; ++i)
+        if (new_to_old[i] >= 0) layer[i] = 0;
+    for (int i = 0; i < n; ++i)
+        if (layer[i] < 0) frontier.push_back(i);
+    std::vector<std::vector<std::pair<int, double>>> rows(n);
+    auto row_of = [&](int k, std::vector<std::pair<int, double>>& out) {
+        if (new_to_old[k] >= 0) {
+            const int j = new_to_old[k];
+            for (int b = Po.rp[j]; b < Po.rp[j + 1]; ++b) out.push_back({Po.ci[b], Po.v[b]});
+        } else {
+            out.insert(out.end(), rows[k].begin(), rows[k].end());
+        }
+    };
+    for (int depth = 1; !frontier.empty(); ++depth) {
+        next.clear();
+        std::vector<int> filled;
+        for (int i : frontier) {
+            std::vector<std::pair<int, double>> acc;
+            int used = 0;
+            for (int a = h_rp[i]; a < h_rp[i + 1]; ++a) {
+                const int k = h_ci[a];
+                if (k == i || layer[k] < 0 || layer[k] >= depth) continue;
+                ++used;
+                row_of(k, acc);
+            }
+            if (!used) { next.push_back(i); continue; }
+            std::stable_sort(acc.begin(), acc.end(),
+                             [](const std::pair<int, double>& x, const std::pair<int, double>& y) { return x.first < y.first; });
+            auto& out = rows[i];
+            for (const auto& e : acc) {
+                if (!out.empty() && out.back().first == e.first) out.back().second += e.second;
+                else out.push_back(e);
+            }
+            for (auto& e : out) e.second /= used;
+            filled.push_back(i);
+        }
+        if (filled.empty()) {   // unreachable new nodes
+            setup(n, nnz, d_row_ptr, d_col_ind, d_val);
+            return;
+        }
+        for (int i : filled) layer[i] = depth;
+        frontier.swap(next);
+    }
     HostCsr P;
     P.rows = n;
     P.cols = Po.cols;
     P.rp.assign(n + 1, 0);
-    std::vector<int> extra_of(n, -1);
-    std::vector<std::vector<std::pair<int, double>>> extra;
-    std::vector<int> pending;
-    for (int i = 0; i < n; ++i)
-        if (new_to_old[i] < 0) pending.push_back(i);
-    std::vector<char> has_row(n);
-    for (int i = 0; i < n; ++i) has_row[i] = new_to_old[i] >= 0;
-    std::vector<std::pair<int, double>> acc;
-    while (!pending.empty()) {
-        std::vector<int> still, set_now;
-        for (int i : pending) {
-            acc.clear();
-            int cnt = 0;
-            for (int a = h_rp[i]; a < h_rp[i + 1]; ++a) {
-                const int k = h_ci[a];
-                if (k == i || !has_row[k]) continue;
-                ++cnt;
-                if (new_to_old[k] >= 0) {
-                    const int j = new_to_old[k];
-                    for (int b = Po.rp[j]; b < Po.rp[j + 1]; ++b) acc.push_back({Po.ci[b], Po.v[b]});
-                } else {
-                    for (const auto& e : extra[extra_of[k]]) acc.push_back(e);
-                }
-            }
-            if (cnt == 0) { still.push_back(i); continue; }
-            std::sort(acc.begin(), acc.end(),
-                      [](const std::pair<int, double>& x, const std::pair<int, double>& y) { return x.first < y.first; });
-            std::vector<std::pair<int, double>> row;
-            for (const auto& e : acc) {
-                if (!row.empty() && row.back().first == e.first) row.back().second += e.second;
-                else row.push_back(e);
-            }
-            for (auto& e : row) e.second /= cnt;
-            extra_of[i] = (int)extra.size();
-            extra.push_back(std::move(row));
-            set_now.push_back(i);
-        }
-        if (set_now.empty()) {
-            setup(n, nnz, d_row_ptr, d_col_ind, d_val);
-            return;
-        }
-        for (int i : set_now) has_row[i] = 1;
-        pending.swap(still);
-    }
     for (int i = 0; i < n; ++i) {
         const int j = new_to_old[i];
-        P.rp[i + 1] = j >= 0 ? Po.rp[j + 1] - Po.rp[j] : (int)extra[extra_of[i]].size();
+        P.rp[i + 1] = P.rp[i] + (j >= 0 ? Po.rp[j + 1] - Po.rp[j] : (int)rows[i].size());
     }
-    for (int i = 0; i < n; ++i) P.rp[i + 1] += P.rp[i];
-    P.ci.resize(P.rp[n]);
-    P.v.resize(P.rp[n]);
+    P.ci.reserve(P.rp[n]);
+    P.v.reserve(P.rp[n]);
     for (int i = 0; i < n; ++i) {
-        const int j = new_to_old[i];
-        int o = P.rp[i];
-        if (j >= 0) {
-            std::copy(Po.ci.begin() + Po.rp[j], Po.ci.begin() + Po.rp[j + 1], P.ci.begin() + o);
-            std::copy(Po.v.begin() + Po.rp[j], Po.v.begin() + Po.rp[j + 1], P.v.begin() + o);
-        } else {
-            for (const auto& e : extra[extra_of[i]]) { P.ci[o] = e.first; P.v[o] = e.second; ++o; }
-        }
+        std::vector<std::pair<int, double>> r;
+        row_of(i, r);
+        for (const auto& e : r) { P.ci.push_back(e.first); P.v.push_back(e.second); }
     }
-    HostCsr R = csr_transpose(P);
+    const cfa_stream_t s = solver_stream();
     Level& L = *levels[0];
     upload_csr_from_device(n, nnz, d_row_ptr, d_col_ind, d_val, L.A);
-    upload_csr(P, L.P);
-    upload_csr(R, L.R);
+    L.P.rows = n;
+    L.P.nnz = P.rp[n];
+    L.P.tpr = csr_lanes_per_row(n, L.P.nnz);
+    L.P.rp.upload(P.rp);
+    L.P.ci.upload(P.ci);
+    L.P.v.upload(P.v);
+    const int cols = P.cols, pn = P.rp[n];
+    L.R.rows = cols;
+    L.R.nnz = pn;
+    L.R.tpr = csr_lanes_per_row(cols, pn);
+    L.R.rp.reserve(cols + 1);
+    L.R.ci.reserve(std::max(pn, 1));
+    L.R.v.reserve(std::max(pn, 1));
+    const size_t tws = cfa_csr_transpose_ws_bytes(n, cols, pn);
+    d_tws_.reserve(std::max<size_t>(tws, 1));
+    GPU_CHECK(cfa_csr_transpose(n, cols, pn, L.P.rp, L.P.ci, L.P.v, d_tws_, tws, L.R.rp, L.R.ci, L.R.v, s));
     ensure_level_vectors(L, n);
-    GPU_CHECK(cfa_inv_diag(n, L.A.rp, L.A.ci, L.A.v, 1.0e-300, L.dinv, solver_stream()));
+    GPU_CHECK(cfa_inv_diag(n, L.A.rp, L.A.ci, L.A.v, 1.0e-300, L.dinv, s));
     P0_host = std::move(P);
     ++remaps;
     refresh_view();

@@ -117,16 +117,12 @@ private:
     KrylovWorkspace ws_;
 };
 
-// Host CSR matrix used by the AMG set-up (src/cfa/linalg.cuh:791-795)
+// Host CSR matrix (the level-0 prolongator kept for remap)
 struct HostCsr {
     int rows = 0, cols = 0;
     std::vector<int> rp, ci;
     std::vector<double> v;
 };
-HostCsr csr_multiply(const HostCsr& A, const HostCsr& B);
-HostCsr csr_transpose(const HostCsr& A);
-
-bool amg_device_setup_default();
 
 class AmgPreconditioner {
 public:
@@ -149,15 +145,13 @@ public:
     int coarse_target = 300;
     int max_levels = 15;
     bool fp32 = false;   // run the cycle in FP32 (coarse solve in FP64); FP32 copies made at set-up
-    // numeric set-up phases (D^-1, spectral radius, P smoothing, transpose, Galerkin product) on
-    // the device, bit-identical to the host path; the aggregation stays on the host. A level
-    // whose products exceed the device kernel's row capacity falls back to the host path.
-    // Measured on MI355X (warm set-up of the pressure matrix): 11.5 ms against the host path's
-    // 14.1 ms at 115k rows, 32.5 ms against 54.2 ms at 460k rows. CFA_AMG_HOST_SETUP=1 in the
-    // environment selects the host path, for comparisons.
-    bool device_setup = amg_device_setup_default();
-    bool keep_p0_host = true;   // download the level-0 prolongator for remap()
-    int device_levels = 0;      // levels the last set-up built on the device
+    // The whole set-up runs on the device, level by level (D^-1, spectral radius, aggregation,
+    // P smoothing, transpose, Galerkin product); only the coarse operator comes to the host for
+    // its dense inverse.
+    bool keep_p0_host = true;      // download the level-0 prolongator for remap()
+    bool keep_aggregates = false;  // download the level-0 aggregates into last_aggregates
+    int device_levels = 0;         // levels the last set-up built
+    long long front_rounds = 0;    // rounds of the aggregation's decision front in the last set-up
 
     std::vector<std::unique_ptr<Level>> levels;
     int coarse_n = 0;
@@ -181,27 +175,20 @@ public:
     void apply(const double* d_r, double* d_z);
     const cfa_amg_t* view() const { return &view_; }
 
-    // host-side set-up steps (src/cfa/linalg.cuh:1276-1431)
-    static double spectral_radius_DinvA(const HostCsr& A, const std::vector<double>& dinv);
-    HostCsr smoothed_prolongator(const HostCsr& A, const std::vector<double>& dinv, double omega,
-                                 std::vector<int>* agg_out) const;
-    // the three-phase greedy aggregation of smoothed_prolongator (row order, sequential)
-    std::vector<int> aggregate(const HostCsr& A, const std::vector<double>& dinv, int& n_agg) const;
-    static std::vector<double> dense_spd_inverse(std::vector<double> M, int n);
 private:
     DevCsr dev_next_, dev_tmp_, dev_next2_;    // device set-up: next operator, products
-    DevArray<int> d_cnt_, d_flag_, d_agg_, d_iota_;
-    DevArray<unsigned char> d_tws_, d_sws_;   // transpose and scan workspaces
+    DevArray<int> d_cnt_, d_flag_, d_agg_, d_iota_, d_info_;
+    DevArray<unsigned char> d_tws_, d_sws_, d_aws_;   // transpose, scan and aggregation workspaces
     DevArray<double> d_p0v_, d_x0_, d_pw_;
     std::vector<double> x0_host_;
     int x0_dev_n_ = 0, iota_dev_n_ = 0;
-    // one level on the device: 1 built (A becomes the coarse operator), 0 stop (coarsening
-    // stalled), -1 fall back to the host for this and the following levels
-    int device_level(HostCsr& A, const int* arp, const int* aci, const double* av,
-                     std::vector<std::unique_ptr<Level>>& pool);
+    // one level: 1 built (the next operator in dev_next_, its size in n_next, nnz_next), 0 stop
+    // (coarsening stalled), -1 a product row beyond the device kernel's capacity
+    int device_level(int n, int nnz, const int* arp, const int* aci, const double* av,
+                     std::vector<std::unique_ptr<Level>>& pool, int& n_next, int& nnz_next);
     // C = A B on the device; left_max_row: the longest row of A when known (rows of at most 16
     // entries take the merge kernel), -1 for the LDS-sort kernel straight away. false: a row is
-    // beyond both kernels (the caller falls back to the host).
+    // beyond both kernels.
     bool device_product(int rows, const int* arp, const int* aci, const double* av, const int* brp, const int* bci,
                         const double* bv, DevCsr& C, int left_max_row);
     std::vector<std::unique_ptr<Level>> spare_;
@@ -216,7 +203,6 @@ private:
     }
     void release();
     void refresh_view();
-    static void upload_csr(const HostCsr& M, DevCsr& D);
     static void upload_csr_from_device(int rows, int nnz, const int* rp, const int* ci, const double* v, DevCsr& D);
     static void ensure_level_vectors(Level& L, int n);
 };

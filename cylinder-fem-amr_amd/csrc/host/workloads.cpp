@@ -31,6 +31,27 @@ Mesh2D uniform_mesh(int levels) {
 // phase times (ms) of the last pressure workload's AMG set-up, then the levels built on the device
 double g_amg_phases[AmgPreconditioner::PH_COUNT + 1] = {};
 
+// Host copies of the last workload systems (cfa_workload_keep_systems): the matrix, the
+// right-hand sides, the initial guesses and the solutions, for checks outside the library.
+struct KeptSystem {
+    int n = 0, nnz = 0, cols = 0;    // cols: right-hand sides (pressure) or 2 (momentum)
+    std::vector<int> rp, ci, tag;
+    std::vector<double> val, b, x0, x;   // b, x0: cols x n; x: pressure cols x n, momentum 4 x n
+    std::vector<int> iters;              // per solution of x
+};
+bool g_keep = false;
+KeptSystem g_kept[2];   // 0 pressure, 1 momentum
+
+void keep_pattern(KeptSystem& k, const Mesh2D& mesh, const DevArray<double>& val, int nnz) {
+    k = KeptSystem();
+    k.n = mesh.num_nodes;
+    k.nnz = nnz;
+    k.rp = mesh.csr_row_ptr;
+    k.ci = mesh.csr_col_ind;
+    k.tag = mesh.bc_tag;
+    k.val = val.to_host(nnz);
+}
+
 double host_ms(Clock::time_point t0) {
     dev_sync();
     return ms_since(t0);
@@ -83,6 +104,10 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
     double solve_ms = 0.0, worst = 0.0;
     long long iters = 0;
     std::vector<double> b(n);
+    if (g_keep) {
+        keep_pattern(g_kept[0], mesh, g.d_Kp_val, g.nnz);
+        g_kept[0].cols = nrhs;
+    }
     for (int k = 0; k < nrhs; ++k) {
         std::mt19937_64 rng(1000 + k);
         std::normal_distribution<double> N(0.0, 1.0);
@@ -98,10 +123,19 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
         dev_zero(g.d_dp, n * sizeof(double));
         dev_sync();
         t0 = Clock::now();
-        iters += solver.solve_pcg_fused(n, g.d_row_ptr, g.d_col_ind, g.d_Kp_val, g.d_rhs_p, g.d_dp, 4000, tol, nullptr,
-                                        &amg);
+        const int it = solver.solve_pcg_fused(n, g.d_row_ptr, g.d_col_ind, g.d_Kp_val, g.d_rhs_p, g.d_dp, 4000, tol,
+                                              nullptr, &amg);
+        iters += it;
         solve_ms += host_ms(t0);
         worst = std::max(worst, solver.last_rel_residual);
+        if (g_keep) {
+            KeptSystem& kp = g_kept[0];
+            kp.b.insert(kp.b.end(), b.begin(), b.end());
+            kp.x0.resize(kp.b.size(), 0.0);
+            const std::vector<double> x = g.d_dp.to_host(n);
+            kp.x.insert(kp.x.end(), x.begin(), x.end());
+            kp.iters.push_back(it);
+        }
     }
     if (kernel_reps > 0) {
         DevArray<double> t_out;
@@ -214,6 +248,16 @@ int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out) {
     x0v.reserve(n);
     dev_copy(x0u, g.d_u, n);
     dev_copy(x0v, g.d_v, n);
+    if (g_keep) {
+        KeptSystem& km = g_kept[1];
+        keep_pattern(km, mesh, g.d_A_val, g.nnz);
+        km.cols = 2;
+        km.b = g.d_rhs_u.to_host(n);
+        const std::vector<double> bv = g.d_rhs_v.to_host(n), x0u_h = x0u.to_host(n), x0v_h = x0v.to_host(n);
+        km.b.insert(km.b.end(), bv.begin(), bv.end());
+        km.x0 = x0u_h;
+        km.x0.insert(km.x0.end(), x0v_h.begin(), x0v_h.end());
+    }
     MulticolourIlu0 ilu;
     ilu.keep_fp32 = true;
     t0 = Clock::now();
@@ -241,6 +285,11 @@ int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out) {
         const std::vector<double> xv = g.d_v.to_host(n);
         xs.insert(xs.end(), xv.begin(), xv.end());
     }
+    if (g_keep) {
+        g_kept[1].x = x64;
+        g_kept[1].x.insert(g_kept[1].x.end(), x32.begin(), x32.end());
+        g_kept[1].iters = {(int)res[0][2], (int)res[0][3], (int)res[1][2], (int)res[1][3]};
+    }
     double diff = 0.0, scale = 0.0;
     for (size_t i = 0; i < x64.size(); ++i) {
         diff = std::max(diff, std::fabs(x32[i] - x64[i]));
@@ -264,6 +313,41 @@ int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out) {
                           res[0][0], res[0][1], res[0][2], res[0][3], res[1][0], res[1][1], res[1][2], res[1][3],
                           scale > 0.0 ? diff / scale : 0.0, its64 > 0 ? 1000.0 * res[0][1] / std::max(res[0][2], res[0][3]) : 0.0};
     for (int k = 0; k < 16; ++k) out[k] = v[k];
+    return CFA_OK;
+}
+
+// Keep host copies of the systems of the following workload calls (on != 0) or drop them.
+int cfa_workload_keep_systems(int on) {
+    g_keep = on != 0;
+    if (!g_keep) g_kept[0] = g_kept[1] = KeptSystem();
+    return CFA_OK;
+}
+
+// Sizes of the kept system `which` (0 pressure, 1 momentum): n, nnz, right-hand sides.
+int cfa_workload_system_sizes(int which, int* n, int* nnz, int* cols) {
+    if (which < 0 || which > 1 || !n || !nnz || !cols) return CFA_E_ARG;
+    *n = g_kept[which].n;
+    *nnz = g_kept[which].nnz;
+    *cols = g_kept[which].cols;
+    return CFA_OK;
+}
+
+// Copies the kept system: rp[n+1], ci[nnz], tag[n], val[nnz], b and x0 [cols n], x [cols n]
+// (pressure: the solution of each right-hand side) or [4 n] (momentum: u, v with FP64 factors,
+// then u, v with FP32 factors), and the iterations of each solution in iters.
+int cfa_workload_system(int which, int* rp, int* ci, int* tag, double* val, double* b, double* x0, double* x,
+                        int* iters) {
+    if (which < 0 || which > 1 || !rp || !ci || !tag || !val || !b || !x0 || !x || !iters) return CFA_E_ARG;
+    const KeptSystem& k = g_kept[which];
+    if (k.n == 0) return CFA_E_NOTREADY;
+    std::copy(k.rp.begin(), k.rp.end(), rp);
+    std::copy(k.ci.begin(), k.ci.end(), ci);
+    std::copy(k.tag.begin(), k.tag.end(), tag);
+    std::copy(k.val.begin(), k.val.end(), val);
+    std::copy(k.b.begin(), k.b.end(), b);
+    std::copy(k.x0.begin(), k.x0.end(), x0);
+    std::copy(k.x.begin(), k.x.end(), x);
+    std::copy(k.iters.begin(), k.iters.end(), iters);
     return CFA_OK;
 }
 

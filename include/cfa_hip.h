@@ -51,6 +51,9 @@ int cfa_memcpy(void* dst, const void* src, size_t bytes, int kind, cfa_stream_t 
 /* zero fill by a kernel (no memset node, safe inside captured graphs) */
 int cfa_zero(void* d_ptr, size_t bytes, cfa_stream_t stream);
 int cfa_stream_sync(cfa_stream_t stream);
+/* a stream that does not synchronise with the default stream (default priority) */
+int cfa_stream_create(cfa_stream_t* stream);
+int cfa_stream_destroy(cfa_stream_t stream);
 int cfa_device_sync(void);
 /* GPU timers for the benchmark: events recorded on `stream` */
 int cfa_timer_create(void** timer);
@@ -399,6 +402,14 @@ int cfa_amg_prolongator(int n, const int* ap_rp, const int* ap_ci, const double*
                         const double* p0v, const double* dinv, double omega, int pass, int* cnt, const int* p_rp,
                         int* p_ci, double* p_v, cfa_stream_t stream);
 size_t cfa_csr_transpose_ws_bytes(int rows, int cols, int nnz);
+/* cfa_amg_aggregate: the three-phase greedy aggregation of smoothed_prolongator (linalg.cuh:
+ * 1306-1356) for the n x n CSR matrix (rp, ci, val) with inverse diagonal dinv and strength
+ * threshold theta: agg [n] (aggregate of each row, numbered as the row-order loop numbers them),
+ * p0v [n] = 1 / sqrt(size of the row's aggregate), info (device int [4]) = aggregates, phase-1
+ * aggregates, longest row, rounds of the decision front. ws: cfa_amg_aggregate_ws_bytes bytes. */
+size_t cfa_amg_aggregate_ws_bytes(int n, int nnz);
+int cfa_amg_aggregate(int n, int nnz, const int* rp, const int* ci, const double* val, const double* dinv,
+                      double theta, void* ws, size_t ws_bytes, int* agg, double* p0v, int* info, cfa_stream_t stream);
 int cfa_csr_transpose(int rows, int cols, int nnz, const int* rp, const int* ci, const double* v, void* ws,
                       size_t ws_bytes, int* t_rp, int* t_ci, double* t_v, cfa_stream_t stream);
 
@@ -434,6 +445,144 @@ int cfa_mesh_tables(int n, int ne, const int* elem_v, const int* row_ptr, const 
 int cfa_mesh_check(int n, int ne, const int* elem_v, const double* x, const double* y, const int* bc_tag,
                    const int* row_ptr, const int* col_ind, int nnz, double h_channel, double l_channel, void* ws,
                    size_t ws_bytes, double* stats, cfa_stream_t stream);
+
+/* ---- newest-vertex-bisection forest on the device (src/cfa/amr.cuh:58-996, host code in the
+ * reference: AdaptiveMesh's forest, marking, look-ahead, coarsening, refinement with conformity
+ * closure, Morton extraction and solution transfer) -----------------------------------------------
+ * cfa_forest_t: the forest in structure-of-arrays form. Element slot e: ev[3e..3e+2] = newest
+ * vertex, refinement edge (v1, v2), counter-clockwise; egen, eparent, ech[2e..2e+1] (-1: leaf),
+ * ealive. Vertex slot v: coordinates, the five solution values (u, v, p and the previous time
+ * level uo, vo), boundary tag, valive, vpar[2v..2v+1] (the elements bisected through v), vedge (the
+ * bisected edge, min id in the high word; 0 for a base vertex). efree / vfree: recycled slots,
+ * taken from the end (LIFO, amr.cuh:761-795). hkey / hval: open-addressing table bisected edge ->
+ * midpoint (hash_cap a power of two, load kept below one half by the caller). log [3 x]: the
+ * midpoints created by the current adaptation {m, a, b} in creation order; cyl_log: those on the
+ * cylinder, whose arc positions the caller may set itself (cfa_nvb_set_xy) before the next pass
+ * uses them. cnt [CFA_NVB_COUNTERS]
+ * (device): the counters below, read and written by the kernels so a whole adaptation runs without
+ * host round trips except one per refinement pass. Capacities are the caller's: before a call the
+ * arrays must hold the slots it may create (bisect: 2 n elements, n vertices, n log entries). */
+enum {
+    CFA_NVB_NE = 0,        /* element slots in use (alive or recycled) */
+    CFA_NVB_NV,            /* vertex slots in use */
+    CFA_NVB_EFREE,         /* entries of efree */
+    CFA_NVB_VFREE,         /* entries of vfree */
+    CFA_NVB_LOG,           /* entries of log */
+    CFA_NVB_ERROR,         /* non-zero: an edge bisected by a third element */
+    CFA_NVB_LIST,          /* length of the last selected list */
+    CFA_NVB_MARKED_REFINE,
+    CFA_NVB_MARKED_COARSEN,
+    CFA_NVB_REMOVED,       /* vertices removed by coarsening */
+    CFA_NVB_NODES,         /* nodes of the last extraction */
+    CFA_NVB_ORDERED,       /* leaves above tolerance in the marking order */
+    CFA_NVB_CANDIDATES,    /* vertices collapsed by the last coarsening sweep */
+    CFA_NVB_CYLINDER,      /* entries of cyl_log */
+    CFA_NVB_COUNTERS = 16
+};
+typedef struct {
+    int* ev;
+    int* egen;
+    int* eparent;
+    int* ech;
+    unsigned char* ealive;
+    double* vx;
+    double* vy;
+    double* vu;
+    double* vv;
+    double* vp;
+    double* vuo;
+    double* vvo;
+    int* vtag;
+    unsigned char* valive;
+    int* vpar;
+    unsigned long long* vedge;
+    int* efree;
+    int* vfree;
+    int* log;
+    int* cyl_log;   /* [3 x]: {m, a, b} of the cylinder midpoints created (any order) */
+    unsigned long long* hkey;
+    int* hval;
+    int hash_cap;
+    int* cnt;
+} cfa_forest_t;
+
+/* Marking phase of one adaptation (amr.cuh:238-455 up to the coarsening sweeps): indicators
+ * normalised by their tolerances (optionally grown with the distance from the cylinder), leaf edge
+ * adjacency, advective look-ahead (traces along the centroid velocity with a walk across edges and
+ * a 0.01 bucket-grid fallback; a leaf keeps the maximum of what reaches it), the refinement marks
+ * largest score first within the element budget (ties in leaf order), the minimum level, two
+ * hole-filling passes, buffer layers, the protected layer and the coarsening flags. Leaves are the
+ * last extraction's, in its order. Forest-indexed outputs av, ai, target, refine, coarsen cover the
+ * element slots; clist receives the coarsening-flagged leaves in leaf order (count in
+ * cnt[CFA_NVB_LIST]). */
+typedef struct {
+    int n_leaves;                  /* L */
+    const int* leaves;             /* [L] forest slot of each leaf */
+    const double* eta;             /* [2 L] vorticity, then interpolation indicator */
+    int max_level, min_level, buffer_layers;
+    long long max_elems;
+    double tol_vort, tol_interp, coarsen_safety, lookahead_time, keep_radius, tol_radius, tol_power;
+    double* av;                    /* [element slots] */
+    double* ai;
+    int* target;
+    unsigned char* refine;
+    unsigned char* coarsen;
+    unsigned char* vflag;          /* [vertex slots] */
+    int* nbr;                      /* [3 L] */
+    int* clist;                    /* [L] */
+    long long* predicted;          /* device: the predicted element count of the marks */
+} cfa_nvb_mark_t;
+
+/* cfa_nvb_tmp_bytes: rocPRIM temporary storage for lists of up to n items (select, scan, sort).
+ * cfa_nvb_scratch_ints: int scratch (device) the calls below need for up to n_elem element slots,
+ * n_vert vertex slots and L leaves.
+ * cfa_nvb_hash_build: empties the table and inserts the live midpoints (vertex slots < nv).
+ * cfa_nvb_leaves: list (ascending slot) of the element slots < ne that are leaves (mode 0), leaves
+ * below target[e] (mode 1: the work of a refinement round, amr.cuh:476-508) or hanging leaves
+ * (mode 2: an edge with a midpoint, amr.cuh:922-969); count in cnt[CFA_NVB_LIST].
+ * cfa_nvb_bisect: bisects the n listed leaves as if one after the other in list order
+ * (amr.cuh:802-880): (c, a, b) -> (m, c, a), (m, b, c); the midpoint of (a, b) is taken from the
+ * table or created (slot from vfree, else appended; cylinder midpoints on the circle, boundary
+ * tags by position, P1 values with exact Dirichlet data, inlet peak inlet_umax; logged); children
+ * take slots from efree, else appended; with inherit they take target[e], else -1.
+ * cfa_nvb_mark: the marking phase above (scratch: cfa_nvb_scratch_ints ints).
+ * cfa_nvb_coarsen_sweep: one sweep of vertex collapses (amr.cuh:391-441, 977-996): the newest
+ * vertices of the listed leaves whose two (or one, on the boundary) bisected elements have
+ * coarsening-flagged leaf children predicted below coarsen_safety are removed in ascending slot
+ * order, the merged parents inherit the flag and the merged indicators (sqrt(2) and 2 times the
+ * larger), freed slots go to the free lists in that order; clist is replaced by the merged
+ * parents (count in cnt[CFA_NVB_LIST], removals added to cnt[CFA_NVB_REMOVED]).
+ * cfa_nvb_extract: compact mesh of the listed leaves (ascending slot, count L): leaves sorted by the
+ * Morton key of the centroid (16 bits per axis, ties by slot), nodes numbered by first use
+ * (amr.cuh:152-228); out_leaf [L], out_node [nodes], elem_v [3 L], gen [L] and the node arrays;
+ * node count in cnt[CFA_NVB_NODES].
+ * cfa_nvb_uniform_targets: target = -1 on the ne element slots, the listed leaves' generation plus
+ * 2 levels (refine_uniform, amr.cuh:461-471).
+ * cfa_nvb_set_xy: vertex slots slots [n] take the coordinates xy [2 n] (x, y pairs).
+ * cfa_nvb_scatter: values [5 n] (u, v, p, uo, vo of n nodes) to the vertex slots nodes [n].
+ * cfa_nvb_replay: recomputes the midpoint values of log entries [first, last) (amr.cuh:112-145,
+ * 896-914); entries of one refinement pass are independent, passes must be replayed in order.
+ * cfa_nvb_gather: values [5 n] from the vertex slots nodes [n]. */
+size_t cfa_nvb_tmp_bytes(int n);
+size_t cfa_nvb_scratch_ints(int n_elem, int n_vert, int n_leaves);
+int cfa_nvb_hash_build(const cfa_forest_t* f, int nv, cfa_stream_t stream);
+int cfa_nvb_leaves(const cfa_forest_t* f, int ne, int mode, const int* target, int* flags, int* list, void* tmp,
+                   size_t tmp_bytes, cfa_stream_t stream);
+int cfa_nvb_bisect(const cfa_forest_t* f, int n, const int* list, int inherit, int* target, double inlet_umax,
+                   int* scratch, void* tmp, size_t tmp_bytes, cfa_stream_t stream);
+int cfa_nvb_mark(const cfa_forest_t* f, const cfa_nvb_mark_t* m, int ne, int nv, int* scratch, void* tmp,
+                 size_t tmp_bytes, cfa_stream_t stream);
+int cfa_nvb_coarsen_sweep(const cfa_forest_t* f, const cfa_nvb_mark_t* m, int nv, int* scratch, void* tmp,
+                          size_t tmp_bytes, cfa_stream_t stream);
+int cfa_nvb_extract(const cfa_forest_t* f, int nv, int n_leaves, const int* leaves, int* scratch, void* tmp,
+                    size_t tmp_bytes, int* out_leaf, int* out_node, int* elem_v, int* gen, double* x, double* y,
+                    int* tag, double* values, cfa_stream_t stream);
+int cfa_nvb_uniform_targets(const cfa_forest_t* f, int ne, int n_leaves, const int* leaves, int levels, int* target,
+                            cfa_stream_t stream);
+int cfa_nvb_set_xy(const cfa_forest_t* f, int n, const int* slots, const double* xy, cfa_stream_t stream);
+int cfa_nvb_scatter(const cfa_forest_t* f, int n, const int* nodes, const double* values, cfa_stream_t stream);
+int cfa_nvb_replay(const cfa_forest_t* f, int first, int last, double inlet_umax, cfa_stream_t stream);
+int cfa_nvb_gather(const cfa_forest_t* f, int n, const int* nodes, double* values, cfa_stream_t stream);
 
 /* ---- discrete adjoint of the step (no counterpart in the reference) --------------------------
  * Vector-Jacobian products of the step's assemblies and updates (the forward operations are

@@ -79,6 +79,18 @@ int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out);
  * aggregation, P smoothing, transpose, Galerkin product, upload, coarse inverse, remap, then the
  * number of levels built on the device. */
 int cfa_workload_amg_phases(double* out);
+/* Host copies of the workload systems, for checks outside the library (no counterpart in the
+ * reference, which has no isolated workloads). cfa_workload_keep_systems(1): the following
+ * workload calls keep their matrix, right-hand sides, initial guesses and solutions (0 drops
+ * them). cfa_workload_system_sizes: n, nnz and right-hand sides of system `which` (0 the
+ * pressure workload, 1 the momentum workload). cfa_workload_system copies rp[n+1], ci[nnz],
+ * tag[n], val[nnz], b[cols n], x0[cols n] and x: pressure [cols n], the solution of each
+ * right-hand side; momentum [4 n], u and v with FP64 factors, then with FP32 factors; and the
+ * Krylov iterations of each of these solutions in iters. */
+int cfa_workload_keep_systems(int on);
+int cfa_workload_system_sizes(int which, int* n, int* nnz, int* cols);
+int cfa_workload_system(int which, int* rp, int* ci, int* tag, double* val, double* b, double* x0, double* x,
+                        int* iters);
 
 /* Solver-study dumps written by --dump-systems N (src/cylinder_fem_amr.cu:331-352 and :375-378:
  * <prefix>_momentum.bin = int32 n, nnz; row_ptr [n + 1], col_ind [nnz] int32; values [nnz],

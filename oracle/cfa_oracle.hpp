@@ -10,16 +10,15 @@
 //     plain left-to-right sums;
 //   * the time loop of src/cylinder_fem_amr.cu:277-538 and the re-projection of
 //     src/cfa/device.cuh:337-464.
-// Meshes come from the product's host mesh/NVB layer (an input, like a data file); that layer
-// is pinned directly by the reference's stored mesh sizes and through this oracle's agreement
-// with the reference's stored force histories.
+// Meshes: the base mesh from the product's generator (pinned by the reference's stored mesh sizes),
+// refined by the oracle's own restatement of the NVB rules (cfa_oracle_amr.hpp).
 #pragma once
 
 #include <functional>
 #include <string>
 #include <vector>
 
-#include "../cylinder-fem-amr_amd/csrc/host/amr.hpp"
+#include "cfa_oracle_amr.hpp"
 
 namespace oracle {
 
@@ -153,6 +152,7 @@ Csr csr_multiply(const Csr& A, const Csr& B);
 Csr csr_transpose(const Csr& A);
 Csr smoothed_prolongator(const Csr& A, const Vec& dinv, double omega, double theta, std::vector<int>* agg_out);
 Vec dense_spd_inverse(Vec M, int n);
+Csr amg_remap_p0(const Csr& P0, int n, const int* rp, const int* ci, const std::vector<int>& new_to_old, bool* ok);
 
 // ---- driver (src/cylinder_fem_amr.cu) ---------------------------------------------------------
 struct RunConfig {

@@ -1,0 +1,104 @@
+// ORACLE — test infrastructure only: a serial restatement of the reference's adaptive-mesh rules
+// (src/cfa/amr.cuh: the newest-vertex-bisection forest, marking with look-ahead, hole filling,
+// buffer layers, coarsening by vertex collapse, refinement with conformity closure, Morton
+// extraction and the P1 solution transfer), written as plain loops over the whole forest. Used by
+// tests/ and the oracle's own time loops as the checker of the product's device NVB; the product
+// never includes or links it.
+//
+// Equal by necessity: every rule that decides a mesh, its numbering or the slot recycling (visit
+// orders, LIFO reuse of element and vertex slots, the Morton key with ties by forest id, merge
+// factors sqrt(2) and 2, the bucket-grid fallback of the look-ahead) follows the reference, since
+// the tests compare meshes and their numbering bit for bit.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <vector>
+
+#include "../cylinder-fem-amr_amd/csrc/host/mesh.hpp"
+
+namespace oracle {
+
+// src/cfa/amr.cuh:9-30
+struct AmrParams {
+    int max_level = 4;
+    int min_level = 2;
+    double tol_vort = 0.1;
+    double tol_interp = 0.01;
+    double coarsen_safety = 0.35;
+    int buffer_layers = 1;
+    int max_elems = 200000;
+    double lookahead_time = 0.0;
+    double keep_radius = 0.25;
+    double tol_radius = 0.0;
+    double tol_power = 1.0;
+};
+
+// src/cfa/amr.cuh:32-49 (counts only)
+struct AmrCounts {
+    int leaves_before = 0, leaves_after = 0, marked_refine = 0, marked_coarsen = 0;
+    int bisections = 0, closure_bisections = 0, removed_vertices = 0;
+};
+
+class NvbForest {
+public:
+    // src/cfa/amr.cuh:71-90: the base mesh as generation-0 roots, u^{n-1} = u^n
+    NvbForest(const cfa::Mesh2D& base, const std::vector<double>& u0, const std::vector<double>& v0,
+              const std::vector<double>& p0);
+
+    double inlet_umax = cfa::Sim::U_MAX;
+
+    // src/cfa/amr.cuh:152-228
+    void extract(cfa::Mesh2D& mesh, std::vector<double>& u, std::vector<double>& v, std::vector<double>& p,
+                 std::vector<double>& uo, std::vector<double>& vo);
+    // src/cfa/amr.cuh:238-455; eta = [vorticity | interpolation] per leaf of the last extract
+    AmrCounts adapt(const std::vector<double>& eta, const AmrParams& prm);
+    // src/cfa/amr.cuh:461-471
+    AmrCounts refine_uniform(int levels);
+    // src/cfa/amr.cuh:96-110
+    void scatter_solution(const std::vector<double>& u, const std::vector<double>& v, const std::vector<double>& p,
+                          const std::vector<double>& uo, const std::vector<double>& vo);
+    // src/cfa/amr.cuh:112-145
+    void retransfer(const std::vector<int>& old_nodes, const std::vector<double>& u, const std::vector<double>& v,
+                    const std::vector<double>& p, const std::vector<double>& uo, const std::vector<double>& vo,
+                    std::vector<double>& nu, std::vector<double>& nv, std::vector<double>& np,
+                    std::vector<double>& nuo, std::vector<double>& nvo);
+
+    const std::vector<int>& node_slots() const { return nodes_; }   // compact node -> vertex slot
+    const std::vector<int>& leaf_slots() const { return leaves_; }  // compact element -> forest slot
+    int vertex_slots() const { return (int)x_.size(); }
+    int element_slots() const { return (int)el_.size(); }
+
+private:
+    struct El {
+        int v[3] = {-1, -1, -1};   // v[0] newest vertex, (v[1], v[2]) refinement edge
+        int gen = 0, parent = -1;
+        int ch[2] = {-1, -1};
+        bool alive = false;
+    };
+    std::vector<double> x_, y_, u_, v_, p_, uo_, vo_;
+    std::vector<int> tag_;
+    std::vector<char> valive_;
+    std::vector<std::array<int, 2>> vpar_;
+    std::vector<uint64_t> vedge_;
+    std::vector<int> vfree_;
+    std::vector<El> el_;
+    std::vector<int> efree_;
+    std::map<uint64_t, int> mid_;                 // bisected edge -> its midpoint
+    std::vector<std::array<int, 3>> created_;     // {m, a, b} of the last adapt, creation order
+    std::vector<int> leaves_, nodes_;             // last extract
+
+    bool leaf(int e) const { return el_[e].alive && el_[e].ch[0] < 0; }
+    double area(int e) const;
+    bool inside(int e, double px, double py, double l[3]) const;
+    int new_vertex();
+    int new_element();
+    void drop_element(int e);
+    void midpoint_values(int m, int a, int b);
+    int midpoint(int a, int b);
+    void split(int e);
+    void refine(std::vector<int>& target, AmrCounts& st);
+};
+
+}  // namespace oracle

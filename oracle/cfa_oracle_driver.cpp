@@ -77,7 +77,7 @@ RunResult run_fixed(const RunConfig& cfg) {
     initial_condition(mesh, hu, hv, hp, inflow0);
     huo = hu;
     hvo = hv;
-    cfa::AdaptiveMesh amr(mesh, hu, hv, hp);
+    NvbForest amr(mesh, hu, hv, hp);
     amr.extract(mesh, hu, hv, hp, huo, hvo);
     cfa::finalize_mesh(mesh);
     if (cfg.uniform_level > 0 || cfg.min_level > 0) {

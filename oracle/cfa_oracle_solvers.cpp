@@ -661,4 +661,80 @@ void Amg::apply(const double* r, double* z) {
     std::copy(levels[0].x.begin(), levels[0].x.end(), z);
 }
 
+// ---- AMG remap (src/cfa/linalg.cuh:1074-1157): the level-0 prolongator carried to a new graph --
+// rp/ci: the new operator's graph; new_to_old[i]: the old row of a kept node, -1 for a new node.
+// New nodes get, pass after pass, the mean of the rows of their neighbours that had one before the
+// pass (entries gathered in neighbour order, sorted by column with std::sort as the reference
+// does, equal columns summed in that order). *ok = false when a pass sets nothing.
+Csr amg_remap_p0(const Csr& P0, int n, const int* rp, const int* ci, const std::vector<int>& new_to_old, bool* ok) {
+    Csr P;
+    P.rows = n;
+    P.cols = P0.cols;
+    P.rp.assign(n + 1, 0);
+    std::vector<int> extra_of(n, -1);
+    std::vector<std::vector<std::pair<int, double>>> extra;
+    std::vector<int> pending;
+    for (int i = 0; i < n; ++i)
+        if (new_to_old[i] < 0) pending.push_back(i);
+    std::vector<char> has_row(n);
+    for (int i = 0; i < n; ++i) has_row[i] = new_to_old[i] >= 0;
+    *ok = true;
+    while (!pending.empty()) {
+        std::vector<int> still, set_now;
+        for (int i : pending) {
+            std::vector<std::pair<int, double>> acc;
+            int cnt = 0;
+            for (int a = rp[i]; a < rp[i + 1]; ++a) {
+                const int k = ci[a];
+                if (k == i || !has_row[k]) continue;
+                ++cnt;
+                if (new_to_old[k] >= 0) {
+                    for (int b = P0.rp[new_to_old[k]]; b < P0.rp[new_to_old[k] + 1]; ++b)
+                        acc.push_back({P0.ci[b], P0.v[b]});
+                } else {
+                    acc.insert(acc.end(), extra[extra_of[k]].begin(), extra[extra_of[k]].end());
+                }
+            }
+            if (cnt == 0) {
+                still.push_back(i);
+                continue;
+            }
+            std::sort(acc.begin(), acc.end(),
+                      [](const std::pair<int, double>& x, const std::pair<int, double>& y) { return x.first < y.first; });
+            std::vector<std::pair<int, double>> row;
+            for (const auto& e : acc) {
+                if (!row.empty() && row.back().first == e.first) row.back().second += e.second;
+                else row.push_back(e);
+            }
+            for (auto& e : row) e.second /= cnt;
+            extra_of[i] = (int)extra.size();
+            extra.push_back(row);
+            set_now.push_back(i);
+        }
+        if (set_now.empty()) {
+            *ok = false;
+            return P;
+        }
+        for (int i : set_now) has_row[i] = 1;
+        pending.swap(still);
+    }
+    for (int i = 0; i < n; ++i) {
+        const int j = new_to_old[i];
+        P.rp[i + 1] = P.rp[i] + (j >= 0 ? P0.rp[j + 1] - P0.rp[j] : (int)extra[extra_of[i]].size());
+    }
+    for (int i = 0; i < n; ++i) {
+        const int j = new_to_old[i];
+        if (j >= 0) {
+            P.ci.insert(P.ci.end(), P0.ci.begin() + P0.rp[j], P0.ci.begin() + P0.rp[j + 1]);
+            P.v.insert(P.v.end(), P0.v.begin() + P0.rp[j], P0.v.begin() + P0.rp[j + 1]);
+        } else {
+            for (const auto& e : extra[extra_of[i]]) {
+                P.ci.push_back(e.first);
+                P.v.push_back(e.second);
+            }
+        }
+    }
+    return P;
+}
+
 }  // namespace oracle

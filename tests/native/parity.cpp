@@ -5,13 +5,14 @@
 // block reductions or other summation orders must match to the stated relative tolerance.
 //
 //   parity <case> [level]    case: momentum pressure correction forces indicator mms oss ilu
-//                                  spmv amg pcg bicgstab mixed topology outlet all
+//                                  spmv amg pcg bicgstab jacobi mixed topology outlet nvb remap all
 //
 // Prints one line per compared quantity and exits 0 when every comparison passes.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <random>
 #include <string>
 
@@ -431,12 +432,11 @@ void case_amg(Fixture& f) {
     const PressureSystem o = case_pressure(f, false);
     oracle::Amg oa;
     oa.setup(host_csr(f, o.Kp));
-    for (const bool on_device : {true, false}) {
-        std::printf("  set-up on the %s\n", on_device ? "device" : "host");
+    {
         AmgPreconditioner amg;
-        amg.device_setup = on_device;
+        amg.keep_aggregates = true;
         amg.setup(f.n, f.nnz, f.g.d_row_ptr, f.g.d_col_ind, f.g.d_Kp_val);
-        if (on_device) expect(amg.device_levels == (int)amg.levels.size(), "every level built on the device");
+        std::printf("  aggregation decision-front rounds: %lld\n", amg.front_rounds);
         expect(amg.num_levels() == oa.num_levels(), "number of levels");
         std::vector<int> rows_g, rows_o, nnz_g, nnz_o;
         Vec om_g, om_o;
@@ -489,6 +489,9 @@ void case_amg(Fixture& f) {
             oracle::smoothed_prolongator(oa.levels[0].A, oa.levels[0].dinv, oa.levels[0].omega, 0.08, &agg);
             same_ints("level-0 aggregates", amg.last_aggregates, agg);
         }
+        if (amg.coarse_n == oa.coarse_n)
+            close_rel("coarse-level inverse", down(amg.d_coarse_inv, (size_t)amg.coarse_n * amg.coarse_n),
+                      oa.coarse_inv, 1e-12);
         Vec r(f.n);
         for (auto& v : r) v = std::uniform_real_distribution<double>(-1, 1)(f.rng);
         DevArray<double> d_r, d_z;
@@ -549,6 +552,228 @@ void topology_pair(const char* name, const Mesh2D& m, bool expect_valid) {
     same_ints("tables from the host graph: positions", down_int(g2.d_inc_pos, (int)ipos.size()), ipos);
 }
 
+// The device NVB forest against the oracle's restatement of the reference's rules: uniform
+// refinement, then adaptation cycles on synthetic indicators (a moving blob, a thin stripe, noise)
+// with look-ahead along a travelling-wave velocity, tight element budgets every fourth cycle, one or
+// two buffer layers, and the overlapped-remesh transfer of a later state every other cycle.
+// Connectivity, generations, boundary tags, coordinates (the cylinder midpoints' arc positions are
+// placed by the host's libm), vertex slots and transferred values must match bit for bit.
+void compare_meshes(const char* what, const Mesh2D& a, const Mesh2D& b, const Vec* fa, const Vec* fb) {
+    std::vector<int> ea, eb;
+    for (const Triangle& t : a.elems) ea.insert(ea.end(), {t.v[0], t.v[1], t.v[2], t.gen});
+    for (const Triangle& t : b.elems) eb.insert(eb.end(), {t.v[0], t.v[1], t.v[2], t.gen});
+    char name[96];
+    std::snprintf(name, sizeof name, "%s: elements", what);
+    same_ints(name, ea, eb);
+    std::snprintf(name, sizeof name, "%s: boundary tags", what);
+    same_ints(name, a.bc_tag, b.bc_tag);
+    if (a.num_nodes != b.num_nodes) return;
+    std::snprintf(name, sizeof name, "%s: coordinates", what);
+    Vec ca = a.x, cb = b.x;
+    ca.insert(ca.end(), a.y.begin(), a.y.end());
+    cb.insert(cb.end(), b.y.begin(), b.y.end());
+    same_bits(name, ca, cb);
+    static const char* fields[5] = {"u", "v", "p", "u_old", "v_old"};
+    for (int q = 0; q < 5; ++q) {
+        std::snprintf(name, sizeof name, "%s: %s", what, fields[q]);
+        same_bits(name, fa[q], fb[q]);
+    }
+}
+
+void case_nvb(Fixture&, int level) {
+    std::printf("[NVB forest on the device vs the oracle]\n");
+    Mesh2D base;
+    generate_cylinder_mesh(base, 8, 7, 28);
+    finalize_mesh(base);
+    const Vec z(base.num_nodes, 0.0);
+    AdaptiveMesh P(base, z, z, z);
+    oracle::NvbForest O(base, z, z, z);
+    Mesh2D mp, mo;
+    Vec fp[5], fo[5];
+    P.extract(mp, fp[0], fp[1], fp[2], fp[3], fp[4]);
+    O.extract(mo, fo[0], fo[1], fo[2], fo[3], fo[4]);
+    compare_meshes("base mesh", mp, mo, fp, fo);
+    P.refine_uniform(2);
+    O.refine_uniform(2);
+    P.extract(mp, fp[0], fp[1], fp[2], fp[3], fp[4]);
+    O.extract(mo, fo[0], fo[1], fo[2], fo[3], fo[4]);
+    compare_meshes("uniform level 2", mp, mo, fp, fo);
+    std::mt19937_64 rng(7);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    const int cycles = 4 * level;
+    for (int c = 0; c < cycles; ++c) {
+        const int n = mp.num_nodes, L = mp.num_elems;
+        Vec st[5];
+        for (Vec& v : st) v.resize(n);
+        for (int i = 0; i < n; ++i) {
+            st[0][i] = 1.0 + 0.3 * std::sin(8.0 * mp.x[i] - 0.3 * c) + 0.01 * U(rng);
+            st[1][i] = 0.2 * std::cos(5.0 * mp.y[i] + 0.2 * c);
+            st[2][i] = U(rng);
+            st[3][i] = 0.99 * st[0][i];
+            st[4][i] = 0.99 * st[1][i];
+        }
+        P.scatter_solution(st[0], st[1], st[2], st[3], st[4]);
+        O.scatter_solution(st[0], st[1], st[2], st[3], st[4]);
+        Vec eta(2 * (size_t)L);
+        const double bx = 0.3 + 0.05 * c, by = 0.2 + 0.05 * std::sin(0.7 * c);
+        for (int e = 0; e < L; ++e) {
+            const int* v = mp.elems[e].v;
+            const double xc = (mp.x[v[0]] + mp.x[v[1]] + mp.x[v[2]]) / 3.0;
+            const double yc = (mp.y[v[0]] + mp.y[v[1]] + mp.y[v[2]]) / 3.0;
+            const double r2 = (xc - bx) * (xc - bx) + (yc - by) * (yc - by);
+            eta[e] = 0.4 * std::exp(-r2 / 0.002) + 0.01 * U(rng) +
+                     (std::fabs(yc - 0.2 - 0.02 * std::sin(10.0 * xc)) < 0.01 ? 0.08 : 0.0);
+            eta[L + e] = 0.03 * std::exp(-r2 / 0.01) + 0.001 * U(rng);
+        }
+        AmrParams pp;
+        oracle::AmrParams po;
+        pp.tol_vort = po.tol_vort = 0.05;
+        pp.tol_interp = po.tol_interp = 0.005;
+        pp.max_level = po.max_level = 5;
+        pp.min_level = po.min_level = 2;
+        pp.max_elems = po.max_elems = (c % 4 == 3) ? (int)(L * 1.05) : 160000;
+        pp.lookahead_time = po.lookahead_time = (c % 3 == 0) ? 0.0 : 0.02 * (1 + c % 5);
+        pp.buffer_layers = po.buffer_layers = 1 + (c % 2);
+        const AmrStats a = P.adapt(eta, pp);
+        const oracle::AmrCounts b = O.adapt(eta, po);
+        std::printf("  cycle %2d: marked %d/%d, coarsen %d/%d, bisections %d/%d + closure %d/%d, removed %d/%d, "
+                    "leaves %d/%d (device/oracle)\n", c, a.marked_refine, b.marked_refine, a.marked_coarsen,
+                    b.marked_coarsen, a.bisections, b.bisections, a.closure_bisections, b.closure_bisections,
+                    a.removed_vertices, b.removed_vertices, a.leaves_after, b.leaves_after);
+        expect(a.marked_refine == b.marked_refine && a.marked_coarsen == b.marked_coarsen &&
+               a.bisections == b.bisections && a.closure_bisections == b.closure_bisections &&
+               a.removed_vertices == b.removed_vertices && a.leaves_after == b.leaves_after,
+               "adaptation counts");
+        const std::vector<int> old_p = P.node_vertices(), old_o = O.node_slots();
+        P.extract(mp, fp[0], fp[1], fp[2], fp[3], fp[4]);
+        O.extract(mo, fo[0], fo[1], fo[2], fo[3], fo[4]);
+        char what[48];
+        std::snprintf(what, sizeof what, "cycle %d", c);
+        compare_meshes(what, mp, mo, fp, fo);
+        same_ints("vertex slots of the nodes", P.node_vertices(), O.node_slots());
+        expect(P.num_vertex_slots() == O.vertex_slots(), "vertex slot count");
+        if (c % 2) {   // the overlapped remesh's transfer of a later state
+            Vec t[5], rp[5], ro[5];
+            for (Vec& v : t) v.resize(old_p.size());
+            for (size_t i = 0; i < old_p.size(); ++i)
+                for (int q = 0; q < 5; ++q) t[q][i] = U(rng);
+            P.inlet_umax = O.inlet_umax = 1.2;
+            P.retransfer(old_p, t[0], t[1], t[2], t[3], t[4], rp[0], rp[1], rp[2], rp[3], rp[4]);
+            O.retransfer(old_o, t[0], t[1], t[2], t[3], t[4], ro[0], ro[1], ro[2], ro[3], ro[4]);
+            for (int q = 0; q < 5; ++q) same_bits("transferred later state", rp[q], ro[q]);
+            P.inlet_umax = O.inlet_umax = Sim::U_MAX;
+        }
+        if (failures) break;
+    }
+}
+
+// AMG remap (src/cfa/linalg.cuh:1074-1157, --amg-remap): the level-0 prolongator of a hierarchy
+// built on one mesh carried to the locally refined mesh that follows it, against the oracle's
+// restatement applied to the oracle's own prolongator of the same matrix. Matrix: the graph
+// Laplacian with a 1e-4 diagonal shift on each mesh. The pattern must match; values to 1e-14 (a
+// new node's row sums equal columns in the order its neighbours list them, the reference in the
+// order of an unstable sort). Then PCG with the remapped hierarchy converges on the new mesh.
+std::vector<double> laplacian_values(const Mesh2D& m) {
+    std::vector<double> a(m.nnz);
+    for (int i = 0; i < m.num_nodes; ++i) {
+        int d = -1, deg = 0;
+        for (int k = m.csr_row_ptr[i]; k < m.csr_row_ptr[i + 1]; ++k) {
+            if (m.csr_col_ind[k] == i) { d = k; continue; }
+            a[k] = -1.0;
+            ++deg;
+        }
+        a[d] = deg * (1.0 + 1.0e-4);
+    }
+    return a;
+}
+
+void case_remap(Fixture& f) {
+    std::printf("[AMG remap to a refined mesh]\n");
+    Mesh2D m = f.mesh;
+    Vec z(m.num_nodes, 0.0), u = z, v = z, p = z, uo = z, vo = z;
+    AdaptiveMesh amr(m, z, z, z);
+    amr.extract(m, u, v, p, uo, vo);
+    finalize_mesh(m);
+    const Vec a0 = laplacian_values(m);
+    DevArray<int> d_rp, d_ci;
+    DevArray<double> d_a;
+    d_rp.upload(m.csr_row_ptr);
+    d_ci.upload(m.csr_col_ind);
+    d_a.upload(a0);
+    AmgPreconditioner amg;
+    amg.keep_p0_host = true;
+    amg.setup(m.num_nodes, m.nnz, d_rp, d_ci, d_a);
+    oracle::Csr A0;
+    A0.rows = A0.cols = m.num_nodes;
+    A0.rp = m.csr_row_ptr;
+    A0.ci = m.csr_col_ind;
+    A0.v = a0;
+    oracle::Amg oa;
+    oa.setup(A0);
+    const std::vector<int> old_vids = amr.node_vertices();
+    const Vec old_x = m.x, old_y = m.y;
+    // refine around (0.4, 0.2) and one level everywhere near the outlet
+    AmrParams prm;
+    prm.max_level = 5;
+    prm.min_level = 0;
+    prm.keep_radius = 0.0;
+    prm.tol_vort = prm.tol_interp = 1.0;
+    Vec eta(2 * m.elems.size(), 0.5);
+    for (size_t e = 0; e < m.elems.size(); ++e) {
+        const Triangle& t = m.elems[e];
+        const double cx = (m.x[t.v[0]] + m.x[t.v[1]] + m.x[t.v[2]]) / 3.0;
+        const double cy = (m.y[t.v[0]] + m.y[t.v[1]] + m.y[t.v[2]]) / 3.0;
+        if (std::hypot(cx - 0.4, cy - 0.2) < 0.03 || cx > 2.0) eta[e] = 8.0;
+    }
+    amr.adapt(eta, prm);
+    amr.extract(m, u, v, p, uo, vo);
+    finalize_mesh(m);
+    const int n = m.num_nodes;
+    std::vector<int> slot_old(amr.num_vertex_slots(), -1), new_to_old(n);
+    for (int j = 0; j < (int)old_vids.size(); ++j) slot_old[old_vids[j]] = j;
+    int kept = 0;
+    for (int i = 0; i < n; ++i) {
+        const int j = slot_old[amr.node_vertices()[i]];
+        new_to_old[i] = (j >= 0 && old_x[j] == m.x[i] && old_y[j] == m.y[i]) ? j : -1;
+        kept += new_to_old[i] >= 0;
+    }
+    std::printf("  %d -> %d nodes, %d kept\n", (int)old_vids.size(), n, kept);
+    const Vec a1 = laplacian_values(m);
+    d_rp.upload(m.csr_row_ptr);
+    d_ci.upload(m.csr_col_ind);
+    d_a.upload(a1);
+    amg.remap(n, m.nnz, d_rp, d_ci, d_a, m.csr_row_ptr.data(), m.csr_col_ind.data(), new_to_old);
+    expect(amg.remaps == 1, "remapped (no fall-back to a new set-up)");
+    bool ok = false;
+    const oracle::Csr Pn = oracle::amg_remap_p0(oa.levels[0].P, n, m.csr_row_ptr.data(), m.csr_col_ind.data(),
+                                                new_to_old, &ok);
+    expect(ok, "oracle remap reached every new node");
+    const auto& L = *amg.levels[0];
+    std::vector<int> pat_g = down_int(L.P.rp, n + 1), ci_g = down_int(L.P.ci, L.P.nnz);
+    pat_g.insert(pat_g.end(), ci_g.begin(), ci_g.end());
+    std::vector<int> pat_o = Pn.rp;
+    pat_o.insert(pat_o.end(), Pn.ci.begin(), Pn.ci.end());
+    same_ints("remapped P pattern", pat_g, pat_o);
+    if (pat_g == pat_o) close_rel("remapped P values", down(L.P.v, L.P.nnz), Pn.v, 1e-14);
+    const oracle::Csr Rn = oracle::csr_transpose(Pn);
+    std::vector<int> rpat_g = down_int(L.R.rp, L.R.rows + 1), rci_g = down_int(L.R.ci, L.R.nnz);
+    rpat_g.insert(rpat_g.end(), rci_g.begin(), rci_g.end());
+    std::vector<int> rpat_o = Rn.rp;
+    rpat_o.insert(rpat_o.end(), Rn.ci.begin(), Rn.ci.end());
+    same_ints("R = P^T pattern", rpat_g, rpat_o);
+    if (rpat_g == rpat_o) close_rel("R = P^T values", down(L.R.v, L.R.nnz), Rn.v, 1e-14);
+    Vec b(n);
+    for (int i = 0; i < n; ++i) b[i] = std::cos(0.11 * i);
+    DevArray<double> d_b, d_x;
+    d_b.upload(b);
+    d_x.upload(Vec(n, 0.0));
+    GpuLinearSolver solver;
+    std::function<void(const double*, double*)> prec = [&amg](const double* r, double* zz) { amg.apply(r, zz); };
+    const int it = solver.solve_pcg(n, d_rp, d_ci, d_a, d_b, d_x, 400, 1.0e-10, nullptr, &prec);
+    std::printf("  PCG with the remapped hierarchy: %d iterations\n", it);
+    expect(solver.converged, "PCG with the remapped hierarchy converges");
+}
+
 void case_topology(Fixture& f) {
     std::printf("[mesh topology on the device]\n");
     topology_pair("uniform mesh", f.mesh, true);
@@ -571,22 +796,6 @@ void case_topology(Fixture& f) {
         amr.adapt(eta, prm);
     }
     amr.extract(m, u, v, p, uo, vo);
-    {   // the Morton ordering of the leaves sorted on the device: the same mesh, numbering included
-        Mesh2D md;
-        Vec ud, vd, pd, uod, vod;
-        DeviceLeafSorter sorter;
-        amr.leaf_sorter = [&sorter](std::vector<std::pair<uint32_t, int>>& keyed) { return sorter(keyed); };
-        amr.extract(md, ud, vd, pd, uod, vod);
-        amr.leaf_sorter = nullptr;
-        std::vector<int> ev_h, ev_d;
-        for (const Triangle& t : m.elems) ev_h.insert(ev_h.end(), {t.v[0], t.v[1], t.v[2], t.gen});
-        for (const Triangle& t : md.elems) ev_d.insert(ev_d.end(), {t.v[0], t.v[1], t.v[2], t.gen});
-        std::printf("  extraction with the device leaf sort\n");
-        same_ints("elements (vertices, generation)", ev_d, ev_h);
-        same_bits("node coordinates", [&] { Vec c = md.x; c.insert(c.end(), md.y.begin(), md.y.end()); return c; }(),
-                  [&] { Vec c = m.x; c.insert(c.end(), m.y.begin(), m.y.end()); return c; }());
-        same_ints("boundary tags", md.bc_tag, m.bc_tag);
-    }
     topology_pair("locally refined mesh", m, true);
     Mesh2D flipped = m;
     std::swap(flipped.elems[flipped.elems.size() / 2].v[1], flipped.elems[flipped.elems.size() / 2].v[2]);
@@ -695,6 +904,53 @@ void case_bicgstab(Fixture& f) {
         close_rel(prec ? "u (FP32 factors)" : "u (FP64 factors)", down(f.g.d_u, f.n), ou, prec ? 1e-8 : 1e-9);
         close_rel(prec ? "v (FP32 factors)" : "v (FP64 factors)", down(f.g.d_v, f.n), ov, prec ? 1e-8 : 1e-9);
     }
+}
+
+// The non-default solver paths: Jacobi-preconditioned BiCGSTAB on one momentum component
+// (--momentum-precond jacobi, src/cfa/linalg.cuh:261-347) and Jacobi PCG on the PPE with the OSS
+// operator (--pressure-precond jacobi, :172-256), against the oracle's restatements.
+void case_jacobi(Fixture& f) {
+    std::printf("[Jacobi BiCGSTAB and Jacobi PCG]\n");
+    case_momentum(f, false);   // leaves the momentum system on the device
+    const Vec A = down(f.g.d_A_val, f.nnz), bu = down(f.g.d_rhs_u, f.n), bv = down(f.g.d_rhs_v, f.n);
+    const Vec u0 = down(f.g.d_u, f.n), v0 = down(f.g.d_v, f.n);
+    const double tol = 1e-10;
+    for (int comp = 0; comp < 2; ++comp) {
+        const Vec& b = comp ? bv : bu;
+        const Vec& x0 = comp ? v0 : u0;
+        Vec xo = x0;
+        const oracle::SolveInfo si = oracle::bicgstab_jacobi(f.n, f.rp, f.ci, A.data(), b.data(), xo.data(), 200, tol);
+        DevArray<double> d_b, d_x;
+        d_b.upload(b);
+        d_x.upload(x0);
+        GpuLinearSolver solver;
+        const int it = solver.solve_bicgstab(f.n, f.g.d_row_ptr, f.g.d_col_ind, f.g.d_A_val, d_b, d_x, 200, tol);
+        std::printf("  component %c: iterations gpu %d oracle %d\n", comp ? 'v' : 'u', it, si.iters);
+        expect(std::abs(it - si.iters) <= 1 && solver.converged && si.converged, "iterations and convergence");
+        close_rel(comp ? "v (Jacobi BiCGSTAB)" : "u (Jacobi BiCGSTAB)", down(d_x, f.n), xo, 1e-9);
+    }
+    const PressureSystem o = case_pressure(f, false);
+    Vec x(f.n, 0.0);
+    f.g.d_dp.upload(x);
+    GpuLinearSolver solver;
+    const cfa_oss_op_t op = f.g.oss_op();
+    const int it = solver.solve_pcg_fused(f.n, f.g.d_row_ptr, f.g.d_col_ind, f.g.d_Kp_val, f.g.d_rhs_p, f.g.d_dp, 4000,
+                                          tol, &op, nullptr);
+    Vec dinv(f.n, 1.0);
+    for (int i = 0; i < f.n; ++i)
+        for (int k = f.rp[i]; k < f.rp[i + 1]; ++k)
+            if (f.ci[k] == i) { dinv[i] = std::fabs(o.Kp[k]) > 1.0e-14 ? 1.0 / o.Kp[k] : 1.0; break; }
+    oracle::Op oss_op = [&](const double* in, double* out) {
+        oracle::oss_apply(f.n, f.rp, f.ci, f.nnz, o.oss.data(), o.inv_m.data(), f.tag, in, out);
+    };
+    oracle::Op prec = [&](const double* r, double* z) { for (int i = 0; i < f.n; ++i) z[i] = dinv[i] * r[i]; };
+    const oracle::SolveInfo si = oracle::pcg(f.n, f.rp, f.ci, o.Kp.data(), o.rhs.data(), x.data(), 4000, tol, &oss_op,
+                                             &prec);
+    std::printf("  Jacobi PCG: iterations gpu %d oracle %d\n", it, si.iters);
+    // hundreds of iterations: the reduction order moves the stopping step by a few
+    expect(solver.converged && si.converged && std::abs(it - si.iters) <= 2 + si.iters / 50,
+           "iterations (within 2 + 2%) and convergence");
+    close_rel("solution (Jacobi PCG)", down(f.g.d_dp, f.n), x, 1e-8);
 }
 
 }  // namespace
@@ -865,9 +1121,12 @@ int main(int argc, char** argv) {
         if (all || what == "amg") case_amg(f);
         if (all || what == "pcg") case_pcg(f);
         if (all || what == "bicgstab") case_bicgstab(f);
+        if (all || what == "jacobi") case_jacobi(f);
         if (all || what == "mixed") case_mixed(f);
         if (all || what == "topology") case_topology(f);
         if (all || what == "outlet") case_outlet(f);
+        if (all || what == "nvb") case_nvb(f, level);
+        if (all || what == "remap") case_remap(f);
         std::printf(failures ? "%d FAILURE(S)\n" : "ALL PASSED\n", failures);
         status = failures ? 1 : 0;
     }   // the fixture's device memory is freed here

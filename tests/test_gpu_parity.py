@@ -11,7 +11,7 @@ import pytest
 
 BIN = Path(__file__).resolve().parent / "native" / "build" / "parity"
 CASES = ["momentum", "pressure", "correction", "forces", "indicator", "mms", "oss", "ilu", "spmv", "amg", "pcg",
-         "bicgstab", "mixed", "topology", "outlet"]
+         "bicgstab", "jacobi", "mixed", "topology", "outlet", "nvb", "remap"]
 
 pytestmark = pytest.mark.gpu
 # (case, level) pairs kept off the GPU: each ended once with SIGSEGV in the parity process, the

@@ -1,5 +1,7 @@
-"""Host mesh layer (generator, NVB uniform refinement, Morton extraction, CSR graph) against the
-reference's known answers, and mesh invariants checked by brute force.
+"""Mesh layer against the reference's known answers, and mesh invariants checked by brute force: the
+generator and CSR graph of the product's host layer with the oracle's restatement of the NVB rules
+(uniform refinement, Morton extraction) on the CPU, and the product's device NVB forest against
+the oracle's meshes on the GPU (connectivity, tags, graph and coordinates identical).
 
 Known answers: base mesh 508 nodes / 896 triangles / 3316 CSR nonzeros, min angle 15.29 deg
 (src/cylinder_fem_amr.cu init line in every runs/*/run.log); node counts 1912, 7408, 29152,
@@ -10,6 +12,7 @@ import numpy as np
 import pytest
 
 import cfa
+import cfa_oracle
 from conftest import read_csv
 
 NODES = {0: 508, 1: 1912, 2: 7408, 3: 29152, 4: 115648}
@@ -18,7 +21,7 @@ NNZ = {0: 3316, 1: 12904, 2: 50896, 3: 202144, 4: 805696}
 
 @pytest.fixture(scope="module", params=[0, 1, 2, 3])
 def mesh(request):
-    return request.param, cfa.uniform_mesh(request.param)
+    return request.param, cfa_oracle.uniform_mesh(request.param)
 
 
 def test_sizes(mesh):
@@ -32,7 +35,7 @@ def test_sizes(mesh):
 
 def test_h_max_matches_reference(golden):
     for level in (1, 2, 3):
-        m = cfa.uniform_mesh(level)
+        m = cfa_oracle.uniform_mesh(level)
         ev = m["ev"].reshape(-1, 3)
         h = 0.0
         for a, b in ((0, 1), (1, 2), (2, 0)):
@@ -103,3 +106,17 @@ def test_boundary_tags(mesh):
     assert (bc[inlet] == 1).all()
     assert (bc[outlet] == 2).all()
     assert (bc[~(cyl | wall | inlet | outlet)] == 0).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("level", [0, 1, 2, 3, 4])
+def test_device_nvb_uniform_matches_oracle(level):
+    print(f"[case] uniform level {level}: device NVB forest vs the oracle", flush=True)
+    d, o = cfa.uniform_mesh(level), cfa_oracle.uniform_mesh(level)
+    assert (d["n"], d["ne"], d["nnz"]) == (o["n"], o["ne"], o["nnz"])
+    if level <= 4:
+        assert d["n"] == NODES[level] and d["nnz"] == NNZ[level]
+    for key in ("ev", "bc", "rp", "ci", "cyl"):
+        np.testing.assert_array_equal(d[key], o[key], err_msg=key)
+    for key in ("x", "y"):
+        np.testing.assert_array_equal(d[key], o[key], err_msg=key)

@@ -1,6 +1,7 @@
 """The C++ unit tests (cylinder-fem-amr_amd/csrc/app/test_cfa.cpp, the checks of the reference's
-tests/test_cfa.cu) run as one program: host-side mesh adaptation checks here, device checks on
-the GPU."""
+tests/test_cfa.cu) run as one program on the GPU: the device NVB forest's adaptation checks
+(conformity, level cap, angle bound, exact transfer of linear fields, refine/coarsen round trip,
+minimum level, retransfer) and the solver checks."""
 import subprocess
 
 import pytest
@@ -15,10 +16,7 @@ def run_units(part):
     assert r.returncode == 0 and "ALL PASSED" in r.stdout, r.stderr[-2000:]
 
 
-def test_units_host():
-    run_units("cpu")
-
-
 @pytest.mark.gpu
 def test_units_device():
+    print("[case] C++ unit tests", flush=True)
     run_units("gpu")

@@ -8,13 +8,15 @@
 // The row-order greedy rules are lexicographically-first independent sets: a row is a phase-1
 // root exactly when no lower root is in conflict with it (it claimed the row, is a strong
 // neighbour of it, or claimed one of its strong neighbours), and a phase-3 root when no lower
-// phase-3 root claimed it. The decisions sweep through the rows as a front (each decision needs
-// the lower conflicting ones), so one workgroup follows the front in a sliding window of rows:
-// every round decides what it can in the window, remembers for an undecided row the lower row it
-// waits on, and moves the window to the first undecided row. Every round decides at least that
-// row, so the loop ends after at most n rounds. Aggregates are numbered in row order by prefix
-// sums, as the greedy loop numbers them; the strength test and the P0 values 1/sqrt(size) are
-// the host's floating-point operations (contraction off).
+// phase-3 root claimed it. Each decision needs the lower conflicting ones, so the decisions
+// sweep through the rows as a front whose depth grows like the square root of the row count on
+// these meshes. The lower conflicts of every row are listed in parallel first; then one workgroup
+// decides the rows window by window (up to 8192 rows), with the window's conflict lists and
+// decisions in LDS, in rounds that cost one barrier each. Aggregates are numbered in row order
+// by prefix sums, as the greedy loop numbers them; the strength test and the P0 values
+// 1/sqrt(size) are the host's floating-point operations (contraction off).
+#include <algorithm>
+
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -26,16 +28,17 @@ using namespace cfa_dev;
 
 namespace {
 
-constexpr int WINDOW = 4096;        // rows per round of the front-following kernel
 constexpr int FRONT_THREADS = 1024;
+constexpr int FRONT_ROWS = 8;                                // window rows per thread
+constexpr int FRONT_WINDOW = FRONT_THREADS * FRONT_ROWS;    // rows per window at most
+constexpr int FRONT_LIST = 61440;                            // staged conflict entries (LDS, 120 KB)
 
 int nblk(long long n) { return blocks_for(n > 0 ? n : 1); }
 
-__device__ __forceinline__ int ld(const int* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void st_(int* p, int v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+// conflict-list slot per row: four times the mean row length, 32 to 128 entries
+int conflict_cap(int n, int nnz) {
+    const long long c = 4LL * nnz / (n > 0 ? n : 1);
+    return (int)std::min(128LL, std::max(32LL, (c + 7) / 8 * 8));
 }
 
 // strong entries per row (|a_ij| >= theta sqrt(|d_i d_j|), j != i, d = 1 / dinv) and the longest row
@@ -86,74 +89,221 @@ __global__ void k_strong_fill(int n, const int* __restrict__ rp, const int* __re
         }
 }
 
-// One workgroup following the decision front. mode 1: phase-1 conflicts (rows that claimed i,
-// strong neighbours of i, rows that claimed a strong neighbour of i); mode 3: rows that claimed i.
-// state: 0 undecided, 1 root, 2 not a root (rows outside the phase are 2 from the start).
-__global__ __launch_bounds__(FRONT_THREADS) void k_front(int n, int mode, const int* __restrict__ s_rp,
-                                                         const int* __restrict__ s_ci, const int* __restrict__ t_rp,
-                                                         const int* __restrict__ t_ci, int* state, int* wait,
-                                                         int* __restrict__ info) {
-    __shared__ int s_lo, s_min;
-    if (threadIdx.x == 0) s_lo = 0;
-    __syncthreads();
-    int rounds = 0;
-    while (true) {
-        const int lo = s_lo;
-        if (lo >= n) break;
-        const int hi = min(n, lo + WINDOW);
-        for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-            if (ld(state + i) != 0) continue;
-            const int w = ld(wait + i);
-            if (w >= 0 && ld(state + w) == 0) continue;
-            int blocker = -1;
-            bool lower_root = false;
-            // rows r < i of a list sorted ascending: a root decides, an undecided one blocks
-            auto scan = [&](const int* list, int b, int e) {
-                for (int k = b; k < e && !lower_root; ++k) {
-                    const int r = list[k];
-                    if (r >= i) break;
-                    const int s = ld(state + r);
-                    if (s == 1) lower_root = true;
-                    else if (s == 0 && blocker < 0) blocker = r;
-                }
-            };
-            scan(t_ci, t_rp[i], t_rp[i + 1]);
-            if (mode == 1) {
-                for (int k = s_rp[i]; k < s_rp[i + 1] && !lower_root; ++k) {
-                    const int r = s_ci[k];
-                    if (r < i) {
-                        const int s = ld(state + r);
-                        if (s == 1) lower_root = true;
-                        else if (s == 0 && blocker < 0) blocker = r;
-                    }
-                }
-                for (int k = s_rp[i]; k < s_rp[i + 1] && !lower_root; ++k) {
-                    const int j = s_ci[k];
-                    scan(t_ci, t_rp[j], t_rp[j + 1]);
-                }
-            }
-            if (lower_root) st_(state + i, 2);
-            else if (blocker >= 0) st_(wait + i, blocker);
-            else st_(state + i, 1);
+// Lower conflicts of each row, deduplicated, in a slot of `cap` entries per row: mode 1 (phase 1)
+// the rows r < i that claimed i (T(i)), the strong neighbours r < i (S(i)) and the rows r < i that
+// claimed a strong neighbour (T(j), j in S(i)); mode 3 (phase 3) T(i) only. len[i] = -1 marks a
+// row with more than `cap` of them (its decisions walk the graph itself).
+__global__ void k_lower_conflicts(int n, int mode, const int* __restrict__ s_rp, const int* __restrict__ s_ci,
+                                  const int* __restrict__ t_rp, const int* __restrict__ t_ci, int cap,
+                                  int* __restrict__ lc, int* __restrict__ len) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int* out = lc + (size_t)i * cap;
+    int c = 0;
+    bool full = false;
+    auto add = [&](int r) {
+        if (full) return;
+        for (int k = 0; k < c; ++k)
+            if (out[k] == r) return;
+        if (c == cap) {
+            full = true;
+            return;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) s_min = hi;
-        __syncthreads();
-        for (int i = lo + threadIdx.x; i < hi; i += blockDim.x)
-            if (ld(state + i) == 0) atomicMin(&s_min, i);
-        __syncthreads();
-        if (threadIdx.x == 0) s_lo = s_min;
-        ++rounds;
-        __syncthreads();
+        out[c++] = r;
+    };
+    for (int k = t_rp[i]; k < t_rp[i + 1]; ++k) {   // columns of T ascend
+        const int r = t_ci[k];
+        if (r >= i) break;
+        add(r);
     }
-    if (threadIdx.x == 0) info[3] += rounds;
+    if (mode == 1)
+        for (int k = s_rp[i]; k < s_rp[i + 1]; ++k) {
+            const int j = s_ci[k];
+            if (j < i) add(j);
+            for (int q = t_rp[j]; q < t_rp[j + 1]; ++q) {
+                const int r = t_ci[q];
+                if (r >= i) break;
+                add(r);
+            }
+        }
+    len[i] = full ? -1 : c;
 }
 
-__global__ void k_init_state(int n, const int* __restrict__ agg, int* __restrict__ state, int* __restrict__ wait) {
+// exclusive scan over the 1024 threads of the workgroup; *total = the sum (s: 16 ints of LDS)
+__device__ int block_scan_front(int v, int* s, int& total) {
+    const int t = threadIdx.x, lane = t & (WAVE - 1), w = t / WAVE;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
+    }
+    if (lane == WAVE - 1) s[w] = x;
+    __syncthreads();
+    if (w == 0) {
+        int y = lane < FRONT_THREADS / WAVE ? s[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < FRONT_THREADS / WAVE; o <<= 1) {
+            const int z = __shfl_up(y, o, WAVE);
+            if (lane >= o) y += z;
+        }
+        if (lane < FRONT_THREADS / WAVE) s[lane] = y;
+    }
+    __syncthreads();
+    const int before = w > 0 ? s[w - 1] : 0;
+    total = s[FRONT_THREADS / WAVE - 1];
+    __syncthreads();
+    return before + x - v;
+}
+
+// One workgroup deciding the greedy roots window by window. A row is a root exactly when none of
+// its lower conflicts is a root; every row below the window is decided, so each window is
+// decided completely by rounds in LDS: its rows' conflict lists are staged as window offsets
+// (the conflicts below the window resolved from their final states while staging), and every
+// round each undecided row either finds a lower root (not a root), or an undecided lower row (it
+// waits on that row, rechecking only it until it is decided), or neither (a root). The lowest
+// undecided row of the window always decides, so the rounds end. state (global): 0 undecided,
+// 1 root, 2 not a root (rows outside the phase are 2 from the start).
+__global__ __launch_bounds__(FRONT_THREADS) void k_front(int n, int mode, const int* __restrict__ s_rp,
+                                                         const int* __restrict__ s_ci, const int* __restrict__ t_rp,
+                                                         const int* __restrict__ t_ci, const int* __restrict__ lc,
+                                                         const int* __restrict__ lc_len, int cap,
+                                                         int* __restrict__ state, int* __restrict__ info) {
+    __shared__ unsigned short s_list[FRONT_LIST];
+    __shared__ unsigned char s_st[FRONT_WINDOW];
+    __shared__ int s_scan[FRONT_THREADS / WAVE];
+    __shared__ int s_flag[3];
+    const int t = threadIdx.x;
+    int rounds = 0;
+    for (int lo = 0; lo < n;) {
+        // the window: as wide as its staged lists allow
+        int width = FRONT_WINDOW, hi = lo, base = 0, total = 0;
+        while (true) {
+            hi = min(n, lo + width);
+            int mine = 0;
+#pragma unroll
+            for (int k = 0; k < FRONT_ROWS; ++k) {
+                const int i = lo + t + k * FRONT_THREADS;
+                if (i < hi) mine += max(lc_len[i], 0);
+            }
+            base = block_scan_front(mine, s_scan, total);
+            if (total <= FRONT_LIST) break;
+            width >>= 1;   // uniform: every thread saw the same total
+        }
+        const int wn = hi - lo;
+        // staging
+        int beg[FRONT_ROWS], cnt[FRONT_ROWS], wt[FRONT_ROWS];
+        int off = base;
+#pragma unroll
+        for (int k = 0; k < FRONT_ROWS; ++k) {
+            const int loc = t + k * FRONT_THREADS;
+            beg[k] = off;
+            cnt[k] = 0;
+            wt[k] = -1;
+            if (loc >= wn) continue;
+            const int i = lo + loc;
+            const int raw = lc_len[i];
+            unsigned char st0 = state[i] != 0 ? 2 : 0;
+            if (raw < 0) {
+                cnt[k] = -1;
+            } else {
+                const int* row = lc + (size_t)i * cap;
+                int c = 0;
+                for (int q = 0; q < raw; ++q) {
+                    const int r = row[q];
+                    if (r < lo) {
+                        if (state[r] == 1) st0 = 2;
+                    } else {
+                        s_list[off + c++] = (unsigned short)(r - lo);
+                    }
+                }
+                cnt[k] = c;
+                off += raw;
+            }
+            s_st[loc] = st0;
+        }
+        if (t < 3) s_flag[t] = 0;
+        __syncthreads();
+        // rounds until every row of the window is decided
+        for (int it = 0;; ++it) {
+            if (t == 0) s_flag[(it + 1) % 3] = 0;
+            bool pending = false;
+#pragma unroll
+            for (int k = 0; k < FRONT_ROWS; ++k) {
+                const int loc = t + k * FRONT_THREADS;
+                if (loc >= wn || s_st[loc] != 0) continue;
+                if (wt[k] >= 0) {
+                    const int sw = s_st[wt[k]];
+                    if (sw == 0) {
+                        pending = true;
+                        continue;
+                    }
+                    if (sw == 1) {
+                        s_st[loc] = 2;
+                        continue;
+                    }
+                }
+                int blocker = -1;
+                bool lower_root = false;
+                if (cnt[k] >= 0) {
+                    for (int q = 0; q < cnt[k] && !lower_root; ++q) {
+                        const int r = s_list[beg[k] + q];
+                        const int sr = s_st[r];
+                        if (sr == 1) lower_root = true;
+                        else if (sr == 0 && blocker < 0) blocker = r;
+                    }
+                } else {   // a row with a long conflict list walks the graph
+                    const int i = lo + loc;
+                    auto see = [&](int r) {
+                        const int sr = r < lo ? state[r] : s_st[r - lo];
+                        if (sr == 1) lower_root = true;
+                        else if (sr == 0 && blocker < 0) blocker = r - lo;
+                    };
+                    for (int q = t_rp[i]; q < t_rp[i + 1] && !lower_root; ++q) {
+                        const int r = t_ci[q];
+                        if (r >= i) break;
+                        see(r);
+                    }
+                    if (mode == 1)
+                        for (int q = s_rp[i]; q < s_rp[i + 1] && !lower_root; ++q) {
+                            const int j = s_ci[q];
+                            if (j < i) see(j);
+                            for (int z = t_rp[j]; z < t_rp[j + 1] && !lower_root; ++z) {
+                                const int r = t_ci[z];
+                                if (r >= i) break;
+                                see(r);
+                            }
+                        }
+                }
+                if (lower_root) {
+                    s_st[loc] = 2;
+                } else if (blocker >= 0) {
+                    wt[k] = blocker;
+                    pending = true;
+                } else {
+                    s_st[loc] = 1;
+                }
+            }
+            if (pending) s_flag[it % 3] = 1;
+            __syncthreads();
+            ++rounds;
+            if (s_flag[it % 3] == 0) break;
+        }
+#pragma unroll
+        for (int k = 0; k < FRONT_ROWS; ++k) {
+            const int loc = t + k * FRONT_THREADS;
+            if (loc < wn) state[lo + loc] = s_st[loc];
+        }
+        __syncthreads();   // LDS reused by the next window; its staging reads these states
+        lo = hi;
+    }
+    if (t == 0) info[3] += rounds;
+}
+
+__global__ void k_init_state(int n, const int* __restrict__ agg, int* __restrict__ state) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     state[i] = (agg && agg[i] >= 0) ? 2 : 0;
-    wait[i] = -1;
 }
 
 __global__ void k_root_flags(int n, const int* __restrict__ state, int* __restrict__ flags) {
@@ -229,7 +379,9 @@ __global__ void k_agg_total(const int* __restrict__ n1, const int* __restrict__ 
 }
 
 struct AggWs {
-    int *flag, *cnt, *s_rp, *s_ci, *rows, *t_ci, *t_cnt, *t_rp, *state, *wait, *agg1, *agg2, *flags, *id, *id1, *size;
+    int *flag, *cnt, *s_rp, *s_ci, *rows, *t_ci, *t_cnt, *t_rp, *state, *lc, *lc_len, *agg1, *agg2, *flags, *id, *id1,
+        *size;
+    int cap;
     unsigned *keys, *keys_sorted;
     void* tmp;
     size_t tmp_bytes;
@@ -264,7 +416,9 @@ size_t layout(int n, int nnz, char* base, AggWs* w) {
     t.t_cnt = reinterpret_cast<int*>(take(N * 4));
     t.t_rp = reinterpret_cast<int*>(take(N * 4));
     t.state = reinterpret_cast<int*>(take(N * 4));
-    t.wait = reinterpret_cast<int*>(take(N * 4));
+    t.cap = conflict_cap(n, nnz);
+    t.lc = reinterpret_cast<int*>(take((size_t)n * t.cap * 4));
+    t.lc_len = reinterpret_cast<int*>(take(N * 4));
     t.agg1 = reinterpret_cast<int*>(take(N * 4));
     t.agg2 = reinterpret_cast<int*>(take(N * 4));
     t.flags = reinterpret_cast<int*>(take(N * 4));
@@ -316,9 +470,10 @@ int cfa_amg_aggregate(int n, int nnz, const int* rp, const int* ci, const double
     e = rocprim::exclusive_scan(w.tmp, tb, w.t_cnt, w.t_rp, 0, (size_t)n + 1, rocprim::plus<int>(), s);
     if (e != hipSuccess) return (int)e;
     // phase 1
-    hipLaunchKernelGGL(k_init_state, gn, b, 0, s, n, nullptr, w.state, w.wait);
-    hipLaunchKernelGGL(k_front, dim3(1), dim3(FRONT_THREADS), 0, s, n, 1, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.state,
-                       w.wait, info);
+    hipLaunchKernelGGL(k_init_state, gn, b, 0, s, n, nullptr, w.state);
+    hipLaunchKernelGGL(k_lower_conflicts, gn, b, 0, s, n, 1, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.cap, w.lc, w.lc_len);
+    hipLaunchKernelGGL(k_front, dim3(1), dim3(FRONT_THREADS), 0, s, n, 1, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.lc,
+                       w.lc_len, w.cap, w.state, info);
     hipLaunchKernelGGL(k_root_flags, gn, b, 0, s, n, w.state, w.flags);
     hipLaunchKernelGGL(k_fill_keys, dim3(1), dim3(WAVE), 0, s, 1, reinterpret_cast<unsigned*>(w.flags + n), 0u);
     tb = w.tmp_bytes;
@@ -330,9 +485,10 @@ int cfa_amg_aggregate(int n, int nnz, const int* rp, const int* ci, const double
     // phase 2
     hipLaunchKernelGGL(k_phase2, gn, b, 0, s, n, w.s_rp, w.s_ci, w.agg1, w.agg2);
     // phase 3
-    hipLaunchKernelGGL(k_init_state, gn, b, 0, s, n, w.agg2, w.state, w.wait);
-    hipLaunchKernelGGL(k_front, dim3(1), dim3(FRONT_THREADS), 0, s, n, 3, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.state,
-                       w.wait, info);
+    hipLaunchKernelGGL(k_init_state, gn, b, 0, s, n, w.agg2, w.state);
+    hipLaunchKernelGGL(k_lower_conflicts, gn, b, 0, s, n, 3, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.cap, w.lc, w.lc_len);
+    hipLaunchKernelGGL(k_front, dim3(1), dim3(FRONT_THREADS), 0, s, n, 3, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.lc,
+                       w.lc_len, w.cap, w.state, info);
     hipLaunchKernelGGL(k_root_flags, gn, b, 0, s, n, w.state, w.flags);
     tb = w.tmp_bytes;
     e = rocprim::exclusive_scan(w.tmp, tb, w.flags, w.id, 0, (size_t)n + 1, rocprim::plus<int>(), s);

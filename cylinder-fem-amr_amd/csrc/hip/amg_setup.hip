@@ -1,15 +1,15 @@
 // SA-AMG set-up on gfx950: the numeric phases of src/cfa/linalg.cuh:995-1060 and 1276-1431
 // (AmgPreconditioner::setup, spectral_radius_DinvA, smoothed_prolongator, csr_multiply,
-// csr_transpose), which the reference runs on the host with OpenMP. The greedy aggregation stays
-// on the host (it is sequential in row order); everything else runs here and reproduces the
-// host's floating-point results bit for bit:
+// csr_transpose), which the reference runs on the host with OpenMP (the greedy aggregation is in
+// aggregate.hip). Every phase reproduces the host's floating-point results bit for bit:
 //   * the power iteration keeps the host's chunking (rows in chunks of 4096, each chunk summed in
 //     row order, the chunk sums added in chunk order);
 //   * a sparse product accumulates every output entry in the host's encounter order (row entries
 //     of the left factor in order, each right row in order), starting from 0.0;
 //   * contraction is off, so every product and sum is the host's own operation.
-// Nothing is summed by atomics: products and P are built row by row, the transpose is a stable
-// radix sort (rocPRIM) of P's entries by column that moves values without combining them.
+// Nothing is summed by atomics: products and P are built row by row (rows beyond the LDS kernel's
+// 4096 products through a stable sort of the listed products in global memory), the transpose is
+// a stable radix sort (rocPRIM) of P's entries by column that moves values without combining them.
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
@@ -398,6 +398,137 @@ size_t transpose_sort_bytes(int nnz, int cols) {
 
 int blocks(int n) { return blocks_for(n > 0 ? n : 1); }
 
+// ---- products beyond the LDS method: expand, stable sort, compress -----------------------------
+// Every product is listed in encounter order (the left row's entries in order, each right row in
+// order) under the key (row, column); a stable radix sort groups equal keys and keeps each group
+// in that order, and each group is summed from 0.0 in order: the host's accumulation order.
+struct CountAt64 {
+    const long long* c;
+    int n;
+    __host__ __device__ long long operator()(int i) const { return i < n ? c[i] : 0; }
+};
+
+__global__ void k_entry_products(int nnz, const int* __restrict__ a_ci, const int* __restrict__ b_rp,
+                                 long long* __restrict__ cnt) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnz) return;
+    const int k = a_ci[a];
+    cnt[a] = b_rp[k + 1] - b_rp[k];
+}
+
+__global__ void k_expand_products(int nnz, const int* __restrict__ row_of, const int* __restrict__ a_ci,
+                                  const double* __restrict__ a_v, const int* __restrict__ b_rp,
+                                  const int* __restrict__ b_ci, const double* __restrict__ b_v,
+                                  const long long* __restrict__ off, unsigned cb, unsigned long long* __restrict__ key,
+                                  double* __restrict__ val) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnz) return;
+    const int k = a_ci[a];
+    const unsigned long long r = (unsigned long long)(unsigned)row_of[a] << cb;
+    const double av = a_v[a];
+    long long o = off[a];
+    for (int q = b_rp[k]; q < b_rp[k + 1]; ++q, ++o) {
+        key[o] = r | (unsigned)b_ci[q];
+        val[o] = av * b_v[q];
+    }
+}
+
+__global__ void k_group_heads(long long n, const unsigned long long* __restrict__ key, int* __restrict__ head) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    head[p] = (p == 0 || key[p] != key[p - 1]) ? 1 : 0;
+}
+
+__global__ void k_group_sums(long long n, const unsigned long long* __restrict__ key, const double* __restrict__ val,
+                             const int* __restrict__ gidx, unsigned cb, int* __restrict__ c_ci,
+                             double* __restrict__ c_v, int* __restrict__ grow) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || (p > 0 && key[p] == key[p - 1])) return;
+    const unsigned long long k = key[p];
+    double acc = 0.0;
+    for (long long q = p; q < n && key[q] == k; ++q) acc += val[q];
+    const int g = gidx[p];
+    c_ci[g] = (int)(k & ((1ull << cb) - 1ull));
+    c_v[g] = acc;
+    grow[g] = (int)(k >> cb);
+}
+
+// c_rp[i] = first group whose row is >= i, for i = 0..rows (grow ascending)
+__global__ void k_row_starts(int rows, int groups, const int* __restrict__ grow, int* __restrict__ c_rp) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > rows) return;
+    int lo = 0, hi = groups;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (grow[mid] < i) lo = mid + 1;
+        else hi = mid;
+    }
+    c_rp[i] = lo;
+}
+
+// workspace of the sorted product, in 256-byte aligned pieces
+struct SortedWs {
+    long long* cnt;            // [a_nnz] products of each left entry
+    long long* off;            // [a_nnz + 1] their offsets (off[a_nnz] = products)
+    int* row_of;               // [a_nnz]
+    unsigned long long* key[2];   // [products] listed, sorted
+    double* val[2];            // [products] listed, sorted
+    int* head;                 // [products] group heads, then the group rows
+    int* gidx;                 // [products + 1] group of each head
+    void* tmp;
+    size_t tmp_bytes;
+};
+
+size_t sorted_tmp_bytes(int a_nnz, long long products, unsigned end_bit) {
+    size_t need = 0, b = 0;
+    const auto in64 = rocprim::make_transform_iterator(rocprim::counting_iterator<int>(0), CountAt64{nullptr, a_nnz});
+    if (rocprim::exclusive_scan(nullptr, b, in64, static_cast<long long*>(nullptr), 0LL, (size_t)a_nnz + 1,
+                                rocprim::plus<long long>()) != hipSuccess)
+        return 0;
+    need = b;
+    if (products > 0) {
+        b = 0;
+        if (rocprim::radix_sort_pairs(nullptr, b, static_cast<const unsigned long long*>(nullptr),
+                                      static_cast<unsigned long long*>(nullptr), static_cast<const double*>(nullptr),
+                                      static_cast<double*>(nullptr), (size_t)products, 0, end_bit) != hipSuccess)
+            return 0;
+        need = b > need ? b : need;
+        b = 0;
+        const auto inh = rocprim::make_transform_iterator(rocprim::counting_iterator<int>(0),
+                                                          CountAt{nullptr, (int)products});
+        if (rocprim::exclusive_scan(nullptr, b, inh, static_cast<int*>(nullptr), 0, (size_t)products + 1,
+                                    rocprim::plus<int>()) != hipSuccess)
+            return 0;
+        need = b > need ? b : need;
+    }
+    return need;
+}
+
+size_t sorted_layout(int a_nnz, long long products, unsigned end_bit, void* base, SortedWs* w) {
+    char* p = static_cast<char*>(base);
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        char* q = p ? p + at : nullptr;
+        at += align_up(bytes > 0 ? bytes : 1);
+        return q;
+    };
+    const size_t P = (size_t)products;
+    SortedWs s;
+    s.cnt = reinterpret_cast<long long*>(take((size_t)a_nnz * sizeof(long long)));
+    s.off = reinterpret_cast<long long*>(take(((size_t)a_nnz + 1) * sizeof(long long)));
+    s.row_of = reinterpret_cast<int*>(take((size_t)a_nnz * sizeof(int)));
+    s.key[0] = reinterpret_cast<unsigned long long*>(take(P * sizeof(unsigned long long)));
+    s.key[1] = reinterpret_cast<unsigned long long*>(take(P * sizeof(unsigned long long)));
+    s.val[0] = reinterpret_cast<double*>(take(P * sizeof(double)));
+    s.val[1] = reinterpret_cast<double*>(take(P * sizeof(double)));
+    s.head = reinterpret_cast<int*>(take(P * sizeof(int)));
+    s.gidx = reinterpret_cast<int*>(take((P + 1) * sizeof(int)));
+    s.tmp_bytes = sorted_tmp_bytes(a_nnz, products, end_bit);
+    s.tmp = take(s.tmp_bytes);
+    if (w) *w = s;
+    return s.tmp_bytes ? at : 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -523,6 +654,73 @@ int cfa_csr_transpose(int rows, int cols, int nnz, const int* rp, const int* ci,
         hipLaunchKernelGGL(k_transpose_gather, dim3(blocks(nnz)), dim3(BLOCK), 0, s, nnz, order, row_of, v, t_ci, t_v);
     }
     hipLaunchKernelGGL(k_column_starts, dim3(blocks(cols + 1)), dim3(BLOCK), 0, s, cols, nnz, keys, t_rp);
+    return launch_status();
+}
+
+size_t cfa_spgemm_sorted_ws_bytes(int rows, int cols, int a_nnz, long long products) {
+    if (rows < 0 || cols < 0 || a_nnz < 0 || products < 0 || products > 0x7fffffffLL) return 0;
+    return sorted_layout(a_nnz, products, column_bits(cols) + column_bits(rows), nullptr, nullptr);
+}
+
+int cfa_spgemm_sorted(int rows, int cols, int a_nnz, const int* a_rp, const int* a_ci, const double* a_v,
+                      const int* b_rp, const int* b_ci, const double* b_v, int pass, long long products, void* ws,
+                      size_t ws_bytes, long long* count, int* c_rp, int* c_ci, double* c_v, cfa_stream_t stream) {
+    if (rows < 0 || cols < 0 || a_nnz < 0 || pass < 0 || pass > 2 || products < 0 || !ws) return CFA_E_ARG;
+    if ((pass < 2 && !count) || (pass == 2 && !c_rp)) return CFA_E_ARG;
+    if (a_nnz > 0 && (!a_rp || !a_ci || !a_v || !b_rp || !b_ci || !b_v)) return CFA_E_ARG;
+    const size_t need = cfa_spgemm_sorted_ws_bytes(rows, cols, a_nnz, pass == 0 ? 0 : products);
+    if (need == 0 || ws_bytes < need) return CFA_E_ARG;
+    const unsigned cb = column_bits(cols), end_bit = cb + column_bits(rows);
+    SortedWs w;
+    sorted_layout(a_nnz, pass == 0 ? 0 : products, end_bit, ws, &w);
+    const hipStream_t s = as_stream(stream);
+    const long long P = products;
+    if (pass < 2) {
+        // products per left entry and their offsets (also the total)
+        if (a_nnz > 0)
+            hipLaunchKernelGGL(k_entry_products, dim3(blocks(a_nnz)), dim3(BLOCK), 0, s, a_nnz, a_ci, b_rp, w.cnt);
+        const auto in = rocprim::make_transform_iterator(rocprim::counting_iterator<int>(0), CountAt64{w.cnt, a_nnz});
+        size_t tb = w.tmp_bytes;
+        hipError_t e = rocprim::exclusive_scan(w.tmp, tb, in, w.off, 0LL, (size_t)a_nnz + 1,
+                                               rocprim::plus<long long>(), s);
+        if (e != hipSuccess) return (int)e;
+        if (pass == 0) {
+            e = hipMemcpyAsync(count, w.off + a_nnz, sizeof(long long), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            return e == hipSuccess ? launch_status() : (int)e;
+        }
+        // list, sort, mark the groups; *count = groups
+        if (P > 0) {
+            hipLaunchKernelGGL(k_row_of_entries, dim3(blocks(rows)), dim3(BLOCK), 0, s, rows, a_rp, w.row_of);
+            hipLaunchKernelGGL(k_expand_products, dim3(blocks(a_nnz)), dim3(BLOCK), 0, s, a_nnz, w.row_of, a_ci, a_v,
+                               b_rp, b_ci, b_v, w.off, cb, w.key[0], w.val[0]);
+            tb = w.tmp_bytes;
+            e = rocprim::radix_sort_pairs(w.tmp, tb, w.key[0], w.key[1], w.val[0], w.val[1], (size_t)P, 0, end_bit, s);
+            if (e != hipSuccess) return (int)e;
+            hipLaunchKernelGGL(k_group_heads, dim3(blocks_for(P)), dim3(BLOCK), 0, s, P, w.key[1], w.head);
+        }
+        const auto inh = rocprim::make_transform_iterator(rocprim::counting_iterator<int>(0), CountAt{w.head, (int)P});
+        tb = w.tmp_bytes;
+        e = rocprim::exclusive_scan(w.tmp, tb, inh, w.gidx, 0, (size_t)P + 1, rocprim::plus<int>(), s);
+        if (e != hipSuccess) return (int)e;
+        int groups = 0;
+        e = hipMemcpyAsync(&groups, w.gidx + P, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return (int)e;
+        *count = groups;
+        return launch_status();
+    }
+    // pass 2: the sums, columns and row starts (the workspace of pass 1, unchanged)
+    int groups = 0;
+    if (P > 0) {
+        hipError_t e = hipMemcpyAsync(&groups, w.gidx + P, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return (int)e;
+        if (groups > 0 && (!c_ci || !c_v)) return CFA_E_ARG;
+        hipLaunchKernelGGL(k_group_sums, dim3(blocks_for(P)), dim3(BLOCK), 0, s, P, w.key[1], w.val[1], w.gidx, cb, c_ci,
+                           c_v, w.head);
+    }
+    hipLaunchKernelGGL(k_row_starts, dim3(blocks(rows + 1)), dim3(BLOCK), 0, s, rows, groups, w.head, c_rp);
     return launch_status();
 }
 

@@ -80,6 +80,7 @@ struct AdaptiveMesh::Device {
     int c[CFA_NVB_COUNTERS] = {};
     int n_leaves = 0, n_nodes = 0;
     std::vector<int> passes;   // log offsets of the bisection passes of the last adaptation
+    std::vector<int> pass_n;   // bisections per pass of the last adaptation
     // host copy of the cylinder vertices' coordinates (by vertex slot): the arc positions of new
     // cylinder midpoints are computed here with the host's atan2 / cos / sin (amr.cuh:802-846)
     std::vector<double> cx, cy;
@@ -193,6 +194,11 @@ struct AdaptiveMesh::Device {
         const cfa_forest_t f = view();
         GPU_CHECK(cfa_nvb_bisect(&f, n, list.ptr, inherit ? 1 : 0, target.ptr, umax, scratch.ptr, tmp.ptr, tmp_bytes,
                                  s));
+        // element slots in use after the pass, as the device counts them (2n children, recycled
+        // slots first), so that the next selection scans the new children too
+        const int etake = std::min(2 * n, c[CFA_NVB_EFREE]);
+        c[CFA_NVB_NE] += 2 * n - etake;
+        c[CFA_NVB_EFREE] -= etake;
     }
     // Refinement rounds (amr.cuh:476-508): every leaf below its target generation is bisected
     // (children inherit the target), then every hanging leaf, pass after pass, until none hangs.
@@ -200,15 +206,18 @@ struct AdaptiveMesh::Device {
     void refine_rounds(double umax, AmrStats& st) {
         size_t occupied = (size_t)c[CFA_NVB_NV];
         passes.assign(1, c[CFA_NVB_LOG]);
+        pass_n.clear();
         while (true) {
             const int n = select(1);
             if (n == 0) break;
+            pass_n.push_back(n);
             bisect(n, true, umax, occupied);
             st.bisections += n;
             while (true) {
                 const int h = select(2);   // also reads the counters of the pass before
                 passes.push_back(c[CFA_NVB_LOG]);
                 if (h == 0) break;
+                pass_n.push_back(-h);
                 bisect(h, false, umax, occupied);
                 st.closure_bisections += h;
             }
@@ -273,6 +282,15 @@ AdaptiveMesh::~AdaptiveMesh() {
 int AdaptiveMesh::num_leaves() const { return d_->n_leaves; }
 int AdaptiveMesh::num_vertex_slots() const { return d_->c[CFA_NVB_NV]; }
 
+const std::vector<int>& AdaptiveMesh::pass_sizes() const { return d_->pass_n; }
+
+void AdaptiveMesh::slot_counts(int out[4]) const {
+    out[0] = d_->c[CFA_NVB_NE];
+    out[1] = d_->c[CFA_NVB_NV];
+    out[2] = d_->c[CFA_NVB_EFREE];
+    out[3] = d_->c[CFA_NVB_VFREE];
+}
+
 void AdaptiveMesh::extract(Mesh2D& mesh, std::vector<double>& h_u, std::vector<double>& h_v,
                            std::vector<double>& h_p, std::vector<double>& h_uo, std::vector<double>& h_vo) {
     Device& d = *d_;
@@ -324,6 +342,8 @@ void AdaptiveMesh::extract(Mesh2D& mesh, std::vector<double>& h_u, std::vector<d
     }
     vert_ids_.resize(nn);
     d.nodes.get(vert_ids_.data(), nn, d.s);
+    leaf_ids_.resize(L);
+    d.leaves.get(leaf_ids_.data(), L, d.s);
     dbg_ms[D_DOWNLOAD] += ms_since(t);
 }
 

@@ -62,7 +62,13 @@ public:
     int num_leaves() const;
     // compact node of the last extract() -> persistent vertex slot
     const std::vector<int>& node_vertices() const { return vert_ids_; }
+    // compact element of the last extract() -> forest element slot
+    const std::vector<int>& leaf_elements() const { return leaf_ids_; }
     int num_vertex_slots() const;
+    // slots in use and recycled slots waiting for reuse: {elements, vertices, free elements, free vertices}
+    void slot_counts(int out[4]) const;
+    // bisections of each refinement pass of the last adapt() / refine_uniform(), in order
+    const std::vector<int>& pass_sizes() const;
 
     void scatter_solution(const std::vector<double>& h_u, const std::vector<double>& h_v,
                           const std::vector<double>& h_p, const std::vector<double>& h_uo,
@@ -80,7 +86,7 @@ public:
 private:
     struct Device;
     std::unique_ptr<Device> d_;
-    std::vector<int> vert_ids_;
+    std::vector<int> vert_ids_, leaf_ids_;
 };
 
 }  // namespace cfa

@@ -382,8 +382,9 @@ void AmgPreconditioner::release() {
 
 // C = A B on the device (host csr_multiply results, bit for bit); false when a row has more
 // products than the device kernel holds
-bool AmgPreconditioner::device_product(int rows, const int* arp, const int* aci, const double* av, const int* brp,
-                                       const int* bci, const double* bv, DevCsr& C, int left_max_row) {
+bool AmgPreconditioner::device_product(int rows, int cols, int a_nnz, const int* arp, const int* aci,
+                                       const double* av, const int* brp, const int* bci, const double* bv, DevCsr& C,
+                                       int left_max_row) {
     const cfa_stream_t s = solver_stream();
     d_cnt_.reserve(std::max(rows, 1));
     d_flag_.reserve(2);
@@ -393,7 +394,8 @@ bool AmgPreconditioner::device_product(int rows, const int* arp, const int* aci,
     // method 0: one thread per row merging the right rows (rows of A up to 16 entries);
     // method 1: one workgroup per row sorting the products in LDS (up to 4096 per row).
     // The count pass, the scan and one download of {overflow flag, total} per attempt.
-    for (int method = left_max_row >= 0 && left_max_row <= 16 ? 0 : 1; method < 2; ++method) {
+    const int first_method = force_sorted_products ? 2 : left_max_row >= 0 && left_max_row <= 16 ? 0 : 1;
+    for (int method = first_method; method < 2; ++method) {
         dev_zero(d_flag_, 2 * sizeof(int));
         GPU_CHECK(cfa_spgemm(rows, arp, aci, av, brp, bci, bv, 0, method, d_cnt_, nullptr, nullptr, nullptr, d_flag_,
                              s));
@@ -410,7 +412,27 @@ bool AmgPreconditioner::device_product(int rows, const int* arp, const int* aci,
         C.tpr = csr_lanes_per_row(rows, total);
         return true;
     }
-    return false;
+    // every product listed in global memory, stably sorted by (row, column), summed per key
+    long long products = 0, entries = 0;
+    size_t ws = cfa_spgemm_sorted_ws_bytes(rows, cols, a_nnz, 0);
+    if (ws == 0) return false;
+    d_gws_.reserve(ws);
+    GPU_CHECK(cfa_spgemm_sorted(rows, cols, a_nnz, arp, aci, av, brp, bci, bv, 0, 0, d_gws_, ws, &products, nullptr,
+                                nullptr, nullptr, s));
+    ws = cfa_spgemm_sorted_ws_bytes(rows, cols, a_nnz, products);
+    if (ws == 0) return false;
+    d_gws_.reserve(ws);
+    GPU_CHECK(cfa_spgemm_sorted(rows, cols, a_nnz, arp, aci, av, brp, bci, bv, 1, products, d_gws_, ws, &entries,
+                                nullptr, nullptr, nullptr, s));
+    C.ci.reserve(std::max<long long>(entries, 1));
+    C.v.reserve(std::max<long long>(entries, 1));
+    GPU_CHECK(cfa_spgemm_sorted(rows, cols, a_nnz, arp, aci, av, brp, bci, bv, 2, products, d_gws_, ws, nullptr, C.rp,
+                                C.ci, C.v, s));
+    C.rows = rows;
+    C.nnz = (int)entries;
+    C.tpr = csr_lanes_per_row(rows, C.nnz);
+    ++sorted_products;
+    return true;
 }
 
 
@@ -523,7 +545,7 @@ int AmgPreconditioner::device_level(int n, int nnz, const int* arp, const int* a
         iota_dev_n_ = m;
     }
     // P = P0 - omega D^-1 A P0
-    if (!device_product(n, arp, aci, av, d_iota_, d_agg_, d_p0v_, dev_tmp_, left_max)) {
+    if (!device_product(n, na, nnz, arp, aci, av, d_iota_, d_agg_, d_p0v_, dev_tmp_, left_max)) {
         give_back();
         return -1;
     }
@@ -557,8 +579,9 @@ int AmgPreconditioner::device_level(int n, int nnz, const int* arp, const int* a
     L->R.tpr = csr_lanes_per_row(na, p_nnz);
     phase_end(PH_TRANSPOSE);
     // Galerkin product R (A P) (rows of R cover an aggregate's neighbourhood: the LDS-sort kernel)
-    if (!device_product(n, arp, aci, av, L->P.rp, L->P.ci, L->P.v, dev_tmp_, left_max) ||
-        !device_product(na, L->R.rp, L->R.ci, L->R.v, dev_tmp_.rp, dev_tmp_.ci, dev_tmp_.v, dev_next2_, -1)) {
+    if (!device_product(n, na, nnz, arp, aci, av, L->P.rp, L->P.ci, L->P.v, dev_tmp_, left_max) ||
+        !device_product(na, na, p_nnz, L->R.rp, L->R.ci, L->R.v, dev_tmp_.rp, dev_tmp_.ci, dev_tmp_.v, dev_next2_,
+                        -1)) {
         give_back();
         return -1;
     }
@@ -593,6 +616,7 @@ void AmgPreconditioner::setup(int n, int nnz, const int* d_row_ptr, const int* d
     last_aggregates.clear();
     device_levels = 0;
     front_rounds = 0;
+    sorted_products = 0;
     // the current level's operator: the caller's matrix, then the Galerkin products
     const int* arp = d_row_ptr;
     const int* aci = d_col_ind;
@@ -602,7 +626,7 @@ void AmgPreconditioner::setup(int n, int nnz, const int* d_row_ptr, const int* d
         int n_next = 0, nnz_next = 0;
         const int r = device_level(rows, entries, arp, aci, av, old, n_next, nnz_next);
         if (r == 0) break;
-        if (r < 0) fatal("[AMG] a Galerkin product row holds more than 4096 products; the hierarchy cannot be built");
+        if (r < 0) fatal("[AMG] a sparse product of the set-up exceeds 2^31 - 1 products; the hierarchy cannot be built");
         ++device_levels;
         arp = dev_next_.rp;
         aci = dev_next_.ci;

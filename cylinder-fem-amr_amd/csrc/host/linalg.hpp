@@ -151,6 +151,8 @@ public:
     bool keep_p0_host = true;      // download the level-0 prolongator for remap()
     bool keep_aggregates = false;  // download the level-0 aggregates into last_aggregates
     int device_levels = 0;         // levels the last set-up built
+    bool force_sorted_products = false;   // every product by the global-memory sort (a check)
+    int sorted_products = 0;       // products of the last set-up that took the global-memory sort
     long long front_rounds = 0;    // rounds of the aggregation's decision front in the last set-up
 
     std::vector<std::unique_ptr<Level>> levels;
@@ -178,19 +180,20 @@ public:
 private:
     DevCsr dev_next_, dev_tmp_, dev_next2_;    // device set-up: next operator, products
     DevArray<int> d_cnt_, d_flag_, d_agg_, d_iota_, d_info_;
-    DevArray<unsigned char> d_tws_, d_sws_, d_aws_;   // transpose, scan and aggregation workspaces
+    DevArray<unsigned char> d_tws_, d_sws_, d_aws_, d_gws_;   // transpose, scan, aggregation, sorted product
     DevArray<double> d_p0v_, d_x0_, d_pw_;
     std::vector<double> x0_host_;
     int x0_dev_n_ = 0, iota_dev_n_ = 0;
     // one level: 1 built (the next operator in dev_next_, its size in n_next, nnz_next), 0 stop
-    // (coarsening stalled), -1 a product row beyond the device kernel's capacity
+    // (coarsening stalled), -1 a product beyond the device kernels' capacity
     int device_level(int n, int nnz, const int* arp, const int* aci, const double* av,
                      std::vector<std::unique_ptr<Level>>& pool, int& n_next, int& nnz_next);
-    // C = A B on the device; left_max_row: the longest row of A when known (rows of at most 16
-    // entries take the merge kernel), -1 for the LDS-sort kernel straight away. false: a row is
-    // beyond both kernels.
-    bool device_product(int rows, const int* arp, const int* aci, const double* av, const int* brp, const int* bci,
-                        const double* bv, DevCsr& C, int left_max_row);
+    // C = A B on the device (A rows x k with a_nnz entries, B k x cols); left_max_row: the longest
+    // row of A when known (rows of at most 16 entries take the merge kernel), -1 for the LDS-sort
+    // kernel straight away; a row beyond both takes the global-memory sort. false: more than
+    // 2^31 - 1 products.
+    bool device_product(int rows, int cols, int a_nnz, const int* arp, const int* aci, const double* av,
+                        const int* brp, const int* bci, const double* bv, DevCsr& C, int left_max_row);
     std::vector<std::unique_ptr<Level>> spare_;
     std::vector<cfa_amg_level_t> desc_;
     cfa_amg_t view_{};

@@ -366,7 +366,7 @@ int cfa_bicgstab_jacobi(int n, const int* rp, const int* ci, const double* val, 
 /* ---- SA-AMG set-up on the device (src/cfa/linalg.cuh:995-1060, 1276-1431, host code in the
  * reference) -----------------------------------------------------------------------------------
  * Bit-identical to the reference's host set-up: the same operations in the same order
- * (contraction off). The greedy aggregation is not here (it stays on the host).
+ * (contraction off), the greedy aggregation included (cfa_amg_aggregate).
  * cfa_amg_dinv: dinv = 1 / a_ii, or 1 where |a_ii| <= 1e-300 (linalg.cuh:1003-1008).
  * cfa_amg_power: spectral_radius_DinvA (linalg.cuh:1276-1300): `iterations` power steps of
  * D^-1 A from the device vector x0 (the reference starts from 1 + 0.1 sin(1 + i)), norms summed
@@ -396,6 +396,19 @@ int cfa_amg_power(int n, const int* rp, const int* ci, const double* val, const 
 int cfa_spgemm(int rows, const int* a_rp, const int* a_ci, const double* a_v, const int* b_rp, const int* b_ci,
                const double* b_v, int pass, int method, int* cnt, const int* c_rp, int* c_ci, double* c_v,
                int* overflow, cfa_stream_t stream);
+/* cfa_spgemm_sorted: C = A B (csr_multiply, linalg.cuh:799-844) for rows beyond cfa_spgemm's
+ * limits: every product is listed in encounter order under the key (row, column), stably radix
+ * sorted, and each key's products are summed from 0 in that order (the same values as
+ * cfa_spgemm). A is rows x (rows of B) with a_nnz entries, B has `cols` columns. Three passes on
+ * one workspace: pass 0 (ws of cfa_spgemm_sorted_ws_bytes(rows, cols, a_nnz, 0) bytes) sets the
+ * host *count = products; pass 1 (ws of cfa_spgemm_sorted_ws_bytes(..., products) bytes) lists and
+ * sorts them and sets the host *count = entries of C; pass 2 on the same workspace writes c_rp
+ * [rows + 1], c_ci and c_v [entries]. Passes 0 and 1 synchronize the stream. At most 2^31 - 1
+ * products (the size query returns 0 beyond). */
+size_t cfa_spgemm_sorted_ws_bytes(int rows, int cols, int a_nnz, long long products);
+int cfa_spgemm_sorted(int rows, int cols, int a_nnz, const int* a_rp, const int* a_ci, const double* a_v,
+                      const int* b_rp, const int* b_ci, const double* b_v, int pass, long long products, void* ws,
+                      size_t ws_bytes, long long* count, int* c_rp, int* c_ci, double* c_v, cfa_stream_t stream);
 size_t cfa_scan_counts_ws_bytes(int n);
 int cfa_scan_counts(int n, const int* cnt, int* out, int* total, void* ws, size_t ws_bytes, cfa_stream_t stream);
 int cfa_amg_prolongator(int n, const int* ap_rp, const int* ap_ci, const double* ap_v, const int* agg,

@@ -228,11 +228,13 @@ void NvbForest::refine(std::vector<int>& target, AmrCounts& st) {
             if (mid_.count(cfa::edge_key(w[k], w[(k + 1) % 3]))) return true;
         return false;
     };
+    pass_n_.clear();
     while (true) {
         std::vector<int> work;
         for (int e = 0; e < (int)el_.size(); ++e)
             if (leaf(e) && e < (int)target.size() && target[e] > el_[e].gen) work.push_back(e);
         if (work.empty()) break;
+        pass_n_.push_back((int)work.size());
         for (int e : work) {
             split(e);
             ++st.bisections;
@@ -245,6 +247,7 @@ void NvbForest::refine(std::vector<int>& target, AmrCounts& st) {
             for (int e = 0; e < (int)el_.size(); ++e)
                 if (leaf(e) && hanging(e)) h.push_back(e);
             if (h.empty()) break;
+            pass_n_.push_back(-(int)h.size());
             for (int e : h) {
                 split(e);
                 ++st.closure_bisections;

@@ -69,6 +69,10 @@ public:
     const std::vector<int>& leaf_slots() const { return leaves_; }  // compact element -> forest slot
     int vertex_slots() const { return (int)x_.size(); }
     int element_slots() const { return (int)el_.size(); }
+    int free_vertex_slots() const { return (int)vfree_.size(); }
+    int free_element_slots() const { return (int)efree_.size(); }
+    // bisections of each refinement pass of the last adapt (work passes positive, closure negative)
+    const std::vector<int>& pass_sizes() const { return pass_n_; }
 
 private:
     struct El {
@@ -88,6 +92,7 @@ private:
     std::map<uint64_t, int> mid_;                 // bisected edge -> its midpoint
     std::vector<std::array<int, 3>> created_;     // {m, a, b} of the last adapt, creation order
     std::vector<int> leaves_, nodes_;             // last extract
+    std::vector<int> pass_n_;
 
     bool leaf(int e) const { return el_[e].alive && el_[e].ch[0] < 0; }
     double area(int e) const;

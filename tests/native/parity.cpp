@@ -432,11 +432,17 @@ void case_amg(Fixture& f) {
     const PressureSystem o = case_pressure(f, false);
     oracle::Amg oa;
     oa.setup(host_csr(f, o.Kp));
-    {
+    // the default products, then every product through the global-memory sort that takes rows
+    // beyond the LDS kernel
+    for (int sorted = 0; sorted < 2; ++sorted) {
         AmgPreconditioner amg;
         amg.keep_aggregates = true;
+        amg.force_sorted_products = sorted == 1;
+        if (sorted) std::printf("  every product by the sorted global-memory method:\n");
         amg.setup(f.n, f.nnz, f.g.d_row_ptr, f.g.d_col_ind, f.g.d_Kp_val);
-        std::printf("  aggregation decision-front rounds: %lld\n", amg.front_rounds);
+        std::printf("  aggregation decision-front rounds: %lld, sorted products %d\n", amg.front_rounds,
+                    amg.sorted_products);
+        if (sorted) expect(amg.sorted_products == 3 * (int)amg.levels.size(), "every product sorted");
         expect(amg.num_levels() == oa.num_levels(), "number of levels");
         std::vector<int> rows_g, rows_o, nnz_g, nnz_o;
         Vec om_g, om_o;
@@ -644,14 +650,30 @@ void case_nvb(Fixture&, int level) {
                a.bisections == b.bisections && a.closure_bisections == b.closure_bisections &&
                a.removed_vertices == b.removed_vertices && a.leaves_after == b.leaves_after,
                "adaptation counts");
+        {
+            const std::vector<int>& a = P.pass_sizes();
+            const std::vector<int>& b = O.pass_sizes();
+            std::string sa, sb;
+            for (int k : a) sa += " " + std::to_string(k);
+            for (int k : b) sb += " " + std::to_string(k);
+            if (a != b) std::printf("  passes (work > 0, closure < 0): device%s\n                                  oracle%s\n", sa.c_str(), sb.c_str());
+            same_ints("bisections per refinement pass", a, b);
+        }
         const std::vector<int> old_p = P.node_vertices(), old_o = O.node_slots();
         P.extract(mp, fp[0], fp[1], fp[2], fp[3], fp[4]);
         O.extract(mo, fo[0], fo[1], fo[2], fo[3], fo[4]);
         char what[48];
         std::snprintf(what, sizeof what, "cycle %d", c);
         compare_meshes(what, mp, mo, fp, fo);
+        same_ints("element slots of the leaves", P.leaf_elements(), O.leaf_slots());
         same_ints("vertex slots of the nodes", P.node_vertices(), O.node_slots());
-        expect(P.num_vertex_slots() == O.vertex_slots(), "vertex slot count");
+        int cnt[4];
+        P.slot_counts(cnt);
+        std::printf("  slots device {elements %d, vertices %d, free %d, %d} oracle {%d, %d, free %d, %d}\n", cnt[0],
+                    cnt[1], cnt[2], cnt[3], O.element_slots(), O.vertex_slots(), O.free_element_slots(),
+                    O.free_vertex_slots());
+        expect(cnt[0] == O.element_slots() && cnt[1] == O.vertex_slots() && cnt[2] == O.free_element_slots() &&
+               cnt[3] == O.free_vertex_slots(), "slot counts");
         if (c % 2) {   // the overlapped remesh's transfer of a later state
             Vec t[5], rp[5], ro[5];
             for (Vec& v : t) v.resize(old_p.size());

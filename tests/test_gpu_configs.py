@@ -102,8 +102,9 @@ def test_u6_momentum_vs_oracle(u6_momentum):
     # BiCGSTAB's residual is not monotone: over 45 iterations the reduction order moves the
     # first step below the tolerance by up to a few iterations
     assert abs(it64[0] - st["it_u"]) <= 3 and abs(it64[1] - st["it_v"]) <= 3
-    # two iterates that both met the 1e-8 stopping rule differ by about the tolerance; both are
-    # compared with the oracle's solution at 1e-12
+    # an iterate that met the 1e-8 residual rule is within about cond(A) * 1e-8 of the solution:
+    # the GPU's iterates are held to three times the oracle's own 1e-8 iterate's distance from the
+    # oracle's solution at 1e-12, and their residuals are recomputed on the host
     uc, vc, stc = cfa_oracle.bicgstab2_ilu(s["rp"], s["ci"], s["val"], s["b"][0], s["b"][1], s["x0"][0],
                                            s["x0"][1], tol=1e-12)
     assert stc["converged_u"] and stc["converged_v"]
@@ -111,11 +112,13 @@ def test_u6_momentum_vs_oracle(u6_momentum):
     for c, exact, o8 in ((0, uc, u), (1, vc, v)):
         e64, e32, eo = rel(s["x"][c], exact), rel(s["x"][2 + c], exact), rel(o8, exact)
         r0 = np.linalg.norm(s["b"][c] - A @ s["x0"][c])
+        r64 = np.linalg.norm(s["b"][c] - A @ s["x"][c]) / r0
         r32 = np.linalg.norm(s["b"][c] - A @ s["x"][2 + c]) / r0
         print(f"  component {c}: error vs the oracle at 1e-12: GPU FP64 factors {e64:.3e}, FP32 factors {e32:.3e}, "
-              f"oracle at 1e-8 {eo:.3e}; FP32 host residual {r32:.3e}", flush=True)
-        assert e64 <= 5e-8 and e32 <= 5e-8 and eo <= 5e-8
-        assert r32 <= 10 * 1e-8
+              f"oracle at 1e-8 {eo:.3e}; host residuals FP64 {r64:.3e}, FP32 {r32:.3e}", flush=True)
+        assert eo <= 1e-6
+        assert e64 <= 3 * eo and e32 <= 3 * eo
+        assert r64 <= 2e-8 and r32 <= 2e-8
 
 
 @pytest.mark.parametrize("steps", [300])

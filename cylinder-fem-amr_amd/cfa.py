@@ -159,7 +159,7 @@ def workload_pressure(level: int, nrhs: int = 32, tol: float = 1e-8, kernel_reps
 
 
 AMG_PHASES = ("download", "dinv_rho", "aggregation", "p_smoothing", "transpose", "galerkin", "upload", "coarse_inverse",
-              "remap", "device_levels")
+              "remap", "device_levels", "front_rounds", "sorted_products")
 
 
 def workload_amg_phases() -> dict:

@@ -31,7 +31,7 @@ namespace {
 constexpr int FRONT_THREADS = 1024;
 constexpr int FRONT_ROWS = 8;                                // window rows per thread
 constexpr int FRONT_WINDOW = FRONT_THREADS * FRONT_ROWS;    // rows per window at most
-constexpr int FRONT_LIST = 61440;                            // staged conflict entries (LDS, 120 KB)
+constexpr int FRONT_LIST = 45056;                            // staged upper-list entries (LDS, 88 KB)
 
 int nblk(long long n) { return blocks_for(n > 0 ? n : 1); }
 
@@ -156,28 +156,71 @@ __device__ int block_scan_front(int v, int* s, int& total) {
     return before + x - v;
 }
 
+// appends `item` for the lanes with `want` to q at the LDS counter *qn (one atomic per wave)
+__device__ __forceinline__ void wave_append(bool want, unsigned short item, unsigned short* q, int* qn) {
+    const unsigned long long m = __ballot(want);
+    if (!m) return;
+    const int lane = threadIdx.x & (WAVE - 1), leader = __ffsll((long long)m) - 1;
+    int b = 0;
+    if (lane == leader) b = atomicAdd(qn, __popcll(m));
+    b = __shfl(b, leader, WAVE);
+    if (want) q[b + __popcll(m & ((1ull << lane) - 1))] = item;
+}
+
+// LDS word of a window row: bits 0-7 its lower conflicts in the window not yet decided, bit 8 a
+// lower root seen, bit 9 a row with a long list (decides by walking the graph), bits 10-11 its
+// decision once processed (1 root, 2 not a root), bits 16-31 the start of its upper list (the
+// window rows that have it as a lower conflict; lists in row order, word wn holds the end).
+constexpr unsigned W_FLAG = 1u << 8, W_LONG = 1u << 9, W_ROOT = 1u << 10, W_NOT = 2u << 10, W_DONE = 3u << 10;
+
+// stage the lower conflicts of the window rows (vector loads of the row's slot): f(r) per entry
+template <typename F>
+__device__ __forceinline__ void for_lower(const int* __restrict__ lc, int cap, int i, int raw, F f) {
+    const int4* row4 = reinterpret_cast<const int4*>(lc + (size_t)i * cap);
+    for (int b = 0; b < raw; b += 16) {
+        int4 v[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (b + 4 * c < raw) v[c] = row4[b / 4 + c];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int e[4] = {v[c].x, v[c].y, v[c].z, v[c].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (b + 4 * c + j >= raw) break;
+                f(e[j]);
+            }
+        }
+    }
+}
+
 // One workgroup deciding the greedy roots window by window. A row is a root exactly when none of
-// its lower conflicts is a root; every row below the window is decided, so each window is
-// decided completely by rounds in LDS: its rows' conflict lists are staged as window offsets
-// (the conflicts below the window resolved from their final states while staging), and every
-// round each undecided row either finds a lower root (not a root), or an undecided lower row (it
-// waits on that row, rechecking only it until it is decided), or neither (a root). The lowest
-// undecided row of the window always decides, so the rounds end. state (global): 0 undecided,
-// 1 root, 2 not a root (rows outside the phase are 2 from the start).
+// its lower conflicts is a root; every row below the window is decided when the window starts,
+// so each window is decided completely inside LDS. Staging resolves the conflicts below the
+// window from their final states and builds, for every row, the count of its undecided lower
+// conflicts in the window and the list of window rows it is a lower conflict of. A row whose
+// count reaches zero without a lower root (a root), or that meets a lower root (not a root), is
+// queued by the update that decided it; each round processes the queue: the row's decision goes
+// to `state`, and to its upper rows (a flag, or a decrement, by LDS atomics whose old value shows
+// whether this update decided that row). Rows with more lower conflicts than their slot holds
+// walk the graph every round until they decide, then join the queue. The lowest undecided row is
+// always decidable, so every window ends. state (global): 0 undecided, 1 root, 2 not a root
+// (rows outside the phase are 2 from the start).
 __global__ __launch_bounds__(FRONT_THREADS) void k_front(int n, int mode, const int* __restrict__ s_rp,
                                                          const int* __restrict__ s_ci, const int* __restrict__ t_rp,
                                                          const int* __restrict__ t_ci, const int* __restrict__ lc,
                                                          const int* __restrict__ lc_len, int cap,
                                                          int* __restrict__ state, int* __restrict__ info) {
-    __shared__ unsigned short s_list[FRONT_LIST];
-    __shared__ unsigned char s_st[FRONT_WINDOW];
+    __shared__ unsigned short s_up[FRONT_LIST];
+    __shared__ unsigned s_word[FRONT_WINDOW + 1];
+    __shared__ unsigned short s_q[2][FRONT_WINDOW];
     __shared__ int s_scan[FRONT_THREADS / WAVE];
-    __shared__ int s_flag[3];
+    __shared__ int s_qn[3], s_pend[3];
     const int t = threadIdx.x;
     int rounds = 0;
     for (int lo = 0; lo < n;) {
-        // the window: as wide as its staged lists allow
-        int width = FRONT_WINDOW, hi = lo, base = 0, total = 0;
+        // the window: as many rows as their listed conflicts allow
+        int width = FRONT_WINDOW, hi = lo, total = 0;
         while (true) {
             hi = min(n, lo + width);
             int mine = 0;
@@ -186,115 +229,195 @@ __global__ __launch_bounds__(FRONT_THREADS) void k_front(int n, int mode, const 
                 const int i = lo + t + k * FRONT_THREADS;
                 if (i < hi) mine += max(lc_len[i], 0);
             }
-            base = block_scan_front(mine, s_scan, total);
+            (void)block_scan_front(mine, s_scan, total);
             if (total <= FRONT_LIST) break;
-            width >>= 1;   // uniform: every thread saw the same total
+            const int cut = (int)((long long)width * FRONT_LIST / total);   // uniform: every thread saw total
+            width = max(1, min(cut, width - 1));
         }
         const int wn = hi - lo;
-        // staging
-        int beg[FRONT_ROWS], cnt[FRONT_ROWS], wt[FRONT_ROWS];
-        int off = base;
+        // staging pass 1: counts of the undecided lower conflicts (registers), upper counts (LDS)
 #pragma unroll
         for (int k = 0; k < FRONT_ROWS; ++k) {
             const int loc = t + k * FRONT_THREADS;
-            beg[k] = off;
-            cnt[k] = 0;
-            wt[k] = -1;
+            if (loc < wn) s_word[loc] = 0;
+        }
+        __syncthreads();
+        unsigned own[FRONT_ROWS];
+        unsigned longs = 0;   // row slots of this thread with a long list, undecided
+#pragma unroll
+        for (int k = 0; k < FRONT_ROWS; ++k) {
+            const int loc = t + k * FRONT_THREADS;
+            own[k] = 0;
             if (loc >= wn) continue;
             const int i = lo + loc;
             const int raw = lc_len[i];
-            unsigned char st0 = state[i] != 0 ? 2 : 0;
+            unsigned w = 0;
             if (raw < 0) {
-                cnt[k] = -1;
+                w = W_LONG;
+                longs |= 1u << k;
             } else {
-                const int* row = lc + (size_t)i * cap;
-                int c = 0;
-                for (int q = 0; q < raw; ++q) {
-                    const int r = row[q];
+                for_lower(lc, cap, i, raw, [&](int r) {
                     if (r < lo) {
-                        if (state[r] == 1) st0 = 2;
+                        if (state[r] == 1) w |= W_FLAG;
                     } else {
-                        s_list[off + c++] = (unsigned short)(r - lo);
+                        ++w;
+                        atomicAdd(&s_word[r - lo], 1u);
                     }
-                }
-                cnt[k] = c;
-                off += raw;
+                });
             }
-            s_st[loc] = st0;
+            if (state[i] != 0) {   // outside the phase: not a root
+                w |= W_FLAG;
+                longs &= ~(1u << k);
+            }
+            own[k] = w;
         }
-        if (t < 3) s_flag[t] = 0;
         __syncthreads();
-        // rounds until every row of the window is decided
-        for (int it = 0;; ++it) {
-            if (t == 0) s_flag[(it + 1) % 3] = 0;
-            bool pending = false;
+        // pass 2: upper-list starts in row order (one scan per row slot), fill cursors in s_word
+        int ucnt[FRONT_ROWS], ubeg[FRONT_ROWS];
 #pragma unroll
-            for (int k = 0; k < FRONT_ROWS; ++k) {
-                const int loc = t + k * FRONT_THREADS;
-                if (loc >= wn || s_st[loc] != 0) continue;
-                if (wt[k] >= 0) {
-                    const int sw = s_st[wt[k]];
-                    if (sw == 0) {
-                        pending = true;
-                        continue;
-                    }
-                    if (sw == 1) {
-                        s_st[loc] = 2;
-                        continue;
-                    }
-                }
-                int blocker = -1;
-                bool lower_root = false;
-                if (cnt[k] >= 0) {
-                    for (int q = 0; q < cnt[k] && !lower_root; ++q) {
-                        const int r = s_list[beg[k] + q];
-                        const int sr = s_st[r];
-                        if (sr == 1) lower_root = true;
-                        else if (sr == 0 && blocker < 0) blocker = r;
-                    }
-                } else {   // a row with a long conflict list walks the graph
-                    const int i = lo + loc;
-                    auto see = [&](int r) {
-                        const int sr = r < lo ? state[r] : s_st[r - lo];
-                        if (sr == 1) lower_root = true;
-                        else if (sr == 0 && blocker < 0) blocker = r - lo;
-                    };
-                    for (int q = t_rp[i]; q < t_rp[i + 1] && !lower_root; ++q) {
-                        const int r = t_ci[q];
-                        if (r >= i) break;
-                        see(r);
-                    }
-                    if (mode == 1)
-                        for (int q = s_rp[i]; q < s_rp[i + 1] && !lower_root; ++q) {
-                            const int j = s_ci[q];
-                            if (j < i) see(j);
-                            for (int z = t_rp[j]; z < t_rp[j + 1] && !lower_root; ++z) {
-                                const int r = t_ci[z];
-                                if (r >= i) break;
-                                see(r);
-                            }
-                        }
-                }
-                if (lower_root) {
-                    s_st[loc] = 2;
-                } else if (blocker >= 0) {
-                    wt[k] = blocker;
-                    pending = true;
-                } else {
-                    s_st[loc] = 1;
-                }
-            }
-            if (pending) s_flag[it % 3] = 1;
-            __syncthreads();
-            ++rounds;
-            if (s_flag[it % 3] == 0) break;
+        for (int k = 0; k < FRONT_ROWS; ++k) {
+            const int loc = t + k * FRONT_THREADS;
+            ucnt[k] = loc < wn ? (int)s_word[loc] : 0;
+        }
+        int run = 0;
+#pragma unroll
+        for (int k = 0; k < FRONT_ROWS; ++k) {
+            int tot = 0;
+            ubeg[k] = run + block_scan_front(ucnt[k], s_scan, tot);
+            run += tot;
         }
 #pragma unroll
         for (int k = 0; k < FRONT_ROWS; ++k) {
             const int loc = t + k * FRONT_THREADS;
-            if (loc < wn) state[lo + loc] = s_st[loc];
+            if (loc < wn) s_word[loc] = ubeg[k];
         }
-        __syncthreads();   // LDS reused by the next window; its staging reads these states
+        __syncthreads();
+        // pass 3: the upper lists
+#pragma unroll
+        for (int k = 0; k < FRONT_ROWS; ++k) {
+            const int loc = t + k * FRONT_THREADS;
+            if (loc >= wn) continue;
+            const int i = lo + loc;
+            const int raw = lc_len[i];
+            if (raw > 0)
+                for_lower(lc, cap, i, raw, [&](int r) {
+                    if (r >= lo) s_up[atomicAdd(&s_word[r - lo], 1u)] = (unsigned short)loc;
+                });
+        }
+        if (t < 3) {
+            s_qn[t] = 0;
+            s_pend[t] = 0;
+        }
+        __syncthreads();
+        // row words and the first queue: rows decided by staging (a lower root below the window,
+        // outside the phase, or no lower conflict in the window)
+#pragma unroll
+        for (int k = 0; k < FRONT_ROWS; ++k) {
+            const int loc = t + k * FRONT_THREADS;
+            bool want = false;
+            unsigned short item = 0;
+            if (loc < wn) {
+                s_word[loc] = ((unsigned)ubeg[k] << 16) | own[k];
+                if (own[k] & W_FLAG) {
+                    want = true;
+                    item = (unsigned short)loc;
+                } else if (!(own[k] & W_LONG) && (own[k] & 0xffu) == 0) {
+                    want = true;
+                    item = (unsigned short)(loc | 0x8000);
+                }
+            }
+            wave_append(want, item, s_q[0], &s_qn[0]);
+        }
+        if (t == 0) s_word[wn] = (unsigned)run << 16;
+        __syncthreads();
+        // rounds: queued rows publish their decision and update their upper rows; long rows walk
+        for (int r = 0;; ++r) {
+            const int cur = r % 3, nxt = (r + 1) % 3;
+            const int qn = s_qn[cur];
+            if (t == 0) {
+                s_qn[(r + 2) % 3] = 0;
+                s_pend[(r + 2) % 3] = 0;
+            }
+            const unsigned short* q = s_q[r & 1];
+            unsigned short* qo = s_q[(r + 1) & 1];
+            for (int base = 0; base < qn; base += FRONT_THREADS / 8) {
+                const int qi = base + t / 8, sub = t & 7;
+                int e = 0, e1 = 0;
+                bool root = false;
+                if (qi < qn) {
+                    const unsigned item = q[qi];
+                    const int loc = item & 0x7fff;
+                    root = (item >> 15) != 0;
+                    e = (int)(s_word[loc] >> 16) + sub;
+                    e1 = (int)(s_word[loc + 1] >> 16);
+                    if (sub == 0) {
+                        state[lo + loc] = root ? 1 : 2;
+                        atomicOr(&s_word[loc], root ? W_ROOT : W_NOT);
+                    }
+                }
+                for (;; e += 8) {
+                    const bool act = e < e1;
+                    if (!__any(act)) break;
+                    bool want = false;
+                    unsigned short item = 0;
+                    if (act) {
+                        const int up = s_up[e];
+                        if (root) {
+                            want = !(atomicOr(&s_word[up], W_FLAG) & W_FLAG);
+                            item = (unsigned short)up;
+                        } else {
+                            const unsigned old = atomicSub(&s_word[up], 1u);
+                            want = (old & 0xffu) == 1 && !(old & W_FLAG);
+                            item = (unsigned short)(up | 0x8000);
+                        }
+                    }
+                    wave_append(want, item, qo, &s_qn[nxt]);
+                }
+            }
+            // rows with long lists: a lower root decides "not a root", all lower rows decided "root"
+            int pending = 0;
+            if (longs) {
+#pragma unroll
+                for (int k = 0; k < FRONT_ROWS; ++k) {
+                    if (!(longs & (1u << k))) continue;
+                    const int loc = t + k * FRONT_THREADS, i = lo + loc;
+                    bool lower_root = false, undecided = false;
+                    auto see = [&](int x) {
+                        const unsigned sx = x < lo ? (unsigned)state[x] << 10 : (s_word[x - lo] & W_DONE);
+                        if (sx == W_ROOT) lower_root = true;
+                        else if (sx == 0) undecided = true;
+                    };
+                    for (int a = t_rp[i]; a < t_rp[i + 1] && !lower_root; ++a) {
+                        const int x = t_ci[a];
+                        if (x >= i) break;
+                        see(x);
+                    }
+                    if (mode == 1)
+                        for (int a = s_rp[i]; a < s_rp[i + 1] && !lower_root; ++a) {
+                            const int j = s_ci[a];
+                            if (j < i) see(j);
+                            for (int z = t_rp[j]; z < t_rp[j + 1] && !lower_root; ++z) {
+                                const int x = t_ci[z];
+                                if (x >= i) break;
+                                see(x);
+                            }
+                        }
+                    if (lower_root || !undecided) {
+                        longs &= ~(1u << k);
+                        qo[atomicAdd(&s_qn[nxt], 1)] = (unsigned short)(lower_root ? loc : (loc | 0x8000));
+                    } else {
+                        ++pending;
+                    }
+                }
+            }
+#pragma unroll
+            for (int o = WAVE / 2; o > 0; o >>= 1) pending += __shfl_xor(pending, o, WAVE);
+            if ((t & (WAVE - 1)) == 0 && pending) atomicAdd(&s_pend[nxt], pending);
+            __syncthreads();
+            ++rounds;
+            if (s_qn[nxt] == 0 && s_pend[nxt] == 0) break;
+        }
+        __syncthreads();   // LDS reused by the next window, whose staging reads these states
         lo = hi;
     }
     if (t == 0) info[3] += rounds;

@@ -29,7 +29,7 @@ Mesh2D uniform_mesh(int levels) {
 }
 
 // phase times (ms) of the last pressure workload's AMG set-up, then the levels built on the device
-double g_amg_phases[AmgPreconditioner::PH_COUNT + 1] = {};
+double g_amg_phases[AmgPreconditioner::PH_COUNT + 3] = {};
 
 // Host copies of the last workload systems (cfa_workload_keep_systems): the matrix, the
 // right-hand sides, the initial guesses and the solutions, for checks outside the library.
@@ -100,6 +100,8 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
     const double setup_ms = host_ms(t0);
     for (int k = 0; k < AmgPreconditioner::PH_COUNT; ++k) g_amg_phases[k] = amg.phase_ms[k] - cold_phases[k];
     g_amg_phases[AmgPreconditioner::PH_COUNT] = amg.device_levels;
+    g_amg_phases[AmgPreconditioner::PH_COUNT + 1] = (double)amg.front_rounds;
+    g_amg_phases[AmgPreconditioner::PH_COUNT + 2] = amg.sorted_products;
     GpuLinearSolver solver;
     double solve_ms = 0.0, worst = 0.0;
     long long iters = 0;
@@ -203,7 +205,7 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
 // FP32: factor ms, solve ms, it_u, it_v; max |x32 - x64| / max |x64|, us per FP64 iteration.
 int cfa_workload_amg_phases(double* out) {
     if (!out) return CFA_E_ARG;
-    for (int k = 0; k <= AmgPreconditioner::PH_COUNT; ++k) out[k] = g_amg_phases[k];
+    for (int k = 0; k < AmgPreconditioner::PH_COUNT + 3; ++k) out[k] = g_amg_phases[k];
     return CFA_OK;
 }
 

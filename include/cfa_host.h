@@ -75,9 +75,10 @@ void cfa_mesh_probes(void* mesh, int* ab, double* w);
  * kernel_reps > 0 also times the main kernels (read with cfa_last_run_kernels). */
 int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, double* out);
 int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out);
-/* AMG set-up of the last pressure workload (its second, timed set-up): out[10] = ms in download, D^-1 + spectral radius,
+/* AMG set-up of the last pressure workload (its second, timed set-up): out[12] = ms in download, D^-1 + spectral radius,
  * aggregation, P smoothing, transpose, Galerkin product, upload, coarse inverse, remap, then the
- * number of levels built on the device. */
+ * number of levels built on the device, the rounds of the aggregation's decision front and the
+ * sparse products that took the global-memory sort. */
 int cfa_workload_amg_phases(double* out);
 /* Host copies of the workload systems, for checks outside the library (no counterpart in the
  * reference, which has no isolated workloads). cfa_workload_keep_systems(1): the following

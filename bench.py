@@ -41,8 +41,8 @@ sys.path.insert(0, str(ROOT / "cylinder-fem-amr_amd"))
 
 HBM_PEAK_GBS = 8000.0
 DOMINANT = "k_csr_op<jacobi> level 0"
-ROOFLINE_LEVEL = 4
-TRAFFIC_FILE = ROOT / "profiles" / "jacobi_level0_uniform_L4_counters.txt"
+ROOFLINE_LEVEL = 7
+TRAFFIC_FILE = ROOT / "profiles" / "jacobi_level0_uniform_L7_counters.txt"
 
 WORKLOADS = {
     # runs/paper/d2_A2_dt05/cmd.txt (--steps set by the benchmark)
@@ -166,7 +166,7 @@ def solver_workloads(level: int, nrhs: int, tol: float, reps: int) -> int:
 def read_traffic() -> dict | None:
     """HBM bytes per launch of the roofline kernel from the committed counter summary (two
     rocprofv3 --pmc passes of `bench.py --roofline-only`: FETCH_SIZE and WRITE_SIZE in KiB per
-    dispatch, each scaled by the same counter over a 64 MiB device copy)."""
+    dispatch, each scaled by the same counter over a 1 GiB device copy)."""
     if not TRAFFIC_FILE.exists():
         return None
     vals = {}
@@ -188,7 +188,8 @@ def read_traffic() -> dict | None:
 
 
 def roofline_measurement(reps: int, tol: float) -> tuple[dict, dict]:
-    """The level-0 smoother of the SA-AMG V-cycle on the uniform level-4 pressure matrix, launched
+    """The level-0 smoother of the SA-AMG V-cycle on the uniform level-7 pressure matrix (7,347,712 rows,
+    51.4 M entries: 881 MB per sweep, above the 256 MB Infinity Cache, so the sweep streams HBM), launched
     `reps` times between HIP events on the solver stream (cfa_workload_pressure timings)."""
     import cfa
     cfa.workload_pressure(ROOFLINE_LEVEL, 1, tol, max(reps, 1))

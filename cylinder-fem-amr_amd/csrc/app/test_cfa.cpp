@@ -381,6 +381,7 @@ void amg_remap() {
     graph_laplacian(w.mesh, d_rp, d_ci, d_a);
     const int n0 = w.mesh.num_nodes;
     AmgPreconditioner amg;
+    amg.keep_p0_host = true;
     amg.setup(n0, w.mesh.nnz, d_rp, d_ci, d_a);
     EXPECT(amg.num_levels() > 2, "hierarchy too shallow (%d levels)", amg.num_levels());
     const std::vector<double> z_setup = vcycle(amg, n0);

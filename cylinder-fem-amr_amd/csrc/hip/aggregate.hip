@@ -92,17 +92,23 @@ __global__ void k_strong_fill(int n, const int* __restrict__ rp, const int* __re
 // Lower conflicts of each row, deduplicated, in a slot of `cap` entries per row: mode 1 (phase 1)
 // the rows r < i that claimed i (T(i)), the strong neighbours r < i (S(i)) and the rows r < i that
 // claimed a strong neighbour (T(j), j in S(i)); mode 3 (phase 3) T(i) only. len[i] = -1 marks a
-// row with more than `cap` of them (its decisions walk the graph itself).
+// row with more than `cap` of them (its decisions walk the graph itself). Rows outside the phase
+// (state != 0) are never roots: they list nothing and are listed by no row.
 __global__ void k_lower_conflicts(int n, int mode, const int* __restrict__ s_rp, const int* __restrict__ s_ci,
-                                  const int* __restrict__ t_rp, const int* __restrict__ t_ci, int cap,
-                                  int* __restrict__ lc, int* __restrict__ len) {
+                                  const int* __restrict__ t_rp, const int* __restrict__ t_ci,
+                                  const int* __restrict__ state, int cap, int* __restrict__ lc,
+                                  int* __restrict__ len) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (state[i] != 0) {   // outside the phase: decided, and never a root, so never a conflict
+        len[i] = 0;
+        return;
+    }
     int* out = lc + (size_t)i * cap;
     int c = 0;
     bool full = false;
     auto add = [&](int r) {
-        if (full) return;
+        if (full || state[r] != 0) return;
         for (int k = 0; k < c; ++k)
             if (out[k] == r) return;
         if (c == cap) {
@@ -594,7 +600,8 @@ int cfa_amg_aggregate(int n, int nnz, const int* rp, const int* ci, const double
     if (e != hipSuccess) return (int)e;
     // phase 1
     hipLaunchKernelGGL(k_init_state, gn, b, 0, s, n, nullptr, w.state);
-    hipLaunchKernelGGL(k_lower_conflicts, gn, b, 0, s, n, 1, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.cap, w.lc, w.lc_len);
+    hipLaunchKernelGGL(k_lower_conflicts, gn, b, 0, s, n, 1, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.state, w.cap, w.lc,
+                       w.lc_len);
     hipLaunchKernelGGL(k_front, dim3(1), dim3(FRONT_THREADS), 0, s, n, 1, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.lc,
                        w.lc_len, w.cap, w.state, info);
     hipLaunchKernelGGL(k_root_flags, gn, b, 0, s, n, w.state, w.flags);
@@ -609,7 +616,8 @@ int cfa_amg_aggregate(int n, int nnz, const int* rp, const int* ci, const double
     hipLaunchKernelGGL(k_phase2, gn, b, 0, s, n, w.s_rp, w.s_ci, w.agg1, w.agg2);
     // phase 3
     hipLaunchKernelGGL(k_init_state, gn, b, 0, s, n, w.agg2, w.state);
-    hipLaunchKernelGGL(k_lower_conflicts, gn, b, 0, s, n, 3, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.cap, w.lc, w.lc_len);
+    hipLaunchKernelGGL(k_lower_conflicts, gn, b, 0, s, n, 3, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.state, w.cap, w.lc,
+                       w.lc_len);
     hipLaunchKernelGGL(k_front, dim3(1), dim3(FRONT_THREADS), 0, s, n, 3, w.s_rp, w.s_ci, w.t_rp, w.t_ci, w.lc,
                        w.lc_len, w.cap, w.state, info);
     hipLaunchKernelGGL(k_root_flags, gn, b, 0, s, n, w.state, w.flags);

@@ -144,12 +144,37 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
+// The lane's entries of the CSR row [k0, k1) (k0 + lane, k0 + lane + TPR, ...), ROW_UNROLL at a
+// time: the column indices and values of a group are all loaded before any is used, then f(a, j)
+// runs for each entry in order. Slots past the row's end give (0, column 0), so every gather f
+// makes is unconditional and the group's gathers are in flight together; an accumulation of
+// a * x[j] adds an exact zero for them. (Non-temporal loads of the matrix were slower: the lanes
+// of a row share cache lines across the group's load instructions.)
+constexpr int ROW_UNROLL = 4;
+template <int TPR, typename T, typename F>
+__device__ __forceinline__ void row_entries(const int* __restrict__ ci, const T* __restrict__ val, int k0, int k1,
+                                            int lane, F f) {
+    for (int k = k0 + lane; k < k1; k += ROW_UNROLL * TPR) {
+        int c[ROW_UNROLL];
+        T v[ROW_UNROLL];
+#pragma unroll
+        for (int u = 0; u < ROW_UNROLL; ++u) {
+            const int idx = k + u * TPR;
+            const bool ok = idx < k1;
+            c[u] = ok ? ci[idx] : 0;
+            v[u] = ok ? val[idx] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < ROW_UNROLL; ++u) f(v[u], c[u]);
+    }
+}
+
 // Row product sum_k val[k] x[ci[k]] over [s, e) with TPR lanes (lane = rank inside the group)
 template <int TPR>
 __device__ __forceinline__ double row_dot(const int* __restrict__ ci, const double* __restrict__ val,
                                           const double* __restrict__ x, int s, int e, int lane) {
     double acc = 0.0;
-    for (int k = s + lane; k < e; k += TPR) acc += val[k] * x[ci[k]];
+    row_entries<TPR>(ci, val, s, e, lane, [&](double a, int j) { acc += a * x[j]; });
     return group_sum<TPR>(acc);
 }
 

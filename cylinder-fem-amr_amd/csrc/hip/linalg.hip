@@ -72,7 +72,7 @@ k_csr_op(int rows, const int* __restrict__ rp, const int* __restrict__ ci, const
     int k0, k1;
     if (!running_with_range(rp, row, rows, state, k0, k1)) return;
     T s = 0;
-    for (int k = k0 + lane; k < k1; k += TPR) s += val[k] * x[ci[k]];
+    row_entries<TPR>(ci, val, k0, k1, lane, [&](T a, int j) { s += a * x[j]; });
     s = group_sum<TPR>(s);
     if (row >= rows || lane != 0) return;
     if (OP == OP_SPMV) y[row] = s;
@@ -93,10 +93,7 @@ k_jacobi_from_zero(int rows, const int* __restrict__ rp, const int* __restrict__
     int k0, k1;
     if (!running_with_range(rp, row, rows, state, k0, k1)) return;
     T s = 0;
-    for (int k = k0 + lane; k < k1; k += TPR) {
-        const int j = ci[k];
-        s += val[k] * (omega * dinv[j] * b[j]);
-    }
+    row_entries<TPR>(ci, val, k0, k1, lane, [&](T a, int j) { s += a * (omega * dinv[j] * b[j]); });
     s = group_sum<TPR>(s);
     if (row >= rows || lane != 0) return;
     const T x0 = omega * dinv[row] * b[row];
@@ -223,12 +220,10 @@ k_spmv_multi(int n, const int* __restrict__ rp, const int* __restrict__ ci, cons
 #pragma unroll
     for (int c = 0; c < NRHS; ++c) s[c] = 0.0;
     if (row < n)
-        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
-            const int j = ci[k];
-            const double a = val[k];
+        row_entries<TPR>(ci, val, rp[row], rp[row + 1], lane, [&](double a, int j) {
 #pragma unroll
             for (int c = 0; c < NRHS; ++c) s[c] += a * x[(size_t)c * n + j];
-        }
+        });
 #pragma unroll
     for (int c = 0; c < NRHS; ++c) s[c] = group_sum<TPR>(s[c]);
     if (row < n && lane == 0)
@@ -498,11 +493,15 @@ k_fp_operator(PcgArgs a, int h) {
     double contrib = 0.0;
     for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < a.n; row += groups) {
         double s = 0.0, sx = 0.0, sy = 0.0;
-        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
-            const int j = ci[k];
-            const double pj = first ? z[j] : fma(beta, po[j], z[j]);
-            s += val[k] * pj;
-            if (OSS) {
+        if (!OSS) {
+            row_entries<TPR>(ci, val, rp[row], rp[row + 1], lane, [&](double av, int j) {
+                s += av * (first ? z[j] : fma(beta, po[j], z[j]));
+            });
+        } else {
+            for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
+                const int j = ci[k];
+                const double pj = first ? z[j] : fma(beta, po[j], z[j]);
+                s += val[k] * pj;
                 sx += a.oss[k] * pj;
                 sy += a.oss[a.nnz + k] * pj;
             }
@@ -608,10 +607,8 @@ k_jacobi_from_zero_check(int rows, const int* __restrict__ rp, const int* __rest
     const int row = g / TPR, lane = g % TPR;
     double s = 0.0;
     if (row < rows)
-        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
-            const int j = ci[k];
-            s += val[k] * (omega * dinv[j] * b[j]);
-        }
+        row_entries<TPR>(ci, val, rp[row], rp[row + 1], lane,
+                         [&](double a, int j) { s += a * (omega * dinv[j] * b[j]); });
     s = group_sum<TPR>(s);
     if (row >= rows || lane != 0) return;
     const double x0 = omega * dinv[row] * b[row];
@@ -634,7 +631,7 @@ k_jacobi_dot(int rows, const int* __restrict__ rp, const int* __restrict__ ci, c
     double contrib = 0.0;
     for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < rows; row += groups) {
         T s = 0;
-        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) s += val[k] * x[ci[k]];
+        row_entries<TPR>(ci, val, rp[row], rp[row + 1], lane, [&](T a, int j) { s += a * x[j]; });
         s = group_sum<TPR>(s);
         if (lane == 0) {
             const double yi = (double)(x[row] + omega * dinv[row] * (b[row] - s));
@@ -795,12 +792,10 @@ k_bicg_init(int n, const int* __restrict__ rp, const int* __restrict__ ci, const
     double acc[1][2] = {{0.0, 0.0}};
     for (int row = g0; row < n; row += groups) {
         double s0 = 0.0, s1 = 0.0;
-        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
-            const int j = ci[k];
-            const double a = val[k];
+        row_entries<TPR>(ci, val, rp[row], rp[row + 1], lane, [&](double a, int j) {
             s0 += a * x[j];
             s1 += a * x[n + j];
-        }
+        });
         s0 = group_sum<TPR>(s0);
         s1 = group_sum<TPR>(s1);
         if (lane == 0) {
@@ -890,12 +885,10 @@ k_spmv2_dots(int n, const int* __restrict__ rp, const int* __restrict__ ci, cons
     double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
     for (int row = g0; row < n; row += groups) {
         double s0 = 0.0, s1 = 0.0;
-        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
-            const int j = ci[k];
-            const double a = val[k];
+        row_entries<TPR>(ci, val, rp[row], rp[row + 1], lane, [&](double a, int j) {
             s0 += a * x[j];
             s1 += a * x[n + j];
-        }
+        });
         s0 = group_sum<TPR>(s0);
         s1 = group_sum<TPR>(s1);
         if (lane == 0) {
@@ -1383,12 +1376,18 @@ int cfa_spmv(int n, const int* rp, const int* ci, const double* val, const doubl
              cfa_stream_t stream) {
     if (n < 0 || !rp || !ci || !val || !x || !y || nrhs < 1 || nrhs > 2) return CFA_E_ARG;
     if (n == 0) return CFA_OK;
-    constexpr int TPR = 8;
-    const int blocks = blocks_for((long long)n * TPR);
-    if (nrhs == 1)
-        hipLaunchKernelGGL((k_spmv_multi<TPR, 1>), dim3(blocks), dim3(BLOCK), 0, S(stream), n, rp, ci, val, x, y);
-    else
-        hipLaunchKernelGGL((k_spmv_multi<TPR, 2>), dim3(blocks), dim3(BLOCK), 0, S(stream), n, rp, ci, val, x, y);
+    // the mesh matrices (5-9 entries per row): two lanes per row from 20k rows (measured fastest
+    // for the level-0 sweeps of the same pattern), a group of eight on small, latency-bound meshes
+    auto launch = [&](auto c) {
+        constexpr int TPR = decltype(c)::value;
+        const int blocks = blocks_for((long long)n * TPR);
+        if (nrhs == 1)
+            hipLaunchKernelGGL((k_spmv_multi<TPR, 1>), dim3(blocks), dim3(BLOCK), 0, S(stream), n, rp, ci, val, x, y);
+        else
+            hipLaunchKernelGGL((k_spmv_multi<TPR, 2>), dim3(blocks), dim3(BLOCK), 0, S(stream), n, rp, ci, val, x, y);
+    };
+    if (n >= 20000) launch(std::integral_constant<int, 2>{});
+    else launch(std::integral_constant<int, 8>{});
     return launch_status();
 }
 

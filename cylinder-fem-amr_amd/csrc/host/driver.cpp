@@ -558,7 +558,8 @@ int run_driver(int argc, char** argv) {
                 prm_now.lookahead_time += opt.amr_overlap * dt;
                 pend.old_vids = amr.node_vertices();
                 pend.join_step = step + opt.amr_overlap;
-                // the worker owns the forest and pend until the join; it never touches the GPU
+                // the worker owns the forest and pend until the join; the forest's passes run on
+                // its own stream, which does not synchronise with the solver's
                 const bool want_ilu = opt.momentum_ilu;
                 pend.job = std::async(std::launch::async, [&amr, &pend, prm_now, want_ilu, err = h_elem_err]() {
                     auto tw = Clock::now();

@@ -148,7 +148,7 @@ public:
     // The whole set-up runs on the device, level by level (D^-1, spectral radius, aggregation,
     // P smoothing, transpose, Galerkin product); only the coarse operator comes to the host for
     // its dense inverse.
-    bool keep_p0_host = true;      // download the level-0 prolongator for remap()
+    bool keep_p0_host = false;     // download the level-0 prolongator for remap() (without it, remap() sets up anew)
     bool keep_aggregates = false;  // download the level-0 aggregates into last_aggregates
     int device_levels = 0;         // levels the last set-up built
     bool force_sorted_products = false;   // every product by the global-memory sort (a check)

@@ -174,7 +174,7 @@ int cfa_dumped_momentum_solve(const char* path, double tol, int max_iter, int pr
         dev_copy(t_in, bu, n);
         dev_copy(t_in + n, bv, n);
         const double dn = n, dnnz = nnz;
-        kernel_timings().push_back({"k_spmv_multi<8,2>", 12.0 * dnnz + 36.0 * dn + 4.0, time_launches(kernel_reps, [&] {
+        kernel_timings().push_back({"cfa_spmv, 2 right-hand sides", 12.0 * dnnz + 36.0 * dn + 4.0, time_launches(kernel_reps, [&] {
             GPU_CHECK(cfa_spmv(n, rp, ci, val, t_in, t_out, 2, solver_stream()));
         }), kernel_reps});
         kernel_timings().push_back({"ilu0 apply, 2 components (all colours)", 12.0 * dnnz + 100.0 * dn,
@@ -220,7 +220,7 @@ int cfa_dumped_pressure_solve(const char* path, double tol, int max_iter, int pr
         DevArray<double> t_out;
         t_out.reserve(n);
         const double dn = n, dnnz = nnz;
-        kernel_timings().push_back({"k_spmv_multi<8,1>", 12.0 * dnnz + 20.0 * dn + 4.0, time_launches(kernel_reps, [&] {
+        kernel_timings().push_back({"cfa_spmv, 1 right-hand side", 12.0 * dnnz + 20.0 * dn + 4.0, time_launches(kernel_reps, [&] {
             GPU_CHECK(cfa_spmv(n, rp, ci, val, xd, t_out, 1, solver_stream()));
         }), kernel_reps});
         kernel_timings().push_back({"V-cycle", vcycle_bytes(amg), time_launches(kernel_reps, [&] { amg.apply(b, t_out); }),

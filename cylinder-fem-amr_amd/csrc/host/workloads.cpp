@@ -69,7 +69,7 @@ extern "C" {
 // with right-hand sides drawn from mt19937_64(1000 + k) N(0,1), mean over the free rows removed,
 // Dirichlet rows zero. out[12] = n, nnz, ne, mesh s, AMG set-up ms, AMG levels, solves,
 // solve ms total, iterations total, ms per solve, us per iteration, max final relative residual.
-// With kernel_reps > 0 the SpMV, one smoothing sweep per level and lanes-per-row choice, a 64 MiB
+// With kernel_reps > 0 the SpMV, one smoothing sweep per level and lanes-per-row choice, a 1 GiB
 // device copy and, last, the level-0 smoother as the V-cycle runs it are timed afterwards
 // (kernel_timings()).
 int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, double* out) {
@@ -143,7 +143,7 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
         DevArray<double> t_out;
         t_out.reserve(n);
         const double nnz = g.nnz, nn = n;
-        kernel_timings().push_back({"k_spmv_multi<8,1>", 12.0 * nnz + 20.0 * nn + 4.0, time_launches(kernel_reps, [&] {
+        kernel_timings().push_back({"cfa_spmv, 1 right-hand side", 12.0 * nnz + 20.0 * nn + 4.0, time_launches(kernel_reps, [&] {
             GPU_CHECK(cfa_spmv(n, g.d_row_ptr, g.d_col_ind, g.d_Kp_val, g.d_dp, t_out, 1, s));
         }), kernel_reps});
         if (!amg.levels.empty()) {
@@ -168,14 +168,15 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
                 lv[l].tpr_a = L.tpr_a;
             }
         }
-        {   // a device copy of 64 MiB: twice the aggregate L2, so every byte crosses to HBM / MALL;
-            // hardware byte counters of the other kernels are calibrated against it
-            const size_t words = (size_t)8 << 20;
+        {   // a device copy of 1 GiB (four times the Infinity Cache): every byte crosses to HBM, so
+            // its time is a measured HBM rate and the hardware byte counters of the other kernels
+            // are calibrated against it
+            const size_t words = (size_t)1 << 27;
             DevArray<double> c_src, c_dst;
             c_src.reserve(words, 0);
             c_dst.reserve(words, 0);
             dev_zero(c_src, words * sizeof(double));
-            kernel_timings().push_back({"copy 64 MiB (counter calibration)", 2.0 * words * sizeof(double),
+            kernel_timings().push_back({"copy 1 GiB (counter calibration)", 2.0 * words * sizeof(double),
                                         time_launches(kernel_reps, [&] { dev_copy(c_dst, c_src, words); }),
                                         kernel_reps});
         }
@@ -304,7 +305,7 @@ int cfa_workload_momentum(int level, double tol, int kernel_reps, double* out) {
         dev_copy(t_in, g.d_rhs_u, n);
         dev_copy(t_in + n, g.d_rhs_v, n);
         const double nnz = g.nnz, nn = n;
-        kernel_timings().push_back({"k_spmv_multi<8,2>", 12.0 * nnz + 36.0 * nn + 4.0, time_launches(kernel_reps, [&] {
+        kernel_timings().push_back({"cfa_spmv, 2 right-hand sides", 12.0 * nnz + 36.0 * nn + 4.0, time_launches(kernel_reps, [&] {
             GPU_CHECK(cfa_spmv(n, g.d_row_ptr, g.d_col_ind, g.d_A_val, t_in, t_out, 2, s));
         }), kernel_reps});
         kernel_timings().push_back({"ilu0 apply, 2 components (all colours)", 12.0 * nnz + 100.0 * nn,

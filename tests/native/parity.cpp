@@ -8,6 +8,7 @@
 //                                  spmv amg pcg bicgstab jacobi mixed topology outlet nvb remap all
 //
 // Prints one line per compared quantity and exits 0 when every comparison passes.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -770,6 +771,47 @@ void case_remap(Fixture& f) {
     const oracle::Csr Pn = oracle::amg_remap_p0(oa.levels[0].P, n, m.csr_row_ptr.data(), m.csr_col_ind.data(),
                                                 new_to_old, &ok);
     expect(ok, "oracle remap reached every new node");
+    {   // the oracle's remap by what its rule fixes: a kept node keeps its old row exactly; a new
+        // node's row (a mean of its neighbours' rows) is nonempty, its columns come from those
+        // rows and its sum lies within theirs
+        const oracle::Csr& P0 = oa.levels[0].P;
+        int kept_bad = 0, new_bad = 0;
+        auto row_sum = [&](int i) {
+            double t = 0.0;
+            for (int a = Pn.rp[i]; a < Pn.rp[i + 1]; ++a) t += Pn.v[a];
+            return t;
+        };
+        for (int i = 0; i < n; ++i) {
+            const int j = new_to_old[i];
+            if (j >= 0) {
+                const bool same = Pn.rp[i + 1] - Pn.rp[i] == P0.rp[j + 1] - P0.rp[j] &&
+                                  std::equal(Pn.ci.begin() + Pn.rp[i], Pn.ci.begin() + Pn.rp[i + 1],
+                                             P0.ci.begin() + P0.rp[j]) &&
+                                  std::equal(Pn.v.begin() + Pn.rp[i], Pn.v.begin() + Pn.rp[i + 1],
+                                             P0.v.begin() + P0.rp[j]);
+                kept_bad += same ? 0 : 1;
+                continue;
+            }
+            std::vector<int> cols;
+            double lo = 1e300, hi = -1e300;
+            for (int a = m.csr_row_ptr[i]; a < m.csr_row_ptr[i + 1]; ++a) {
+                const int k = m.csr_col_ind[a];
+                if (k == i) continue;
+                cols.insert(cols.end(), Pn.ci.begin() + Pn.rp[k], Pn.ci.begin() + Pn.rp[k + 1]);
+                const double sk = row_sum(k);
+                lo = std::min(lo, sk);
+                hi = std::max(hi, sk);
+            }
+            bool good = Pn.rp[i + 1] > Pn.rp[i];
+            for (int a = Pn.rp[i]; a < Pn.rp[i + 1] && good; ++a)
+                good = std::find(cols.begin(), cols.end(), Pn.ci[a]) != cols.end();
+            const double si = row_sum(i), slack = 1e-12 * std::max(1.0, std::fabs(si));
+            if (!(good && si >= lo - slack && si <= hi + slack)) ++new_bad;
+        }
+        std::printf("  oracle remap invariants: kept rows changed %d, new rows outside their neighbours %d\n",
+                    kept_bad, new_bad);
+        expect(kept_bad == 0 && new_bad == 0, "oracle remap: kept rows kept, new rows means of neighbours");
+    }
     const auto& L = *amg.levels[0];
     std::vector<int> pat_g = down_int(L.P.rp, n + 1), ci_g = down_int(L.P.ci, L.P.nnz);
     pat_g.insert(pat_g.end(), ci_g.begin(), ci_g.end());

@@ -178,4 +178,8 @@ __device__ __forceinline__ double row_dot(const int* __restrict__ ci, const doub
     return group_sum<TPR>(acc);
 }
 
+// Looks up, once per device, the device address of the fixed workspace of the no-allocation
+// entry point (fem.hip), for the current device; cfa_init calls it.
+int scratch_setup();
+
 }  // namespace cfa_dev

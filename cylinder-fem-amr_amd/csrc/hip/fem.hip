@@ -7,6 +7,8 @@
 // There are no atomics, the result is deterministic, and with contraction disabled it is the
 // same sum, in the same order, as a serial element loop. Element quantities are recomputed by
 // each of the three owners; the kernels are bound by the gathers, not by that arithmetic.
+#include <mutex>
+
 #include "cfa_device.hpp"
 
 using namespace cfa_dev;
@@ -866,7 +868,58 @@ inline OutletMode outlet_of(const cfa_mesh_t* m) {
     return OutletMode{m->outlet_dirichlet, m->p_pin, m->outlet_scale};
 }
 
+// rhs -= (sum rhs / sum mass) mass, rhs[pin] = 0, with the block sums and totals in ws
+int remove_mean(const cfa_mesh_t* m, const double* mass, double* rhs, double* ws, cfa_stream_t stream) {
+    const int n = m->n_nodes;
+    const int nb = blocks_for(n, MEAN_BLOCK);
+    hipLaunchKernelGGL(k_mean_block_sums, dim3(nb), dim3(MEAN_BLOCK), 0, as_stream(stream), n, rhs, mass, ws);
+    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(WAVE), 0, as_stream(stream), nb, ws, ws + 2 * (size_t)nb);
+    hipLaunchKernelGGL(k_subtract_mean, dim3(blocks_for(n)), dim3(BLOCK), 0, as_stream(stream), n,
+                       ws + 2 * (size_t)nb, mass, m->p_pin, rhs);
+    return launch_status();
+}
+
 }  // namespace
+
+// ---- the mean removal without allocation -------------------------------------------------------
+// Workspace of cfa_pressure_remove_mean_noalloc: the block sums of at most CFA_MEAN_NOALLOC_MAX_NODES
+// nodes (two per block of MEAN_BLOCK) and the two totals, 512 KiB + 16 B, one copy per GPU held
+// until the process ends. Host code never passes the array's name (on the host it is only a
+// stand-in): it uses the device address hipGetSymbolAddress gives for the current device, taken
+// once per device and kept in scratch_ptr.
+constexpr size_t MEAN_NOALLOC_DOUBLES = 2 * (size_t)(CFA_MEAN_NOALLOC_MAX_NODES / 256) + 2;
+static_assert(CFA_MEAN_NOALLOC_MAX_NODES % 256 == 0, "whole blocks of the mean removal");
+static __device__ __align__(256) double scratch[MEAN_NOALLOC_DOUBLES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double* scratch_ptr[MAX_DEVICES] = {};
+
+// the device address of `scratch` on the current device (looked up on the first request)
+int scratch_address(double** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= MAX_DEVICES) return CFA_E_ARG;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = scratch_ptr[dev];
+    return CFA_OK;
+}
+}  // namespace
+
+namespace cfa_dev {
+int scratch_setup() {
+    double* p = nullptr;
+    return scratch_address(&p);
+}
+}  // namespace cfa_dev
 
 extern "C" {
 
@@ -946,14 +999,18 @@ size_t cfa_pressure_mean_ws_doubles(int n) {
 int cfa_pressure_remove_mean(const cfa_mesh_t* m, const double* mass, double* rhs, double* ws,
                              cfa_stream_t stream) {
     if (!mesh_ok(m) || !m->outlet_dirichlet || !mass || !rhs || !ws) return CFA_E_ARG;
-    const int n = m->n_nodes;
-    if (n == 0) return CFA_OK;
-    const int nb = blocks_for(n, MEAN_BLOCK);
-    hipLaunchKernelGGL(k_mean_block_sums, dim3(nb), dim3(MEAN_BLOCK), 0, as_stream(stream), n, rhs, mass, ws);
-    hipLaunchKernelGGL(k_mean_final, dim3(1), dim3(WAVE), 0, as_stream(stream), nb, ws, ws + 2 * (size_t)nb);
-    hipLaunchKernelGGL(k_subtract_mean, dim3(blocks_for(n)), dim3(BLOCK), 0, as_stream(stream), n,
-                       ws + 2 * (size_t)nb, mass, m->p_pin, rhs);
-    return launch_status();
+    if (m->n_nodes == 0) return CFA_OK;
+    return remove_mean(m, mass, rhs, ws, stream);
+}
+
+int cfa_pressure_remove_mean_noalloc(const cfa_mesh_t* m, const double* mass, double* rhs, cfa_stream_t stream) {
+    if (!mesh_ok(m) || !m->outlet_dirichlet || !mass || !rhs) return CFA_E_ARG;
+    if (m->n_nodes > CFA_MEAN_NOALLOC_MAX_NODES) return CFA_E_SIZE;   // before any device work
+    if (m->n_nodes == 0) return CFA_OK;
+    double* ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc != CFA_OK) return rc;
+    return remove_mean(m, mass, rhs, ws, stream);
 }
 
 int cfa_assemble_oss_operator(const cfa_mesh_t* m, const double* elem_s, const double* mass,

@@ -24,7 +24,8 @@ int cfa_init(int device) {
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return (int)e;
     e = hipFree(nullptr);   // create the context now
-    return e == hipSuccess ? CFA_OK : (int)e;
+    if (e != hipSuccess) return (int)e;
+    return scratch_setup();   // the no-allocation entry point's workspace address on this device
 }
 
 const char* cfa_error_string(int code) {
@@ -34,6 +35,7 @@ const char* cfa_error_string(int code) {
         case CFA_E_ROWLEN: return "CSR row too long for the kernel";
         case CFA_E_NOTREADY: return "object not set up";
         case CFA_E_IO: return "file missing or not in the expected layout";
+        case CFA_E_SIZE: return "input larger than the fixed workspace of a no-allocation entry point";
         default: return code > 0 ? hipGetErrorString((hipError_t)code) : "unknown error";
     }
 }

@@ -33,7 +33,8 @@ enum {
     CFA_E_ARG = -1,      /* invalid argument (null pointer, negative size, ...) */
     CFA_E_ROWLEN = -2,   /* a CSR row longer than the kernel limit (ILU: 32, as linalg.cuh:361) */
     CFA_E_NOTREADY = -3, /* object not set up */
-    CFA_E_IO = -4        /* file missing or not in the expected layout */
+    CFA_E_IO = -4,       /* file missing or not in the expected layout */
+    CFA_E_SIZE = -5      /* input larger than the fixed workspace of a no-allocation entry point */
 };
 
 /* ---------------------------------------------------------------------------------------------
@@ -141,6 +142,18 @@ int cfa_assemble_pressure(const cfa_mesh_t* m, const double* u_star, const doubl
 size_t cfa_pressure_mean_ws_doubles(int n);
 int cfa_pressure_remove_mean(const cfa_mesh_t* m, const double* mass, double* rhs, double* ws,
                              cfa_stream_t stream);
+
+/* cfa_pressure_remove_mean without a workspace argument and without any allocation. No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The block sums go to a
+ * fixed device array of the library, 512 KiB per GPU, held until the process ends (its address is
+ * looked up once per device, by cfa_init or by the first call), so meshes of at most
+ * CFA_MEAN_NOALLOC_MAX_NODES nodes are accepted; a larger one returns CFA_E_SIZE before any device
+ * work. Results are bit-identical to cfa_pressure_remove_mean's. Every call on a GPU uses the same
+ * array: calls that overlap on one GPU (other streams or threads) race. Nothing in the array needs
+ * clearing (each call writes every sum it reads). */
+#define CFA_MEAN_NOALLOC_MAX_NODES 8388608
+int cfa_pressure_remove_mean_noalloc(const cfa_mesh_t* m, const double* mass, double* rhs, cfa_stream_t stream);
 
 /* OSS operator alone (src/cfa/device.cuh:304-311 assemble_oss_projection: fem_kernels.cuh:807-852
  * k_assemble_oss_operator and k_inverse_lumped_mass): writes oss_val [Bx|By|BTx|BTy] and inv_m

@@ -511,6 +511,50 @@ void case_amg(Fixture& f) {
     }
 }
 
+// The no-allocation mean removal against cfa_pressure_remove_mean: on the fixture's mesh, then
+// on a smaller one (the second call must not see the first's sums in the shared array), bit for
+// bit; a mesh beyond the fixed array is refused before any device work.
+void case_noalloc(Fixture& f, int level) {
+    std::printf("[pressure mean removal without allocation]\n");
+    Mesh2D small = make_mesh(level >= 2 ? level - 2 : 0);
+    DeviceMeshBuffers gs;
+    gs.outlet_dirichlet = true;
+    const Vec zs(small.num_nodes, 0.0), z(f.n, 0.0);
+    gs.sync_from_host(small, zs, zs, zs, zs, zs);
+    f.g.outlet_dirichlet = true;
+    f.g.sync_from_host(f.mesh, z, z, z, z, z);
+    struct Case {
+        const char* name;
+        const cfa_mesh_t* view;
+        int n;
+    };
+    const Case cases[2] = {{"fixture mesh", f.g.view(), f.n}, {"smaller mesh", gs.view(), small.num_nodes}};
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    DevArray<double> d_mass, d_a, d_b, d_ws;
+    for (const Case& c : cases) {
+        Vec mass(c.n), rhs(c.n);
+        for (int i = 0; i < c.n; ++i) {
+            mass[i] = 1.0e-4 * (1.5 + U(f.rng));
+            rhs[i] = U(f.rng);
+        }
+        d_mass.upload(mass);
+        d_a.upload(rhs);
+        d_b.upload(rhs);
+        d_ws.reserve(cfa_pressure_mean_ws_doubles(c.n));
+        GPU_CHECK(cfa_pressure_remove_mean(c.view, d_mass, d_a, d_ws, f.s()));
+        GPU_CHECK(cfa_pressure_remove_mean_noalloc(c.view, d_mass, d_b, f.s()));
+        std::printf("  %s: %d nodes\n", c.name, c.n);
+        same_bits("  mean-free, pinned rhs (no allocation vs workspace)", down(d_b, c.n), down(d_a, c.n));
+    }
+    cfa_mesh_t big = *f.g.view();
+    big.n_nodes = CFA_MEAN_NOALLOC_MAX_NODES + 1;
+    const int rc = cfa_pressure_remove_mean_noalloc(&big, d_mass, d_b, f.s());
+    std::printf("  %d nodes: %s\n", big.n_nodes, cfa_error_string(rc));
+    expect(rc == CFA_E_SIZE, "a mesh beyond the fixed array is refused");
+    f.g.outlet_dirichlet = false;   // back to the default outlet for later cases
+    f.g.sync_from_host(f.mesh, z, z, z, z, z);
+}
+
 // Mesh topology on the device against the host build: the CSR graph, the incidence and position
 // tables, the cylinder edges and the checks, on the fixture's uniform mesh and on a locally
 // refined NVB mesh; then on two planted defects (a flipped element, a missing element), which
@@ -1191,6 +1235,7 @@ int main(int argc, char** argv) {
         if (all || what == "outlet") case_outlet(f);
         if (all || what == "nvb") case_nvb(f, level);
         if (all || what == "remap") case_remap(f);
+        if (all || what == "noalloc") case_noalloc(f, level);
         std::printf(failures ? "%d FAILURE(S)\n" : "ALL PASSED\n", failures);
         status = failures ? 1 : 0;
     }   // the fixture's device memory is freed here

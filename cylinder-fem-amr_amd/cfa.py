@@ -143,7 +143,8 @@ def last_kernels() -> dict:
 
 
 PRESSURE_FIELDS = ("n", "nnz", "ne", "mesh_s", "amg_setup_ms", "amg_levels", "solves", "solve_ms", "iterations",
-                   "ms_per_solve", "us_per_iteration", "max_rel_residual", "amg_setup_cold_ms")
+                   "ms_per_solve", "us_per_iteration", "max_rel_residual", "amg_setup_cold_ms",
+                   "fp32_ms_per_solve", "fp32_iterations", "fp32_max_rel_residual")
 MOMENTUM_FIELDS = ("n", "nnz", "ne", "mesh_s", "colouring_ms", "colours", "fp64_factor_ms", "fp64_solve_ms",
                    "fp64_it_u", "fp64_it_v", "fp32_factor_ms", "fp32_solve_ms", "fp32_it_u", "fp32_it_v",
                    "rel_diff_fp32_fp64", "us_per_fp64_iteration")
@@ -151,7 +152,7 @@ MOMENTUM_FIELDS = ("n", "nnz", "ne", "mesh_s", "colouring_ms", "colours", "fp64_
 
 def workload_pressure(level: int, nrhs: int = 32, tol: float = 1e-8, kernel_reps: int = 0) -> dict:
     _, host = load()
-    out = np.zeros(13)
+    out = np.zeros(len(PRESSURE_FIELDS))
     rc = host.cfa_workload_pressure(level, nrhs, tol, kernel_reps, _dptr(out))
     if rc != 0:
         raise RuntimeError(f"cfa_workload_pressure failed: {rc}")

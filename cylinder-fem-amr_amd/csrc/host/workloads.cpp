@@ -110,7 +110,7 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
         keep_pattern(g_kept[0], mesh, g.d_Kp_val, g.nnz);
         g_kept[0].cols = nrhs;
     }
-    for (int k = 0; k < nrhs; ++k) {
+    auto make_rhs = [&](int k) {
         std::mt19937_64 rng(1000 + k);
         std::normal_distribution<double> N(0.0, 1.0);
         double mean = 0.0;
@@ -121,6 +121,9 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
         }
         mean /= std::max(free_rows, 1);
         for (int i = 0; i < n; ++i) b[i] = mesh.bc_tag[i] == BC_OUTLET ? 0.0 : b[i] - mean;
+    };
+    for (int k = 0; k < nrhs; ++k) {
+        make_rhs(k);
         g.d_rhs_p.upload(b);
         dev_zero(g.d_dp, n * sizeof(double));
         dev_sync();
@@ -138,6 +141,24 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
             kp.x.insert(kp.x.end(), x.begin(), x.end());
             kp.iters.push_back(it);
         }
+    }
+    // the same right-hand sides with the FP32 cycle (FP32 copies of a second hierarchy; coarse
+    // solve, Krylov vectors and operator FP64), as --precision mixed runs the pressure solve
+    AmgPreconditioner amg32;
+    amg32.fp32 = true;
+    amg32.setup(n, g.nnz, g.d_row_ptr, g.d_col_ind, g.d_Kp_val);
+    double solve32_ms = 0.0, worst32 = 0.0;
+    long long iters32 = 0;
+    for (int k = 0; k < nrhs; ++k) {
+        make_rhs(k);
+        g.d_rhs_p.upload(b);
+        dev_zero(g.d_dp, n * sizeof(double));
+        dev_sync();
+        t0 = Clock::now();
+        iters32 += solver.solve_pcg_fused(n, g.d_row_ptr, g.d_col_ind, g.d_Kp_val, g.d_rhs_p, g.d_dp, 4000, tol,
+                                          nullptr, &amg32);
+        solve32_ms += host_ms(t0);
+        worst32 = std::max(worst32, solver.last_rel_residual);
     }
     if (kernel_reps > 0) {
         DevArray<double> t_out;
@@ -180,6 +201,16 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
                                         time_launches(kernel_reps, [&] { dev_copy(c_dst, c_src, words); }),
                                         kernel_reps});
         }
+        if (!amg32.levels.empty()) {   // the FP32 cycle's level-0 sweep (values, vectors in FP32)
+            auto& L = *amg32.levels[0];
+            const double n0 = L.A.rows, nnz0 = L.A.nnz;
+            dev_zero(L.x32, (size_t)L.A.rows * sizeof(float));
+            dev_zero(L.b32, (size_t)L.A.rows * sizeof(float));
+            kernel_timings().push_back({"k_csr_op<jacobi, float> level 0", 8.0 * nnz0 + 20.0 * n0 + 4.0,
+                                        time_launches(kernel_reps, [&] {
+                GPU_CHECK(cfa_amg_smooth_f32(amg32.view(), 0, L.x32, L.b32, L.Ax32, s));
+            }), kernel_reps});
+        }
         // the level-0 sweep as the V-cycle runs it, timed last so that a profiler's last
         // kernel_reps dispatches of this kernel are exactly these
         if (!amg.levels.empty()) {
@@ -190,10 +221,11 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
             }), kernel_reps});
         }
     }
-    const double v[13] = {(double)n, (double)g.nnz, (double)g.n_elems, mesh_s, setup_ms, (double)amg.num_levels(),
+    const double v[16] = {(double)n, (double)g.nnz, (double)g.n_elems, mesh_s, setup_ms, (double)amg.num_levels(),
                           (double)nrhs, solve_ms, (double)iters, solve_ms / nrhs,
-                          iters ? 1000.0 * solve_ms / iters : 0.0, worst, setup_cold_ms};
-    for (int k = 0; k < 13; ++k) out[k] = v[k];
+                          iters ? 1000.0 * solve_ms / iters : 0.0, worst, setup_cold_ms,
+                          solve32_ms / nrhs, (double)iters32, worst32};
+    for (int k = 0; k < 16; ++k) out[k] = v[k];
     return CFA_OK;
 }
 

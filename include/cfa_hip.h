@@ -283,6 +283,10 @@ int cfa_to_f32(size_t n, const double* in, float* out, cfa_stream_t stream);
  * V-cycle (src/cfa/linalg.cuh:1177-1181 jacobi_step); one kernel. */
 int cfa_amg_smooth(const cfa_amg_t* amg, int level, const double* x, const double* b, double* y,
                    cfa_stream_t stream);
+/* The same step in FP32 on the level's FP32 copies (amg->use_f32; the mixed-precision cycle's
+ * smoother); CFA_E_NOTREADY without them. */
+int cfa_amg_smooth_f32(const cfa_amg_t* amg, int level, const float* x, const float* b, float* y,
+                       cfa_stream_t stream);
 
 /* Host callback that enqueues out = op(in) (or out += op(in) for an extra operator) on the
  * stream; the generic form of the std::function arguments of GpuLinearSolver::solve_pcg. */

@@ -65,9 +65,10 @@ void cfa_mesh_probes(void* mesh, int* ab, double* w);
  * its momentum study harness scripts/momentum_study does the same on dumped systems).
  * Pressure: the PPE Laplacian (src/cfa/fem_kernels.cuh:201-282 with stab = 0, :460-531 BCs), one
  * SA-AMG set-up, nrhs seeded zero-mean right-hand sides (mt19937_64(1000 + k), N(0,1)) solved
- * from 0 to tol. out[13] = n, nnz, ne, mesh s, AMG set-up ms (a second set-up of the same matrix:
+ * from 0 to tol. out[16] = n, nnz, ne, mesh s, AMG set-up ms (a second set-up of the same matrix:
  * buffers sized, kernels loaded, as in a time-stepping run), AMG levels, solves, solve ms,
- * iterations, ms per solve, us per iteration, max final relative residual, first set-up ms.
+ * iterations, ms per solve, us per iteration, max final relative residual, first set-up ms, then
+ * the same solves with the FP32 cycle: ms per solve, iterations, max final relative residual.
  * Momentum: the BDF2 momentum matrix (fem_kernels.cuh:40-181, :411-453) for a seeded smooth
  * divergence-free field, BiCGSTAB2 with FP64 and with FP32 ILU(0) factors to tol. out[16] = n,
  * nnz, ne, mesh s, colouring ms, colours, FP64 factor ms, solve ms, it_u, it_v, FP32 factor ms,

@@ -1,6 +1,7 @@
 """Isolated solver workloads (the path's PCG + SA-AMG and ILU-BiCGSTAB2 on frozen uniform meshes)
 at a small level: residual reached, iteration counts sane, FP32 and FP64 preconditioners give the
-same solution to the solver tolerance."""
+same solution to the solver tolerance (the pressure solves with the FP32 cycle meet the same rule
+within two more iterations per solve)."""
 import pytest
 
 import cfa
@@ -15,6 +16,9 @@ def test_pressure_workload_level3():
     assert w["max_rel_residual"] < 1e-8
     assert 5 <= w["iterations"] / w["solves"] <= 200
     assert w["amg_levels"] >= 3
+    # the FP32 cycle inside the FP64 PCG: the same stopping rule met, at most a few more iterations
+    assert w["fp32_max_rel_residual"] < 1e-8
+    assert w["fp32_iterations"] <= w["iterations"] + 2 * w["solves"]
 
 
 def test_momentum_workload_level3():

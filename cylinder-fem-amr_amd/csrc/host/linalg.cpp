@@ -200,8 +200,12 @@ MulticolourIlu0::Pattern MulticolourIlu0::build_pattern(int n, const std::vector
     for (int i = 0; i < n; ++i) P.colour_start[colour[i] + 1]++;
     for (int c = 0; c < P.n_colours; ++c) P.colour_start[c + 1] += P.colour_start[c];
     P.rows.resize(n);
+    P.pos.resize(n);
     std::vector<int> fill(P.colour_start.begin(), P.colour_start.end() - 1);
-    for (int i = 0; i < n; ++i) P.rows[fill[colour[i]]++] = i;
+    for (int i = 0; i < n; ++i) {
+        P.pos[i] = fill[colour[i]]++;
+        P.rows[P.pos[i]] = i;
+    }
     // split pattern in colour-row order: L = earlier colours, U = later colours
     P.lp.assign(n + 1, 0);
     P.up.assign(n + 1, 0);
@@ -231,8 +235,8 @@ MulticolourIlu0::Pattern MulticolourIlu0::build_pattern(int n, const std::vector
         for (int a = rp[i]; a < rp[i + 1]; ++a) {
             const int j = ci[a];
             if (j == i) P.d_src[t] = a;
-            else if (colour[j] < colour[i]) { P.lc[l] = j; P.l_src[l++] = a; }
-            else { P.uc[u] = j; P.u_src[u++] = a; }
+            else if (colour[j] < colour[i]) { P.lc[l] = P.pos[j]; P.l_src[l++] = a; }
+            else { P.uc[u] = P.pos[j]; P.u_src[u++] = a; }
         }
     }
     ilu_elimination_schedule(n, rp.data(), ci.data(), colour, P.rows, P.diag, P.el_ptr, P.el, P.upd);
@@ -268,6 +272,9 @@ void MulticolourIlu0::set_pattern(const Pattern& P, const int* d_rp, const int* 
     d_l_src.upload(P.l_src);
     d_u_src.upload(P.u_src);
     d_d_src.upload(P.d_src);
+    d_pos.upload(P.pos);
+    d_r_perm.reserve(2 * (size_t)n, 64);
+    d_z_perm.reserve(2 * (size_t)n, 64);
     d_lu.reserve(nnz, 256);
     d_lv.reserve(std::max(nl, 1), 256);
     d_uv.reserve(std::max(nu, 1), 256);
@@ -298,6 +305,9 @@ void MulticolourIlu0::refresh_view() {
     view_.up = d_up;
     view_.lc = d_lc;
     view_.uc = d_uc;
+    view_.pos = d_pos;
+    view_.r_perm = d_r_perm;
+    view_.z_perm = d_z_perm;
     view_.l_src = d_l_src;
     view_.u_src = d_u_src;
     view_.d_src = d_d_src;

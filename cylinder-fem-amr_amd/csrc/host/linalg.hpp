@@ -65,7 +65,8 @@ public:
     DevArray<double> d_lu;
     const int* d_row_ptr = nullptr;
     const int* d_col_ind = nullptr;
-    DevArray<int> d_lp, d_up, d_lc, d_uc, d_l_src, d_u_src, d_d_src;
+    DevArray<int> d_lp, d_up, d_lc, d_uc, d_l_src, d_u_src, d_d_src, d_pos;
+    DevArray<double> d_r_perm, d_z_perm;   // the apply's vectors in colour order
     DevArray<int> d_el_ptr, d_el, d_upd;   // elimination schedule of the factorization
     DevArray<double> d_lv, d_uv, d_dinv;
     DevArray<float> d_lv32, d_uv32, d_dinv32;   // FP32 factors (mixed precision)
@@ -81,7 +82,8 @@ public:
     // device memory, so a remesh worker thread can prepare it
     struct Pattern {
         int n = 0, nnz = 0, n_colours = 0, nl = 0, nu = 0;
-        std::vector<int> colour_start, rows, colour, diag, lp, up, lc, uc, l_src, u_src, d_src, el_ptr, el, upd;
+        // lc, uc: colour-order positions of the split entries' columns; pos: of each row
+        std::vector<int> colour_start, rows, colour, diag, lp, up, lc, uc, l_src, u_src, d_src, el_ptr, el, upd, pos;
         double build_ms = 0.0;
     };
     static Pattern build_pattern(int n, const std::vector<int>& rp, const std::vector<int>& ci);

@@ -346,7 +346,10 @@ typedef struct {
     const int* row_ptr; const int* col_ind;
     double* lu;              /* [nnz] factor in CSR layout */
     const int* lp; const int* up;       /* [n + 1] split pattern, colour-row order */
-    const int* lc; const int* uc;       /* [nl], [nu] column of each split entry */
+    const int* lc; const int* uc;       /* [nl], [nu] column of each split entry, as its colour-order
+                                           position (pos[column]) */
+    const int* pos;                     /* [n] colour-order position of row i (rows[pos[i]] = i) */
+    double* r_perm; double* z_perm;     /* [2 n] work vectors of the apply in colour order */
     const int* l_src; const int* u_src; /* CSR position of each split entry */
     const int* d_src;                   /* [n] CSR position of the diagonal, colour-row order */
     double* lv; double* uv; double* dinv;
@@ -361,7 +364,10 @@ typedef struct {
 /* Factor the values (linalg.cuh:531-541): lu = ILU(0) of val in colour order, then lv, uv,
  * dinv (and the FP32 copies when present). */
 int cfa_ilu0_factor(const cfa_ilu_t* ilu, const double* val, cfa_stream_t stream);
-/* z = (LU)^{-1} r for nrhs vectors of stride n (linalg.cuh:544-555). */
+/* z = (LU)^{-1} r for nrhs vectors of stride n (linalg.cuh:544-555). The sweeps run in colour
+ * order: r is permuted into r_perm, every colour's rows, factor entries and vector entries are
+ * then contiguous, and z is permuted back; each row's arithmetic is the reference's, in its
+ * order. */
 int cfa_ilu0_apply(const cfa_ilu_t* ilu, const double* r, double* z, int nrhs,
                    cfa_stream_t stream);
 

@@ -1,11 +1,11 @@
-// Solver classes over the C ABI, and the host smoothed-aggregation set-up.
-// The set-up restates src/cfa/linalg.cuh:995-1060 and 1276-1431: strength |a_ij| >=
-// theta sqrt(|a_ii a_jj|) (theta 0.08), three-phase greedy aggregation in row order, tentative
-// prolongator 1/sqrt(|aggregate|), one damped Jacobi smoothing step with omega = 4 / (3 * 1.05 *
-// rho(D^-1 A)) from 20 power iterations, Galerkin R A P with R = P^T, stop at <= 300 rows,
-// 15 levels or when P.cols >= 0.85 rows, dense Cholesky inverse of the coarsest operator. Every
-// parallel loop writes disjoint rows and each sum keeps its serial order, so the hierarchy does
-// not depend on the thread count.
+// Solver classes over the C ABI. The smoothed-aggregation set-up (src/cfa/linalg.cuh:995-1060 and
+// 1276-1431: strength |a_ij| >= theta sqrt(|a_ii a_jj|) with theta 0.08, three-phase greedy
+// aggregation in row order, tentative prolongator 1/sqrt(|aggregate|), one damped Jacobi
+// smoothing step with omega = 4 / (3 * 1.05 * rho(D^-1 A)) from 20 power iterations, Galerkin
+// R A P with R = P^T, stop at <= 300 rows, 15 levels or when P.cols >= 0.85 rows, dense Cholesky
+// inverse of the coarsest operator) is sequenced here level by level over the device kernels of
+// amg_setup.hip and aggregate.hip; the host inverts the coarse operator and, for remap(), fills
+// the prolongator rows of new nodes layer by layer.
 #include "linalg.hpp"
 
 #include <algorithm>

@@ -1,7 +1,7 @@
 // Solver classes with the reference's interfaces (src/cfa/linalg.cuh): GpuLinearSolver
 // (:120-348), MulticolourIlu0 (:435-568), GpuBicgstab2 (:642-779), AmgPreconditioner
-// (:939-1432). The numerical work runs in the gfx950 kernels behind include/cfa_hip.h; the
-// smoothed-aggregation set-up runs on the host (OpenMP), as in the reference.
+// (:939-1432). The numerical work runs in the gfx950 kernels behind include/cfa_hip.h, the
+// smoothed-aggregation set-up included (the reference runs that on the host with OpenMP).
 #pragma once
 
 #include <functional>

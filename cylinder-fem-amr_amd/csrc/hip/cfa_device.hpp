@@ -169,6 +169,31 @@ __device__ __forceinline__ void row_entries(const int* __restrict__ ci, const T*
     }
 }
 
+// row_entries over NV value arrays on one pattern (the OSS operator's K, Bx, By): f(v, j) gets the
+// entry's NV values v[0..NV-1] and its column; padded slots give zeros and column 0.
+template <int NV, typename T>
+struct RowValues {
+    const T* p[NV];
+};
+template <int TPR, int NV, typename T, typename F>
+__device__ __forceinline__ void row_entries_n(const int* __restrict__ ci, const RowValues<NV, T>& vals, int k0,
+                                              int k1, int lane, F f) {
+    for (int k = k0 + lane; k < k1; k += ROW_UNROLL * TPR) {
+        int c[ROW_UNROLL];
+        T v[ROW_UNROLL][NV];
+#pragma unroll
+        for (int u = 0; u < ROW_UNROLL; ++u) {
+            const int idx = k + u * TPR;
+            const bool ok = idx < k1;
+            c[u] = ok ? ci[idx] : 0;
+#pragma unroll
+            for (int m = 0; m < NV; ++m) v[u][m] = ok ? vals.p[m][idx] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < ROW_UNROLL; ++u) f(v[u], c[u]);
+    }
+}
+
 // Row product sum_k val[k] x[ci[k]] over [s, e) with TPR lanes (lane = rank inside the group)
 template <int TPR>
 __device__ __forceinline__ double row_dot(const int* __restrict__ ci, const double* __restrict__ val,

@@ -432,11 +432,12 @@ __global__ void k_oss_B(int n, int nnz, const int* __restrict__ rp, const int* _
     const int k = gtid / TPR, lane = gtid % TPR;
     double sx = 0.0, sy = 0.0;
     if (k < n)
-        for (int a = rp[k] + lane; a < rp[k + 1]; a += TPR) {
-            const double xj = in[ci[a]];
-            sx += oss[a] * xj;
-            sy += oss[nnz + a] * xj;
-        }
+        row_entries_n<TPR>(ci, RowValues<2, double>{{oss, oss + (size_t)nnz}}, rp[k], rp[k + 1], lane,
+                           [&](const double* v, int j) {
+                               const double xj = in[j];
+                               sx += v[0] * xj;
+                               sy += v[1] * xj;
+                           });
     sx = group_sum<TPR>(sx);
     sy = group_sum<TPR>(sy);
     if (k < n && lane == 0) {
@@ -456,10 +457,8 @@ __global__ void k_oss_BT(int n, int nnz, const int* __restrict__ rp, const int* 
     double s = 0.0;
     const bool active = i < n && !pressure_dirichlet(i, tag[i], om);
     if (active)
-        for (int a = rp[i] + lane; a < rp[i + 1]; a += TPR) {
-            const int k = ci[a];
-            s += BTx[a] * gx[k] + BTy[a] * gy[k];
-        }
+        row_entries_n<TPR>(ci, RowValues<2, double>{{BTx, BTy}}, rp[i], rp[i + 1], lane,
+                           [&](const double* v, int k) { s += v[0] * gx[k] + v[1] * gy[k]; });
     s = group_sum<TPR>(s);
     if (active && lane == 0) out[i] -= s;
 }

@@ -373,14 +373,15 @@ k_apply_K_B(int n, int nnz, const int* __restrict__ rp, const int* __restrict__ 
     if (halted(st)) return;
     const int groups = gridDim.x * (blockDim.x / TPR);
     const int lane = threadIdx.x % TPR;
+    const RowValues<3, double> kb{{val, oss, oss + (size_t)nnz}};
     for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < n; row += groups) {
         double s = 0.0, sx = 0.0, sy = 0.0;
-        for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
-            const double xj = in[ci[k]];
-            s += val[k] * xj;
-            sx += oss[k] * xj;
-            sy += oss[nnz + k] * xj;
-        }
+        row_entries_n<TPR>(ci, kb, rp[row], rp[row + 1], lane, [&](const double* v, int j) {
+            const double xj = in[j];
+            s += v[0] * xj;
+            sx += v[1] * xj;
+            sy += v[2] * xj;
+        });
         s = group_sum<TPR>(s);
         sx = group_sum<TPR>(sx);
         sy = group_sum<TPR>(sy);
@@ -406,15 +407,14 @@ k_apply_BT_dot(int n, int nnz, const int* __restrict__ rp, const int* __restrict
     const double* BTy = oss + 3 * (size_t)nnz;
     const int groups = gridDim.x * (blockDim.x / TPR);
     const int lane = threadIdx.x % TPR;
+    const RowValues<2, double> bt{{BTx, BTy}};
     double contrib = 0.0;
     for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < n; row += groups) {
         const bool active = !pressure_dirichlet(row, tag[row], om);
         double s = 0.0;
         if (active)
-            for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
-                const int c = ci[k];
-                s += BTx[k] * gx[c] + BTy[k] * gy[c];
-            }
+            row_entries_n<TPR>(ci, bt, rp[row], rp[row + 1], lane,
+                               [&](const double* v, int c) { s += v[0] * gx[c] + v[1] * gy[c]; });
         s = group_sum<TPR>(s);
         if (lane == 0) {
             double qi = q[row];
@@ -490,6 +490,7 @@ k_fp_operator(PcgArgs a, int h) {
     const double* __restrict__ val = a.val;
     const int groups = gridDim.x * (blockDim.x / TPR);
     const int lane = threadIdx.x % TPR;
+    const RowValues<3, double> kb{{val, a.oss, a.oss + (size_t)a.nnz}};
     double contrib = 0.0;
     for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < a.n; row += groups) {
         double s = 0.0, sx = 0.0, sy = 0.0;
@@ -498,13 +499,12 @@ k_fp_operator(PcgArgs a, int h) {
                 s += av * (first ? z[j] : fma(beta, po[j], z[j]));
             });
         } else {
-            for (int k = rp[row] + lane; k < rp[row + 1]; k += TPR) {
-                const int j = ci[k];
+            row_entries_n<TPR>(ci, kb, rp[row], rp[row + 1], lane, [&](const double* v, int j) {
                 const double pj = first ? z[j] : fma(beta, po[j], z[j]);
-                s += val[k] * pj;
-                sx += a.oss[k] * pj;
-                sy += a.oss[a.nnz + k] * pj;
-            }
+                s += v[0] * pj;
+                sx += v[1] * pj;
+                sy += v[2] * pj;
+            });
         }
         s = group_sum<TPR>(s);
         if (OSS) {
@@ -707,22 +707,12 @@ __global__ void k_ilu_gather(int nl, int nu, int n, const int* __restrict__ l_sr
     if (e < n) { const double v = 1.0 / lu[d_src[e]]; dinv[e] = v; if (dinv32) dinv32[e] = (float)v; }
 }
 
-// The ILU(0) apply runs in colour order: k_ilu_permute puts r there (read in row order, written
-// into one run per colour), the sweeps then stream every colour's rows, vector entries and factor
-// entries, and k_ilu_unpermute brings z back (written in row order). In row order a colour's rows
-// are spread over the whole vector, so each colour's sweep touched every cache line of r and z:
-// measured at U7, 2.2x the algorithmic bytes crossed HBM.
-template <int NRHS>
-__global__ void k_ilu_permute(int n, const int* __restrict__ pos, const double* __restrict__ r,
-                              double* __restrict__ rp, const int* __restrict__ st) {
-    if (halted(st)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int q = pos[i];
-#pragma unroll
-    for (int c = 0; c < NRHS; ++c) rp[(size_t)c * n + q] = r[(size_t)c * n + i];
-}
-
+// The ILU(0) apply runs in colour order: r is permuted there (read in row order, written into one
+// run per colour), the sweeps then stream every colour's rows, vector entries and factor entries,
+// and z is permuted back (written in row order). In row order a colour's rows are spread over the
+// whole vector, so each colour's sweep touched every cache line of r and z: measured at U7, 2.2x
+// the algorithmic bytes crossed HBM.
+// z back to row order (one colour: no backward sweep)
 template <int NRHS>
 __global__ void k_ilu_unpermute(int n, const int* __restrict__ pos, const double* __restrict__ zp,
                                 double* __restrict__ z, const int* __restrict__ st) {
@@ -732,6 +722,60 @@ __global__ void k_ilu_unpermute(int n, const int* __restrict__ pos, const double
     const int q = pos[i];
 #pragma unroll
     for (int c = 0; c < NRHS; ++c) z[(size_t)c * n + i] = zp[(size_t)c * n + q];
+}
+
+// The permutation into colour order fused with the forward sweep of colour 0. L holds the columns
+// of earlier colours, so colour 0 has no L entries and its sweep is z = r (times dinv when it is
+// also the last colour): its z entries are written here, and its r entries are not needed.
+template <int NRHS, typename T>
+__global__ void k_ilu_permute_first(int n, int m0, const int* __restrict__ pos, const double* __restrict__ r,
+                                    const T* __restrict__ dinv, int last, double* __restrict__ rp,
+                                    double* __restrict__ zp, const int* __restrict__ st) {
+    if (halted(st)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int q = pos[i];
+    if (q < m0) {
+        const double d = (double)dinv[q];
+#pragma unroll
+        for (int c = 0; c < NRHS; ++c) {
+            const double acc = r[(size_t)c * n + i];
+            zp[(size_t)c * n + q] = last ? acc * d : acc;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NRHS; ++c) rp[(size_t)c * n + q] = r[(size_t)c * n + i];
+    }
+}
+
+// The backward sweep of colour 0 fused with the permutation back to row order, one thread per
+// row i in row order: the rows of colour 0 (positions below m0) finish their sweep and write z
+// directly, every other row copies its final value from z_perm.
+template <int NRHS, typename T>
+__global__ void __launch_bounds__(BLOCK)
+k_ilu_last_unpermute(int n, int m0, const int* __restrict__ pos, const int* __restrict__ up,
+                     const int* __restrict__ uc, const T* __restrict__ uv, const T* __restrict__ dinv,
+                     const double* __restrict__ zp, double* __restrict__ z, const int* __restrict__ st) {
+    if (halted(st)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int q = pos[i];
+    if (q >= m0) {
+#pragma unroll
+        for (int c = 0; c < NRHS; ++c) z[(size_t)c * n + i] = zp[(size_t)c * n + q];
+        return;
+    }
+    double acc[NRHS];
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) acc[c] = zp[(size_t)c * n + q];
+    row_entries<1>(uc, uv, up[q], up[q + 1], 0, [&](T ut, int j) {
+        const double u = (double)ut;
+#pragma unroll
+        for (int c = 0; c < NRHS; ++c) acc[c] -= u * zp[(size_t)c * n + j];
+    });
+    const double d = (double)dinv[q];
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) z[(size_t)c * n + i] = acc[c] * d;
 }
 
 // Forward sweep over the rows cs .. cs + m - 1 of one colour (colour order), all components per
@@ -1256,22 +1300,31 @@ template <int NRHS, typename T>
 void ilu_apply_impl(const cfa_ilu_t* ilu, const T* lv, const T* uv, const T* dinv, const double* r, double* z,
                     const int* st, hipStream_t s) {
     const int nc = ilu->n_colours, n = ilu->n;
+    if (n == 0 || nc == 0) return;
     double* rp = ilu->r_perm;
     double* zp = ilu->z_perm;
-    hipLaunchKernelGGL((k_ilu_permute<NRHS>), dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, ilu->pos, r, rp, st);
-    for (int c = 0; c < nc; ++c) {
+    const int m0 = ilu->colour_start[1] - ilu->colour_start[0];
+    // colour 0 forward with the permutation, colours 1.. forward, colours nc-2..1 backward, colour 0
+    // backward with the permutation back (the last colour has no U entries)
+    hipLaunchKernelGGL((k_ilu_permute_first<NRHS, T>), dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, m0, ilu->pos, r,
+                       dinv, nc == 1 ? 1 : 0, rp, zp, st);
+    for (int c = 1; c < nc; ++c) {
         const int cs = ilu->colour_start[c], m = ilu->colour_start[c + 1] - cs;
         if (m == 0) continue;
         hipLaunchKernelGGL((k_ilu_lower<NRHS, T>), dim3(blocks_for(m)), dim3(BLOCK), 0, s, m, n, cs, ilu->lp + cs,
                            ilu->lc, lv, dinv + cs, c == nc - 1 ? 1 : 0, rp, zp, st);
     }
-    for (int c = nc - 2; c >= 0; --c) {
+    for (int c = nc - 2; c >= 1; --c) {
         const int cs = ilu->colour_start[c], m = ilu->colour_start[c + 1] - cs;
         if (m == 0) continue;
         hipLaunchKernelGGL((k_ilu_upper<NRHS, T>), dim3(blocks_for(m)), dim3(BLOCK), 0, s, m, n, cs, ilu->up + cs,
                            ilu->uc, uv, dinv + cs, zp, st);
     }
-    hipLaunchKernelGGL((k_ilu_unpermute<NRHS>), dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, ilu->pos, zp, z, st);
+    if (nc >= 2)
+        hipLaunchKernelGGL((k_ilu_last_unpermute<NRHS, T>), dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, m0, ilu->pos,
+                           ilu->up, ilu->uc, uv, dinv, zp, z, st);
+    else
+        hipLaunchKernelGGL((k_ilu_unpermute<NRHS>), dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, ilu->pos, zp, z, st);
 }
 
 void ilu_apply_any(const cfa_ilu_t* ilu, int use_f32, const double* r, double* z, int nrhs, const int* st,

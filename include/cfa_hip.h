@@ -345,7 +345,9 @@ typedef struct {
     const int* diag_pos;     /* [n] CSR position of the diagonal of row i */
     const int* row_ptr; const int* col_ind;
     double* lu;              /* [nnz] factor in CSR layout */
-    const int* lp; const int* up;       /* [n + 1] split pattern, colour-row order */
+    const int* lp; const int* up;       /* [n + 1] split pattern, colour-row order: L holds the
+                                           columns of earlier colours (none in colour 0), U those
+                                           of later colours (none in the last) */
     const int* lc; const int* uc;       /* [nl], [nu] column of each split entry, as its colour-order
                                            position (pos[column]) */
     const int* pos;                     /* [n] colour-order position of row i (rows[pos[i]] = i) */

@@ -38,72 +38,89 @@ __global__ void k_amg_dinv(int n, const int* __restrict__ rp, const int* __restr
     dinv[i] = fabs(d) > 1.0e-300 ? 1.0 / d : 1.0;
 }
 
-// One power step y = D^-1 A x, one thread per row, entries in row order. The step's x is formed
-// where it is read: x0 on the first step, else y_prev / nrm with nrm = sqrt(sum of the previous
-// step's chunk sums, in chunk order), which every workgroup recomputes (the same division the
-// host applies to its stored x). Each row also stores its own x_i for the chunk sums.
+// Sum of cnt values in index order by thread 0 (t += v[c], c = 0, 1, ...), the values staged in
+// LDS by the whole workgroup a tile at a time, so the loads are in flight together instead of one
+// dependent global load per term. All threads call it; the result is valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double ordered_sum(const double* __restrict__ v, int cnt, double* tile) {
+    double t = 0.0;
+    for (int c0 = 0; c0 < cnt; c0 += NT) {
+        const int m = min(NT, cnt - c0);
+        if ((int)threadIdx.x < m) tile[threadIdx.x] = v[c0 + threadIdx.x];
+        __syncthreads();
+        if (threadIdx.x == 0)
+#pragma unroll 8
+            for (int c = 0; c < m; ++c) t += tile[c];
+        __syncthreads();
+    }
+    return t;
+}
+
+// One power step y = D^-1 A x, one thread per row, entries in row order (loaded ahead of use).
+// The step's x is formed where it is read: x0 on the first step, else y_prev / nrm with
+// nrm = sqrt(sum of the previous step's chunk sums, in chunk order), which every workgroup
+// recomputes (the same division the host applies to its stored x). Each row also stores its own
+// x_i for the chunk sums.
 __global__ void __launch_bounds__(BLOCK)
 k_power_step(int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
              const double* __restrict__ dinv, const double* __restrict__ x0, const double* __restrict__ y_prev,
              const double* __restrict__ py_prev, int nch, int first, double* __restrict__ x,
              double* __restrict__ y) {
+    __shared__ double tile[BLOCK];
     __shared__ double s_nrm;
     if (!first) {
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int c = 0; c < nch; ++c) t += py_prev[c];
-            s_nrm = sqrt(t);
-        }
+        const double t = ordered_sum<BLOCK>(py_prev, nch, tile);
+        if (threadIdx.x == 0) s_nrm = sqrt(t);
         __syncthreads();
     }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double nrm = first ? 1.0 : s_nrm;
     double s = 0.0;
-    for (int a = rp[i]; a < rp[i + 1]; ++a) {
-        const int j = ci[a];
+    row_entries<1>(ci, v, rp[i], rp[i + 1], 0, [&](double a, int j) {
         const double xj = first ? x0[j] : y_prev[j] / nrm;
-        s += v[a] * xj;
-    }
+        s += a * xj;
+    });
     x[i] = first ? x0[i] : y_prev[i] / nrm;
     y[i] = dinv[i] * s;
 }
 
-// per chunk of 4096 rows: sum y_i^2 and x_i^2 in row order. One wavefront per chunk stages the
-// squares in LDS; then lane 0 adds the y^2 and lane 1 the x^2 in row order, two dependent chains
-// running side by side in one instruction stream with their LDS reads issued ahead.
-__global__ void __launch_bounds__(WAVE) k_power_chunks(int n, const double* __restrict__ x,
-                                                       const double* __restrict__ y,
-                                                       double* __restrict__ py, double* __restrict__ px) {
+// per chunk of 4096 rows: sum y_i^2 and x_i^2 in row order. A workgroup of 1024 threads per chunk
+// stages the squares in LDS; then lane 0 adds the y^2 and lane 1 the x^2 in row order, two
+// dependent chains running side by side in one instruction stream with their LDS reads issued
+// ahead.
+constexpr int CHUNK_THREADS = 1024;
+__global__ void __launch_bounds__(CHUNK_THREADS) k_power_chunks(int n, const double* __restrict__ x,
+                                                                const double* __restrict__ y,
+                                                                double* __restrict__ py, double* __restrict__ px) {
     __shared__ double sq[2][CHUNK];
-    const int c = blockIdx.x, lane = threadIdx.x;
+    const int c = blockIdx.x, t = threadIdx.x;
     const int lo = c * CHUNK, cnt = min(n, lo + CHUNK) - lo;
-    for (int k = lane; k < cnt; k += WAVE) {
+#pragma unroll 4
+    for (int k = t; k < cnt; k += CHUNK_THREADS) {
         const double yi = y[lo + k], xi = x[lo + k];
         sq[0][k] = yi * yi;
         sq[1][k] = xi * xi;
     }
     __syncthreads();
-    if (lane < 2) {
-        const double* q = sq[lane];
+    if (t < 2) {
+        const double* q = sq[t];
         double acc = 0.0;
 #pragma unroll 16
         for (int k = 0; k < cnt; ++k) acc += q[k];
-        if (lane == 0) py[c] = acc;
+        if (t == 0) py[c] = acc;
         else px[c] = acc;
     }
 }
 
 // nrm = sqrt(sum of chunk sums), lambda = nrm / sqrt(sum of x^2), in chunk order; out[0] = nrm,
 // out[1] = lambda
-__global__ void k_power_final(int nch, const double* __restrict__ py, const double* __restrict__ px,
-                              double* __restrict__ out) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    double nrm = 0.0, xn = 0.0;
-    for (int c = 0; c < nch; ++c) {
-        nrm += py[c];
-        xn += px[c];
-    }
+__global__ void __launch_bounds__(BLOCK) k_power_final(int nch, const double* __restrict__ py,
+                                                       const double* __restrict__ px, double* __restrict__ out) {
+    __shared__ double tile[BLOCK];
+    double nrm = ordered_sum<BLOCK>(py, nch, tile);
+    const double xn = ordered_sum<BLOCK>(px, nch, tile);
+    if (threadIdx.x != 0) return;
     nrm = sqrt(nrm);
     out[0] = nrm;
     out[1] = nrm / sqrt(xn);
@@ -179,6 +196,138 @@ k_spgemm_merge(int rows, const int* __restrict__ a_rp, const int* __restrict__ a
         ++out;
     }
     if (pass == 0) cnt[row] = out;
+}
+
+// Rows of A with up to 64 entries and up to WMERGE_CAP products: one wavefront per row. Lane l
+// holds entry l of A's row; the wave stages the B rows in LDS (columns, and in pass 1 the
+// products a_il b_lj rounded as the host rounds them) with independent loads, then merges them:
+// per output column j the wave takes the minimum head column, and the lanes whose head is j add
+// their products from 0.0 in lane order, which is A's entry order (the host's encounter order).
+// Each output column costs a few wave operations on LDS instead of the bitonic sort's stages.
+// Output entries are buffered one per lane and written 64 at a time.
+constexpr int WMERGE_CAP = 1024;
+
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, WAVE));
+    return v;
+}
+
+// v of lane l (l wave-uniform)
+__device__ __forceinline__ double read_lane(double v, int l) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__global__ void __launch_bounds__(WAVE)
+k_spgemm_wave(int rows, const int* __restrict__ a_rp, const int* __restrict__ a_ci, const double* __restrict__ a_v,
+              const int* __restrict__ b_rp, const int* __restrict__ b_ci, const double* __restrict__ b_v, int pass,
+              int* __restrict__ cnt, const int* __restrict__ c_rp, int* __restrict__ c_ci, double* __restrict__ c_v,
+              int* __restrict__ overflow) {
+    __shared__ int s_col[WMERGE_CAP];
+    __shared__ double s_prod[WMERGE_CAP];
+    __shared__ int s_off[WAVE + 1], s_b0[WAVE];
+    __shared__ double s_av[WAVE];
+    const int row = blockIdx.x, lane = threadIdx.x;
+    if (row >= rows) return;
+    const int a0 = a_rp[row], len = a_rp[row + 1] - a0;
+    if (len > WAVE) {
+        if (lane == 0) {
+            overflow[0] = 1;
+            if (pass == 0) cnt[row] = 0;
+        }
+        return;
+    }
+    int b0 = 0, blen = 0;
+    double av = 0.0;
+    if (lane < len) {
+        const int k = a_ci[a0 + lane];
+        av = a_v[a0 + lane];
+        b0 = b_rp[k];
+        blen = b_rp[k + 1] - b0;
+    }
+    // inclusive scan of the B row lengths over the wave
+    int incl = blen;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int t = __shfl_up(incl, o, WAVE);
+        if (lane >= o) incl += t;
+    }
+    const int total = __shfl(incl, WAVE - 1, WAVE);
+    if (total > WMERGE_CAP) {
+        if (lane == 0) {
+            overflow[0] = 1;
+            if (pass == 0) cnt[row] = 0;
+        }
+        return;
+    }
+    const int off = incl - blen;
+    s_off[lane] = off;
+    if (lane == WAVE - 1) s_off[WAVE] = total;
+    s_b0[lane] = b0;
+    s_av[lane] = av;
+    __syncthreads();
+    // stage: product p belongs to the lane whose range [s_off[l], s_off[l + 1]) holds it
+    for (int p = lane; p < total; p += WAVE) {
+        int lo = 0, hi = WAVE - 1;   // the last l with s_off[l] <= p
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= p) lo = mid;
+            else hi = mid - 1;
+        }
+        const int src = s_b0[lo] + (p - s_off[lo]);
+        s_col[p] = b_ci[src];
+        if (pass == 1) {
+            const double prod = s_av[lo] * b_v[src];
+            s_prod[p] = prod;
+        }
+    }
+    __syncthreads();
+    int cur = off;
+    const int end = off + blen;
+    int head = cur < end ? s_col[cur] : NO_COL;
+    int out = 0, my_j = 0;
+    double my_v = 0.0;
+    const int o0 = pass == 1 ? c_rp[row] : 0;
+    while (true) {
+        const int j = wave_min_int(head);
+        if (j == NO_COL) break;
+        const bool hit = head == j;
+        if (pass == 1) {
+            const double prod = hit ? s_prod[cur] : 0.0;
+            unsigned long long mask = __ballot(hit);
+            double acc = 0.0;
+            while (mask) {
+                const int l = __ffsll((long long)mask) - 1;
+                acc += read_lane(prod, l);
+                mask &= mask - 1;
+            }
+            const int slot = out & (WAVE - 1);
+            if (lane == slot) {
+                my_j = j;
+                my_v = acc;
+            }
+            if (slot == WAVE - 1) {
+                c_ci[o0 + out - (WAVE - 1) + lane] = my_j;
+                c_v[o0 + out - (WAVE - 1) + lane] = my_v;
+            }
+        }
+        if (hit) {
+            ++cur;
+            head = cur < end ? s_col[cur] : NO_COL;
+        }
+        ++out;
+    }
+    if (pass == 1) {
+        const int rem = out & (WAVE - 1);
+        if (lane < rem) {
+            c_ci[o0 + out - rem + lane] = my_j;
+            c_v[o0 + out - rem + lane] = my_v;
+        }
+    }
+    if (pass == 0 && lane == 0) cnt[row] = out;
 }
 
 // Products of long rows: one workgroup per row of A. The products a_ik b_kj are listed in encounter order (entries of
@@ -573,9 +722,9 @@ int cfa_amg_power(int n, const int* rp, const int* ci, const double* val, const 
         const int cur = it & 1, prev = cur ^ 1;
         hipLaunchKernelGGL(k_power_step, dim3(blocks(n)), dim3(BLOCK), 0, s, n, rp, ci, val, dinv, x0, w.y[prev],
                            w.py[prev], nch, it == 0 ? 1 : 0, w.x, w.y[cur]);
-        hipLaunchKernelGGL(k_power_chunks, dim3(nch), dim3(WAVE), 0, s, n, w.x, w.y[cur], w.py[cur], w.px);
+        hipLaunchKernelGGL(k_power_chunks, dim3(nch), dim3(CHUNK_THREADS), 0, s, n, w.x, w.y[cur], w.py[cur], w.px);
     }
-    hipLaunchKernelGGL(k_power_final, dim3(1), dim3(WAVE), 0, s, nch, w.py[(iterations - 1) & 1], w.px, w.out);
+    hipLaunchKernelGGL(k_power_final, dim3(1), dim3(BLOCK), 0, s, nch, w.py[(iterations - 1) & 1], w.px, w.out);
     int rc = launch_status();
     if (rc || !lambda) return rc;
     hipError_t e = hipMemcpyAsync(lambda, w.out + 1, sizeof(double), hipMemcpyDeviceToHost, s);
@@ -586,13 +735,16 @@ int cfa_amg_power(int n, const int* rp, const int* ci, const double* val, const 
 int cfa_spgemm(int rows, const int* a_rp, const int* a_ci, const double* a_v, const int* b_rp, const int* b_ci,
                const double* b_v, int pass, int method, int* cnt, const int* c_rp, int* c_ci, double* c_v,
                int* overflow, cfa_stream_t stream) {
-    if (rows < 0 || (pass != 0 && pass != 1) || (method != 0 && method != 1) || !overflow) return CFA_E_ARG;
+    if (rows < 0 || (pass != 0 && pass != 1) || method < 0 || method > 2 || !overflow) return CFA_E_ARG;
     if (rows == 0) return CFA_OK;
     if (!a_rp || !a_ci || !a_v || !b_rp || !b_ci || !b_v) return CFA_E_ARG;
     if ((pass == 0 && !cnt) || (pass == 1 && (!c_rp || !c_ci || !c_v))) return CFA_E_ARG;
     if (method == 0)
         hipLaunchKernelGGL(k_spgemm_merge, dim3(blocks(rows)), dim3(BLOCK), 0, as_stream(stream), rows, a_rp, a_ci,
                            a_v, b_rp, b_ci, b_v, pass, cnt, c_rp, c_ci, c_v, overflow);
+    else if (method == 2)
+        hipLaunchKernelGGL(k_spgemm_wave, dim3(rows), dim3(WAVE), 0, as_stream(stream), rows, a_rp, a_ci, a_v, b_rp,
+                           b_ci, b_v, pass, cnt, c_rp, c_ci, c_v, overflow);
     else
         hipLaunchKernelGGL(k_spgemm_rows, dim3(rows), dim3(SPGEMM_THREADS), 0, as_stream(stream), rows, a_rp, a_ci,
                            a_v, b_rp, b_ci, b_v, pass, cnt, c_rp, c_ci, c_v, overflow);

@@ -402,10 +402,17 @@ bool AmgPreconditioner::device_product(int rows, int cols, int a_nnz, const int*
     d_sws_.reserve(std::max<size_t>(sws, 1));
     C.rp.reserve(rows + 1);
     // method 0: one thread per row merging the right rows (rows of A up to 16 entries);
+    // method 2: one wavefront per row merging them in LDS (up to 64 entries, 1024 products);
     // method 1: one workgroup per row sorting the products in LDS (up to 4096 per row).
-    // The count pass, the scan and one download of {overflow flag, total} per attempt.
-    const int first_method = force_sorted_products ? 2 : left_max_row >= 0 && left_max_row <= 16 ? 0 : 1;
-    for (int method = first_method; method < 2; ++method) {
+    // The count pass, the scan and one download of {overflow flag, total} per attempt; a method
+    // whose limit a row exceeds hands the product to the next.
+    std::vector<int> methods;
+    if (!force_sorted_products) {
+        if (left_max_row >= 0 && left_max_row <= 16) methods.push_back(0);
+        if (left_max_row < 0 || left_max_row <= 64) methods.push_back(2);
+        methods.push_back(1);
+    }
+    for (const int method : methods) {
         dev_zero(d_flag_, 2 * sizeof(int));
         GPU_CHECK(cfa_spgemm(rows, arp, aci, av, brp, bci, bv, 0, method, d_cnt_, nullptr, nullptr, nullptr, d_flag_,
                              s));

@@ -402,8 +402,9 @@ int cfa_bicgstab_jacobi(int n, const int* rp, const int* ci, const double* val, 
  * caller) when a row is beyond the method's limit; pass 1 writes the sorted columns and the
  * values, each summed from 0 in the order the products are met, at c_rp (from
  * cfa_scan_counts). method 0: one thread per row merging the right rows (rows of A with at
- * most 16 entries); method 1: one workgroup per row sorting up to 4096 products in LDS. Rows of
- * B must have sorted, unique columns.
+ * most 16 entries); method 1: one workgroup per row sorting up to 4096 products in LDS;
+ * method 2: one wavefront per row merging the right rows staged in LDS (rows of A with at most
+ * 64 entries and 1024 products). Rows of B must have sorted, unique columns.
  * cfa_scan_counts: out[0..n] = exclusive prefix sums of cnt (out[n] = total; also copied to the
  * device int *total when given); ws: device workspace of cfa_scan_counts_ws_bytes(n) bytes.
  * cfa_amg_power with lambda == NULL does not synchronize: {||D^-1 A x||, lambda} of the last step

@@ -476,14 +476,6 @@ k_fp_operator(PcgArgs a, int h) {
         return;
     }
     const bool first = it == 0;
-    // the first row's range loads while the partials are summed
-    const int lane = threadIdx.x % TPR;
-    const int row0 = (blockIdx.x * blockDim.x + threadIdx.x) / TPR;
-    int k0 = 0, k1 = 0;
-    if (row0 < a.n) {
-        k0 = a.rp[row0];
-        k1 = a.rp[row0 + 1];
-    }
     double beta = 0.0;
     if (!first) {
         double rz[2];
@@ -497,20 +489,17 @@ k_fp_operator(PcgArgs a, int h) {
     const int* __restrict__ ci = a.ci;
     const double* __restrict__ val = a.val;
     const int groups = gridDim.x * (blockDim.x / TPR);
+    const int lane = threadIdx.x % TPR;
     const RowValues<3, double> kb{{val, a.oss, a.oss + (size_t)a.nnz}};
     double contrib = 0.0;
-    for (int row = row0; row < a.n; row += groups) {
-        if (row != row0) {
-            k0 = rp[row];
-            k1 = rp[row + 1];
-        }
+    for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < a.n; row += groups) {
         double s = 0.0, sx = 0.0, sy = 0.0;
         if (!OSS) {
-            row_entries<TPR>(ci, val, k0, k1, lane, [&](double av, int j) {
+            row_entries<TPR>(ci, val, rp[row], rp[row + 1], lane, [&](double av, int j) {
                 s += av * (first ? z[j] : fma(beta, po[j], z[j]));
             });
         } else {
-            row_entries_n<TPR>(ci, kb, k0, k1, lane, [&](const double* v, int j) {
+            row_entries_n<TPR>(ci, kb, rp[row], rp[row + 1], lane, [&](const double* v, int j) {
                 const double pj = first ? z[j] : fma(beta, po[j], z[j]);
                 s += v[0] * pj;
                 sx += v[1] * pj;
@@ -547,16 +536,6 @@ k_fp_update(PcgArgs a, int h, int nb_pq) {
     __shared__ double lds[2 * BLOCK / WAVE];
     int it;
     if (block_state(a.st, &it) != ST_RUNNING) return;
-    const double* __restrict__ p = a.p[h];
-    const double* __restrict__ q = a.q;
-    double* __restrict__ x = a.x;
-    double* __restrict__ r = a.r;
-    // the first element's operands load while the partials are summed
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
-    double pre[4] = {0.0, 0.0, 0.0, 0.0};
-    if (i0 < a.n) {
-        pre[0] = x[i0]; pre[1] = p[i0]; pre[2] = r[i0]; pre[3] = q[i0];
-    }
     double sums[2];   // p.q, r.z
     sum_partials_k<2>({a.ppq, a.prz[h]}, {nb_pq, a.nb_rz}, 1, sums, lds);
     const double pq = sums[0];
@@ -569,15 +548,17 @@ k_fp_update(PcgArgs a, int h, int nb_pq) {
         return;
     }
     const double alpha = sums[1] / pq;
+    const double* __restrict__ p = a.p[h];
+    const double* __restrict__ q = a.q;
+    double* __restrict__ x = a.x;
+    double* __restrict__ r = a.r;
     double acc = 0.0;
-    auto body = [&](int i, double xi, double pi, double ri0, double qi) {
-        x[i] = xi + alpha * pi;
-        const double ri = ri0 + (-alpha) * qi;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] + (-alpha) * q[i];
         r[i] = ri;
         acc += ri * ri;
-    };
-    if (i0 < a.n) body(i0, pre[0], pre[1], pre[2], pre[3]);
-    for (int i = i0 + gridDim.x * blockDim.x; i < a.n; i += gridDim.x * blockDim.x) body(i, x[i], p[i], r[i], q[i]);
+    }
     acc = block_sum(acc, lds);
     if (threadIdx.x == 0) {
         a.prr[blockIdx.x] = acc;
@@ -621,18 +602,13 @@ k_jacobi_from_zero_check(int rows, const int* __restrict__ rp, const int* __rest
                          const double* __restrict__ prr, int nb_rr, double* sc, int* st, int* status) {
     __shared__ double lds[BLOCK / WAVE];
     if (block_state(st, nullptr) != ST_RUNNING) return;
-    // the row's range loads while the residual test sums its partials
+    if (!fp_residual_test(prr, nb_rr, sc, st, status, lds)) return;
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = g / TPR, lane = g % TPR;
-    int k0 = 0, k1 = 0;
-    if (row < rows) {
-        k0 = rp[row];
-        k1 = rp[row + 1];
-    }
-    if (!fp_residual_test(prr, nb_rr, sc, st, status, lds)) return;
     double s = 0.0;
     if (row < rows)
-        row_entries<TPR>(ci, val, k0, k1, lane, [&](double a, int j) { s += a * (omega * dinv[j] * b[j]); });
+        row_entries<TPR>(ci, val, rp[row], rp[row + 1], lane,
+                         [&](double a, int j) { s += a * (omega * dinv[j] * b[j]); });
     s = group_sum<TPR>(s);
     if (row >= rows || lane != 0) return;
     const double x0 = omega * dinv[row] * b[row];
@@ -673,13 +649,11 @@ k_jacobi_dot(int rows, const int* __restrict__ rp, const int* __restrict__ ci, c
 __global__ void k_to_f32_check(int n, const double* __restrict__ r, float* __restrict__ b32,
                                const double* __restrict__ prr, int nb_rr, double* sc, int* st, int* status) {
     __shared__ double lds[BLOCK / WAVE];
-    if (st && block_state(st, nullptr) != ST_RUNNING) return;
-    // the first element loads while the residual test sums its partials
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
-    const double r_0 = i0 < n ? r[i0] : 0.0;
-    if (st && prr && !fp_residual_test(prr, nb_rr, sc, st, status, lds)) return;
-    if (i0 < n) b32[i0] = (float)r_0;
-    for (int i = i0 + gridDim.x * blockDim.x; i < n; i += gridDim.x * blockDim.x) b32[i] = (float)r[i];
+    if (st) {
+        if (block_state(st, nullptr) != ST_RUNNING) return;
+        if (prr && !fp_residual_test(prr, nb_rr, sc, st, status, lds)) return;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) b32[i] = (float)r[i];
 }
 
 __global__ void k_f64_to_f32(size_t n, const double* __restrict__ in, float* __restrict__ out) {
@@ -1024,13 +998,6 @@ k_bicg_s(int n, const double* __restrict__ r, const double* __restrict__ v, doub
     if (threadIdx.x == 0) { s_state = st[0]; s_act[0] = st[2]; s_act[1] = st[3]; }
     __syncthreads();
     if (s_state != ST_RUNNING) return;
-    // the first element's operands load while the partials are summed
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
-    double pre[2] = {0.0, 0.0};
-    if (i0 < 2 * n) {
-        pre[0] = r[i0];
-        pre[1] = v[i0];
-    }
     double ca[2];
     bool act[2];
     double o2[2];
@@ -1051,14 +1018,12 @@ k_bicg_s(int n, const double* __restrict__ r, const double* __restrict__ v, doub
     }
     if (!(act[0] || act[1])) return;
     double acc[1][2] = {{0.0, 0.0}};
-    auto body = [&](int i, double ri, double vi) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += gridDim.x * blockDim.x) {
         const int c = i >= n;
-        const double si = ri - ca[c] * vi;
+        const double si = r[i] - ca[c] * v[i];
         s[i] = si;
         acc[0][c] += si * si;
-    };
-    if (i0 < 2 * n) body(i0, pre[0], pre[1]);
-    for (int i = i0 + gridDim.x * blockDim.x; i < 2 * n; i += gridDim.x * blockDim.x) body(i, r[i], v[i]);
+    }
     write_partials2<1>(part, acc, lds);
 }
 
@@ -1074,12 +1039,6 @@ k_bicg_xr(int n, const double* __restrict__ ph, const double* __restrict__ sh, c
     if (threadIdx.x == 0) { s_state = st[0]; s_act[0] = st[2]; s_act[1] = st[3]; }
     __syncthreads();
     if (s_state != ST_RUNNING) return;
-    // the first element's operands load while the partials are summed
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
-    double pre[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (i0 < 2 * n) {
-        pre[0] = x[i0]; pre[1] = ph[i0]; pre[2] = sh[i0]; pre[3] = s[i0]; pre[4] = t[i0]; pre[5] = r0[i0];
-    }
     double a1[2], a2[2], om[2];
     int half[2];
     double sums[6];   // s.s, t.t, t.s of both components
@@ -1110,17 +1069,14 @@ k_bicg_xr(int n, const double* __restrict__ ph, const double* __restrict__ sh, c
             if (s_act[c] && !half[c]) sc[B_OMEGA + c] = om[c];
         }
     double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-    auto body = [&](int i, double xi, double phi, double shi, double si, double ti, double r0i) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * n; i += gridDim.x * blockDim.x) {
         const int c = i >= n;
-        x[i] = xi + (a1[c] * phi + a2[c] * shi);
-        const double ri = si - a2[c] * ti;
+        x[i] += a1[c] * ph[i] + a2[c] * sh[i];
+        const double ri = s[i] - a2[c] * t[i];
         r[i] = ri;
         acc[0][c] += ri * ri;
-        acc[1][c] += r0i * ri;
-    };
-    if (i0 < 2 * n) body(i0, pre[0], pre[1], pre[2], pre[3], pre[4], pre[5]);
-    for (int i = i0 + gridDim.x * blockDim.x; i < 2 * n; i += gridDim.x * blockDim.x)
-        body(i, x[i], ph[i], sh[i], s[i], t[i], r0[i]);
+        acc[1][c] += r0[i] * ri;
+    }
     write_partials2<2>(part, acc, lds);
 }
 

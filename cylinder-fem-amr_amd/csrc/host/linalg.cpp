@@ -349,15 +349,14 @@ void GpuBicgstab2::solve(int n, const int* d_row_ptr, const int* d_col_ind, cons
 // SA-AMG (src/cfa/linalg.cuh:939-1432): the set-up runs on the device level by level
 // ---------------------------------------------------------------------------------------------
 int csr_lanes_per_row(int rows, long long nnz) {
-    // Measured on MI355X (one Jacobi sweep per level, lanes 1..32, uniform level-4 to level-7
+    // Measured on MI355X (one Jacobi sweep per level, lanes 1..32, uniform level-4 and level-5
     // pressure hierarchies): rows of ~7 entries run fastest with 2-4 lanes (one lane is 2.2x
-    // slower on 460k rows; at 1.8M rows 2 and 4 lanes tie, at 7.3M rows 4 lanes are 4.5 % faster),
-    // ~15 entries with 4-8, and short levels (< 20k rows) are latency bound and run fastest when a
-    // group covers the whole row. So: about four entries per lane, at least 2 lanes (4 for
-    // operator rows of 6 or more entries from 4M rows); one lane per entry (up to 32) below 20k rows.
+    // slower on 460k rows), ~15 entries with 4-8, and short levels (< 20k rows) are latency bound
+    // and run fastest when a group covers the whole row. So: about four entries per lane, at
+    // least 2 lanes; one lane per entry (up to 32) below 20k rows.
     const double per_row = rows > 0 ? (double)nnz / rows : 1.0;
     const double target = rows < 20000 ? per_row : per_row / 4.0;
-    int tpr = rows >= 4000000 && per_row >= 6.0 ? 4 : 2;
+    int tpr = 2;
     while (tpr < 32 && tpr < target) tpr *= 2;
     return tpr;
 }

@@ -138,6 +138,12 @@ int cfa_timer_record(void* timer, cfa_stream_t stream) {
     return e == hipSuccess ? CFA_OK : (int)e;
 }
 
+int cfa_stream_wait(cfa_stream_t stream, void* event) {
+    if (!event) return CFA_E_ARG;
+    const hipError_t e = hipStreamWaitEvent(as_stream(stream), static_cast<hipEvent_t>(event), 0);
+    return e == hipSuccess ? CFA_OK : (int)e;
+}
+
 int cfa_timer_elapsed_ms(void* t_begin, void* t_end, float* ms) {
     hipError_t e = hipEventSynchronize(static_cast<hipEvent_t>(t_end));
     if (e != hipSuccess) return (int)e;

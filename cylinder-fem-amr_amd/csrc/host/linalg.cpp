@@ -384,6 +384,21 @@ void AmgPreconditioner::ensure_level_vectors(Level& L, int n) {
     L.Ax.reserve(n, 16);
 }
 
+void AmgPreconditioner::AuxStream::ensure() {
+    if (s) return;
+    GPU_CHECK(cfa_stream_create(&s));
+    GPU_CHECK(cfa_timer_create(&ready));
+    GPU_CHECK(cfa_timer_create(&done));
+}
+
+AmgPreconditioner::AuxStream::~AuxStream() {
+    if (!s) return;
+    (void)cfa_stream_sync(s);
+    (void)cfa_timer_destroy(ready);
+    (void)cfa_timer_destroy(done);
+    (void)cfa_stream_destroy(s);
+}
+
 void AmgPreconditioner::release() {
     for (auto& L : levels) spare_.push_back(std::move(L));
     levels.clear();
@@ -532,7 +547,12 @@ int AmgPreconditioner::device_level(int n, int nnz, const int* arp, const int* a
     }
     const size_t pw = cfa_amg_power_ws_doubles(n);
     d_pw_.reserve(pw);
-    GPU_CHECK(cfa_amg_power(n, arp, aci, av, L->dinv, d_x0_, 20, d_pw_, nullptr, s));
+    // the power iteration on the second stream, beside the aggregation below
+    aux_.ensure();
+    GPU_CHECK(cfa_timer_record(aux_.ready, s));
+    GPU_CHECK(cfa_stream_wait(aux_.s, aux_.ready));
+    GPU_CHECK(cfa_amg_power(n, arp, aci, av, L->dinv, d_x0_, 20, d_pw_, nullptr, aux_.s));
+    GPU_CHECK(cfa_timer_record(aux_.done, aux_.s));
     phase_end(PH_RHO);
     // aggregation; then one download: {aggregates, phase-1 aggregates, longest row, rounds} and lambda
     const size_t aws = cfa_amg_aggregate_ws_bytes(n, nnz);
@@ -544,6 +564,7 @@ int AmgPreconditioner::device_level(int n, int nnz, const int* arp, const int* a
     int info[4] = {0, 0, 0, 0};
     d_info_.download(info, 4);
     double lambda = 0.0;
+    GPU_CHECK(cfa_stream_wait(s, aux_.done));
     GPU_CHECK(cfa_memcpy(&lambda, d_pw_.ptr + pw - 1, sizeof(double), 1, s));
     phase_end(PH_AGG);
     const int na = info[0], left_max = info[2];

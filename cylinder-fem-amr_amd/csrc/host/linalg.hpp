@@ -180,6 +180,19 @@ public:
     const cfa_amg_t* view() const { return &view_; }
 
 private:
+    // the set-up's second stream: each level's power iteration runs on it beside the aggregation
+    // (both only read the level's operator and D^-1), joined by two events
+    struct AuxStream {
+        cfa_stream_t s = nullptr;
+        void* ready = nullptr;   // D^-1 written (recorded on the solver stream)
+        void* done = nullptr;    // power iteration finished (recorded on s)
+        AuxStream() = default;
+        AuxStream(const AuxStream&) = delete;
+        AuxStream& operator=(const AuxStream&) = delete;
+        ~AuxStream();
+        void ensure();
+    };
+    AuxStream aux_;
     DevCsr dev_next_, dev_tmp_, dev_next2_;    // device set-up: next operator, products
     DevArray<int> d_cnt_, d_flag_, d_agg_, d_iota_, d_info_;
     DevArray<unsigned char> d_tws_, d_sws_, d_aws_, d_gws_;   // transpose, scan, aggregation, sorted product

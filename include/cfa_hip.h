@@ -61,6 +61,9 @@ int cfa_timer_create(void** timer);
 int cfa_timer_destroy(void* timer);
 int cfa_timer_record(void* timer, cfa_stream_t stream);
 int cfa_timer_elapsed_ms(void* t_begin, void* t_end, float* ms);
+/* Work enqueued on `stream` after this call waits for `event` (a timer recorded on another
+ * stream): the dependency between the SA-AMG set-up's two streams. */
+int cfa_stream_wait(cfa_stream_t stream, void* event);
 
 /* ---------------------------------------------------------------------------------------------
  * Mesh description on the device. Besides the reference's arrays (elem_v, x, y, bc_tag, CSR

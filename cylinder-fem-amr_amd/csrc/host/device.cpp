@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <thread>
 #include <iomanip>
 #include <iostream>
 
@@ -382,9 +383,20 @@ void apply_oss_projection(DeviceMeshBuffers& g, const double* in, double* out) {
 }
 
 // src/cfa/device.cuh:337-408
+ReprojectionAux::ReprojectionAux() {
+    GPU_CHECK(cfa_stream_create(&s));
+    GPU_CHECK(cfa_timer_create(&ready));
+}
+
+ReprojectionAux::~ReprojectionAux() {
+    (void)cfa_stream_sync(s);
+    (void)cfa_timer_destroy(ready);
+    (void)cfa_stream_destroy(s);
+}
+
 int project_scheme_consistent(DeviceMeshBuffers& g, GpuLinearSolver& solver, double dt_proj, double stab,
                               AmgPreconditioner* amg, int& nonconverged, const std::function<void()>& build_amg,
-                              bool both_levels, double* phi_out, double tol) {
+                              bool both_levels, double* phi_out, double tol, ReprojectionAux* aux) {
     cfa_stream_t s = solver_stream();
     const bool oss = stab > 0.0;
     const cfa_mesh_t* m = g.view();
@@ -392,6 +404,54 @@ int project_scheme_consistent(DeviceMeshBuffers& g, GpuLinearSolver& solver, dou
     if (oss) GPU_CHECK(cfa_oss_weights(m, g.d_u, g.d_v, dt_proj, Sim::RHO, Sim::NU, stab, g.d_elem_s, s));
     const cfa_oss_op_t op = g.oss_op();
     int iters = 0;
+    if (both_levels && amg && aux) {
+        // The two projections do not depend on each other (the correction leaves p unchanged:
+        // p_update 0): both systems are assembled, the hierarchy built from the first, and the two
+        // PCG solves run at the same time, the second on its own stream, solver, right-hand side,
+        // solution, OSS scratch and hierarchy work vectors; then the two corrections in order. Every
+        // solve and correction sees the values it sees when they run one after the other.
+        const int n = g.n_nodes;
+        for (DevArray<double>* v : {&aux->rhs, &aux->div_num, &aux->dp, &aux->gx, &aux->gy}) v->reserve(n, 64);
+        double* rhs[2] = {g.d_rhs_p, aux->rhs};
+        double* div[2] = {g.d_div_num, aux->div_num};
+        double* dp[2] = {g.d_dp, aux->dp};
+        double* uv[2][2] = {{g.d_u_old, g.d_v_old}, {g.d_u, g.d_v}};
+        if (oss) GPU_CHECK(cfa_scalar_gradient(m, g.d_p, g.d_elem_s, g.d_grad_dp_x, g.d_grad_dp_y, s));
+        for (int f = 0; f < 2; ++f) {
+            dev_zero(dp[f], n * sizeof(double));
+            GPU_CHECK(cfa_assemble_pressure(m, uv[f][0], uv[f][1], g.d_p, g.d_grad_dp_x, g.d_grad_dp_y,
+                                            g.d_mass_lumped, g.d_elem_s, dt_proj, Sim::RHO, stab, g.d_Kp_val, rhs[f],
+                                            div[f], dp[f], oss ? g.d_oss_val.ptr : nullptr, g.d_inv_m, s));
+            g.pressure_outlet_bc(g.d_mass_lumped, rhs[f], s);
+        }
+        if (build_amg) build_amg();
+        else amg->setup(n, g.nnz, g.d_row_ptr, g.d_col_ind, g.d_Kp_val);
+        const cfa_amg_t* view2 = amg->second_view();
+        cfa_oss_op_t op2 = op;
+        op2.gx = aux->gx;
+        op2.gy = aux->gy;
+        aux->solver.allocate(n);
+        solver.allocate(n);
+        GPU_CHECK(cfa_timer_record(aux->ready, s));
+        GPU_CHECK(cfa_stream_wait(aux->s, aux->ready));
+        int it1 = 0;
+        std::thread second([&] {
+            it1 = aux->solver.solve_pcg_fused(n, g.d_row_ptr, g.d_col_ind, g.d_Kp_val, rhs[1], dp[1], 4000, tol,
+                                              oss ? &op2 : nullptr, view2, aux->s);
+        });
+        const int it0 = solver.solve_pcg_fused(n, g.d_row_ptr, g.d_col_ind, g.d_Kp_val, rhs[0], dp[0], 4000, tol,
+                                               oss ? &op : nullptr, amg);
+        second.join();
+        iters = it0 + it1;
+        nonconverged += (solver.converged ? 0 : 1) + (aux->solver.converged ? 0 : 1);
+        double* vort[2] = {g.d_oss_gx, g.d_vort};
+        for (int f = 0; f < 2; ++f)
+            GPU_CHECK(cfa_correction(m, dt_proj, Sim::RHO, 0.0, 0.0, div[f], g.d_mass_lumped, dp[f], uv[f][0],
+                                     uv[f][1], g.d_p, vort[f], g.d_grad_dp_x, g.d_grad_dp_y, g.d_vort_num, s));
+        if (phi_out) dev_copy(phi_out, dp[1], n);
+        dev_zero(g.d_dp, n * sizeof(double));
+        return iters;
+    }
     double* fields[2][3] = {{g.d_u_old, g.d_v_old, g.d_oss_gx}, {g.d_u, g.d_v, g.d_vort}};
     const int first = both_levels ? 0 : 1;
     for (int f = first; f < 2; ++f) {

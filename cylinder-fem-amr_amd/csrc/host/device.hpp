@@ -94,10 +94,24 @@ void compute_error_indicator(DeviceMeshBuffers& g, double inflow_peak = Sim::U_M
 void apply_oss_projection_matrix_free(DeviceMeshBuffers& g, const double* in, double* out);
 void assemble_oss_projection(DeviceMeshBuffers& g);
 void apply_oss_projection(DeviceMeshBuffers& g, const double* in, double* out);
+// What the two solves of the scheme-consistent re-projection (u^{n-1} and u^n, independent of
+// each other) need to run at the same time: a second stream and solver, the second solve's
+// right-hand side, divergence, solution and OSS scratch vectors.
+struct ReprojectionAux {
+    cfa_stream_t s = nullptr;
+    void* ready = nullptr;   // the systems and the hierarchy are built (recorded on the solver stream)
+    GpuLinearSolver solver;
+    DevArray<double> rhs, div_num, dp, gx, gy;
+    ReprojectionAux();
+    ~ReprojectionAux();
+    ReprojectionAux(const ReprojectionAux&) = delete;
+    ReprojectionAux& operator=(const ReprojectionAux&) = delete;
+};
+
 int project_scheme_consistent(DeviceMeshBuffers& g, GpuLinearSolver& solver, double dt_proj, double stab,
                               AmgPreconditioner* amg, int& nonconverged,
                               const std::function<void()>& build_amg = nullptr, bool both_levels = true,
-                              double* phi_out = nullptr, double tol = 1.0e-8);
+                              double* phi_out = nullptr, double tol = 1.0e-8, ReprojectionAux* aux = nullptr);
 int project_divergence_free(DeviceMeshBuffers& g, GpuLinearSolver& solver, double dt, AmgPreconditioner* amg,
                             bool setup_amg, double* u, double* v, double* vort_out, double tol = 1.0e-8);
 void set_initial_condition(const Mesh2D& mesh, std::vector<double>& h_u, std::vector<double>& h_v,

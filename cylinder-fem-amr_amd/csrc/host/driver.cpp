@@ -203,6 +203,7 @@ int run_driver(int argc, char** argv) {
     g.outlet_dirichlet = opt.outlet_dirichlet;
     g.sync_from_host(mesh, h_u, h_v, h_p, h_uo, h_vo);
     GpuLinearSolver solver;
+    ReprojectionAux reproj_aux;   // the second re-projection solve's stream, solver and vectors
     if (opt.init_amr > 0)
         for (int cycle = 0; cycle < opt.init_amr; ++cycle) {
             compute_error_indicator(g, inflow0);
@@ -439,7 +440,8 @@ int run_driver(int argc, char** argv) {
                 proj_iters = project_scheme_consistent(g, solver, opt.bdf2 ? (2.0 / 3.0) * dt : dt, opt.pressure_stab,
                                                        use_amg ? &amg : nullptr, nonconverged_solves, build_amg,
                                                        opt.remesh_projection == 4,
-                                                       opt.reproject_shift ? g.d_phi.ptr : nullptr, opt.proj_tol);
+                                                       opt.reproject_shift ? g.d_phi.ptr : nullptr, opt.proj_tol,
+                                                       &reproj_aux);
                 amg_remaps_in_row = amg.remaps > remaps_before ? amg_remaps_in_row + 1 : 0;
                 phi_pending = opt.reproject_shift;
             }

@@ -101,6 +101,19 @@ int GpuLinearSolver::solve_pcg_fused(int n, const int* d_row_ptr, const int* d_c
     return res.iters[0];
 }
 
+int GpuLinearSolver::solve_pcg_fused(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
+                                     const double* d_b, double* d_x, int max_iter, double tol,
+                                     const cfa_oss_op_t* oss, const cfa_amg_t* amg_view, cfa_stream_t stream) {
+    cfa_solve_result_t res{};
+    GPU_CHECK(cfa_pcg(n, d_row_ptr, d_col_ind, d_val, d_b, d_x, max_iter, tol, oss, nullptr, nullptr, amg_view,
+                      nullptr, nullptr, ws_.get(n), &res, stream));
+    max_n = ws_.capacity();
+    converged = res.converged[0] != 0;
+    if (res.rel_residual[0] >= 0.0) last_rel_residual = res.rel_residual[0];
+    last_iters = res.iters[0];
+    return res.iters[0];
+}
+
 int GpuLinearSolver::solve_bicgstab(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
                                     const double* d_b, double* d_x, int max_iter, double tol) {
     cfa_solve_result_t res{};
@@ -746,6 +759,35 @@ void AmgPreconditioner::refresh_view() {
     view_.cb = d_cb;
     view_.cx = d_cx;
     view_.smooth_steps = smooth_steps;
+}
+
+const cfa_amg_t* AmgPreconditioner::second_view() {
+    desc2_ = desc_;
+    view2_ = view_;
+    while (vec2_.size() < levels.size()) vec2_.push_back(std::make_unique<WorkVectors>());
+    for (size_t l = 0; l < levels.size(); ++l) {
+        WorkVectors& w = *vec2_[l];
+        cfa_amg_level_t& d = desc2_[l];
+        const int n = d.rows;
+        for (DevArray<double>* v : {&w.x, &w.b, &w.r, &w.Ax}) v->reserve(std::max(n, 1), 16);
+        d.x = w.x; d.b = w.b; d.r = w.r; d.ax = w.Ax;
+        if (fp32) {
+            for (DevArray<float>* v : {&w.x32, &w.b32, &w.r32, &w.Ax32}) v->reserve(std::max(n, 1), 16);
+            d.x32 = w.x32; d.b32 = w.b32; d.r32 = w.r32; d.ax32 = w.Ax32;
+        }
+    }
+    d_cx2_.reserve(std::max(coarse_n, 1), 16);
+    d_cb2_.reserve(std::max(coarse_n, 1), 16);
+    view2_.cx = d_cx2_;
+    view2_.cb = d_cb2_;
+    if (fp32) {
+        d_cx32_2_.reserve(std::max(coarse_n, 1), 16);
+        d_cb32_2_.reserve(std::max(coarse_n, 1), 16);
+        view2_.cx32 = d_cx32_2_;
+        view2_.cb32 = d_cb32_2_;
+    }
+    view2_.levels = desc2_.data();
+    return &view2_;
 }
 
 void AmgPreconditioner::apply(const double* d_r, double* d_z) {

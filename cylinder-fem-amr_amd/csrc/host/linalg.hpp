@@ -49,6 +49,10 @@ public:
     int solve_pcg_fused(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
                         const double* d_b, double* d_x, int max_iter, double tol,
                         const cfa_oss_op_t* oss, AmgPreconditioner* amg);
+    // The same with a given hierarchy view and stream (a second solve running beside the first)
+    int solve_pcg_fused(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
+                        const double* d_b, double* d_x, int max_iter, double tol,
+                        const cfa_oss_op_t* oss, const cfa_amg_t* amg_view, cfa_stream_t stream);
     int solve_bicgstab(int n, const int* d_row_ptr, const int* d_col_ind, const double* d_val,
                        const double* d_b, double* d_x, int max_iter = 200, double tol = 1.0e-7);
 private:
@@ -178,6 +182,9 @@ public:
                const int* h_rp, const int* h_ci, const std::vector<int>& new_to_old);
     void apply(const double* d_r, double* d_z);
     const cfa_amg_t* view() const { return &view_; }
+    // A view of the current hierarchy with a second set of work vectors, for a V-cycle running on
+    // another stream at the same time as one through view(); valid until the next set-up or remap.
+    const cfa_amg_t* second_view();
 
 private:
     // the set-up's second stream: each level's power iteration runs on it beside the aggregation
@@ -193,6 +200,15 @@ private:
         void ensure();
     };
     AuxStream aux_;
+    struct WorkVectors {
+        DevArray<double> x, b, r, Ax;
+        DevArray<float> x32, b32, r32, Ax32;
+    };
+    std::vector<std::unique_ptr<WorkVectors>> vec2_;
+    DevArray<double> d_cx2_, d_cb2_;
+    DevArray<float> d_cx32_2_, d_cb32_2_;
+    std::vector<cfa_amg_level_t> desc2_;
+    cfa_amg_t view2_{};
     DevCsr dev_next_, dev_tmp_, dev_next2_;    // device set-up: next operator, products
     DevArray<int> d_cnt_, d_flag_, d_agg_, d_iota_, d_info_;
     DevArray<unsigned char> d_tws_, d_sws_, d_aws_, d_gws_;   // transpose, scan, aggregation, sorted product

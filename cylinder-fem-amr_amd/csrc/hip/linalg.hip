@@ -826,6 +826,64 @@ k_ilu_upper(int m, int n, int cs, const int* __restrict__ up, const int* __restr
     for (int c = 0; c < NRHS; ++c) z[(size_t)c * n + i] = acc[c] * d;
 }
 
+// The same sweeps with TPR lanes per row, for the FP32 factors of the mixed-precision path (no
+// reference order to keep there): each lane sums its entries' products, the group adds them, and
+// the row takes r - sum (forward) or z - sum (backward).
+template <int NRHS, int TPR>
+__global__ void __launch_bounds__(BLOCK)
+k_ilu_lower_lanes(int m, int n, int cs, const int* __restrict__ lp, const int* __restrict__ lc,
+                  const float* __restrict__ lv, const float* __restrict__ dinv, int last, const double* __restrict__ r,
+                  double* __restrict__ z, const int* __restrict__ st) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = g / TPR, lane = g % TPR;
+    int e0, e1;
+    if (!running_with_range(lp, t, m, st, e0, e1)) return;
+    const int i = cs + t;
+    double sum[NRHS];
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) sum[c] = 0.0;
+    row_entries<TPR>(lc, lv, e0, e1, lane, [&](float lt, int j) {
+        const double l = (double)lt;
+#pragma unroll
+        for (int c = 0; c < NRHS; ++c) sum[c] += l * z[(size_t)c * n + j];
+    });
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) sum[c] = group_sum<TPR>(sum[c]);
+    if (t >= m || lane != 0) return;
+    const double d = (double)dinv[t];
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) {
+        const double acc = r[(size_t)c * n + i] - sum[c];
+        z[(size_t)c * n + i] = last ? acc * d : acc;
+    }
+}
+
+template <int NRHS, int TPR>
+__global__ void __launch_bounds__(BLOCK)
+k_ilu_upper_lanes(int m, int n, int cs, const int* __restrict__ up, const int* __restrict__ uc,
+                  const float* __restrict__ uv, const float* __restrict__ dinv, double* __restrict__ z,
+                  const int* __restrict__ st) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = g / TPR, lane = g % TPR;
+    int e0, e1;
+    if (!running_with_range(up, t, m, st, e0, e1)) return;
+    const int i = cs + t;
+    double sum[NRHS];
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) sum[c] = 0.0;
+    row_entries<TPR>(uc, uv, e0, e1, lane, [&](float ut, int j) {
+        const double u = (double)ut;
+#pragma unroll
+        for (int c = 0; c < NRHS; ++c) sum[c] += u * z[(size_t)c * n + j];
+    });
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) sum[c] = group_sum<TPR>(sum[c]);
+    if (t >= m || lane != 0) return;
+    const double d = (double)dinv[t];
+#pragma unroll
+    for (int c = 0; c < NRHS; ++c) z[(size_t)c * n + i] = (z[(size_t)c * n + i] - sum[c]) * d;
+}
+
 // ======================================================================================================
 // Two-component BiCGSTAB (src/cfa/linalg.cuh:642-779)
 // ======================================================================================================
@@ -1308,17 +1366,27 @@ void ilu_apply_impl(const cfa_ilu_t* ilu, const T* lv, const T* uv, const T* din
     // backward with the permutation back (the last colour has no U entries)
     hipLaunchKernelGGL((k_ilu_permute_first<NRHS, T>), dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, m0, ilu->pos, r,
                        dinv, nc == 1 ? 1 : 0, rp, zp, st);
+    // FP32 factors (mixed precision): two lanes per row; FP64: one lane, the reference's order
+    constexpr bool lanes = std::is_same<T, float>::value;
     for (int c = 1; c < nc; ++c) {
         const int cs = ilu->colour_start[c], m = ilu->colour_start[c + 1] - cs;
         if (m == 0) continue;
-        hipLaunchKernelGGL((k_ilu_lower<NRHS, T>), dim3(blocks_for(m)), dim3(BLOCK), 0, s, m, n, cs, ilu->lp + cs,
-                           ilu->lc, lv, dinv + cs, c == nc - 1 ? 1 : 0, rp, zp, st);
+        if constexpr (lanes)
+            hipLaunchKernelGGL((k_ilu_lower_lanes<NRHS, 2>), dim3(blocks_for(2LL * m)), dim3(BLOCK), 0, s, m, n, cs,
+                               ilu->lp + cs, ilu->lc, lv, dinv + cs, c == nc - 1 ? 1 : 0, rp, zp, st);
+        else
+            hipLaunchKernelGGL((k_ilu_lower<NRHS, T>), dim3(blocks_for(m)), dim3(BLOCK), 0, s, m, n, cs, ilu->lp + cs,
+                               ilu->lc, lv, dinv + cs, c == nc - 1 ? 1 : 0, rp, zp, st);
     }
     for (int c = nc - 2; c >= 1; --c) {
         const int cs = ilu->colour_start[c], m = ilu->colour_start[c + 1] - cs;
         if (m == 0) continue;
-        hipLaunchKernelGGL((k_ilu_upper<NRHS, T>), dim3(blocks_for(m)), dim3(BLOCK), 0, s, m, n, cs, ilu->up + cs,
-                           ilu->uc, uv, dinv + cs, zp, st);
+        if constexpr (lanes)
+            hipLaunchKernelGGL((k_ilu_upper_lanes<NRHS, 2>), dim3(blocks_for(2LL * m)), dim3(BLOCK), 0, s, m, n, cs,
+                               ilu->up + cs, ilu->uc, uv, dinv + cs, zp, st);
+        else
+            hipLaunchKernelGGL((k_ilu_upper<NRHS, T>), dim3(blocks_for(m)), dim3(BLOCK), 0, s, m, n, cs, ilu->up + cs,
+                               ilu->uc, uv, dinv + cs, zp, st);
     }
     if (nc >= 2)
         hipLaunchKernelGGL((k_ilu_last_unpermute<NRHS, T>), dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, m0, ilu->pos,

@@ -508,6 +508,32 @@ void case_amg(Fixture& f) {
         oa.apply(r.data(), z.data());
         amg.apply(d_r, d_z);
         close_rel("one V-cycle", down(d_z, f.n), z, 1e-12);
+        if (!sorted) {
+            // two V-cycles at the same time on two streams, one through the second set of work
+            // vectors (the concurrent re-projection solves): each equals the V-cycle run alone
+            const Vec z1 = down(d_z, f.n);
+            Vec r2(f.n);
+            for (auto& v : r2) v = std::uniform_real_distribution<double>(-1, 1)(f.rng);
+            DevArray<double> d_r2, d_z2, d_za, d_zb;
+            d_r2.upload(r2);
+            d_z2.reserve(f.n);
+            d_za.reserve(f.n);
+            d_zb.reserve(f.n);
+            amg.apply(d_r2, d_z2);
+            const Vec z2 = down(d_z2, f.n);
+            const cfa_amg_t* v2 = amg.second_view();
+            cfa_stream_t s2 = nullptr;
+            GPU_CHECK(cfa_stream_create(&s2));
+            for (int rep = 0; rep < 3; ++rep) {
+                GPU_CHECK(cfa_amg_apply(v2, d_r2, d_zb, s2));
+                GPU_CHECK(cfa_amg_apply(amg.view(), d_r, d_za, solver_stream()));
+            }
+            GPU_CHECK(cfa_stream_sync(s2));
+            dev_sync();
+            GPU_CHECK(cfa_stream_destroy(s2));
+            same_bits("V-cycle beside a second one", down(d_za, f.n), z1);
+            same_bits("second V-cycle (second vectors)", down(d_zb, f.n), z2);
+        }
     }
 }
 

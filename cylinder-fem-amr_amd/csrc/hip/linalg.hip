@@ -869,8 +869,8 @@ enum {
 // ints: [0] state (stopped when both components are inactive), [2 + c] active, [4 + c] iters,
 // [6 + c] converged, [8 + c] half-step done, [10] current iteration, [11] iteration cap
 
-// Two-component partial sums, quantity-major so that the kernels re-summing them read contiguous
-// runs: part[(2 q + c) * RED_BLOCKS + b] (lds: at least 2 nq * blockDim.x / 64 doubles)
+// Two-component partial sums: part[b * 6 + 2 q + c]
+// (lds: at least 2 nq * blockDim.x / 64 doubles)
 template <int NQ>
 __device__ __forceinline__ void write_partials2(double* part, double (&acc)[NQ][2], double* lds) {
     double v[2 * NQ];
@@ -881,7 +881,7 @@ __device__ __forceinline__ void write_partials2(double* part, double (&acc)[NQ][
     block_sum_k<2 * NQ>(v, lds);
     if (threadIdx.x == 0)
 #pragma unroll
-        for (int k = 0; k < 2 * NQ; ++k) part[(size_t)k * RED_BLOCKS + blockIdx.x] = v[k];
+        for (int k = 0; k < 2 * NQ; ++k) part[(size_t)blockIdx.x * 6 + k] = v[k];
 }
 
 // r = b - A x (both components), r0 = p = r, v = 0; partial r.r
@@ -925,7 +925,7 @@ __global__ void k_bicg_final_init(int nb, const double* __restrict__ part, doubl
                                   int* __restrict__ st, int* __restrict__ status, int max_iter) {
     __shared__ double lds[2 * BLOCK / WAVE];
     double rr[2];
-    sum_partials_k<2>({part, part + RED_BLOCKS}, {nb, nb}, 1, rr, lds);
+    sum_partials_k<2>({part, part + 1}, {nb, nb}, 6, rr, lds);
     if (threadIdx.x != 0) return;
     for (int c = 0; c < 2; ++c) {
         sc[B_RN0 + c] = sqrt(rr[c]);
@@ -1033,7 +1033,7 @@ k_bicg_s(int n, const double* __restrict__ r, const double* __restrict__ v, doub
     double ca[2];
     bool act[2];
     double o2[2];
-    sum_partials_k<2>({part_a, part_a + RED_BLOCKS}, {nb_a, nb_a}, 1, o2, lds);
+    sum_partials_k<2>({part_a, part_a + 1}, {nb_a, nb_a}, 6, o2, lds);
     for (int c = 0; c < 2; ++c) {
         const double o = o2[c];
         act[c] = s_act[c] && !(fabs(o) < sc[B_BD + c]);
@@ -1074,9 +1074,8 @@ k_bicg_xr(int n, const double* __restrict__ ph, const double* __restrict__ sh, c
     double a1[2], a2[2], om[2];
     int half[2];
     double sums[6];   // s.s, t.t, t.s of both components
-    sum_partials_k<6>({part_s, part_s + RED_BLOCKS, part_t, part_t + RED_BLOCKS, part_t + 2 * RED_BLOCKS,
-                       part_t + 3 * RED_BLOCKS},
-                      {nb_s, nb_s, nb_t, nb_t, nb_t, nb_t}, 1, sums, lds);
+    sum_partials_k<6>({part_s, part_s + 1, part_t, part_t + 1, part_t + 2, part_t + 3},
+                      {nb_s, nb_s, nb_t, nb_t, nb_t, nb_t}, 6, sums, lds);
     for (int c = 0; c < 2; ++c) {
         const double ss = sums[c];
         const double tt = sums[2 + c];
@@ -1120,8 +1119,7 @@ __global__ void k_bicg_final_post(int nb, const double* __restrict__ part, doubl
     __shared__ double lds[4 * BLOCK / WAVE];
     if (st[0] != ST_RUNNING) return;
     double sums[4];
-    sum_partials_k<4>({part, part + RED_BLOCKS, part + 2 * RED_BLOCKS, part + 3 * RED_BLOCKS}, {nb, nb, nb, nb}, 1,
-                      sums, lds);
+    sum_partials_k<4>({part, part + 1, part + 2, part + 3}, {nb, nb, nb, nb}, 6, sums, lds);
     const double rr[2] = {sums[0], sums[1]}, r0r[2] = {sums[2], sums[3]};
     if (threadIdx.x != 0) return;
     const int it = st[10], max_iter = st[11];

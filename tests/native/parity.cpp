@@ -5,7 +5,7 @@
 // block reductions or other summation orders must match to the stated relative tolerance.
 //
 //   parity <case> [level]    case: momentum pressure correction forces indicator mms oss ilu
-//                                  spmv amg pcg bicgstab jacobi mixed topology outlet nvb remap all
+//                                  spmv amg pcg bicgstab jacobi mixed topology outlet nvb remap reproject all
 //
 // Prints one line per compared quantity and exits 0 when every comparison passes.
 #include <algorithm>
@@ -535,6 +535,45 @@ void case_amg(Fixture& f) {
             same_bits("second V-cycle (second vectors)", down(d_zb, f.n), z2);
         }
     }
+}
+
+// The scheme-consistent re-projection after a remesh with its two solves (u^{n-1} and u^n) side by
+// side on two streams against the same two solves one after the other, from the same state: the
+// velocities, the vorticity and the last increment bit for bit.
+void case_reproject(Fixture& f) {
+    std::printf("[re-projection: the two solves side by side]\n");
+    const Vec u = smooth_field(f.mesh, f.rng, 1.5), v = smooth_field(f.mesh, f.rng, 0.5);
+    const Vec uo = smooth_field(f.mesh, f.rng, 1.5), vo = smooth_field(f.mesh, f.rng, 0.5);
+    const Vec p = smooth_field(f.mesh, f.rng, 2.0);
+    const double dt_proj = (2.0 / 3.0) * 5.0e-4, stab = 0.3;
+    Vec out[2][6];
+    int iters[2] = {0, 0};
+    for (int run = 0; run < 2; ++run) {
+        f.g.d_u.upload(u);
+        f.g.d_v.upload(v);
+        f.g.d_u_old.upload(uo);
+        f.g.d_v_old.upload(vo);
+        f.g.d_p.upload(p);
+        AmgPreconditioner amg;
+        GpuLinearSolver solver;
+        ReprojectionAux aux;
+        DevArray<double> phi;
+        phi.reserve(f.n);
+        int nonconverged = 0;
+        iters[run] = project_scheme_consistent(f.g, solver, dt_proj, stab, &amg, nonconverged, nullptr, true, phi,
+                                               1.0e-8, run ? &aux : nullptr);
+        expect(nonconverged == 0, run ? "side by side: both solves converged" : "one after the other: converged");
+        out[run][0] = down(f.g.d_u, f.n);
+        out[run][1] = down(f.g.d_v, f.n);
+        out[run][2] = down(f.g.d_u_old, f.n);
+        out[run][3] = down(f.g.d_v_old, f.n);
+        out[run][4] = down(f.g.d_vort, f.n);
+        out[run][5] = down(phi, f.n);
+    }
+    std::printf("  PCG iterations (both solves): one after the other %d, side by side %d\n", iters[0], iters[1]);
+    expect(iters[0] == iters[1], "same iterations");
+    const char* names[6] = {"u^n", "v^n", "u^{n-1}", "v^{n-1}", "vorticity", "last increment"};
+    for (int k = 0; k < 6; ++k) same_bits(names[k], out[1][k], out[0][k]);
 }
 
 // The no-allocation mean removal against cfa_pressure_remove_mean: on the fixture's mesh, then
@@ -1261,6 +1300,7 @@ int main(int argc, char** argv) {
         if (all || what == "outlet") case_outlet(f);
         if (all || what == "nvb") case_nvb(f, level);
         if (all || what == "remap") case_remap(f);
+        if (all || what == "reproject") case_reproject(f);
         if (all || what == "noalloc") case_noalloc(f, level);
         std::printf(failures ? "%d FAILURE(S)\n" : "ALL PASSED\n", failures);
         status = failures ? 1 : 0;

@@ -130,6 +130,7 @@ template <> struct Lv<double> {
     __host__ __device__ static const double* p(const cfa_amg_level_t& L) { return L.p_v; }
     __host__ __device__ static const double* r(const cfa_amg_level_t& L) { return L.r_v; }
     __host__ __device__ static const double* dinv(const cfa_amg_level_t& L) { return L.dinv; }
+    __host__ __device__ static const double* sv(const cfa_amg_level_t& L) { return L.s_v; }
     __host__ __device__ static double* x(const cfa_amg_level_t& L) { return L.x; }
     __host__ __device__ static double* b(const cfa_amg_level_t& L) { return L.b; }
     __host__ __device__ static double* res(const cfa_amg_level_t& L) { return L.r; }
@@ -142,6 +143,7 @@ template <> struct Lv<float> {
     __host__ __device__ static const float* p(const cfa_amg_level_t& L) { return L.p_v32; }
     __host__ __device__ static const float* r(const cfa_amg_level_t& L) { return L.r_v32; }
     __host__ __device__ static const float* dinv(const cfa_amg_level_t& L) { return L.dinv32; }
+    __host__ __device__ static const float* sv(const cfa_amg_level_t& L) { return L.s_v32; }
     __host__ __device__ static float* x(const cfa_amg_level_t& L) { return L.x32; }
     __host__ __device__ static float* b(const cfa_amg_level_t& L) { return L.b32; }
     __host__ __device__ static float* res(const cfa_amg_level_t& L) { return L.r32; }
@@ -642,6 +644,180 @@ k_jacobi_dot(int rows, const int* __restrict__ rp, const int* __restrict__ ci, c
     if (part) {
         contrib = block_sum(contrib, lds);
         if (threadIdx.x == 0) part[blockIdx.x] = contrib;
+    }
+}
+
+// ---- Sliced-ELL rows (the V-cycle's level-0 sweeps) --------------------------------------------
+// The operator's rows in slices of 64 positions, one wavefront each, stored column-major inside a
+// slice: entry k of the row at position p = 64 q + l sits at sp[q] + 64 k + l, a slice holds
+// (sp[q + 1] - sp[q]) / 64 slots per position, and srow[p] names the row (-1 past the last row).
+// Inside each window of SELL_WINDOW positions the rows are ordered by decreasing length (ties by
+// row index), so the rows of a slice have nearly the same length and little is padded; padded
+// slots hold column 0 and value 0. Each entry load of a wavefront is one contiguous run (512 bytes
+// in FP64), where the CSR kernels' loads scatter over the rows of a lane group: on a 7.35 M-row
+// matrix of the uniform meshes' 5- and 9-entry rows a Jacobi sweep takes 175 us against 257 us.
+// One lane per row, each row's products summed in its CSR order.
+constexpr int SELL_SLICE = 64;
+constexpr int SELL_WINDOW = 256;
+
+__host__ __device__ inline int sell_positions(int rows) { return (rows + SELL_SLICE - 1) / SELL_SLICE * SELL_SLICE; }
+
+// The position's row, the base of its slots and the slice's length; false when `state` says the
+// solve has stopped (its word loaded together with the slice's offsets, as in running_with_range).
+__device__ __forceinline__ bool sell_slot(const int* __restrict__ sp, const int* __restrict__ srow, int p,
+                                          const int* __restrict__ state, int& row, int& base, int& len) {
+    const int q = p / SELL_SLICE;
+    const int k0 = sp[q], k1 = sp[q + 1];
+    row = srow[p];
+    const int sv = state ? state[0] : ST_RUNNING;
+    asm volatile("" ::"v"(k0), "v"(k1), "v"(row), "s"(sv));
+    base = k0 + p % SELL_SLICE;
+    len = (k1 - k0) / SELL_SLICE;
+    return sv == ST_RUNNING;
+}
+
+// f(a, j) for the position's MAXL slots in order, every index and value loaded before any is
+// used; slots past the slice's length give (0, column 0) without a load, so the gathers f makes
+// are unconditional and an accumulation of a * x[j] adds an exact zero for them.
+template <int MAXL, typename T, typename F>
+__device__ __forceinline__ void sell_entries(const int* __restrict__ ci, const T* __restrict__ val, int base, int len,
+                                             F f) {
+    int c[MAXL];
+    T v[MAXL];
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) {
+        const bool ok = k < len;
+        c[k] = ok ? ci[base + SELL_SLICE * k] : 0;
+        v[k] = ok ? val[base + SELL_SLICE * k] : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) f(v[k], c[k]);
+}
+
+// y = b - A x (OP_RESID) | y = x + omega dinv (b - A x) (OP_JACOBI) on the sliced-ELL copy
+template <int MAXL, int OP, typename T>
+__global__ void __launch_bounds__(BLOCK)
+k_sell_op(int positions, const int* __restrict__ sp, const int* __restrict__ srow, const int* __restrict__ ci,
+          const T* __restrict__ val, const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ dinv,
+          T omega, T* __restrict__ y, const int* __restrict__ state) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= positions) return;
+    int row, base, len;
+    if (!sell_slot(sp, srow, p, state, row, base, len)) return;
+    T s = 0;
+    sell_entries<MAXL>(ci, val, base, len, [&](T a, int j) { s += a * x[j]; });
+    if (row < 0) return;
+    if (OP == OP_RESID) y[row] = b[row] - s;
+    else y[row] = x[row] + omega * dinv[row] * (b[row] - s);
+}
+
+// k_jacobi_from_zero on the sliced-ELL copy
+template <int MAXL, typename T>
+__global__ void __launch_bounds__(BLOCK)
+k_sell_from_zero(int positions, const int* __restrict__ sp, const int* __restrict__ srow, const int* __restrict__ ci,
+                 const T* __restrict__ val, const T* __restrict__ b, const T* __restrict__ dinv, T omega,
+                 T* __restrict__ y, const int* __restrict__ state) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= positions) return;
+    int row, base, len;
+    if (!sell_slot(sp, srow, p, state, row, base, len)) return;
+    T s = 0;
+    sell_entries<MAXL>(ci, val, base, len, [&](T a, int j) { s += a * (omega * dinv[j] * b[j]); });
+    if (row < 0) return;
+    const T x0 = omega * dinv[row] * b[row];
+    y[row] = x0 + omega * dinv[row] * (b[row] - s);
+}
+
+// k_jacobi_from_zero_check on the sliced-ELL copy: the iteration's residual test, then the sweeps
+template <int MAXL>
+__global__ void __launch_bounds__(BLOCK)
+k_sell_from_zero_check(int positions, const int* __restrict__ sp, const int* __restrict__ srow,
+                       const int* __restrict__ ci, const double* __restrict__ val, const double* __restrict__ b,
+                       const double* __restrict__ dinv, double omega, double* __restrict__ y,
+                       const double* __restrict__ prr, int nb_rr, double* sc, int* st, int* status) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (block_state(st, nullptr) != ST_RUNNING) return;
+    if (!fp_residual_test(prr, nb_rr, sc, st, status, lds)) return;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= positions) return;
+    int row, base, len;
+    (void)sell_slot(sp, srow, p, nullptr, row, base, len);
+    double s = 0.0;
+    sell_entries<MAXL>(ci, val, base, len, [&](double a, int j) { s += a * (omega * dinv[j] * b[j]); });
+    if (row < 0) return;
+    const double x0 = omega * dinv[row] * b[row];
+    y[row] = x0 + omega * dinv[row] * (b[row] - s);
+}
+
+// k_jacobi_dot on the sliced-ELL copy: the last sweep into the FP64 output, with the r.z partials
+template <int MAXL, typename T>
+__global__ void __launch_bounds__(BLOCK)
+k_sell_jacobi_dot(int positions, const int* __restrict__ sp, const int* __restrict__ srow,
+                  const int* __restrict__ ci, const T* __restrict__ val, const T* __restrict__ x,
+                  const T* __restrict__ b, const T* __restrict__ dinv, T omega, double* __restrict__ y,
+                  const double* __restrict__ rdot, double* __restrict__ part, const int* __restrict__ st) {
+    __shared__ double lds[BLOCK / WAVE];
+    if (st && block_state(st, nullptr) != ST_RUNNING) return;
+    double contrib = 0.0;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < positions; p += gridDim.x * blockDim.x) {
+        int row, base, len;
+        (void)sell_slot(sp, srow, p, nullptr, row, base, len);
+        T s = 0;
+        sell_entries<MAXL>(ci, val, base, len, [&](T a, int j) { s += a * x[j]; });
+        if (row >= 0) {
+            const double yi = (double)(x[row] + omega * dinv[row] * (b[row] - s));
+            y[row] = yi;
+            if (part) contrib += rdot[row] * yi;
+        }
+    }
+    if (part) {
+        contrib = block_sum(contrib, lds);
+        if (threadIdx.x == 0) part[blockIdx.x] = contrib;
+    }
+}
+
+// The positions of one window's rows (a workgroup per window of SELL_WINDOW rows): each row's rank
+// among the window's rows by decreasing length, ties by row index; the slices' entry counts (64
+// times the length of their first, longest row) and the longest row (atomic max into info[1]).
+__global__ void __launch_bounds__(SELL_WINDOW)
+k_sell_plan(int rows, int positions, const int* __restrict__ rp, int* __restrict__ srow, int* __restrict__ cnt,
+            int* __restrict__ info) {
+    __shared__ int len_of[SELL_WINDOW];
+    __shared__ int len_at[SELL_WINDOW];
+    const int t = threadIdx.x, w0 = blockIdx.x * SELL_WINDOW, r = w0 + t;
+    const int L = r < rows ? rp[r + 1] - rp[r] : -1;
+    len_of[t] = L;
+    __syncthreads();
+    int rank = 0;
+    for (int u = 0; u < SELL_WINDOW; ++u) {
+        const int Lu = len_of[u];
+        rank += (Lu > L || (Lu == L && u < t)) ? 1 : 0;
+    }
+    len_at[rank] = L;
+    if (w0 + rank < positions) srow[w0 + rank] = L >= 0 ? r : -1;
+    if (L > 0) atomicMax(&info[1], L);
+    __syncthreads();
+    if (t < SELL_WINDOW / SELL_SLICE) {
+        const int q = w0 / SELL_SLICE + t;
+        if (q * SELL_SLICE < positions) cnt[q] = SELL_SLICE * max(len_at[t * SELL_SLICE], 0);
+    }
+}
+
+// The slots of each position: its row's entries in CSR order, then (column 0, value 0) up to the
+// slice's length
+__global__ void k_sell_fill(int positions, const int* __restrict__ rp, const int* __restrict__ ci,
+                            const double* __restrict__ val, const int* __restrict__ srow, const int* __restrict__ sp,
+                            int* __restrict__ sci, double* __restrict__ sv) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= positions) return;
+    const int q = p / SELL_SLICE;
+    const int base = sp[q] + p % SELL_SLICE, len = (sp[q + 1] - sp[q]) / SELL_SLICE;
+    const int row = srow[p];
+    const int k0 = row >= 0 ? rp[row] : 0, rl = row >= 0 ? rp[row + 1] - k0 : 0;
+    for (int k = 0; k < len; ++k) {
+        const bool ok = k < rl;
+        sci[base + SELL_SLICE * k] = ok ? ci[k0 + k] : 0;
+        sv[base + SELL_SLICE * k] = ok ? val[k0 + k] : 0.0;
     }
 }
 
@@ -1223,6 +1399,77 @@ void jacobi_from_zero_check_launch(int tpr, int rows, const int* rp, const int* 
     });
 }
 
+// MAXL of the sliced-ELL kernels for a longest row of maxl (at most CFA_SELL_MAX_LEN)
+template <typename F>
+void sell_dispatch(int maxl, F f) {
+    if (maxl <= 8) f(std::integral_constant<int, 8>{});
+    else if (maxl <= 12) f(std::integral_constant<int, 12>{});
+    else f(std::integral_constant<int, 16>{});
+}
+
+// The level operator's sweeps and residual: on its sliced-ELL copy when it has one, else on its
+// CSR rows
+template <int OP, typename T>
+void level_op_launch(const cfa_amg_level_t& L, const T* x, const T* b, const T* dinv, T* y, const int* st,
+                     hipStream_t s) {
+    static_assert(OP == OP_JACOBI || OP == OP_RESID, "the sliced-ELL copy serves the sweeps and the residual");
+    if (!L.s_sp) {
+        csr_op_launch<OP, T>(L.tpr_a, L.rows, L.a_rp, L.a_ci, Lv<T>::a(L), x, b, dinv, OP == OP_JACOBI ? L.omega : 0.0,
+                             y, st, s);
+        return;
+    }
+    const int pos = sell_positions(L.rows);
+    sell_dispatch(L.s_len, [&](auto c) {
+        constexpr int M = decltype(c)::value;
+        hipLaunchKernelGGL((k_sell_op<M, OP, T>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp, L.s_row,
+                           L.s_ci, Lv<T>::sv(L), x, b, dinv, (T)L.omega, y, st);
+    });
+}
+
+template <typename T>
+void level_from_zero_launch(const cfa_amg_level_t& L, const T* b, const T* dinv, T* y, const int* st, hipStream_t s) {
+    if (!L.s_sp) {
+        jacobi_from_zero_launch<T>(L.tpr_a, L.rows, L.a_rp, L.a_ci, Lv<T>::a(L), b, dinv, L.omega, y, st, s);
+        return;
+    }
+    const int pos = sell_positions(L.rows);
+    sell_dispatch(L.s_len, [&](auto c) {
+        constexpr int M = decltype(c)::value;
+        hipLaunchKernelGGL((k_sell_from_zero<M, T>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp, L.s_row,
+                           L.s_ci, Lv<T>::sv(L), b, dinv, (T)L.omega, y, st);
+    });
+}
+
+void level_from_zero_check_launch(const cfa_amg_level_t& L, const double* b, double* y, const PrecHooks& hk,
+                                  hipStream_t s) {
+    if (!L.s_sp) {
+        jacobi_from_zero_check_launch(L.tpr_a, L.rows, L.a_rp, L.a_ci, L.a_v, b, L.dinv, L.omega, y, hk, s);
+        return;
+    }
+    const int pos = sell_positions(L.rows);
+    sell_dispatch(L.s_len, [&](auto c) {
+        constexpr int M = decltype(c)::value;
+        hipLaunchKernelGGL(k_sell_from_zero_check<M>, dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp, L.s_row,
+                           L.s_ci, L.s_v, b, L.dinv, L.omega, y, hk.prr, hk.nb_rr, hk.sc, hk.st_w, hk.status);
+    });
+}
+
+template <typename T>
+int level_jacobi_dot_launch(const cfa_amg_level_t& L, const T* x, const T* b, double* y, const double* rdot,
+                            double* part, const int* st, hipStream_t s) {
+    if (!L.s_sp)
+        return jacobi_dot_launch<T>(L.tpr_a, L.rows, L.a_rp, L.a_ci, Lv<T>::a(L), x, b, Lv<T>::dinv(L), L.omega, y,
+                                    rdot, part, st, s);
+    const int pos = sell_positions(L.rows);
+    const int nb = std::min(blocks_for(pos), RED_BLOCKS);
+    sell_dispatch(L.s_len, [&](auto c) {
+        constexpr int M = decltype(c)::value;
+        hipLaunchKernelGGL((k_sell_jacobi_dot<M, T>), dim3(nb), dim3(BLOCK), 0, s, pos, L.s_sp, L.s_row, L.s_ci,
+                           Lv<T>::sv(L), x, b, Lv<T>::dinv(L), (T)L.omega, y, rdot, part, st);
+    });
+    return nb;
+}
+
 // One V-cycle (src/cfa/linalg.cuh:1184-1206) for level l with right-hand side b, in FP64 or FP32
 // (T). Each level's x / ax buffers form a ping-pong pair; the function returns the buffer holding
 // the level's iterate. On level 0 the last smoothing step writes straight into the FP64 `out`
@@ -1235,7 +1482,6 @@ T* vcycle(const cfa_amg_t* amg, int l, const T* b, double* out, const double* rd
     const int ss = amg->smooth_steps;
     T* xa = Lv<T>::x(L);
     T* xb = Lv<T>::ax(L);
-    const T* av = Lv<T>::a(L);
     const T* dv = Lv<T>::dinv(L);
     // the residual test of a fused PCG iteration (the FP32 cycle takes it in its entry kernel)
     const bool check_here = std::is_same<T, double>::value && hk && hk->prr && l == 0;
@@ -1243,13 +1489,13 @@ T* vcycle(const cfa_amg_t* amg, int l, const T* b, double* out, const double* rd
         bool done = false;
         if constexpr (std::is_same<T, double>::value) {
             if (check_here) {
-                jacobi_from_zero_check_launch(L.tpr_a, n, L.a_rp, L.a_ci, av, b, dv, L.omega, xa, *hk, s);
+                level_from_zero_check_launch(L, b, xa, *hk, s);
                 done = true;
             }
         }
-        if (!done) jacobi_from_zero_launch<T>(L.tpr_a, n, L.a_rp, L.a_ci, av, b, dv, L.omega, xa, st, s);
+        if (!done) level_from_zero_launch<T>(L, b, dv, xa, st, s);
         for (int k = 2; k < ss; ++k) {
-            csr_op_launch<OP_JACOBI, T>(L.tpr_a, n, L.a_rp, L.a_ci, av, xa, b, dv, L.omega, xb, st, s);
+            level_op_launch<OP_JACOBI, T>(L, xa, b, dv, xb, st, s);
             std::swap(xa, xb);
         }
     } else {   // x = omega dinv b is always taken, also for fewer than one smoothing step
@@ -1257,7 +1503,7 @@ T* vcycle(const cfa_amg_t* amg, int l, const T* b, double* out, const double* rd
             hipLaunchKernelGGL(k_fp_check, dim3(1), dim3(BLOCK), 0, s, hk->prr, hk->nb_rr, hk->sc, hk->st_w, hk->status);
         hipLaunchKernelGGL(k_jacobi_init<T>, dim3(blocks_for(n)), dim3(BLOCK), 0, s, n, (T)L.omega, dv, b, xa, st);
     }
-    csr_op_launch<OP_RESID, T>(L.tpr_a, n, L.a_rp, L.a_ci, av, xa, b, nullptr, 0.0, Lv<T>::res(L), st, s);
+    level_op_launch<OP_RESID, T>(L, xa, b, nullptr, Lv<T>::res(L), st, s);
     const bool deeper = l + 1 < amg->n_levels;
     T* cb = deeper ? Lv<T>::b(amg->levels[l + 1]) : Lv<T>::cb(amg);
     csr_op_launch<OP_SPMV, T>(L.tpr_r, L.cols, L.r_rp, L.r_ci, Lv<T>::r(L), Lv<T>::res(L), nullptr, nullptr, 0.0, cb,
@@ -1273,12 +1519,11 @@ T* vcycle(const cfa_amg_t* amg, int l, const T* b, double* out, const double* rd
     csr_op_launch<OP_ADD, T>(L.tpr_p, n, L.p_rp, L.p_ci, Lv<T>::p(L), cx, nullptr, nullptr, 0.0, xa, st, s);
     for (int k = 0; k < ss; ++k) {
         if (out && k == ss - 1) {
-            const int nb = jacobi_dot_launch<T>(L.tpr_a, n, L.a_rp, L.a_ci, av, xa, b, dv, L.omega, out, rdot,
-                                                hk ? hk->dot_part : nullptr, st, s);
+            const int nb = level_jacobi_dot_launch<T>(L, xa, b, out, rdot, hk ? hk->dot_part : nullptr, st, s);
             if (hk && hk->dot_part) hk->dot_blocks = nb;
             return nullptr;   // the result is in out
         }
-        csr_op_launch<OP_JACOBI, T>(L.tpr_a, n, L.a_rp, L.a_ci, av, xa, b, dv, L.omega, xb, st, s);
+        level_op_launch<OP_JACOBI, T>(L, xa, b, dv, xb, st, s);
         std::swap(xa, xb);
     }
     return xa;
@@ -1293,7 +1538,12 @@ bool amg_valid(const cfa_amg_t* amg) {
         for (int l = 0; l < amg->n_levels; ++l) {
             const cfa_amg_level_t& L = amg->levels[l];
             if (!L.a_v32 || !L.p_v32 || !L.r_v32 || !L.dinv32 || !L.x32 || !L.b32 || !L.r32 || !L.ax32) return false;
+            if (L.s_sp && !L.s_v32) return false;
         }
+    }
+    for (int l = 0; l < amg->n_levels; ++l) {
+        const cfa_amg_level_t& L = amg->levels[l];
+        if (L.s_sp && (!L.s_row || !L.s_ci || !L.s_v || L.s_len < 0 || L.s_len > CFA_SELL_MAX_LEN)) return false;
     }
     return true;
 }
@@ -1742,8 +1992,8 @@ int cfa_amg_smooth(const cfa_amg_t* amg, int level, const double* x, const doubl
                    cfa_stream_t stream) {
     if (!amg || !x || !b || !y || level < 0 || level >= amg->n_levels || !amg->levels) return CFA_E_ARG;
     const cfa_amg_level_t& L = amg->levels[level];
-    csr_op_launch<OP_JACOBI, double>(L.tpr_a, L.rows, L.a_rp, L.a_ci, L.a_v, x, b, L.dinv, L.omega, y, nullptr,
-                                      S(stream));
+    if (L.s_sp && (!L.s_row || !L.s_ci || !L.s_v || L.s_len < 0 || L.s_len > CFA_SELL_MAX_LEN)) return CFA_E_ARG;
+    level_op_launch<OP_JACOBI, double>(L, x, b, L.dinv, y, nullptr, S(stream));
     return launch_status();
 }
 
@@ -1751,9 +2001,47 @@ int cfa_amg_smooth_f32(const cfa_amg_t* amg, int level, const float* x, const fl
                        cfa_stream_t stream) {
     if (!amg || !x || !b || !y || level < 0 || level >= amg->n_levels || !amg->levels) return CFA_E_ARG;
     const cfa_amg_level_t& L = amg->levels[level];
-    if (!amg->use_f32 || !L.a_v32 || !L.dinv32) return CFA_E_NOTREADY;
-    csr_op_launch<OP_JACOBI, float>(L.tpr_a, L.rows, L.a_rp, L.a_ci, L.a_v32, x, b, L.dinv32, L.omega, y, nullptr,
-                                     S(stream));
+    if (!amg->use_f32 || !L.a_v32 || !L.dinv32 || (L.s_sp && !L.s_v32)) return CFA_E_NOTREADY;
+    if (L.s_sp && (!L.s_row || !L.s_ci || L.s_len < 0 || L.s_len > CFA_SELL_MAX_LEN)) return CFA_E_ARG;
+    level_op_launch<OP_JACOBI, float>(L, x, b, L.dinv32, y, nullptr, S(stream));
+    return launch_status();
+}
+
+size_t cfa_sell_ws_bytes(int rows) {
+    if (rows < 0) return 0;
+    const int slices = sell_positions(rows) / SELL_SLICE;
+    const size_t scan = cfa_scan_counts_ws_bytes(slices);
+    return scan ? ((size_t)slices * sizeof(int) + 255) / 256 * 256 + scan : 0;
+}
+
+int cfa_sell_plan(int rows, const int* rp, int* sell_row, int* sell_sp, int* info, void* ws, size_t ws_bytes,
+                  cfa_stream_t stream) {
+    if (rows < 0 || !info || !sell_sp) return CFA_E_ARG;
+    if (rows > 0 && (!rp || !sell_row)) return CFA_E_ARG;
+    const size_t need = cfa_sell_ws_bytes(rows);
+    if (need == 0 || !ws || ws_bytes < need) return CFA_E_ARG;
+    const hipStream_t s = S(stream);
+    const int positions = sell_positions(rows), slices = positions / SELL_SLICE;
+    int* cnt = static_cast<int*>(ws);
+    const size_t cnt_bytes = ((size_t)slices * sizeof(int) + 255) / 256 * 256;
+    CFA_TRY(hipMemsetAsync(info, 0, 2 * sizeof(int), s));
+    if (rows > 0)
+        hipLaunchKernelGGL(k_sell_plan, dim3(blocks_for(rows, SELL_WINDOW)), dim3(SELL_WINDOW), 0, s, rows, positions,
+                           rp, sell_row, cnt, info);
+    const int e = launch_status();
+    if (e != CFA_OK) return e;
+    return cfa_scan_counts(slices, cnt, sell_sp, info, static_cast<unsigned char*>(ws) + cnt_bytes, need - cnt_bytes,
+                           stream);
+}
+
+int cfa_sell_fill(int rows, const int* rp, const int* ci, const double* val, const int* sell_row,
+                  const int* sell_sp, int* sell_ci, double* sell_v, cfa_stream_t stream) {
+    if (rows < 0) return CFA_E_ARG;
+    if (rows == 0) return CFA_OK;
+    if (!rp || !ci || !val || !sell_row || !sell_sp || !sell_ci || !sell_v) return CFA_E_ARG;
+    const int positions = sell_positions(rows);
+    hipLaunchKernelGGL(k_sell_fill, dim3(blocks_for(positions)), dim3(BLOCK), 0, S(stream), positions, rp, ci, val,
+                       sell_row, sell_sp, sell_ci, sell_v);
     return launch_status();
 }
 

@@ -704,7 +704,9 @@ int run_driver(int argc, char** argv) {
         }
         if (use_amg && !amg.levels.empty()) {
             const double n0 = amg.levels[0]->A.rows, nnz0 = amg.levels[0]->A.nnz;
-            add("k_csr_op<jacobi> level 0", 12.0 * nnz0 + 36.0 * n0 + 4.0, time_launches(R, [&] {
+            add(amg.levels[0]->has_sell ? "level-0 Jacobi sweep (k_sell_op, sliced ELL)"
+                                        : "level-0 Jacobi sweep (k_csr_op)",
+                12.0 * nnz0 + 36.0 * n0 + 4.0, time_launches(R, [&] {
                     GPU_CHECK(cfa_amg_smooth(amg.view(), 0, t_in, g.d_rhs_p, t_out, s));
                 }), R);
         }

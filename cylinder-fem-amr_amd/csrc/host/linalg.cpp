@@ -707,7 +707,39 @@ void AmgPreconditioner::setup(int n, int nnz, const int* d_row_ptr, const int* d
 }
 
 
+// The sliced-ELL copy of a level's operator (CFA_SELL=0 keeps the CSR rows): every entry load of a
+// sweep's wavefront is one contiguous run; levels with a row longer than CFA_SELL_MAX_LEN keep the
+// CSR kernels.
+void AmgPreconditioner::build_sell(Level& L) {
+    L.has_sell = false;
+    const int n = L.A.rows;
+    const char* env = std::getenv("CFA_SELL");
+    if (n <= 0 || (env && env[0] == '0')) return;
+    const cfa_stream_t s = solver_stream();
+    const int slices = (n + 63) / 64;
+    L.s_row.reserve((size_t)slices * 64);
+    L.s_sp.reserve((size_t)slices + 1);
+    const size_t wsb = cfa_sell_ws_bytes(n);
+    if (wsb == 0) return;
+    d_sell_ws_.reserve(wsb, 0);
+    d_sell_info_.reserve(2, 0);
+    GPU_CHECK(cfa_sell_plan(n, L.A.rp, L.s_row, L.s_sp, d_sell_info_, d_sell_ws_, wsb, s));
+    int info[2] = {0, 0};
+    d_sell_info_.download(info, 2);
+    if (info[1] > CFA_SELL_MAX_LEN) return;
+    L.s_entries = info[0];
+    L.s_len = info[1];
+    L.s_ci.reserve(std::max(L.s_entries, 1), 256);
+    L.s_v.reserve(std::max(L.s_entries, 1), 256);
+    GPU_CHECK(cfa_sell_fill(n, L.A.rp, L.A.ci, L.A.v, L.s_row, L.s_sp, L.s_ci, L.s_v, s));
+    L.has_sell = true;
+}
+
 void AmgPreconditioner::refresh_view() {
+    for (size_t l = 0; l < levels.size(); ++l) {   // level 0 only: the coarser levels' rows are longer
+        if (l == 0) build_sell(*levels[l]);
+        else levels[l]->has_sell = false;
+    }
     desc_.resize(levels.size());
     for (size_t l = 0; l < levels.size(); ++l) {
         const Level& L = *levels[l];
@@ -728,6 +760,12 @@ void AmgPreconditioner::refresh_view() {
         d.x = L.x; d.b = L.b; d.r = L.r; d.ax = L.Ax;
         d.a_v32 = d.p_v32 = d.r_v32 = d.dinv32 = nullptr;
         d.x32 = d.b32 = d.r32 = d.ax32 = nullptr;
+        d.s_len = L.has_sell ? L.s_len : 0;
+        d.s_sp = L.has_sell ? L.s_sp.ptr : nullptr;
+        d.s_row = L.has_sell ? L.s_row.ptr : nullptr;
+        d.s_ci = L.has_sell ? L.s_ci.ptr : nullptr;
+        d.s_v = L.has_sell ? L.s_v.ptr : nullptr;
+        d.s_v32 = nullptr;
     }
     if (fp32) {   // FP32 copies of the values and FP32 work vectors
         const cfa_stream_t s = solver_stream();
@@ -745,6 +783,11 @@ void AmgPreconditioner::refresh_view() {
             GPU_CHECK(cfa_to_f32(n, L.dinv, L.dinv32, s));
             d.a_v32 = L.A32; d.p_v32 = L.P32; d.r_v32 = L.R32; d.dinv32 = L.dinv32;
             d.x32 = L.x32; d.b32 = L.b32; d.r32 = L.r32; d.ax32 = L.Ax32;
+            if (L.has_sell) {
+                L.s_v32.reserve(std::max(L.s_entries, 1), 256);
+                GPU_CHECK(cfa_to_f32(L.s_entries, L.s_v, L.s_v32, s));
+                d.s_v32 = L.s_v32;
+            }
         }
         d_cx32.reserve(std::max(coarse_n, 1), 16);
         d_cb32.reserve(std::max(coarse_n, 1), 16);

@@ -144,6 +144,12 @@ public:
         DevArray<double> x, b, r, Ax;
         // FP32 copies and work vectors of the mixed-precision cycle
         DevArray<float> A32, P32, R32, dinv32, x32, b32, r32, Ax32;
+        // sliced-ELL copy of A that the level's sweeps and residual read (level 0; cfa_sell_plan)
+        bool has_sell = false;
+        int s_len = 0, s_entries = 0;
+        DevArray<int> s_sp, s_row, s_ci;
+        DevArray<double> s_v;
+        DevArray<float> s_v32;
     };
 
     int smooth_steps = 2;
@@ -212,6 +218,10 @@ private:
     DevCsr dev_next_, dev_tmp_, dev_next2_;    // device set-up: next operator, products
     DevArray<int> d_cnt_, d_flag_, d_agg_, d_iota_, d_info_;
     DevArray<unsigned char> d_tws_, d_sws_, d_aws_, d_gws_;   // transpose, scan, aggregation, sorted product
+    DevArray<unsigned char> d_sell_ws_;
+    DevArray<int> d_sell_info_;
+    // the sliced-ELL copy of the level's operator when its rows are at most CFA_SELL_MAX_LEN long
+    void build_sell(Level& L);
     DevArray<double> d_p0v_, d_x0_, d_pw_;
     std::vector<double> x0_host_;
     int x0_dev_n_ = 0, iota_dev_n_ = 0;

@@ -168,25 +168,33 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
             GPU_CHECK(cfa_spmv(n, g.d_row_ptr, g.d_col_ind, g.d_Kp_val, g.d_dp, t_out, 1, s));
         }), kernel_reps});
         if (!amg.levels.empty()) {
-            // one smoothing sweep per level with each lanes-per-row choice (the set-up picks one
-            // by the mean row length and the level size; these timings are what that rule is
-            // checked against)
+            // one smoothing sweep per level with each lanes-per-row choice of the CSR kernel (the
+            // set-up picks one by the mean row length and the level size; these timings are what
+            // that rule is checked against), and on the level's sliced-ELL copy when it has one
             cfa_amg_t v = *amg.view();
             std::vector<cfa_amg_level_t> lv(v.levels, v.levels + v.n_levels);
             v.levels = lv.data();
             for (int l = 0; l < v.n_levels; ++l) {
                 const cfa_amg_level_t& L = amg.view()->levels[l];
                 const double nl = L.rows, nnzl = L.a_nnz;
+                const std::string head = "jacobi level " + std::to_string(l) + " rows " + std::to_string(L.rows) +
+                                         " nnz/row " + std::to_string((int)std::lround(nnzl / nl));
+                lv[l].s_sp = nullptr;
                 for (int tpr = 1; tpr <= 32; tpr *= 2) {
                     lv[l].tpr_a = tpr;
-                    const std::string name = "jacobi level " + std::to_string(l) + " rows " + std::to_string(L.rows) +
-                                             " nnz/row " + std::to_string((int)std::lround(nnzl / nl)) + " tpr " +
-                                             std::to_string(tpr) + (tpr == L.tpr_a ? " (chosen)" : "");
+                    const std::string name =
+                        head + " tpr " + std::to_string(tpr) + (!L.s_sp && tpr == L.tpr_a ? " (chosen)" : "");
                     kernel_timings().push_back({name, 12.0 * nnzl + 36.0 * nl + 4.0, time_launches(kernel_reps, [&] {
                         GPU_CHECK(cfa_amg_smooth(&v, l, L.x, L.b, L.ax, s));
                     }), kernel_reps});
                 }
                 lv[l].tpr_a = L.tpr_a;
+                lv[l].s_sp = L.s_sp;
+                if (L.s_sp)
+                    kernel_timings().push_back({head + " sliced ELL (chosen)", 12.0 * nnzl + 36.0 * nl + 4.0,
+                                                time_launches(kernel_reps, [&] {
+                        GPU_CHECK(cfa_amg_smooth(&v, l, L.x, L.b, L.ax, s));
+                    }), kernel_reps});
             }
         }
         {   // a device copy of 1 GiB (four times the Infinity Cache): every byte crosses to HBM, so
@@ -206,7 +214,9 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
             const double n0 = L.A.rows, nnz0 = L.A.nnz;
             dev_zero(L.x32, (size_t)L.A.rows * sizeof(float));
             dev_zero(L.b32, (size_t)L.A.rows * sizeof(float));
-            kernel_timings().push_back({"k_csr_op<jacobi, float> level 0", 8.0 * nnz0 + 20.0 * n0 + 4.0,
+            kernel_timings().push_back({L.has_sell ? "level-0 Jacobi sweep, FP32 (k_sell_op, sliced ELL)"
+                                                   : "level-0 Jacobi sweep, FP32 (k_csr_op)",
+                                        8.0 * nnz0 + 20.0 * n0 + 4.0,
                                         time_launches(kernel_reps, [&] {
                 GPU_CHECK(cfa_amg_smooth_f32(amg32.view(), 0, L.x32, L.b32, L.Ax32, s));
             }), kernel_reps});
@@ -215,7 +225,9 @@ int cfa_workload_pressure(int level, int nrhs, double tol, int kernel_reps, doub
         // kernel_reps dispatches of this kernel are exactly these
         if (!amg.levels.empty()) {
             const double n0 = amg.levels[0]->A.rows, nnz0 = amg.levels[0]->A.nnz;
-            kernel_timings().push_back({"k_csr_op<jacobi> level 0", 12.0 * nnz0 + 36.0 * n0 + 4.0,
+            kernel_timings().push_back({amg.levels[0]->has_sell ? "level-0 Jacobi sweep (k_sell_op, sliced ELL)"
+                                                                : "level-0 Jacobi sweep (k_csr_op)",
+                                        12.0 * nnz0 + 36.0 * n0 + 4.0,
                                         time_launches(kernel_reps, [&] {
                 GPU_CHECK(cfa_amg_smooth(amg.view(), 0, g.d_dp, g.d_rhs_p, t_out, s));
             }), kernel_reps});

@@ -258,6 +258,11 @@ typedef struct {
      * (cfa_amg_t.use_f32); NULL when the hierarchy has none */
     const float* a_v32; const float* p_v32; const float* r_v32; const float* dinv32;
     float* x32; float* b32; float* r32; float* ax32;
+    /* Sliced-ELL copy of A (cfa_sell_plan / cfa_sell_fill) that the level's smoothing sweeps and
+     * residual read instead of the CSR arrays: s_sp NULL for none; s_len its longest row (at most
+     * CFA_SELL_MAX_LEN); s_v32 its FP32 values (use_f32). */
+    int s_len;
+    const int* s_sp; const int* s_row; const int* s_ci; const double* s_v; const float* s_v32;
 } cfa_amg_level_t;
 
 /* SA-AMG hierarchy for the V(ss, ss) cycle (src/cfa/linalg.cuh:1161-1206).
@@ -290,6 +295,27 @@ int cfa_amg_smooth(const cfa_amg_t* amg, int level, const double* x, const doubl
  * smoother); CFA_E_NOTREADY without them. */
 int cfa_amg_smooth_f32(const cfa_amg_t* amg, int level, const float* x, const float* b, float* y,
                        cfa_stream_t stream);
+
+/* Sliced-ELL copy of a CSR operator for the V-cycle's sweeps (no counterpart in the reference,
+ * whose V-cycle reads the CSR arrays, linalg.cuh:884-901; the sums are the same products in each
+ * row's CSR order). Positions come in slices of 64 (one wavefront). Inside each window of 256
+ * positions the rows are ordered by decreasing length, ties by row index, so the rows of a slice
+ * have nearly the same length. The slice length len_q is the length of its first row. Entry k of
+ * the row at position 64 q + l lies at sell_sp[q] + 64 k + l; slots past a row's end hold column 0
+ * and value 0.
+ * cfa_sell_plan writes:
+ *   sell_row [64 * slices], slices = ceil(rows / 64): the row at each position, -1 past the last row;
+ *   sell_sp [slices + 1]: the slice offsets;
+ *   info (device int [2]): the entries of the copy and the longest row.
+ * Its ws holds cfa_sell_ws_bytes(rows) bytes.
+ * cfa_sell_fill then writes sell_ci and sell_v [info[0]].
+ * The V-cycle takes the copy when the longest row is at most CFA_SELL_MAX_LEN. */
+#define CFA_SELL_MAX_LEN 16
+size_t cfa_sell_ws_bytes(int rows);
+int cfa_sell_plan(int rows, const int* rp, int* sell_row, int* sell_sp, int* info, void* ws, size_t ws_bytes,
+                  cfa_stream_t stream);
+int cfa_sell_fill(int rows, const int* rp, const int* ci, const double* val, const int* sell_row,
+                  const int* sell_sp, int* sell_ci, double* sell_v, cfa_stream_t stream);
 
 /* Host callback that enqueues out = op(in) (or out += op(in) for an extra operator) on the
  * stream; the generic form of the std::function arguments of GpuLinearSolver::solve_pcg. */

@@ -5,7 +5,8 @@
 // block reductions or other summation orders must match to the stated relative tolerance.
 //
 //   parity <case> [level]    case: momentum pressure correction forces indicator mms oss ilu
-//                                  spmv amg pcg bicgstab jacobi mixed topology outlet nvb remap reproject all
+//                                  spmv amg sell pcg bicgstab jacobi mixed topology outlet nvb remap reproject
+//                                  all
 //
 // Prints one line per compared quantity and exits 0 when every comparison passes.
 #include <algorithm>
@@ -533,6 +534,118 @@ void case_amg(Fixture& f) {
             GPU_CHECK(cfa_stream_destroy(s2));
             same_bits("V-cycle beside a second one", down(d_za, f.n), z1);
             same_bits("second V-cycle (second vectors)", down(d_zb, f.n), z2);
+        }
+    }
+}
+
+// The sliced-ELL copy of the level-0 operator: its layout against the CSR rows (every position's
+// row, the window order by decreasing length, every entry and the zero padding), and its sweeps
+// against the CSR kernel with one lane per row, which sums each row in the same order: FP64 and
+// FP32 bit for bit; a V-cycle on it against a V-cycle on the CSR rows.
+void case_sell(Fixture& f) {
+    std::printf("[sliced-ELL level-0 operator]\n");
+    (void)case_pressure(f, false);
+    for (int f32 = 0; f32 < 2; ++f32) {
+        AmgPreconditioner amg;
+        amg.fp32 = f32 == 1;
+        amg.setup(f.n, f.nnz, f.g.d_row_ptr, f.g.d_col_ind, f.g.d_Kp_val);
+        if (amg.levels.empty()) {
+            expect(false, "a hierarchy with a level 0");
+            return;
+        }
+        const auto& L = *amg.levels[0];
+        expect(L.has_sell, "level 0 carries the sliced-ELL copy");
+        if (!L.has_sell) return;
+        const int n = L.A.rows, positions = (n + 63) / 64 * 64, slices = positions / 64;
+        if (!f32) {
+            const std::vector<int> rp = down_int(L.A.rp, n + 1), ci = down_int(L.A.ci, L.A.nnz);
+            const Vec av = down(L.A.v, L.A.nnz);
+            const std::vector<int> srow = down_int(L.s_row, positions), sp = down_int(L.s_sp, slices + 1);
+            const std::vector<int> sci = down_int(L.s_ci, std::max(L.s_entries, 1));
+            const Vec sv = down(L.s_v, std::max(L.s_entries, 1));
+            int longest = 0, bad_perm = 0, bad_order = 0, bad_len = 0, bad_pad = 0;
+            std::vector<int> seen(n, 0);
+            std::vector<int> ci_g, ci_o;
+            Vec v_g, v_o;
+            for (int r = 0; r < n; ++r) longest = std::max(longest, rp[r + 1] - rp[r]);
+            for (int p = 0; p < positions; ++p) {
+                const int r = srow[p], w0 = p / 256 * 256;
+                if (r < 0) {   // padding only past the last row
+                    bad_perm += p < n ? 1 : 0;
+                    continue;
+                }
+                if (r >= n || r < w0 || r >= w0 + 256 || seen[r]++) {
+                    ++bad_perm;
+                    continue;
+                }
+                const int len = rp[r + 1] - rp[r];
+                if (p > w0 && srow[p - 1] >= 0) {   // decreasing length, ties by row
+                    const int q = srow[p - 1], lq = rp[q + 1] - rp[q];
+                    bad_order += (lq < len || (lq == len && q > r)) ? 1 : 0;
+                }
+                const int sl = (sp[p / 64 + 1] - sp[p / 64]) / 64;
+                bad_len += (p % 64 == 0 && sl != len) || sl < len ? 1 : 0;
+                for (int k = 0; k < sl; ++k) {
+                    const int at = sp[p / 64] + 64 * k + p % 64;
+                    if (k < len) {
+                        ci_g.push_back(sci[at]);
+                        ci_o.push_back(ci[rp[r] + k]);
+                        v_g.push_back(sv[at]);
+                        v_o.push_back(av[rp[r] + k]);
+                    } else {
+                        bad_pad += (sci[at] != 0 || sv[at] != 0.0) ? 1 : 0;
+                    }
+                }
+            }
+            for (int r = 0; r < n; ++r) bad_perm += seen[r] == 1 ? 0 : 1;
+            std::printf("  %d rows, %d slices, %d slots for %d entries, longest row %d (recorded %d)\n", n, slices,
+                        sp[slices], L.A.nnz, longest, L.s_len);
+            expect(bad_perm == 0, "every row at one position of its window");
+            expect(bad_order == 0, "window order: decreasing length, ties by row");
+            expect(bad_len == 0 && longest == L.s_len && sp[slices] == L.s_entries, "slice lengths and sizes");
+            expect(bad_pad == 0, "padding slots (column 0, value 0)");
+            same_ints("column indices in CSR order", ci_g, ci_o);
+            same_bits("values in CSR order", v_g, v_o);
+        }
+        // one sweep on the copy against the CSR kernel with one lane per row
+        const cfa_amg_t* view = amg.view();
+        cfa_amg_t csr_view = *view;
+        std::vector<cfa_amg_level_t> lv(view->levels, view->levels + view->n_levels);
+        lv[0].s_sp = nullptr;
+        lv[0].tpr_a = 1;
+        csr_view.levels = lv.data();
+        Vec x(n), b(n);
+        for (auto& v : x) v = std::uniform_real_distribution<double>(-1, 1)(f.rng);
+        for (auto& v : b) v = std::uniform_real_distribution<double>(-1, 1)(f.rng);
+        if (!f32) {
+            DevArray<double> d_x, d_b, d_y1, d_y2;
+            d_x.upload(x);
+            d_b.upload(b);
+            d_y1.reserve(n);
+            d_y2.reserve(n);
+            GPU_CHECK(cfa_amg_smooth(view, 0, d_x, d_b, d_y1, f.s()));
+            GPU_CHECK(cfa_amg_smooth(&csr_view, 0, d_x, d_b, d_y2, f.s()));
+            same_bits("FP64 sweep: sliced ELL = CSR, one lane per row", down(d_y1, n), down(d_y2, n));
+            // a whole V-cycle on the copy against one on the CSR rows (the default lanes per row)
+            lv[0].tpr_a = L.A.tpr;
+            DevArray<double> d_z1, d_z2;
+            d_z1.reserve(n);
+            d_z2.reserve(n);
+            GPU_CHECK(cfa_amg_apply(view, d_x, d_z1, f.s()));
+            GPU_CHECK(cfa_amg_apply(&csr_view, d_x, d_z2, f.s()));
+            close_rel("V-cycle: sliced ELL against CSR", down(d_z1, n), down(d_z2, n), 1e-13);
+        } else {
+            std::vector<float> x32(x.begin(), x.end()), b32(b.begin(), b.end());
+            DevArray<float> d_x, d_b, d_y1, d_y2;
+            d_x.upload(x32);
+            d_b.upload(b32);
+            d_y1.reserve(n);
+            d_y2.reserve(n);
+            GPU_CHECK(cfa_amg_smooth_f32(view, 0, d_x, d_b, d_y1, f.s()));
+            GPU_CHECK(cfa_amg_smooth_f32(&csr_view, 0, d_x, d_b, d_y2, f.s()));
+            const std::vector<float> y1 = d_y1.to_host(n), y2 = d_y2.to_host(n);
+            same_bits("FP32 sweep: sliced ELL = CSR, one lane per row", Vec(y1.begin(), y1.end()),
+                      Vec(y2.begin(), y2.end()));
         }
     }
 }
@@ -1292,6 +1405,7 @@ int main(int argc, char** argv) {
         if (all || what == "ilu") case_ilu(f);
         if (all || what == "spmv") case_spmv(f);
         if (all || what == "amg") case_amg(f);
+        if (all || what == "sell") case_sell(f);
         if (all || what == "pcg") case_pcg(f);
         if (all || what == "bicgstab") case_bicgstab(f);
         if (all || what == "jacobi") case_jacobi(f);

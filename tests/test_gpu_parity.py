@@ -3,16 +3,17 @@ BCs, OSS operator, correction, indicators bit for bit; forces, MMS integrals, Sp
 V-cycle, PCG and BiCGSTAB2 to the relative tolerances printed by the program; the mesh topology
 built on the device (CSR graph, incidence, checks) against the host build, on uniform and locally
 refined meshes and two planted defects; the re-projection's two solves side by side against one
-after the other, bit for bit. Meshes: uniform levels 1 and 3 of the reference's generator; fields
-seeded."""
+after the other, bit for bit; the sliced-ELL copy of the finest AMG level (layout against the CSR
+rows, sweeps bit for bit against the CSR kernel with one lane per row). Meshes: uniform levels 1
+and 3 of the reference's generator; fields seeded."""
 import subprocess
 from pathlib import Path
 
 import pytest
 
 BIN = Path(__file__).resolve().parent / "native" / "build" / "parity"
-CASES = ["momentum", "pressure", "correction", "forces", "indicator", "mms", "oss", "ilu", "spmv", "amg", "pcg",
-         "bicgstab", "jacobi", "mixed", "topology", "outlet", "nvb", "remap", "reproject"]
+CASES = ["momentum", "pressure", "correction", "forces", "indicator", "mms", "oss", "ilu", "spmv", "amg", "sell",
+         "pcg", "bicgstab", "jacobi", "mixed", "topology", "outlet", "nvb", "remap", "reproject"]
 
 pytestmark = pytest.mark.gpu
 # (case, level) pairs kept off the GPU: each ended once with SIGSEGV in the parity process, the

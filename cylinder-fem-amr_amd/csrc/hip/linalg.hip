@@ -656,7 +656,9 @@ k_jacobi_dot(int rows, const int* __restrict__ rp, const int* __restrict__ ci, c
 // slots hold column 0 and value 0. Each entry load of a wavefront is one contiguous run (512 bytes
 // in FP64), where the CSR kernels' loads scatter over the rows of a lane group: on a 7.35 M-row
 // matrix of the uniform meshes' 5- and 9-entry rows a Jacobi sweep takes 175 us against 257 us.
-// One lane per row, each row's products summed in its CSR order.
+// One lane per row; the row's products are summed in the order of the CSR kernels with G lanes per
+// row (G interleaved partial sums, joined by the lane group's tree), so the results are the CSR
+// kernels' bit for bit.
 constexpr int SELL_SLICE = 64;
 constexpr int SELL_WINDOW = 256;
 
@@ -676,12 +678,14 @@ __device__ __forceinline__ bool sell_slot(const int* __restrict__ sp, const int*
     return sv == ST_RUNNING;
 }
 
-// f(a, j) for the position's MAXL slots in order, every index and value loaded before any is
-// used; slots past the slice's length give (0, column 0) without a load, so the gathers f makes
-// are unconditional and an accumulation of a * x[j] adds an exact zero for them.
-template <int MAXL, typename T, typename F>
-__device__ __forceinline__ void sell_entries(const int* __restrict__ ci, const T* __restrict__ val, int base, int len,
-                                             F f) {
+// sum over the row's entries k of a_k xv(j_k), every index and value loaded before any is used
+// (slots past the slice's length give (0, column 0) without a load, so the gathers are
+// unconditional and add exact zeros), in the order of row_entries<G> + group_sum<G>: entry k goes
+// to partial k mod G, each partial in entry order, the partials joined as the lane tree joins them
+// (G = 2: p0 + p1; G = 4: (p0 + p2) + (p1 + p3)).
+template <int MAXL, int G, typename T, typename F>
+__device__ __forceinline__ T sell_row_sum(const int* __restrict__ ci, const T* __restrict__ val, int base, int len,
+                                         F xv) {
     int c[MAXL];
     T v[MAXL];
 #pragma unroll
@@ -690,12 +694,20 @@ __device__ __forceinline__ void sell_entries(const int* __restrict__ ci, const T
         c[k] = ok ? ci[base + SELL_SLICE * k] : 0;
         v[k] = ok ? val[base + SELL_SLICE * k] : T(0);
     }
+    T acc[G];
 #pragma unroll
-    for (int k = 0; k < MAXL; ++k) f(v[k], c[k]);
+    for (int g = 0; g < G; ++g) acc[g] = 0;
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) acc[k % G] += v[k] * xv(c[k]);
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int g = 0; g < o; ++g) acc[g] += acc[g + o];
+    return acc[0];
 }
 
 // y = b - A x (OP_RESID) | y = x + omega dinv (b - A x) (OP_JACOBI) on the sliced-ELL copy
-template <int MAXL, int OP, typename T>
+template <int MAXL, int G, int OP, typename T>
 __global__ void __launch_bounds__(BLOCK)
 k_sell_op(int positions, const int* __restrict__ sp, const int* __restrict__ srow, const int* __restrict__ ci,
           const T* __restrict__ val, const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ dinv,
@@ -704,15 +716,14 @@ k_sell_op(int positions, const int* __restrict__ sp, const int* __restrict__ sro
     if (p >= positions) return;
     int row, base, len;
     if (!sell_slot(sp, srow, p, state, row, base, len)) return;
-    T s = 0;
-    sell_entries<MAXL>(ci, val, base, len, [&](T a, int j) { s += a * x[j]; });
+    const T s = sell_row_sum<MAXL, G>(ci, val, base, len, [&](int j) { return x[j]; });
     if (row < 0) return;
     if (OP == OP_RESID) y[row] = b[row] - s;
     else y[row] = x[row] + omega * dinv[row] * (b[row] - s);
 }
 
 // k_jacobi_from_zero on the sliced-ELL copy
-template <int MAXL, typename T>
+template <int MAXL, int G, typename T>
 __global__ void __launch_bounds__(BLOCK)
 k_sell_from_zero(int positions, const int* __restrict__ sp, const int* __restrict__ srow, const int* __restrict__ ci,
                  const T* __restrict__ val, const T* __restrict__ b, const T* __restrict__ dinv, T omega,
@@ -721,15 +732,14 @@ k_sell_from_zero(int positions, const int* __restrict__ sp, const int* __restric
     if (p >= positions) return;
     int row, base, len;
     if (!sell_slot(sp, srow, p, state, row, base, len)) return;
-    T s = 0;
-    sell_entries<MAXL>(ci, val, base, len, [&](T a, int j) { s += a * (omega * dinv[j] * b[j]); });
+    const T s = sell_row_sum<MAXL, G>(ci, val, base, len, [&](int j) { return omega * dinv[j] * b[j]; });
     if (row < 0) return;
     const T x0 = omega * dinv[row] * b[row];
     y[row] = x0 + omega * dinv[row] * (b[row] - s);
 }
 
 // k_jacobi_from_zero_check on the sliced-ELL copy: the iteration's residual test, then the sweeps
-template <int MAXL>
+template <int MAXL, int G>
 __global__ void __launch_bounds__(BLOCK)
 k_sell_from_zero_check(int positions, const int* __restrict__ sp, const int* __restrict__ srow,
                        const int* __restrict__ ci, const double* __restrict__ val, const double* __restrict__ b,
@@ -742,38 +752,10 @@ k_sell_from_zero_check(int positions, const int* __restrict__ sp, const int* __r
     if (p >= positions) return;
     int row, base, len;
     (void)sell_slot(sp, srow, p, nullptr, row, base, len);
-    double s = 0.0;
-    sell_entries<MAXL>(ci, val, base, len, [&](double a, int j) { s += a * (omega * dinv[j] * b[j]); });
+    const double s = sell_row_sum<MAXL, G>(ci, val, base, len, [&](int j) { return omega * dinv[j] * b[j]; });
     if (row < 0) return;
     const double x0 = omega * dinv[row] * b[row];
     y[row] = x0 + omega * dinv[row] * (b[row] - s);
-}
-
-// k_jacobi_dot on the sliced-ELL copy: the last sweep into the FP64 output, with the r.z partials
-template <int MAXL, typename T>
-__global__ void __launch_bounds__(BLOCK)
-k_sell_jacobi_dot(int positions, const int* __restrict__ sp, const int* __restrict__ srow,
-                  const int* __restrict__ ci, const T* __restrict__ val, const T* __restrict__ x,
-                  const T* __restrict__ b, const T* __restrict__ dinv, T omega, double* __restrict__ y,
-                  const double* __restrict__ rdot, double* __restrict__ part, const int* __restrict__ st) {
-    __shared__ double lds[BLOCK / WAVE];
-    if (st && block_state(st, nullptr) != ST_RUNNING) return;
-    double contrib = 0.0;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < positions; p += gridDim.x * blockDim.x) {
-        int row, base, len;
-        (void)sell_slot(sp, srow, p, nullptr, row, base, len);
-        T s = 0;
-        sell_entries<MAXL>(ci, val, base, len, [&](T a, int j) { s += a * x[j]; });
-        if (row >= 0) {
-            const double yi = (double)(x[row] + omega * dinv[row] * (b[row] - s));
-            y[row] = yi;
-            if (part) contrib += rdot[row] * yi;
-        }
-    }
-    if (part) {
-        contrib = block_sum(contrib, lds);
-        if (threadIdx.x == 0) part[blockIdx.x] = contrib;
-    }
 }
 
 // The positions of one window's rows (a workgroup per window of SELL_WINDOW rows): each row's rank
@@ -1399,12 +1381,24 @@ void jacobi_from_zero_check_launch(int tpr, int rows, const int* rp, const int* 
     });
 }
 
-// MAXL of the sliced-ELL kernels for a longest row of maxl (at most CFA_SELL_MAX_LEN)
+// MAXL (slots loaded per lane) of the sliced-ELL kernels for a longest row of maxl (at most
+// CFA_SELL_MAX_LEN) and G, the CSR lanes per row whose summation order they keep (1, 2 or 4)
 template <typename F>
-void sell_dispatch(int maxl, F f) {
-    if (maxl <= 8) f(std::integral_constant<int, 8>{});
-    else if (maxl <= 12) f(std::integral_constant<int, 12>{});
-    else f(std::integral_constant<int, 16>{});
+void sell_dispatch(int maxl, int lanes, F f) {
+    auto with_g = [&](auto m) {
+        if (lanes == 1) f(m, std::integral_constant<int, 1>{});
+        else if (lanes == 2) f(m, std::integral_constant<int, 2>{});
+        else f(m, std::integral_constant<int, 4>{});
+    };
+    if (maxl <= 8) with_g(std::integral_constant<int, 8>{});
+    else if (maxl <= 12) with_g(std::integral_constant<int, 12>{});
+    else with_g(std::integral_constant<int, 16>{});
+}
+
+// A level's sliced-ELL copy as the kernels take it
+bool sell_usable(const cfa_amg_level_t& L) {
+    return L.s_row && L.s_ci && L.s_v && L.s_len >= 0 && L.s_len <= CFA_SELL_MAX_LEN &&
+           (L.tpr_a == 1 || L.tpr_a == 2 || L.tpr_a == 4);
 }
 
 // The level operator's sweeps and residual: on its sliced-ELL copy when it has one, else on its
@@ -1419,9 +1413,9 @@ void level_op_launch(const cfa_amg_level_t& L, const T* x, const T* b, const T* 
         return;
     }
     const int pos = sell_positions(L.rows);
-    sell_dispatch(L.s_len, [&](auto c) {
-        constexpr int M = decltype(c)::value;
-        hipLaunchKernelGGL((k_sell_op<M, OP, T>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp, L.s_row,
+    sell_dispatch(L.s_len, L.tpr_a, [&](auto m, auto g) {
+        constexpr int M = decltype(m)::value, G = decltype(g)::value;
+        hipLaunchKernelGGL((k_sell_op<M, G, OP, T>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp, L.s_row,
                            L.s_ci, Lv<T>::sv(L), x, b, dinv, (T)L.omega, y, st);
     });
 }
@@ -1433,10 +1427,10 @@ void level_from_zero_launch(const cfa_amg_level_t& L, const T* b, const T* dinv,
         return;
     }
     const int pos = sell_positions(L.rows);
-    sell_dispatch(L.s_len, [&](auto c) {
-        constexpr int M = decltype(c)::value;
-        hipLaunchKernelGGL((k_sell_from_zero<M, T>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp, L.s_row,
-                           L.s_ci, Lv<T>::sv(L), b, dinv, (T)L.omega, y, st);
+    sell_dispatch(L.s_len, L.tpr_a, [&](auto m, auto g) {
+        constexpr int M = decltype(m)::value, G = decltype(g)::value;
+        hipLaunchKernelGGL((k_sell_from_zero<M, G, T>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp,
+                           L.s_row, L.s_ci, Lv<T>::sv(L), b, dinv, (T)L.omega, y, st);
     });
 }
 
@@ -1447,27 +1441,11 @@ void level_from_zero_check_launch(const cfa_amg_level_t& L, const double* b, dou
         return;
     }
     const int pos = sell_positions(L.rows);
-    sell_dispatch(L.s_len, [&](auto c) {
-        constexpr int M = decltype(c)::value;
-        hipLaunchKernelGGL(k_sell_from_zero_check<M>, dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp, L.s_row,
-                           L.s_ci, L.s_v, b, L.dinv, L.omega, y, hk.prr, hk.nb_rr, hk.sc, hk.st_w, hk.status);
+    sell_dispatch(L.s_len, L.tpr_a, [&](auto m, auto g) {
+        constexpr int M = decltype(m)::value, G = decltype(g)::value;
+        hipLaunchKernelGGL((k_sell_from_zero_check<M, G>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp,
+                           L.s_row, L.s_ci, L.s_v, b, L.dinv, L.omega, y, hk.prr, hk.nb_rr, hk.sc, hk.st_w, hk.status);
     });
-}
-
-template <typename T>
-int level_jacobi_dot_launch(const cfa_amg_level_t& L, const T* x, const T* b, double* y, const double* rdot,
-                            double* part, const int* st, hipStream_t s) {
-    if (!L.s_sp)
-        return jacobi_dot_launch<T>(L.tpr_a, L.rows, L.a_rp, L.a_ci, Lv<T>::a(L), x, b, Lv<T>::dinv(L), L.omega, y,
-                                    rdot, part, st, s);
-    const int pos = sell_positions(L.rows);
-    const int nb = std::min(blocks_for(pos), RED_BLOCKS);
-    sell_dispatch(L.s_len, [&](auto c) {
-        constexpr int M = decltype(c)::value;
-        hipLaunchKernelGGL((k_sell_jacobi_dot<M, T>), dim3(nb), dim3(BLOCK), 0, s, pos, L.s_sp, L.s_row, L.s_ci,
-                           Lv<T>::sv(L), x, b, Lv<T>::dinv(L), (T)L.omega, y, rdot, part, st);
-    });
-    return nb;
 }
 
 // One V-cycle (src/cfa/linalg.cuh:1184-1206) for level l with right-hand side b, in FP64 or FP32
@@ -1519,7 +1497,9 @@ T* vcycle(const cfa_amg_t* amg, int l, const T* b, double* out, const double* rd
     csr_op_launch<OP_ADD, T>(L.tpr_p, n, L.p_rp, L.p_ci, Lv<T>::p(L), cx, nullptr, nullptr, 0.0, xa, st, s);
     for (int k = 0; k < ss; ++k) {
         if (out && k == ss - 1) {
-            const int nb = level_jacobi_dot_launch<T>(L, xa, b, out, rdot, hk ? hk->dot_part : nullptr, st, s);
+            // on the CSR rows: its r.z block partials keep their grouping by rows
+            const int nb = jacobi_dot_launch<T>(L.tpr_a, n, L.a_rp, L.a_ci, Lv<T>::a(L), xa, b, dv, L.omega, out,
+                                                rdot, hk ? hk->dot_part : nullptr, st, s);
             if (hk && hk->dot_part) hk->dot_blocks = nb;
             return nullptr;   // the result is in out
         }
@@ -1543,7 +1523,7 @@ bool amg_valid(const cfa_amg_t* amg) {
     }
     for (int l = 0; l < amg->n_levels; ++l) {
         const cfa_amg_level_t& L = amg->levels[l];
-        if (L.s_sp && (!L.s_row || !L.s_ci || !L.s_v || L.s_len < 0 || L.s_len > CFA_SELL_MAX_LEN)) return false;
+        if (L.s_sp && !sell_usable(L)) return false;
     }
     return true;
 }
@@ -1992,7 +1972,7 @@ int cfa_amg_smooth(const cfa_amg_t* amg, int level, const double* x, const doubl
                    cfa_stream_t stream) {
     if (!amg || !x || !b || !y || level < 0 || level >= amg->n_levels || !amg->levels) return CFA_E_ARG;
     const cfa_amg_level_t& L = amg->levels[level];
-    if (L.s_sp && (!L.s_row || !L.s_ci || !L.s_v || L.s_len < 0 || L.s_len > CFA_SELL_MAX_LEN)) return CFA_E_ARG;
+    if (L.s_sp && !sell_usable(L)) return CFA_E_ARG;
     level_op_launch<OP_JACOBI, double>(L, x, b, L.dinv, y, nullptr, S(stream));
     return launch_status();
 }
@@ -2002,7 +1982,7 @@ int cfa_amg_smooth_f32(const cfa_amg_t* amg, int level, const float* x, const fl
     if (!amg || !x || !b || !y || level < 0 || level >= amg->n_levels || !amg->levels) return CFA_E_ARG;
     const cfa_amg_level_t& L = amg->levels[level];
     if (!amg->use_f32 || !L.a_v32 || !L.dinv32 || (L.s_sp && !L.s_v32)) return CFA_E_NOTREADY;
-    if (L.s_sp && (!L.s_row || !L.s_ci || L.s_len < 0 || L.s_len > CFA_SELL_MAX_LEN)) return CFA_E_ARG;
+    if (L.s_sp && !sell_usable(L)) return CFA_E_ARG;
     level_op_launch<OP_JACOBI, float>(L, x, b, L.dinv32, y, nullptr, S(stream));
     return launch_status();
 }

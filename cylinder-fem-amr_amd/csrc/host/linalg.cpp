@@ -708,13 +708,13 @@ void AmgPreconditioner::setup(int n, int nnz, const int* d_row_ptr, const int* d
 
 
 // The sliced-ELL copy of a level's operator (CFA_SELL=0 keeps the CSR rows): every entry load of a
-// sweep's wavefront is one contiguous run; levels with a row longer than CFA_SELL_MAX_LEN keep the
-// CSR kernels.
+// sweep's wavefront is one contiguous run, and the sums keep the CSR kernels' order for the level's
+// lanes per row (1, 2 or 4). Levels with a row longer than CFA_SELL_MAX_LEN keep the CSR kernels.
 void AmgPreconditioner::build_sell(Level& L) {
     L.has_sell = false;
     const int n = L.A.rows;
     const char* env = std::getenv("CFA_SELL");
-    if (n <= 0 || (env && env[0] == '0')) return;
+    if (n <= 0 || (env && env[0] == '0') || (L.A.tpr != 1 && L.A.tpr != 2 && L.A.tpr != 4)) return;
     const cfa_stream_t s = solver_stream();
     const int slices = (n + 63) / 64;
     L.s_row.reserve((size_t)slices * 64);

@@ -258,9 +258,12 @@ typedef struct {
      * (cfa_amg_t.use_f32); NULL when the hierarchy has none */
     const float* a_v32; const float* p_v32; const float* r_v32; const float* dinv32;
     float* x32; float* b32; float* r32; float* ax32;
-    /* Sliced-ELL copy of A (cfa_sell_plan / cfa_sell_fill) that the level's smoothing sweeps and
-     * residual read instead of the CSR arrays: s_sp NULL for none; s_len its longest row (at most
-     * CFA_SELL_MAX_LEN); s_v32 its FP32 values (use_f32). */
+    /* Sliced-ELL copy of A (cfa_sell_plan / cfa_sell_fill) that the level's smoothing sweeps
+     * and residual read instead of the CSR arrays (the last sweep of a V-cycle, which emits the
+     * r.z partials, stays on the CSR rows). Each row's products are summed in the order of the
+     * CSR kernels with tpr_a lanes (1, 2 or 4), so the results are theirs bit for bit. s_sp is NULL
+     * for none; s_len is its longest row (at most CFA_SELL_MAX_LEN); s_v32 holds its FP32 values
+     * (use_f32). */
     int s_len;
     const int* s_sp; const int* s_row; const int* s_ci; const double* s_v; const float* s_v32;
 } cfa_amg_level_t;
@@ -297,8 +300,7 @@ int cfa_amg_smooth_f32(const cfa_amg_t* amg, int level, const float* x, const fl
                        cfa_stream_t stream);
 
 /* Sliced-ELL copy of a CSR operator for the V-cycle's sweeps (no counterpart in the reference,
- * whose V-cycle reads the CSR arrays, linalg.cuh:884-901; the sums are the same products in each
- * row's CSR order). Positions come in slices of 64 (one wavefront). Inside each window of 256
+ * whose V-cycle reads the CSR arrays, linalg.cuh:884-901; the sums are the CSR kernels'). Positions come in slices of 64 (one wavefront). Inside each window of 256
  * positions the rows are ordered by decreasing length, ties by row index, so the rows of a slice
  * have nearly the same length. The slice length len_q is the length of its first row. Entry k of
  * the row at position 64 q + l lies at sell_sp[q] + 64 k + l; slots past a row's end hold column 0
@@ -309,7 +311,8 @@ int cfa_amg_smooth_f32(const cfa_amg_t* amg, int level, const float* x, const fl
  *   info (device int [2]): the entries of the copy and the longest row.
  * Its ws holds cfa_sell_ws_bytes(rows) bytes.
  * cfa_sell_fill then writes sell_ci and sell_v [info[0]].
- * The V-cycle takes the copy when the longest row is at most CFA_SELL_MAX_LEN. */
+ * The V-cycle takes the copy when the longest row is at most CFA_SELL_MAX_LEN and tpr_a is 1, 2
+ * or 4. */
 #define CFA_SELL_MAX_LEN 16
 size_t cfa_sell_ws_bytes(int rows);
 int cfa_sell_plan(int rows, const int* rp, int* sell_row, int* sell_sp, int* info, void* ws, size_t ws_bytes,

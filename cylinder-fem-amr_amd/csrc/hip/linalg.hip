@@ -658,9 +658,13 @@ k_jacobi_dot(int rows, const int* __restrict__ rp, const int* __restrict__ ci, c
 // matrix of the uniform meshes' 5- and 9-entry rows a Jacobi sweep takes 175 us against 257 us.
 // One lane per row; the row's products are summed in the order of the CSR kernels with G lanes per
 // row (G interleaved partial sums, joined by the lane group's tree), so the results are the CSR
-// kernels' bit for bit.
+// kernels' bit for bit. Each cache line of the copy is read by one load instruction, so a sweep
+// whose bytes exceed the 256 MB Infinity Cache loads it non-temporally, leaving the caches to the
+// gathered vector (uniform level 7: 188 -> 160-166 us); a smaller one keeps plain loads, so that
+// the copy stays cached between sweeps (on the level-6 adaptive meshes the hint cost 8.0 -> 9.2 us).
 constexpr int SELL_SLICE = 64;
 constexpr int SELL_WINDOW = 256;
+constexpr double SELL_STREAM_BYTES = 256.0 * 1024 * 1024;
 
 __host__ __device__ inline int sell_positions(int rows) { return (rows + SELL_SLICE - 1) / SELL_SLICE * SELL_SLICE; }
 
@@ -682,8 +686,8 @@ __device__ __forceinline__ bool sell_slot(const int* __restrict__ sp, const int*
 // (slots past the slice's length give (0, column 0) without a load, so the gathers are
 // unconditional and add exact zeros), in the order of row_entries<G> + group_sum<G>: entry k goes
 // to partial k mod G, each partial in entry order, the partials joined as the lane tree joins them
-// (G = 2: p0 + p1; G = 4: (p0 + p2) + (p1 + p3)).
-template <int MAXL, int G, typename T, typename F>
+// (G = 2: p0 + p1; G = 4: (p0 + p2) + (p1 + p3)). NT: non-temporal loads of the copy.
+template <int MAXL, int G, bool NT, typename T, typename F>
 __device__ __forceinline__ T sell_row_sum(const int* __restrict__ ci, const T* __restrict__ val, int base, int len,
                                          F xv) {
     int c[MAXL];
@@ -691,8 +695,14 @@ __device__ __forceinline__ T sell_row_sum(const int* __restrict__ ci, const T* _
 #pragma unroll
     for (int k = 0; k < MAXL; ++k) {
         const bool ok = k < len;
-        c[k] = ok ? ci[base + SELL_SLICE * k] : 0;
-        v[k] = ok ? val[base + SELL_SLICE * k] : T(0);
+        const int at = base + SELL_SLICE * k;
+        if constexpr (NT) {
+            c[k] = ok ? __builtin_nontemporal_load(ci + at) : 0;
+            v[k] = ok ? __builtin_nontemporal_load(val + at) : T(0);
+        } else {
+            c[k] = ok ? ci[at] : 0;
+            v[k] = ok ? val[at] : T(0);
+        }
     }
     T acc[G];
 #pragma unroll
@@ -707,7 +717,7 @@ __device__ __forceinline__ T sell_row_sum(const int* __restrict__ ci, const T* _
 }
 
 // y = b - A x (OP_RESID) | y = x + omega dinv (b - A x) (OP_JACOBI) on the sliced-ELL copy
-template <int MAXL, int G, int OP, typename T>
+template <int MAXL, int G, bool NT, int OP, typename T>
 __global__ void __launch_bounds__(BLOCK)
 k_sell_op(int positions, const int* __restrict__ sp, const int* __restrict__ srow, const int* __restrict__ ci,
           const T* __restrict__ val, const T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ dinv,
@@ -716,14 +726,14 @@ k_sell_op(int positions, const int* __restrict__ sp, const int* __restrict__ sro
     if (p >= positions) return;
     int row, base, len;
     if (!sell_slot(sp, srow, p, state, row, base, len)) return;
-    const T s = sell_row_sum<MAXL, G>(ci, val, base, len, [&](int j) { return x[j]; });
+    const T s = sell_row_sum<MAXL, G, NT>(ci, val, base, len, [&](int j) { return x[j]; });
     if (row < 0) return;
     if (OP == OP_RESID) y[row] = b[row] - s;
     else y[row] = x[row] + omega * dinv[row] * (b[row] - s);
 }
 
 // k_jacobi_from_zero on the sliced-ELL copy
-template <int MAXL, int G, typename T>
+template <int MAXL, int G, bool NT, typename T>
 __global__ void __launch_bounds__(BLOCK)
 k_sell_from_zero(int positions, const int* __restrict__ sp, const int* __restrict__ srow, const int* __restrict__ ci,
                  const T* __restrict__ val, const T* __restrict__ b, const T* __restrict__ dinv, T omega,
@@ -732,14 +742,14 @@ k_sell_from_zero(int positions, const int* __restrict__ sp, const int* __restric
     if (p >= positions) return;
     int row, base, len;
     if (!sell_slot(sp, srow, p, state, row, base, len)) return;
-    const T s = sell_row_sum<MAXL, G>(ci, val, base, len, [&](int j) { return omega * dinv[j] * b[j]; });
+    const T s = sell_row_sum<MAXL, G, NT>(ci, val, base, len, [&](int j) { return omega * dinv[j] * b[j]; });
     if (row < 0) return;
     const T x0 = omega * dinv[row] * b[row];
     y[row] = x0 + omega * dinv[row] * (b[row] - s);
 }
 
 // k_jacobi_from_zero_check on the sliced-ELL copy: the iteration's residual test, then the sweeps
-template <int MAXL, int G>
+template <int MAXL, int G, bool NT>
 __global__ void __launch_bounds__(BLOCK)
 k_sell_from_zero_check(int positions, const int* __restrict__ sp, const int* __restrict__ srow,
                        const int* __restrict__ ci, const double* __restrict__ val, const double* __restrict__ b,
@@ -752,7 +762,7 @@ k_sell_from_zero_check(int positions, const int* __restrict__ sp, const int* __r
     if (p >= positions) return;
     int row, base, len;
     (void)sell_slot(sp, srow, p, nullptr, row, base, len);
-    const double s = sell_row_sum<MAXL, G>(ci, val, base, len, [&](int j) { return omega * dinv[j] * b[j]; });
+    const double s = sell_row_sum<MAXL, G, NT>(ci, val, base, len, [&](int j) { return omega * dinv[j] * b[j]; });
     if (row < 0) return;
     const double x0 = omega * dinv[row] * b[row];
     y[row] = x0 + omega * dinv[row] * (b[row] - s);
@@ -1381,15 +1391,22 @@ void jacobi_from_zero_check_launch(int tpr, int rows, const int* rp, const int* 
     });
 }
 
-// MAXL (slots loaded per lane) of the sliced-ELL kernels for a longest row of maxl (at most
-// CFA_SELL_MAX_LEN) and G, the CSR lanes per row whose summation order they keep (1, 2 or 4)
+// MAXL (slots loaded per lane) of the sliced-ELL kernels for the level's longest row (at most
+// CFA_SELL_MAX_LEN), G, the CSR lanes per row whose summation order they keep (1, 2 or 4), and NT,
+// non-temporal loads when a sweep's bytes (values of vb bytes) exceed the Infinity Cache
 template <typename F>
-void sell_dispatch(int maxl, int lanes, F f) {
+void sell_dispatch(const cfa_amg_level_t& L, int vb, F f) {
+    const bool nt = (4.0 + vb) * L.a_nnz + (4.0 + 4.0 * vb) * L.rows > SELL_STREAM_BYTES;
     auto with_g = [&](auto m) {
-        if (lanes == 1) f(m, std::integral_constant<int, 1>{});
-        else if (lanes == 2) f(m, std::integral_constant<int, 2>{});
-        else f(m, std::integral_constant<int, 4>{});
+        auto with_nt = [&](auto g) {
+            if (nt) f(m, g, std::true_type{});
+            else f(m, g, std::false_type{});
+        };
+        if (L.tpr_a == 1) with_nt(std::integral_constant<int, 1>{});
+        else if (L.tpr_a == 2) with_nt(std::integral_constant<int, 2>{});
+        else with_nt(std::integral_constant<int, 4>{});
     };
+    const int maxl = L.s_len;
     if (maxl <= 8) with_g(std::integral_constant<int, 8>{});
     else if (maxl <= 12) with_g(std::integral_constant<int, 12>{});
     else with_g(std::integral_constant<int, 16>{});
@@ -1413,9 +1430,10 @@ void level_op_launch(const cfa_amg_level_t& L, const T* x, const T* b, const T* 
         return;
     }
     const int pos = sell_positions(L.rows);
-    sell_dispatch(L.s_len, L.tpr_a, [&](auto m, auto g) {
+    sell_dispatch(L, sizeof(T), [&](auto m, auto g, auto nt) {
         constexpr int M = decltype(m)::value, G = decltype(g)::value;
-        hipLaunchKernelGGL((k_sell_op<M, G, OP, T>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp, L.s_row,
+        constexpr bool NT = decltype(nt)::value;
+        hipLaunchKernelGGL((k_sell_op<M, G, NT, OP, T>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp, L.s_row,
                            L.s_ci, Lv<T>::sv(L), x, b, dinv, (T)L.omega, y, st);
     });
 }
@@ -1427,9 +1445,10 @@ void level_from_zero_launch(const cfa_amg_level_t& L, const T* b, const T* dinv,
         return;
     }
     const int pos = sell_positions(L.rows);
-    sell_dispatch(L.s_len, L.tpr_a, [&](auto m, auto g) {
+    sell_dispatch(L, sizeof(T), [&](auto m, auto g, auto nt) {
         constexpr int M = decltype(m)::value, G = decltype(g)::value;
-        hipLaunchKernelGGL((k_sell_from_zero<M, G, T>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp,
+        constexpr bool NT = decltype(nt)::value;
+        hipLaunchKernelGGL((k_sell_from_zero<M, G, NT, T>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp,
                            L.s_row, L.s_ci, Lv<T>::sv(L), b, dinv, (T)L.omega, y, st);
     });
 }
@@ -1441,9 +1460,10 @@ void level_from_zero_check_launch(const cfa_amg_level_t& L, const double* b, dou
         return;
     }
     const int pos = sell_positions(L.rows);
-    sell_dispatch(L.s_len, L.tpr_a, [&](auto m, auto g) {
+    sell_dispatch(L, sizeof(double), [&](auto m, auto g, auto nt) {
         constexpr int M = decltype(m)::value, G = decltype(g)::value;
-        hipLaunchKernelGGL((k_sell_from_zero_check<M, G>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp,
+        constexpr bool NT = decltype(nt)::value;
+        hipLaunchKernelGGL((k_sell_from_zero_check<M, G, NT>), dim3(blocks_for(pos)), dim3(BLOCK), 0, s, pos, L.s_sp,
                            L.s_row, L.s_ci, L.s_v, b, L.dinv, L.omega, y, hk.prr, hk.nb_rr, hk.sc, hk.st_w, hk.status);
     });
 }

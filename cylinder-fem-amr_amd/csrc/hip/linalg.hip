@@ -875,6 +875,68 @@ __global__ void k_ilu_gather(int nl, int nu, int n, const int* __restrict__ l_sr
     if (e < n) { const double v = 1.0 / lu[d_src[e]]; dinv[e] = v; if (dinv32) dinv32[e] = (float)v; }
 }
 
+// The sliced-ELL values of the split rows after a factorization: slot k takes split entry src[k]
+// (FP64 and, when kept, FP32 from the FP32 split values, so they equal the CSR path's), 0 for padding
+__global__ void k_ilu_sell_values(int ls, int us, const int* __restrict__ lsrc, const int* __restrict__ usrc,
+                                  const double* __restrict__ lv, const double* __restrict__ uv,
+                                  const float* __restrict__ lv32, const float* __restrict__ uv32,
+                                  double* __restrict__ slv, double* __restrict__ suv, float* __restrict__ slv32,
+                                  float* __restrict__ suv32) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < ls) {
+        const int e = lsrc[k];
+        slv[k] = e >= 0 ? lv[e] : 0.0;
+        if (slv32) slv32[k] = e >= 0 ? lv32[e] : 0.0f;
+    }
+    if (k < us) {
+        const int e = usrc[k];
+        suv[k] = e >= 0 ? uv[e] : 0.0;
+        if (suv32) suv32[k] = e >= 0 ? uv32[e] : 0.0f;
+    }
+}
+
+// One colour's sweep on the sliced-ELL rows, one lane per row: forward (UPPER false: acc = r_i -
+// sum l z, times dinv on the last colour) or backward (acc = z_i - sum u z, times dinv). All MAXL
+// slots are loaded before the gathers; a padding slot gathers nothing and subtracts an exact +0,
+// so each row's result is k_ilu_lower / k_ilu_upper's. mpos: the colour's positions (a multiple
+// of 64) from global position p0.
+template <int NRHS, int MAXL, bool UPPER, typename T>
+__global__ void __launch_bounds__(BLOCK)
+k_ilu_sweep_sell(int mpos, int n, int cs, int p0, const int* __restrict__ srow, const int* __restrict__ sp,
+                 const int* __restrict__ sc, const T* __restrict__ sv, const T* __restrict__ dinv, int last,
+                 const double* __restrict__ r, double* __restrict__ z, const int* __restrict__ st) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= mpos) return;
+    int t, base, len;
+    if (!sell_slot(sp, srow, p0 + p, st, t, base, len)) return;
+    int c[MAXL];
+    T v[MAXL];
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) {
+        const bool ok = k < len;
+        c[k] = ok ? sc[base + SELL_SLICE * k] : 0;
+        v[k] = ok ? sv[base + SELL_SLICE * k] : T(0);
+    }
+    if (t < 0) return;
+    const int i = cs + t;
+    double acc[NRHS];
+#pragma unroll
+    for (int q = 0; q < NRHS; ++q) acc[q] = UPPER ? z[(size_t)q * n + i] : r[(size_t)q * n + i];
+#pragma unroll
+    for (int k = 0; k < MAXL; ++k) {
+        const double a = (double)v[k];
+        const bool ok = k < len;
+#pragma unroll
+        for (int q = 0; q < NRHS; ++q) {
+            const double zj = ok ? z[(size_t)q * n + c[k]] : 0.0;
+            acc[q] -= a * zj;
+        }
+    }
+    const double d = (double)dinv[t];
+#pragma unroll
+    for (int q = 0; q < NRHS; ++q) z[(size_t)q * n + i] = (UPPER || last) ? acc[q] * d : acc[q];
+}
+
 // The ILU(0) apply runs in colour order: r is permuted there (read in row order, written into one
 // run per colour), the sweeps then stream every colour's rows, vector entries and factor entries,
 // and z is permuted back (written in row order). In row order a colour's rows are spread over the
@@ -1593,10 +1655,29 @@ void ilu_apply_impl(const cfa_ilu_t* ilu, const T* lv, const T* uv, const T* din
     // forward sweeps with the FP32 factors (mixed precision): two lanes per row; FP64: one lane, the
     // reference's order (the backward sweeps take one lane either way: two were no faster)
     constexpr bool lanes = std::is_same<T, float>::value;
+    // the one-lane sweeps on the sliced-ELL rows when the factor has them
+    const T* slv = std::is_same<T, float>::value ? (const T*)(const void*)ilu->s_lv32 : (const T*)(const void*)ilu->s_lv;
+    const T* suv = std::is_same<T, float>::value ? (const T*)(const void*)ilu->s_uv32 : (const T*)(const void*)ilu->s_uv;
+    const bool sell_l = ilu->s_lrow && slv && ilu->s_pos0, sell_u = ilu->s_urow && suv && ilu->s_pos0;
+    auto sell_sweep = [&](auto upper, int c, int len, const int* srow, const int* sp, const int* sc, const T* sv) {
+        const int cs = ilu->colour_start[c], mpos = (ilu->colour_start[c + 1] - cs + 63) / 64 * 64;
+        auto go = [&](auto mx) {
+            constexpr int M = decltype(mx)::value;
+            constexpr bool U = decltype(upper)::value;
+            hipLaunchKernelGGL((k_ilu_sweep_sell<NRHS, M, U, T>), dim3(blocks_for(mpos)), dim3(BLOCK), 0, s, mpos, n,
+                               cs, ilu->s_pos0[c], srow, sp, sc, sv, dinv + cs, (!U && c == nc - 1) ? 1 : 0, rp, zp,
+                               st);
+        };
+        if (len <= 8) go(std::integral_constant<int, 8>{});
+        else if (len <= 12) go(std::integral_constant<int, 12>{});
+        else go(std::integral_constant<int, 16>{});
+    };
     for (int c = 1; c < nc; ++c) {
         const int cs = ilu->colour_start[c], m = ilu->colour_start[c + 1] - cs;
         if (m == 0) continue;
-        if constexpr (lanes)
+        if (!lanes && sell_l)
+            sell_sweep(std::false_type{}, c, ilu->s_llen, ilu->s_lrow, ilu->s_lsp, ilu->s_lc, slv);
+        else if constexpr (lanes)
             hipLaunchKernelGGL((k_ilu_lower_lanes<NRHS, 2>), dim3(blocks_for(2LL * m)), dim3(BLOCK), 0, s, m, n, cs,
                                ilu->lp + cs, ilu->lc, lv, dinv + cs, c == nc - 1 ? 1 : 0, rp, zp, st);
         else
@@ -1606,6 +1687,10 @@ void ilu_apply_impl(const cfa_ilu_t* ilu, const T* lv, const T* uv, const T* din
     for (int c = nc - 2; c >= 1; --c) {
         const int cs = ilu->colour_start[c], m = ilu->colour_start[c + 1] - cs;
         if (m == 0) continue;
+        if (sell_u) {
+            sell_sweep(std::true_type{}, c, ilu->s_ulen, ilu->s_urow, ilu->s_usp, ilu->s_uc, suv);
+            continue;
+        }
         hipLaunchKernelGGL((k_ilu_upper<NRHS, T>), dim3(blocks_for(m)), dim3(BLOCK), 0, s, m, n, cs, ilu->up + cs,
                            ilu->uc, uv, dinv + cs, zp, st);
     }
@@ -2192,6 +2277,13 @@ int cfa_ilu0_factor(const cfa_ilu_t* ilu, const double* val, cfa_stream_t stream
     const int len = std::max(ilu->n, std::max(ilu->nl, ilu->nu));
     hipLaunchKernelGGL(k_ilu_gather, dim3(blocks_for(len)), dim3(BLOCK), 0, s, ilu->nl, ilu->nu, ilu->n, ilu->l_src,
                        ilu->u_src, ilu->d_src, ilu->lu, ilu->lv, ilu->uv, ilu->dinv, ilu->lv32, ilu->uv32, ilu->dinv32);
+    if (ilu->s_lrow && ilu->s_urow && ilu->s_lsrc && ilu->s_usrc && ilu->s_lv && ilu->s_uv) {
+        const int slots = std::max(std::max(ilu->s_lslots, ilu->s_uslots), 1);
+        hipLaunchKernelGGL(k_ilu_sell_values, dim3(blocks_for(slots)), dim3(BLOCK), 0, s, ilu->s_lslots, ilu->s_uslots,
+                           ilu->s_lsrc, ilu->s_usrc, ilu->lv, ilu->uv, ilu->lv32 ? ilu->lv32 : nullptr,
+                           ilu->uv32 ? ilu->uv32 : nullptr, ilu->s_lv, ilu->s_uv, ilu->lv32 ? ilu->s_lv32 : nullptr,
+                           ilu->uv32 ? ilu->s_uv32 : nullptr);
+    }
     return launch_status();
 }
 

@@ -184,7 +184,64 @@ void ilu_elimination_schedule(int n, const int* rp, const int* ci, const std::ve
     }
 }
 
-MulticolourIlu0::Pattern MulticolourIlu0::build_pattern(int n, const std::vector<int>& rp, const std::vector<int>& ci) {
+namespace {
+// Sliced-ELL layout of split rows (row pointer ptr over colour-order rows, columns col) colour by
+// colour: colour c's rows take positions pos0[c] + t (pos0[c] a multiple of 256); inside each
+// 256-position window the rows are ordered by decreasing length, ties by row; 64-position slices
+// hold their slots column-major, as many per position as the slice's first (longest) row has.
+// row: the colour's row at each position (-1 past its last); src: the split entry of each slot
+// (-1 for padding, whose column is 0).
+struct SellRows {
+    std::vector<int> row, sp, col, src;
+    int longest = 0;
+};
+SellRows sell_rows(const std::vector<int>& cs, const std::vector<int>& pos0, const std::vector<int>& ptr,
+                   const std::vector<int>& col) {
+    SellRows S;
+    const int nc = (int)cs.size() - 1, total = pos0[nc];
+    S.row.assign(total, -1);
+    std::vector<int> order;
+    for (int c = 0; c < nc; ++c) {
+        const int m = cs[c + 1] - cs[c];
+        auto len = [&](int t) { return ptr[cs[c] + t + 1] - ptr[cs[c] + t]; };
+        for (int w0 = 0; w0 < m; w0 += 256) {
+            order.clear();
+            for (int t = w0; t < std::min(m, w0 + 256); ++t) order.push_back(t);
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return len(a) > len(b); });
+            for (size_t k = 0; k < order.size(); ++k) S.row[pos0[c] + w0 + k] = order[k];
+        }
+    }
+    const int slices = total / 64;
+    S.sp.assign(slices + 1, 0);
+    for (int q = 0; q < slices; ++q) {
+        int L = 0;
+        for (int l = 0; l < 64; ++l) {
+            const int p = 64 * q + l, t = S.row[p];
+            if (t < 0) continue;
+            const int c = (int)(std::upper_bound(pos0.begin(), pos0.end(), p) - pos0.begin()) - 1;
+            L = std::max(L, ptr[cs[c] + t + 1] - ptr[cs[c] + t]);
+        }
+        S.sp[q + 1] = S.sp[q] + 64 * L;
+        S.longest = std::max(S.longest, L);
+    }
+    S.col.assign(S.sp[slices], 0);
+    S.src.assign(S.sp[slices], -1);
+    for (int c = 0; c < nc; ++c)
+        for (int p = pos0[c]; p < pos0[c + 1]; ++p) {
+            const int t = S.row[p];
+            if (t < 0) continue;
+            const int i = cs[c] + t, base = S.sp[p / 64] + p % 64;
+            for (int k = ptr[i]; k < ptr[i + 1]; ++k) {
+                S.col[base + 64 * (k - ptr[i])] = col[k];
+                S.src[base + 64 * (k - ptr[i])] = k;
+            }
+        }
+    return S;
+}
+}  // namespace
+
+MulticolourIlu0::Pattern MulticolourIlu0::build_pattern(int n, const std::vector<int>& rp, const std::vector<int>& ci,
+                                                        int sell_min_rows) {
     const auto t0 = Clock::now();
     Pattern P;
     P.n = n;
@@ -253,14 +310,25 @@ MulticolourIlu0::Pattern MulticolourIlu0::build_pattern(int n, const std::vector
         }
     }
     ilu_elimination_schedule(n, rp.data(), ci.data(), colour, P.rows, P.diag, P.el_ptr, P.el, P.upd);
+    if (n > 0 && n >= sell_min_rows && P.n_colours > 0) {
+        P.s_pos0.assign(P.n_colours + 1, 0);
+        for (int c = 0; c < P.n_colours; ++c)
+            P.s_pos0[c + 1] = P.s_pos0[c] + (P.colour_start[c + 1] - P.colour_start[c] + 255) / 256 * 256;
+        SellRows L = sell_rows(P.colour_start, P.s_pos0, P.lp, P.lc), U = sell_rows(P.colour_start, P.s_pos0, P.up, P.uc);
+        P.s_lrow = std::move(L.row); P.s_lsp = std::move(L.sp); P.s_lc = std::move(L.col); P.s_lsrc = std::move(L.src);
+        P.s_urow = std::move(U.row); P.s_usp = std::move(U.sp); P.s_uc = std::move(U.col); P.s_usrc = std::move(U.src);
+        P.s_llen = L.longest;
+        P.s_ulen = U.longest;
+    }
     P.build_ms = ms_since(t0);
     return P;
 }
 
 void MulticolourIlu0::set_pattern(int n_, const std::vector<int>& rp, const std::vector<int>& ci, const int* d_rp,
                                   const int* d_ci) {
-    set_pattern(build_pattern(n_, rp, ci), d_rp, d_ci);
+    set_pattern(build_pattern(n_, rp, ci, sell_min_rows), d_rp, d_ci);
 }
+
 
 void MulticolourIlu0::set_pattern(const Pattern& P, const int* d_rp, const int* d_ci) {
     const auto t0 = Clock::now();
@@ -296,6 +364,33 @@ void MulticolourIlu0::set_pattern(const Pattern& P, const int* d_rp, const int* 
         d_lv32.reserve(std::max(nl, 1), 256);
         d_uv32.reserve(std::max(nu, 1), 256);
         d_dinv32.reserve(n, 64);
+    }
+    // the sliced-ELL copies of the split rows for the one-lane sweeps
+    has_sell = false;
+    const char* env = std::getenv("CFA_SELL");
+    if (!P.s_pos0.empty() && !(env && env[0] == '0')) {
+        if (P.s_llen <= CFA_SELL_MAX_LEN && P.s_ulen <= CFA_SELL_MAX_LEN) {
+            s_pos0 = P.s_pos0;
+            s_llen = P.s_llen;
+            s_ulen = P.s_ulen;
+            s_lslots = (int)P.s_lc.size();
+            s_uslots = (int)P.s_uc.size();
+            s_lrow.upload(P.s_lrow);
+            s_lsp.upload(P.s_lsp);
+            s_lc.upload(P.s_lc.empty() ? std::vector<int>(1, 0) : P.s_lc);
+            s_lsrc.upload(P.s_lsrc.empty() ? std::vector<int>(1, -1) : P.s_lsrc);
+            s_urow.upload(P.s_urow);
+            s_usp.upload(P.s_usp);
+            s_uc.upload(P.s_uc.empty() ? std::vector<int>(1, 0) : P.s_uc);
+            s_usrc.upload(P.s_usrc.empty() ? std::vector<int>(1, -1) : P.s_usrc);
+            s_lv.reserve(std::max(s_lslots, 1), 256);
+            s_uv.reserve(std::max(s_uslots, 1), 256);
+            if (keep_fp32) {
+                s_lv32.reserve(std::max(s_lslots, 1), 256);
+                s_uv32.reserve(std::max(s_uslots, 1), 256);
+            }
+            has_sell = true;
+        }
     }
     refresh_view();
     colour_ms += P.build_ms + ms_since(t0);
@@ -333,6 +428,23 @@ void MulticolourIlu0::refresh_view() {
     view_.el_ptr = d_el_ptr;
     view_.el = d_el;
     view_.upd = d_upd;
+    view_.s_pos0 = has_sell ? s_pos0.data() : nullptr;
+    view_.s_lrow = has_sell ? s_lrow.ptr : nullptr;
+    view_.s_lsp = has_sell ? s_lsp.ptr : nullptr;
+    view_.s_lc = has_sell ? s_lc.ptr : nullptr;
+    view_.s_lsrc = has_sell ? s_lsrc.ptr : nullptr;
+    view_.s_lv = has_sell ? s_lv.ptr : nullptr;
+    view_.s_lv32 = has_sell && keep_fp32 ? s_lv32.ptr : nullptr;
+    view_.s_urow = has_sell ? s_urow.ptr : nullptr;
+    view_.s_usp = has_sell ? s_usp.ptr : nullptr;
+    view_.s_uc = has_sell ? s_uc.ptr : nullptr;
+    view_.s_usrc = has_sell ? s_usrc.ptr : nullptr;
+    view_.s_uv = has_sell ? s_uv.ptr : nullptr;
+    view_.s_uv32 = has_sell && keep_fp32 ? s_uv32.ptr : nullptr;
+    view_.s_llen = has_sell ? s_llen : 0;
+    view_.s_ulen = has_sell ? s_ulen : 0;
+    view_.s_lslots = has_sell ? s_lslots : 0;
+    view_.s_uslots = has_sell ? s_uslots : 0;
 }
 
 void MulticolourIlu0::factor(const double* d_val) { GPU_CHECK(cfa_ilu0_factor(&view_, d_val, solver_stream())); }

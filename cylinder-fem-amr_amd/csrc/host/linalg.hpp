@@ -88,10 +88,28 @@ public:
         int n = 0, nnz = 0, n_colours = 0, nl = 0, nu = 0;
         // lc, uc: colour-order positions of the split entries' columns; pos: of each row
         std::vector<int> colour_start, rows, colour, diag, lp, up, lc, uc, l_src, u_src, d_src, el_ptr, el, upd, pos;
+        // sliced-ELL layout of the split rows (cfa_ilu_t.s_*), built with the pattern
+        std::vector<int> s_pos0, s_lrow, s_lsp, s_lc, s_lsrc, s_urow, s_usp, s_uc, s_usrc;
+        int s_llen = 0, s_ulen = 0;
         double build_ms = 0.0;
     };
-    static Pattern build_pattern(int n, const std::vector<int>& rp, const std::vector<int>& ci);
+    // Below about a million rows the two-component apply's bytes (about 184 per row) stay in the
+    // 256 MB Infinity Cache and the CSR sweeps are as fast: the level-6 adaptive benchmark (300k
+    // rows) ran 62.9 against 63.6-64.1 steps/s with the copies
+    static constexpr int SELL_MIN_ROWS = 1 << 20;
+    static Pattern build_pattern(int n, const std::vector<int>& rp, const std::vector<int>& ci,
+                                 int sell_min_rows = SELL_MIN_ROWS);
     void set_pattern(const Pattern& P, const int* d_rp, const int* d_ci);
+    // sliced-ELL copies of the split rows (cfa_ilu_t.s_*): built with the pattern when it has at
+    // least sell_min_rows rows, every split row has at most CFA_SELL_MAX_LEN entries and CFA_SELL
+    // is not 0
+    int sell_min_rows = SELL_MIN_ROWS;
+    bool has_sell = false;
+    std::vector<int> s_pos0;
+    DevArray<int> s_lrow, s_lsp, s_lc, s_lsrc, s_urow, s_usp, s_uc, s_usrc;
+    DevArray<double> s_lv, s_uv;
+    DevArray<float> s_lv32, s_uv32;
+    int s_llen = 0, s_ulen = 0, s_lslots = 0, s_uslots = 0;
     void factor(const double* d_val);
     void apply(const double* r, double* z, int nrhs = 1) const;
     const cfa_ilu_t* view() const { return &view_; }

@@ -393,6 +393,23 @@ typedef struct {
      * of l_ik, CSR position of u_kk, first and end update); upd [2 per update] = (position of
      * a_ij, position of u_kj): a_ij -= l_ik u_kj */
     const int* el_ptr; const int* el; const int* upd;
+    /* Sliced-ELL copies of the split rows for the one-lane colour sweeps: the FP64 forward and
+     * backward sweeps and the FP32 backward sweep (cfa_sell_plan's layout, built per colour).
+     * NULL s_lrow / s_urow selects the CSR rows. Colour c's rows take positions s_pos0[c] + t,
+     * t < 64 ceil(m_c / 64); s_pos0 is a host array [n_colours + 1], each entry a multiple of 256.
+     * Inside 256-position windows the rows are ordered by decreasing length, ties by row.
+     *   s_lrow / s_urow: the colour's row t at each position, -1 past its last;
+     *   s_lsp / s_usp: slice offsets; entry k of the row at position p lies at sp[p / 64] + 64 k + p % 64;
+     *   s_lc / s_uc: colour-order columns (0 in padding);
+     *   s_lsrc / s_usrc: the index into lv / uv of each slot (-1 in padding);
+     *   s_lv / s_uv and their FP32 copies: the values, refreshed by cfa_ilu0_factor;
+     *   s_llen / s_ulen: the longest L / U row (at most CFA_SELL_MAX_LEN).
+     * The sweeps sum each row in its CSR order, and padding adds exact zeros, so the results are
+     * the CSR kernels'. */
+    const int* s_pos0;
+    const int* s_lrow; const int* s_lsp; const int* s_lc; const int* s_lsrc; double* s_lv; float* s_lv32;
+    const int* s_urow; const int* s_usp; const int* s_uc; const int* s_usrc; double* s_uv; float* s_uv32;
+    int s_llen, s_ulen, s_lslots, s_uslots;
 } cfa_ilu_t;
 
 /* Factor the values (linalg.cuh:531-541): lu = ILU(0) of val in colour order, then lv, uv,

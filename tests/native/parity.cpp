@@ -380,6 +380,7 @@ void case_ilu(Fixture& f) {
     f.g.d_A_val.upload(a);
     MulticolourIlu0 ilu;
     ilu.keep_fp32 = true;
+    ilu.sell_min_rows = 0;   // the sliced-ELL copies at this size too (by default from 2^20 rows)
     ilu.set_pattern(f.n, f.mesh.csr_row_ptr, f.mesh.csr_col_ind, f.g.d_row_ptr, f.g.d_col_ind);
     ilu.factor(f.g.d_A_val);
     oracle::Ilu0 o;
@@ -397,6 +398,22 @@ void case_ilu(Fixture& f) {
     Vec z(2 * (size_t)f.n);
     o.apply(r.data(), z.data(), 2);
     close_rel("(LU)^-1 r, two components", down(d_z, 2 * (size_t)f.n), z, 1e-13);
+    // the one-lane sweeps on the sliced-ELL rows against the same sweeps on the CSR rows: bit for
+    // bit (the rows sum in their CSR order; padding subtracts exact zeros)
+    expect(ilu.has_sell, "split rows carry the sliced-ELL copies");
+    if (ilu.has_sell) {
+        std::printf("  sliced ELL: L %d slots (%d entries), U %d slots (%d entries), longest rows %d / %d\n",
+                    ilu.s_lslots, ilu.nl, ilu.s_uslots, ilu.nu, ilu.s_llen, ilu.s_ulen);
+        cfa_ilu_t csr = *ilu.view();
+        csr.s_lrow = csr.s_urow = nullptr;
+        DevArray<double> d_z2;
+        d_z2.reserve(2 * (size_t)f.n);
+        GPU_CHECK(cfa_ilu0_apply(&csr, d_r, d_z2, 2, f.s()));
+        same_bits("(LU)^-1 r: sliced ELL = CSR rows", down(d_z, 2 * (size_t)f.n), down(d_z2, 2 * (size_t)f.n));
+        GPU_CHECK(cfa_ilu0_apply(ilu.view(), d_r, d_z, 1, f.s()));
+        GPU_CHECK(cfa_ilu0_apply(&csr, d_r, d_z2, 1, f.s()));
+        same_bits("(LU)^-1 r, one component: sliced ELL = CSR", down(d_z, f.n), down(d_z2, f.n));
+    }
 }
 
 void case_spmv(Fixture& f) {

@@ -1210,6 +1210,38 @@ void case_bicgstab(Fixture& f) {
         close_rel(prec ? "u (FP32 factors)" : "u (FP64 factors)", down(f.g.d_u, f.n), ou, prec ? 1e-8 : 1e-9);
         close_rel(prec ? "v (FP32 factors)" : "v (FP64 factors)", down(f.g.d_v, f.n), ov, prec ? 1e-8 : 1e-9);
     }
+    // the same solves with the sweeps on the split rows' sliced-ELL copies (built at this size on
+    // request; by default from 2^20 rows): iterates bit for bit, FP64 and FP32 factors
+    MulticolourIlu0 ilu_csr, ilu_sell;
+    ilu_csr.keep_fp32 = ilu_sell.keep_fp32 = true;
+    ilu_csr.sell_min_rows = 1 << 30;
+    ilu_sell.sell_min_rows = 0;
+    ilu_csr.set_pattern(f.n, f.mesh.csr_row_ptr, f.mesh.csr_col_ind, f.g.d_row_ptr, f.g.d_col_ind);
+    ilu_sell.set_pattern(f.n, f.mesh.csr_row_ptr, f.mesh.csr_col_ind, f.g.d_row_ptr, f.g.d_col_ind);
+    expect(!ilu_csr.has_sell && ilu_sell.has_sell, "CSR rows / sliced-ELL copies as requested");
+    ilu_csr.factor(f.g.d_A_val);
+    ilu_sell.factor(f.g.d_A_val);
+    for (int prec = 0; prec < 2; ++prec) {
+        Vec res[2][2];
+        int its[2][2];
+        for (int k = 0; k < 2; ++k) {
+            f.g.d_u.upload(u0);
+            f.g.d_v.upload(v0);
+            GpuBicgstab2 sv;
+            sv.precond_fp32 = prec == 1;
+            sv.solve(f.n, f.g.d_row_ptr, f.g.d_col_ind, f.g.d_A_val, f.g.d_rhs_u, f.g.d_rhs_v, f.g.d_u, f.g.d_v, 150,
+                     1e-10, k ? ilu_sell : ilu_csr);
+            res[k][0] = down(f.g.d_u, f.n);
+            res[k][1] = down(f.g.d_v, f.n);
+            its[k][0] = sv.iters[0];
+            its[k][1] = sv.iters[1];
+        }
+        expect(its[0][0] == its[1][0] && its[0][1] == its[1][1],
+               prec ? "iterations: sliced ELL = CSR (FP32 factors)" : "iterations: sliced ELL = CSR (FP64 factors)");
+        same_bits(prec ? "u, v: sliced ELL = CSR (FP32 factors)" : "u, v: sliced ELL = CSR (FP64 factors)",
+                  [&] { Vec a = res[1][0]; a.insert(a.end(), res[1][1].begin(), res[1][1].end()); return a; }(),
+                  [&] { Vec a = res[0][0]; a.insert(a.end(), res[0][1].begin(), res[0][1].end()); return a; }());
+    }
 }
 
 // The non-default solver paths: Jacobi-preconditioned BiCGSTAB on one momentum component

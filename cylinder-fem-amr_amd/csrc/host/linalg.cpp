@@ -310,7 +310,8 @@ MulticolourIlu0::Pattern MulticolourIlu0::build_pattern(int n, const std::vector
         }
     }
     ilu_elimination_schedule(n, rp.data(), ci.data(), colour, P.rows, P.diag, P.el_ptr, P.el, P.upd);
-    if (n > 0 && n >= sell_min_rows && P.n_colours > 0) {
+    const char* sell_env = std::getenv("CFA_SELL");
+    if (n > 0 && n >= sell_min_rows && P.n_colours > 0 && !(sell_env && sell_env[0] == '0')) {
         P.s_pos0.assign(P.n_colours + 1, 0);
         for (int c = 0; c < P.n_colours; ++c)
             P.s_pos0[c + 1] = P.s_pos0[c] + (P.colour_start[c + 1] - P.colour_start[c] + 255) / 256 * 256;
@@ -375,14 +376,19 @@ void MulticolourIlu0::set_pattern(const Pattern& P, const int* d_rp, const int* 
             s_ulen = P.s_ulen;
             s_lslots = (int)P.s_lc.size();
             s_uslots = (int)P.s_uc.size();
+            // a copy without slots still gets one-element arrays, so that its pointers are set
+            auto put = [](DevArray<int>& d, const std::vector<int>& h, int fill) {
+                if (h.empty()) d.upload(std::vector<int>(1, fill));
+                else d.upload(h);
+            };
             s_lrow.upload(P.s_lrow);
             s_lsp.upload(P.s_lsp);
-            s_lc.upload(P.s_lc.empty() ? std::vector<int>(1, 0) : P.s_lc);
-            s_lsrc.upload(P.s_lsrc.empty() ? std::vector<int>(1, -1) : P.s_lsrc);
+            put(s_lc, P.s_lc, 0);
+            put(s_lsrc, P.s_lsrc, -1);
             s_urow.upload(P.s_urow);
             s_usp.upload(P.s_usp);
-            s_uc.upload(P.s_uc.empty() ? std::vector<int>(1, 0) : P.s_uc);
-            s_usrc.upload(P.s_usrc.empty() ? std::vector<int>(1, -1) : P.s_usrc);
+            put(s_uc, P.s_uc, 0);
+            put(s_usrc, P.s_usrc, -1);
             s_lv.reserve(std::max(s_lslots, 1), 256);
             s_uv.reserve(std::max(s_uslots, 1), 256);
             if (keep_fp32) {

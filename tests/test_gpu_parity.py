@@ -4,8 +4,9 @@ V-cycle, PCG and BiCGSTAB2 to the relative tolerances printed by the program; th
 built on the device (CSR graph, incidence, checks) against the host build, on uniform and locally
 refined meshes and two planted defects; the re-projection's two solves side by side against one
 after the other, bit for bit; the sliced-ELL copy of the finest AMG level (layout against the CSR
-rows, sweeps bit for bit against the CSR kernel with one lane per row). Meshes: uniform levels 1
-and 3 of the reference's generator; fields seeded."""
+rows; sweeps and V-cycles bit for bit against the CSR kernels, also at the uniform level-7 size).
+Meshes: uniform levels 1 and 3 (and 7 for the sliced ELL) of the reference's generator; fields
+seeded."""
 import subprocess
 from pathlib import Path
 
@@ -25,6 +26,10 @@ SKIPPED = {("mms", 1): "ended once with SIGSEGV at exit; cause not found, kept o
            ("correction", 1): "ended once with SIGSEGV after ALL PASSED; cause not found, kept off the GPU"}
 PARAMS = [pytest.param(c, lv, marks=pytest.mark.skip(reason=SKIPPED[(c, lv)])) if (c, lv) in SKIPPED else (c, lv)
           for c in CASES for lv in (1, 3)]
+# the sliced-ELL level 0 at the uniform level-7 size (7,347,712 rows, 51.4 M entries), where a sweep
+# exceeds the Infinity Cache and the copy is loaded non-temporally: layout, sweeps and V-cycles bit
+# for bit against the CSR kernels
+PARAMS.append(("sell", 7))
 
 
 @pytest.mark.parametrize("case,level", PARAMS)

@@ -442,16 +442,90 @@ void amg_remap() {
                 it_remap, it_setup);
 }
 
+// Host check of the ILU(0) pattern's sliced-ELL layout (MulticolourIlu0::build_pattern, no device):
+// on a 120 x 120 grid graph with the nine-point stencil, for L and U, every colour's rows sit once
+// in their 256-position window, in decreasing length with ties by row, the slice offsets give each
+// slice its first row's length, every slot of a row holds its split entry (column and index) in
+// order, and padding holds column 0 and index -1.
+void ilu_sell_layout() {
+    const int side = 120, n = side * side;
+    std::vector<int> rp(1, 0), ci;
+    for (int j = 0; j < side; ++j)
+        for (int i = 0; i < side; ++i) {
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int di = -1; di <= 1; ++di) {
+                    const int a = i + di, b = j + dj;
+                    if (a >= 0 && b >= 0 && a < side && b < side) ci.push_back(b * side + a);
+                }
+            rp.push_back((int)ci.size());
+        }
+    const MulticolourIlu0::Pattern P = MulticolourIlu0::build_pattern(n, rp, ci, 0);
+    EXPECT(!P.s_pos0.empty(), "no sliced-ELL layout was built");
+    if (P.s_pos0.empty()) return;
+    const int nc = P.n_colours;
+    for (int part = 0; part < 2; ++part) {
+        const std::vector<int>& ptr = part ? P.up : P.lp;
+        const std::vector<int>& col = part ? P.uc : P.lc;
+        const std::vector<int>& row = part ? P.s_urow : P.s_lrow;
+        const std::vector<int>& sp = part ? P.s_usp : P.s_lsp;
+        const std::vector<int>& sc = part ? P.s_uc : P.s_lc;
+        const std::vector<int>& src = part ? P.s_usrc : P.s_lsrc;
+        int bad_place = 0, bad_order = 0, bad_len = 0, bad_slot = 0, bad_pad = 0, longest = 0;
+        for (int c = 0; c < nc; ++c) {
+            const int cs = P.colour_start[c], m = P.colour_start[c + 1] - cs;
+            std::vector<int> seen(m, 0);
+            for (int p = P.s_pos0[c]; p < P.s_pos0[c + 1]; ++p) {
+                const int t = row[p], w0 = P.s_pos0[c] + (p - P.s_pos0[c]) / 256 * 256;
+                const int q = p / 64, sl = (sp[q + 1] - sp[q]) / 64;
+                if (t < 0) {
+                    bad_place += p - P.s_pos0[c] < m ? 1 : 0;
+                    for (int k = 0; k < sl; ++k) bad_pad += sc[sp[q] + 64 * k + p % 64] != 0 || src[sp[q] + 64 * k + p % 64] != -1;
+                    continue;
+                }
+                if (t >= m || t < w0 - P.s_pos0[c] || t >= w0 - P.s_pos0[c] + 256 || seen[t]++) {
+                    ++bad_place;
+                    continue;
+                }
+                const int i = cs + t, len = ptr[i + 1] - ptr[i];
+                longest = std::max(longest, len);
+                if (p > w0 && row[p - 1] >= 0) {
+                    const int tp = row[p - 1], lp = ptr[cs + tp + 1] - ptr[cs + tp];
+                    bad_order += (lp < len || (lp == len && tp > t)) ? 1 : 0;
+                }
+                bad_len += (sl < len || (p % 64 == 0 && sl != len)) ? 1 : 0;
+                for (int k = 0; k < sl; ++k) {
+                    const int at = sp[q] + 64 * k + p % 64;
+                    if (k < len) bad_slot += sc[at] != col[ptr[i] + k] || src[at] != ptr[i] + k;
+                    else bad_pad += sc[at] != 0 || src[at] != -1;
+                }
+            }
+            for (int t = 0; t < m; ++t) bad_place += seen[t] == 1 ? 0 : 1;
+        }
+        const char* what = part ? "U" : "L";
+        EXPECT(bad_place == 0, "%s rows misplaced: %d", what, bad_place);
+        EXPECT(bad_order == 0, "%s window order broken: %d", what, bad_order);
+        EXPECT(bad_len == 0, "%s slice lengths wrong: %d", what, bad_len);
+        EXPECT(bad_slot == 0 && bad_pad == 0, "%s slots wrong: %d, padding wrong: %d", what, bad_slot, bad_pad);
+        EXPECT(longest == (part ? P.s_ulen : P.s_llen), "%s longest row %d, recorded %d", what, longest,
+               part ? P.s_ulen : P.s_llen);
+        std::printf("  ILU sliced ELL (%s): %d colours, %d slots for %d entries, longest row %d\n", what, nc,
+                    (int)sc.size(), (int)col.size(), longest);
+    }
+    const MulticolourIlu0::Pattern Q = MulticolourIlu0::build_pattern(n, rp, ci);
+    EXPECT(Q.s_pos0.empty(), "a layout below the default row threshold");
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
     const std::string what = argc > 1 ? argv[1] : "all";
-    if (what != "gpu" && what != "all") {
-        std::fprintf(stderr, "usage: test_cfa [gpu|all] (every check needs the GPU: the NVB forest lives there)\n");
+    if (what != "gpu" && what != "host" && what != "all") {
+        std::fprintf(stderr, "usage: test_cfa [gpu|host|all] (host: the checks that need no device)\n");
         return 2;
     }
     std::printf("cylinder_fem_amr unit tests (%s)\n", what.c_str());
-    {
+    if (what != "gpu") ilu_sell_layout();
+    if (what != "host") {
         GPU_CHECK(cfa_init(0));
         random_adaptation();
         round_trip();
